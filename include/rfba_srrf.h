@@ -1,0 +1,184 @@
+/*
+ * rfba_srrf.h -- C ABI of the MI355X (gfx950) 3D SRRF radiality library
+ * (librfba_srrf.so).
+ *
+ * Three entry points keep the reference's DLL contract exactly (same names,
+ * argument meaning, struct layout and INI file semantics):
+ *
+ *   SRRFGradientCalc     replaces RFBA Main/GradientCalc/GradientCalc.h:4
+ *                        (GradientCalc.cpp:169-173, tiling :120-163)
+ *   SRRFGetMag           replaces GradientCalc.h:5 (GradientCalc.cpp:176-178)
+ *   SRRFGradientCalcGPU  replaces RFBA Main/GradientCalc/kernel.cuh:26
+ *                        (kernel.cu:560-657)
+ *
+ * The rfba_srrf_* entry points are additions: explicit sampling sets,
+ * uint16 input, device-resident buffers, and the batched form.
+ *
+ * Conventions: every volume is x-fastest, [t][z][y][x] for series and
+ * [z][y][x] for SR outputs, exactly as getIdx3 (kernel.cu:36-40).
+ * Functions return RFBA_OK (0) on success, a negative rfba_status otherwise;
+ * rfba_last_error() describes the last failure of the calling thread.
+ */
+#ifndef RFBA_SRRF_H
+#define RFBA_SRRF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The reference never reports errors (it always returns 0);
+ * this library returns 0 on success exactly as the reference does and a
+ * negative code on failure. */
+typedef enum rfba_status {
+    RFBA_OK = 0,
+    RFBA_EINVAL = -1,  /* invalid argument / configuration */
+    RFBA_ENOMEM = -2,  /* host or device allocation failed */
+    RFBA_EDEVICE = -3, /* HIP runtime error (launch, copy, sync) */
+    RFBA_ENODEV = -4   /* no usable gfx950 device */
+} rfba_status;
+
+/* Byte-for-byte the reference's SRRFConf (kernel.cuh:3-24): 17 x 4 bytes. */
+typedef struct SRRFConf {
+    float RingRadius;   /* ring radius in SR pixels (kernel.cu:220-222) */
+    int Magnification;  /* M */
+    int AxesinRing;     /* nRing = 2*AxesinRing, nRing^2 samples */
+    int MainRange;      /* gradient box half-width along the derivative axis */
+    int XRange;
+    int YRange;
+    int ZRange;
+    int Method;         /* 0 TRM, 1 TRA, 2 TRPPM, 3 TRAC */
+    int ITC;            /* TRAC: integrate lag times (binning levels) */
+    int TRACOrder;      /* 2, 3 or 4 */
+    int RemoveCon;      /* 1 applies max(radiality, 0) (naming as reference) */
+    int Renormalize;    /* 1: 0.5 + radiality/2 */
+    int GradientSmooth; /* 0: central difference, 1: box sums */
+    int IntensityW;     /* intensity weighting */
+    int blockSizeX;     /* tiling of SRRFGradientCalc (default 16) */
+    int blockSizeY;
+    int blockSizeZ;
+} SRRFConf;
+
+/* ---- reference-compatible entry points ------------------------------- */
+
+/* Whole-volume SRRF with the reference's implicit configuration: reads
+ * ".\SRRF.ini" (here "./SRRF.ini") on every call, defaults when absent
+ * (GradientCalc.cpp:15-42), splits the volume into blockSize^3 tiles with a
+ * 2-voxel border and stitches them with the reference's put-back rule
+ * (GradientCalc.cpp:47-116): the global low band [0, 2M) of every axis is
+ * left untouched, later tiles overwrite earlier ones.
+ * inImage: float[T][Z][Y][X] host; imageDim = {X, Y, Z, T};
+ * outImage: caller-allocated float[Z*M][Y*M][X*M] host. */
+int SRRFGradientCalc(float *inImage, int *imageDim, float *outImage);
+
+/* Magnification from "./SRRF.ini" ([SRRF] Magnification, default 5). */
+int SRRFGetMag(void);
+
+/* One stack, no tiling, explicit configuration, reference ring sampling. */
+int SRRFGradientCalcGPU(float *inImage, int *imageDim, float *outImage,
+                        SRRFConf *conf);
+
+/* ---- configuration helpers ------------------------------------------- */
+
+/* Parse an SRRF.ini-style file with the reference's section/key names and
+ * defaults (GradientCalc.cpp:15-42). path == NULL reads "./SRRF.ini".
+ * A missing file yields the defaults and RFBA_OK. */
+int rfba_srrf_load_ini(const char *path, SRRFConf *conf);
+
+/* Fill conf with the reference defaults (GradientCalc.cpp:19-41). */
+void rfba_srrf_default_conf(SRRFConf *conf);
+
+/* ---- sampling sets ------------------------------------------------------
+ * A sampling set is an ordered list of n offsets (x, y, z interleaved, in SR
+ * pixels) added to the SR voxel centre, plus the divisor applied to the
+ * signed sum of the per-sample terms.  Reference ring mode: n = nRing^2,
+ * divisor = nRing (kernel.cu:23-33, 244). */
+typedef struct rfba_srrf_samples {
+    int n;              /* number of samples, 1..RFBA_SRRF_MAX_SAMPLES */
+    float divisor;      /* radiality = signed sum / divisor */
+    const float *xyz;   /* host pointer, 3*n floats */
+} rfba_srrf_samples;
+
+#define RFBA_SRRF_MAX_SAMPLES 1024
+
+/* Reference ring: nRing = 2*axes_in_ring; writes 3*nRing^2 floats in the
+ * reference's order (idx = i1*nRing + i2) and returns nRing^2 (or < 0). */
+int rfba_srrf_build_ring(int axes_in_ring, float radius, float *xyz_out);
+
+/* Axis-aligned (kind 6) or 3x3x3-neighbourhood (kind 26) direction sets
+ * scaled to `radius`; offsets are computed in double and rounded once.
+ * Samples sharing an (x, y) offset are listed consecutively.  Returns n. */
+int rfba_srrf_build_dirs(int kind, float radius, float *xyz_out);
+
+/* ---- explicit single-stack entry points --------------------------------
+ * dtype: RFBA_U16 (uint16 frames) or RFBA_F32 (float frames).
+ * samples == NULL selects the reference ring from conf. */
+#define RFBA_U16 1
+#define RFBA_F32 2
+
+/* Host buffers in and out (includes H2D/D2H). */
+int rfba_srrf_stack(const void *in, int dtype, const int dims[4],
+                    const SRRFConf *conf, const rfba_srrf_samples *samples,
+                    float *out);
+
+/* Device-resident buffers: d_in and d_out are device pointers; the call is
+ * asynchronous on `stream` (hipStream_t, NULL = default stream) unless
+ * kernel_ms != NULL, in which case it synchronises and reports the time
+ * between HIP events recorded around the kernel launches on that stream. */
+int rfba_srrf_stack_device(const void *d_in, int dtype, const int dims[4],
+                           const SRRFConf *conf,
+                           const rfba_srrf_samples *samples, float *d_out,
+                           void *stream, float *kernel_ms);
+
+/* ---- batched form over uneven blocks -----------------------------------
+ * A block is a sub-volume (origin, extent) of a source series with its own
+ * magnification, or a whole stack; every block is computed exactly as if it
+ * were passed alone to rfba_srrf_stack (its own clamp-to-edge borders), and
+ * its SR result goes to its own output buffer.  One persistent work-queue
+ * launch covers all blocks. */
+typedef struct rfba_srrf_block {
+    const void *d_src;  /* device pointer to the source series */
+    int src_dims[4];    /* X, Y, Z, T of the source series */
+    int origin[3];      /* block origin inside the source (low-res voxels) */
+    int dims[3];        /* block extent (low-res voxels) */
+    int frames;         /* number of frames used (<= src_dims[3]) */
+    int mag;            /* magnification of this block */
+    float *d_out;       /* device float[dims.z*mag][dims.y*mag][dims.x*mag] */
+} rfba_srrf_block;
+
+typedef struct rfba_srrf_batch rfba_srrf_batch; /* opaque plan */
+
+/* Build a plan (work queue, tables, autotuned tile shape) for n blocks that
+ * share dtype, conf (Method, smoothing, ...) and samples.  The plan owns its
+ * device tables and is reusable while the block list is unchanged. */
+int rfba_srrf_batch_create(const rfba_srrf_block *blocks, int n, int dtype,
+                           const SRRFConf *conf,
+                           const rfba_srrf_samples *samples,
+                           rfba_srrf_batch **plan);
+/* Run: one persistent launch over all blocks (captured into a hipGraph on
+ * the first call when use_graph != 0, replayed afterwards). */
+int rfba_srrf_batch_run(rfba_srrf_batch *plan, void *stream, int use_graph,
+                        float *kernel_ms);
+/* Reference form for comparison: one launch per block on the same stream. */
+int rfba_srrf_batch_run_per_block(rfba_srrf_batch *plan, void *stream,
+                                  float *kernel_ms);
+void rfba_srrf_batch_destroy(rfba_srrf_batch *plan);
+
+/* ---- runtime helpers ---------------------------------------------------- */
+const char *rfba_last_error(void);
+int rfba_device_count(void);
+int rfba_malloc_device(void **ptr, size_t bytes);
+int rfba_free_device(void *ptr);
+int rfba_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int rfba_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int rfba_memset_device(void *dst, int value, size_t bytes);
+int rfba_device_synchronize(void);
+/* Release the library's cached device scratch (kept between calls). */
+void rfba_release_cache(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RFBA_SRRF_H */

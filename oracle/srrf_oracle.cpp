@@ -1,0 +1,546 @@
+// srrf_oracle.cpp -- TEST INFRASTRUCTURE ONLY (the oracle).
+//
+// Host restatement of the reference SRRF arithmetic, used by tests/, by
+// __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker.
+// No product code path links or calls it.
+//
+// Every function follows RFBA Main/GradientCalc/kernel.cu in identical
+// operation order (file:line cited per function).  The FP32 build is compiled
+// with -ffp-contract=off so that a*b+c stays two roundings, as written.  The
+// same templates instantiated with double give the FP64 re-evaluation.
+//
+// Deliberate definitions where the reference is undefined:
+//   * interpolation taps at u == width (weight exactly 0, kernel.cu:146-166
+//     reads the next row) read the clamped neighbour instead; 0*finite == 0
+//     so the sum is unchanged;
+//   * TRAC/ITC binning reads past the last frame at the first level
+//     (kernel.cu:533-536 with odd T) read 0.0 instead of foreign memory.
+//
+// Parity status of each function (what pins it) is listed in tests/.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+struct SRRFConf {
+    float RingRadius;
+    int Magnification, AxesinRing, MainRange, XRange, YRange, ZRange;
+    int Method, ITC, TRACOrder, RemoveCon, Renormalize, GradientSmooth,
+        IntensityW, blockSizeX, blockSizeY, blockSizeZ;
+};
+
+namespace {
+
+template <typename R> struct Consts;
+template <> struct Consts<float> {
+    static float half() { return 0.5f; }
+};
+template <> struct Consts<double> {
+    static double half() { return 0.5; }
+};
+
+inline size_t idx3(int x, int y, int z, int t, int w, int h, int d) {
+    // getIdx3, kernel.cu:36-40 (size_t here: the int form overflows).
+    return (size_t)t * w * h * d + (size_t)z * w * h + (size_t)y * w + x;
+}
+
+// buildRing, kernel.cu:23-33.  FP32: float angle, float sin/cos overloads.
+void build_ring_f32(float *xyz, float r, int n) {
+    const float PI = 3.141592654f;
+    float step = (PI * 2.0f) / (float)n;
+    for (int a1 = 0; a1 < n; a1++)
+        for (int a2 = 0; a2 < n; a2++) {
+            int i = a1 * n + a2;
+            xyz[3 * i + 0] = r * std::sin(step * a1) * std::cos(step * a2);
+            xyz[3 * i + 1] = r * std::sin(step * a1) * std::sin(step * a2);
+            xyz[3 * i + 2] = r * std::cos(step * a1);
+        }
+}
+void build_ring_f64(double *xyz, double r, int n) {
+    const double PI = 3.141592654;
+    double step = (PI * 2.0) / (double)n;
+    for (int a1 = 0; a1 < n; a1++)
+        for (int a2 = 0; a2 < n; a2++) {
+            int i = a1 * n + a2;
+            xyz[3 * i + 0] = r * std::sin(step * a1) * std::cos(step * a2);
+            xyz[3 * i + 1] = r * std::sin(step * a1) * std::sin(step * a2);
+            xyz[3 * i + 2] = r * std::cos(step * a1);
+        }
+}
+
+inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// gradKernel, kernel.cu:44-125, for one (x, y, z, t).  The reference clamps
+// through fminf/fmaxf on ints; for the sizes handled (< 2^24) that equals an
+// integer clamp.
+template <typename R>
+void grad_at(const R *p, int x, int y, int z, int t, int W, int H, int D,
+             const SRRFConf &c, R &gx, R &gy, R &gz) {
+    if (c.GradientSmooth == 0) {
+        int x0 = std::max(x - 1, 0), x1 = std::min(x + 1, W - 1);
+        int y0 = std::max(y - 1, 0), y1 = std::min(y + 1, H - 1);
+        int z0 = std::max(z - 1, 0), z1 = std::min(z + 1, D - 1);
+        gx = -p[idx3(x0, y, z, t, W, H, D)] + p[idx3(x1, y, z, t, W, H, D)];
+        gy = -p[idx3(x, y0, z, t, W, H, D)] + p[idx3(x, y1, z, t, W, H, D)];
+        gz = -p[idx3(x, y, z0, t, W, H, D)] + p[idx3(x, y, z1, t, W, H, D)];
+        return;
+    }
+    const int mR = c.MainRange, xR = c.XRange, yR = c.YRange, zR = c.ZRange;
+    R v = 0;
+    for (int i = -mR; i <= mR; i++)
+        for (int j = -yR; j <= yR; j++)
+            for (int k = -zR; k <= zR; k++) {
+                int x_ = clampi(x + i, 0, W - 1), y_ = clampi(y + j, 0, H - 1),
+                    z_ = clampi(z + k, 0, D - 1);
+                if (i < 0) v -= p[idx3(x_, y_, z_, t, W, H, D)];
+                else if (i > 0) v += p[idx3(x_, y_, z_, t, W, H, D)];
+            }
+    gx = v;
+    v = 0;
+    for (int j = -mR; j <= mR; j++)
+        for (int i = -xR; i <= xR; i++)
+            for (int k = -zR; k <= zR; k++) {
+                int x_ = clampi(x + i, 0, W - 1), y_ = clampi(y + j, 0, H - 1),
+                    z_ = clampi(z + k, 0, D - 1);
+                if (j < 0) v -= p[idx3(x_, y_, z_, t, W, H, D)];
+                else if (j > 0) v += p[idx3(x_, y_, z_, t, W, H, D)];
+            }
+    gy = v;
+    v = 0;
+    for (int k = -mR; k <= mR; k++)
+        for (int i = -xR; i <= xR; i++)
+            for (int j = -yR; j <= yR; j++) {
+                int x_ = clampi(x + i, 0, W - 1), y_ = clampi(y + j, 0, H - 1),
+                    z_ = clampi(z + k, 0, D - 1);
+                if (k < 0) v -= p[idx3(x_, y_, z_, t, W, H, D)];
+                else if (k > 0) v += p[idx3(x_, y_, z_, t, W, H, D)];
+            }
+    gz = v;
+}
+
+// cubic, kernel.cu:128-138 (Catmull-Rom, a = 0.5).
+template <typename R> R cubic(R xx) {
+    const R a = Consts<R>::half();
+    R x = xx;
+    if (x < (R)0) x = -x;
+    R z = 0;
+    if (x < (R)1)
+        z = x * x * (x * (-a + (R)2) + (a - (R)3)) + (R)1;
+    else if (x < (R)2)
+        z = -a * x * x * x + (R)5 * a * x * x - (R)8 * a * x + (R)4 * a;
+    return z;
+}
+
+// Read of a tap inside one frame; u == width (weight 0) reads the clamped
+// neighbour (see header).
+template <typename R>
+inline R tap(const R *a, int u, int v, int w, int t, int W, int H, int D) {
+    u = clampi(u, 0, W - 1);
+    v = clampi(v, 0, H - 1);
+    w = clampi(w, 0, D - 1);
+    return a[idx3(u, v, w, t, W, H, D)];
+}
+
+// interpolateG, kernel.cu:141-169 (also interpolateIntensity without the
+// final clamp, kernel.cu:257-287).
+template <typename R>
+R interp(R x, R y, R z, int t, int mag, int W, int H, int D, const R *arr) {
+    x = x / (R)mag;
+    y = y / (R)mag;
+    z = z / (R)mag;
+    if (x < (R)1.5 || x > (R)W - (R)1.5 || y < (R)1.5 || y > (R)H - (R)1.5 ||
+        z < (R)1.5 || z > (R)D - (R)1.5)
+        return 0;
+    int u0 = (int)std::floor(x - (R)0.5);
+    int v0 = (int)std::floor(y - (R)0.5);
+    int w0 = (int)std::floor(z - (R)0.5);
+    R s = 0;
+    for (int k = 0; k <= 3; k++) {
+        int w = w0 - 1 + k;
+        R q = 0;
+        for (int j = 0; j <= 3; j++) {
+            int v = v0 - 1 + j;
+            R p = 0;
+            for (int i = 0; i <= 3; i++) {
+                int u = u0 - 1 + i;
+                p = p + tap(arr, u, v, w, t, W, H, D) * cubic<R>(x - ((R)u + (R)0.5));
+            }
+            q = q + p * cubic<R>(y - ((R)v + (R)0.5));
+        }
+        s = s + q * cubic<R>(z - ((R)w + (R)0.5));
+    }
+    return s;
+}
+
+template <typename R>
+R interp_intensity(R x, R y, R z, int t, int mag, int W, int H, int D, const R *px) {
+    // interpolateIntensity, kernel.cu:257-287: 0 outside, else max(s, 0).
+    R xs = x / (R)mag, ys = y / (R)mag, zs = z / (R)mag;
+    if (xs < (R)1.5 || xs > (R)W - (R)1.5 || ys < (R)1.5 || ys > (R)H - (R)1.5 ||
+        zs < (R)1.5 || zs > (R)D - (R)1.5)
+        return 0;
+    R s = interp(x, y, z, t, mag, W, H, D, px);
+    return std::fmax(s, (R)0);
+}
+
+// _calculateDk, kernel.cu:174-183 (the last argument receives GMag).
+template <typename R>
+R calc_dk(R x, R y, R z, R Xc, R Yc, R Zc, R Gx, R Gy, R Gz, R GMag) {
+    R tt = (Gx * (x - Xc) + Gy * (y - Yc) + Gz * (z - Zc)) / (GMag * GMag);
+    R xc = Gx * tt + Xc;
+    R yc = Gy * tt + Yc;
+    R zc = Gz * tt + Zc;
+    return std::sqrt((x - xc) * (x - xc) + (y - yc) * (y - yc) + (z - zc) * (z - zc));
+}
+
+template <typename R> struct Job {
+    const R *img;      // [T][D][H][W]
+    const R *gx, *gy, *gz;
+    R *rad;            // [T][D*M][H*M][W*M]
+    R *out;            // [D*M][H*M][W*M]
+    const R *ring;     // 3*n
+    int n;
+    R divisor;
+    int W, H, D, T, M;
+    SRRFConf c;
+};
+
+// calculateRadiality, kernel.cu:186-253, for SR voxels [v0, v1).
+template <typename R> void radiality_range(const Job<R> &j, size_t v0, size_t v1) {
+    const int WM = j.W * j.M, HM = j.H * j.M, DM = j.D * j.M;
+    const R radius = (R)j.c.RingRadius;
+    for (size_t pix = v0; pix < v1; pix++) {
+        int X = (int)(pix % WM), Y = (int)((pix / WM) % HM), Z = (int)((pix / ((size_t)WM * HM)) % DM);
+        for (int t = 0; t < j.T; t++) {
+            R Xc = (R)X + (R)0.5, Yc = (R)Y + (R)0.5, Zc = (R)Z + (R)0.5;
+            R DivD = 0;
+            for (int s = 0; s < j.n; s++) {
+                const R rx = j.ring[3 * s], ry = j.ring[3 * s + 1], rz = j.ring[3 * s + 2];
+                R x0 = Xc + rx, y0 = Yc + ry, z0 = Zc + rz;
+                R Gx = interp(x0, y0, z0, t, j.M, WM / j.M, HM / j.M, DM / j.M, j.gx);
+                R Gy = interp(x0, y0, z0, t, j.M, WM / j.M, HM / j.M, DM / j.M, j.gy);
+                R Gz = interp(x0, y0, z0, t, j.M, WM / j.M, HM / j.M, DM / j.M, j.gz);
+                R GMag = std::sqrt(Gx * Gx + Gy * Gy + Gz * Gz);
+                R GdotR = (Gx * rx + Gy * ry + Gz * rz) / (GMag * radius);
+                bool diverging = (GdotR > (R)0 || GdotR != GdotR);
+                R Dk = (R)1 - calc_dk(x0, y0, z0, Xc, Yc, Zc, Gx, Gy, Gz, GMag) / radius;
+                Dk = Dk * Dk;
+                if (diverging) DivD -= Dk;
+                else DivD += Dk;
+            }
+            DivD = DivD / j.divisor;
+            if (j.c.Renormalize == 1) DivD = (R)0.5 + (DivD / (R)2);
+            R CGH = (j.c.RemoveCon == 1) ? std::fmax(DivD, (R)0) : DivD;
+            j.rad[(size_t)t * WM * HM * DM + pix] = CGH;
+        }
+    }
+}
+
+// Temporal reductions for SR voxels [v0, v1): calculateMIP :332-366,
+// calculateAVE :291-327, calculatePairwiseProductSum :372-419,
+// calculateACRF :423-555.
+template <typename R> void temporal_range(const Job<R> &j, size_t v0, size_t v1) {
+    const int WM = j.W * j.M, HM = j.H * j.M, DM = j.D * j.M;
+    const size_t N = (size_t)WM * HM * DM;
+    const int T = j.T, M = j.M;
+    const int W = j.W, H = j.H, D = j.D;
+    std::vector<R> ser(T);
+    for (size_t pix = v0; pix < v1; pix++) {
+        int x = (int)(pix % WM), y = (int)((pix / WM) % HM), z = (int)((pix / ((size_t)WM * HM)) % DM);
+        R _x = (R)x + (R)0.5, _y = (R)y + (R)0.5, _z = (R)z + (R)0.5;
+        auto I = [&](int t) { return interp_intensity(_x, _y, _z, t, M, W, H, D, j.img); };
+        auto rad = [&](int t) { return j.rad[(size_t)t * N + pix]; };
+        R out = 0;
+        const int iw = j.c.IntensityW;
+        if (j.c.Method == 0) {
+            R MIP = 0;
+            for (int t = 0; t < T; t++)
+                MIP = std::fmax(iw == 1 ? rad(t) * I(t) : rad(t), MIP);
+            out = MIP;
+        } else if (j.c.Method == 1) {
+            R mean = 0;
+            for (int t = 0; t < T; t++) mean += (iw == 1) ? rad(t) * I(t) : rad(t);
+            out = mean / (R)T;
+        } else if (j.c.Method == 2) {
+            R pps = 0, r0, r1, IW = 0;
+            int counter = 0;
+            for (int t0 = 0; t0 < T; t0++) {
+                r0 = std::fmax(rad(t0), (R)0);
+                if (r0 > 0) {
+                    for (int t1 = t0; t1 < T; t1++) {
+                        r1 = std::fmax(rad(t1), (R)0);
+                        pps += r0 * r1;
+                        counter++;
+                    }
+                } else
+                    counter += T - t0;
+            }
+            pps /= std::fmax((R)counter, (R)1);
+            if (iw == 1) {
+                for (int t = 0; t < T; t++) IW += I(t) / (R)T;
+                pps *= IW;
+            }
+            out = pps;
+        } else if (j.c.Method == 3) {
+            const int ord = j.c.TRACOrder;
+            R mean = 0, IW = 0;
+            for (int t = 0; t < T; t++) {
+                IW += rad(t) * I(t);
+                mean += rad(t);
+            }
+            mean = mean / (R)T;
+            IW = IW / (R)T;
+            for (int t = 0; t < T; t++) ser[t] = rad(t);
+            // Series read with the ITC out-of-range rule (0 past the end).
+            auto r = [&](int t) -> R { return t < T ? ser[t] : (R)0; };
+            R ABCD = 0, ABC = 0, AB = 0, CD = 0, AC = 0, BD = 0, AD = 0, BC = 0;
+            R A, B, C, Dd;
+            R acc = 0;
+            if (j.c.ITC != 1) {
+                int t = 0;
+                while (t < (T - ord)) {
+                    AB += ((r(t) - mean) * (r(t + 1) - mean));
+                    if (ord == 3) ABC += ((r(t) - mean) * (r(t + 1) - mean) * (r(t + 2) - mean));
+                    if (ord == 4) {
+                        A = r(t) - mean; B = r(t + 1) - mean; C = r(t + 2) - mean; Dd = r(t + 3) - mean;
+                        ABCD += A * B * C * Dd;
+                        CD += C * Dd; AC += A * C; BD += B * Dd; AD += A * Dd; BC += B * C;
+                    }
+                    t++;
+                }
+                if (ord == 3) acc = std::fabs(ABC) / (R)T;
+                else if (ord == 4) acc = std::fabs(ABCD - AB * CD - AC * BD - AD * BC) / (R)T;
+                else acc = std::fabs(AB) / (R)T;
+            } else {
+                int nB = T;
+                while (nB > ord) {
+                    int t = 0;
+                    AB = 0;
+                    while (t < (nB - ord)) {
+                        int tBin = t * ord;
+                        AB += ((r(t) - mean) * (r(t + 1) - mean));
+                        if (ord == 3) ABC += ((r(t) - mean) * (r(t + 1) - mean) * (r(t + 2) - mean));
+                        if (ord == 4) {
+                            A = r(t) - mean; B = r(t + 1) - mean; C = r(t + 2) - mean; Dd = r(t + 3) - mean;
+                            ABCD += A * B * C * Dd;
+                            CD += C * Dd; AC += A * C; BD += B * Dd; AD += A * Dd; BC += B * C;
+                        }
+                        ser[t] = 0;
+                        if (tBin < nB)
+                            for (int _t = 0; _t < ord; _t++) ser[t] += r(tBin + _t) / (R)ord;
+                        t++;
+                    }
+                    if (ord == 3) acc += std::fabs(ABC) / (R)nB;
+                    else if (ord == 4) acc += std::fabs((ABCD - AB * CD - AC * BD - AD * BC)) / (R)nB;
+                    else acc += std::fabs(AB) / (R)nB;
+                    nB = nB / ord;
+                }
+            }
+            if (iw == 1) acc = IW * acc;
+            out = acc;
+        }
+        j.out[pix] = out;
+    }
+}
+
+template <typename F> void parallel_for(size_t n, int threads, F f) {
+    if (threads <= 1 || n < 64) {
+        f(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    size_t chunk = (n + threads - 1) / threads;
+    for (int i = 0; i < threads; i++) {
+        size_t a = i * chunk, b = std::min(n, a + chunk);
+        if (a >= b) break;
+        th.emplace_back([=] { f(a, b); });
+    }
+    for (auto &t : th) t.join();
+}
+
+template <typename R>
+int stack_impl(const R *img, const int *dims, const SRRFConf *conf, const R *ring, int n,
+               R divisor, R *out, R *rad_out, int threads) {
+    Job<R> j;
+    j.W = dims[0]; j.H = dims[1]; j.D = dims[2]; j.T = dims[3];
+    j.M = conf->Magnification;
+    j.c = *conf;
+    j.n = n;
+    j.divisor = divisor;
+    j.ring = ring;
+    j.img = img;
+    if (j.W <= 0 || j.H <= 0 || j.D <= 0 || j.T <= 0 || j.M <= 0 || n <= 0) return -1;
+    const size_t N = (size_t)j.W * j.H * j.D;
+    const size_t NT = N * j.T;
+    std::vector<R> gx(NT), gy(NT), gz(NT);
+    parallel_for(NT, threads, [&](size_t a, size_t b) {
+        for (size_t i = a; i < b; i++) {
+            int x = (int)(i % j.W), y = (int)((i / j.W) % j.H), z = (int)((i / ((size_t)j.W * j.H)) % j.D),
+                t = (int)(i / N);
+            grad_at(img, x, y, z, t, j.W, j.H, j.D, j.c, gx[i], gy[i], gz[i]);
+        }
+    });
+    j.gx = gx.data(); j.gy = gy.data(); j.gz = gz.data();
+    const size_t NSR = N * (size_t)j.M * j.M * j.M;
+    std::vector<R> radbuf;
+    if (!rad_out) {
+        radbuf.resize(NSR * j.T);
+        rad_out = radbuf.data();
+    }
+    j.rad = rad_out;
+    j.out = out;
+    parallel_for(NSR, threads, [&](size_t a, size_t b) { radiality_range(j, a, b); });
+    parallel_for(NSR, threads, [&](size_t a, size_t b) { temporal_range(j, a, b); });
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int oracle_build_ring_f32(int axes, float radius, float *xyz) {
+    int n = 2 * axes;
+    if (n <= 0) return -1;
+    build_ring_f32(xyz, radius, n);
+    return n * n;
+}
+int oracle_build_ring_f64(int axes, double radius, double *xyz) {
+    int n = 2 * axes;
+    if (n <= 0) return -1;
+    build_ring_f64(xyz, radius, n);
+    return n * n;
+}
+
+float oracle_cubic_f32(float x) { return cubic<float>(x); }
+double oracle_cubic_f64(double x) { return cubic<double>(x); }
+
+// gradKernel over a whole series (kernel.cu:44-125).
+int oracle_gradient_f32(const float *img, const int *dims, const SRRFConf *c, float *gx,
+                        float *gy, float *gz) {
+    const int W = dims[0], H = dims[1], D = dims[2], T = dims[3];
+    size_t N = (size_t)W * H * D;
+    for (size_t i = 0; i < N * T; i++) {
+        int x = (int)(i % W), y = (int)((i / W) % H), z = (int)((i / ((size_t)W * H)) % D), t = (int)(i / N);
+        grad_at(img, x, y, z, t, W, H, D, *c, gx[i], gy[i], gz[i]);
+    }
+    return 0;
+}
+
+// interpolateG at npts SR positions of frame t (kernel.cu:141-169).
+int oracle_interp_f32(const float *arr, const int *dims, int mag, int t, const float *pts, int npts,
+                      float *out) {
+    for (int i = 0; i < npts; i++)
+        out[i] = interp<float>(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], t, mag, dims[0], dims[1],
+                               dims[2], arr);
+    return 0;
+}
+int oracle_interp_f64(const double *arr, const int *dims, int mag, int t, const double *pts,
+                      int npts, double *out) {
+    for (int i = 0; i < npts; i++)
+        out[i] = interp<double>(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], t, mag, dims[0], dims[1],
+                                dims[2], arr);
+    return 0;
+}
+
+// Whole stack: SRRFGradientCalcGPU semantics (kernel.cu:560-657) with an
+// explicit sampling list.  rad_out (optional) receives the per-frame
+// radiality [T][DM][HM][WM].
+int oracle_srrf_stack_f32(const float *img, const int *dims, const SRRFConf *conf, const float *ring,
+                          int n, float divisor, float *out, float *rad_out, int threads) {
+    return stack_impl<float>(img, dims, conf, ring, n, divisor, out, rad_out, threads);
+}
+int oracle_srrf_stack_f64(const double *img, const int *dims, const SRRFConf *conf,
+                          const double *ring, int n, double divisor, double *out, double *rad_out,
+                          int threads) {
+    return stack_impl<double>(img, dims, conf, ring, n, divisor, out, rad_out, threads);
+}
+
+// Temporal reduction alone on a given radiality series rad[T][N] with
+// intensities from img (kernel.cu:291-555); dims are the low-res dims.
+int oracle_temporal_f32(const float *img, const float *rad, const int *dims, const SRRFConf *conf,
+                        float *out) {
+    Job<float> j;
+    j.W = dims[0]; j.H = dims[1]; j.D = dims[2]; j.T = dims[3];
+    j.M = conf->Magnification;
+    j.c = *conf;
+    j.img = img;
+    j.rad = const_cast<float *>(rad);
+    j.out = out;
+    size_t NSR = (size_t)j.W * j.H * j.D * j.M * j.M * j.M;
+    temporal_range(j, 0, NSR);
+    return 0;
+}
+int oracle_temporal_f64(const double *img, const double *rad, const int *dims, const SRRFConf *conf,
+                        double *out) {
+    Job<double> j;
+    j.W = dims[0]; j.H = dims[1]; j.D = dims[2]; j.T = dims[3];
+    j.M = conf->Magnification;
+    j.c = *conf;
+    j.img = img;
+    j.rad = const_cast<double *>(rad);
+    j.out = out;
+    size_t NSR = (size_t)j.W * j.H * j.D * j.M * j.M * j.M;
+    temporal_range(j, 0, NSR);
+    return 0;
+}
+
+// SRRFGradientCalc's tiling (GradientCalc.cpp:47-163) around the FP32 stack
+// oracle with the reference ring from conf.  Only the voxels the reference
+// writes are written.
+int oracle_srrf_tiled_f32(const float *in, const int *imageDim, const SRRFConf *conf, float *out,
+                          int threads) {
+    const int border = 2;
+    const int mag = conf->Magnification;
+    int maxDim[3] = {conf->blockSizeX, conf->blockSizeY, conf->blockSizeZ};
+    int splitN[3];
+    for (int i = 0; i < 3; i++) {  // getSplitCount :47-53
+        splitN[i] = (imageDim[i] - 2 * border) / (maxDim[i] - 2 * border);
+        if ((imageDim[i] - 2 * border) % (maxDim[i] - 2 * border) != 0) splitN[i] = splitN[i] + 1;
+    }
+    int outDim[3] = {imageDim[0] * mag, imageDim[1] * mag, imageDim[2] * mag};
+    int nRing = conf->AxesinRing * 2;
+    std::vector<float> ring(3 * nRing * nRing);
+    build_ring_f32(ring.data(), conf->RingRadius, nRing);
+    for (int k = 0; k < splitN[2]; k++)
+        for (int jj = 0; jj < splitN[1]; jj++)
+            for (int i = 0; i < splitN[0]; i++) {
+                int posN[3] = {i, jj, k};
+                int resDim[4], startPos[3];
+                for (int a = 0; a < 3; a++) {  // doSplit :55-90
+                    resDim[a] = (posN[a] == splitN[a] - 1) ? imageDim[a] - (splitN[a] - 1) * (maxDim[a] - 2 * border)
+                                                           : maxDim[a];
+                    startPos[a] = (posN[a] == 0) ? 0 : posN[a] * (maxDim[a] - 2 * border);
+                }
+                resDim[3] = imageDim[3];
+                std::vector<float> st((size_t)resDim[0] * resDim[1] * resDim[2] * resDim[3]);
+                for (int t = 0; t < imageDim[3]; t++)
+                    for (int z = 0; z < resDim[2]; z++)
+                        for (int y = 0; y < resDim[1]; y++)
+                            for (int x = 0; x < resDim[0]; x++)
+                                st[idx3(x, y, z, t, resDim[0], resDim[1], resDim[2])] =
+                                    in[idx3(x + startPos[0], y + startPos[1], z + startPos[2], t, imageDim[0],
+                                            imageDim[1], imageDim[2])];
+                int outRes[3] = {resDim[0] * mag, resDim[1] * mag, resDim[2] * mag};
+                std::vector<float> stOut((size_t)outRes[0] * outRes[1] * outRes[2]);
+                int rc = stack_impl<float>(st.data(), resDim, conf, ring.data(), nRing * nRing, (float)nRing,
+                                           stOut.data(), nullptr, threads);
+                if (rc) return rc;
+                int borderM = border * mag;  // doPutback :92-116
+                for (int z = borderM; z < outRes[2]; z++)
+                    for (int y = borderM; y < outRes[1]; y++)
+                        for (int x = borderM; x < outRes[0]; x++) {
+                            size_t o = (size_t)(z + startPos[2] * mag) * outDim[0] * outDim[1] +
+                                       (size_t)(y + startPos[1] * mag) * outDim[0] + x + startPos[0] * mag;
+                            out[o] = stOut[(size_t)z * outRes[0] * outRes[1] + (size_t)y * outRes[0] + x];
+                        }
+            }
+    return 0;
+}
+
+int oracle_hardware_threads(void) { return (int)std::thread::hardware_concurrency(); }
+
+} // extern "C"

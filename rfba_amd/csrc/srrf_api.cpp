@@ -1,0 +1,963 @@
+// srrf_api.cpp -- host side of librfba_srrf.so: the C ABI declared in
+// include/rfba_srrf.h, SRRF.ini parsing, sampling-set analysis, tile and
+// LDS planning, device buffer caching and kernel launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "srrf_internal.h"
+
+namespace rfba {
+template <typename Tin, int TZ> __global__ void srrf_fused(LaunchParams P);
+__global__ void srrf_reset(int *counter, int *err);
+}  // namespace rfba
+
+using namespace rfba;
+
+// ---------------------------------------------------------------------------
+// errors
+static thread_local char g_err[512] = "";
+
+static int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail(e_ == hipErrorOutOfMemory ? RFBA_ENOMEM : RFBA_EDEVICE, "%s: %s", \
+                        #expr, hipGetErrorString(e_));                                    \
+    } while (0)
+
+extern "C" const char *rfba_last_error(void) { return g_err; }
+
+// ---------------------------------------------------------------------------
+// SRRF.ini (GradientCalc.cpp:15-42).  Same section and key names, same
+// defaults.  Parsing follows GetPrivateProfile*: case-insensitive section and
+// key match, first occurrence wins, ';' starts a comment line, integers are
+// parsed like atoi (leading sign and digits), missing keys take the default.
+namespace {
+
+std::string lower(std::string s) {
+    for (auto &c : s) c = (char)std::tolower((unsigned char)c);
+    return s;
+}
+std::string trim(const std::string &s) {
+    size_t a = 0, b = s.size();
+    while (a < b && std::isspace((unsigned char)s[a])) a++;
+    while (b > a && std::isspace((unsigned char)s[b - 1])) b--;
+    return s.substr(a, b - a);
+}
+
+struct Ini {
+    std::vector<std::pair<std::string, std::pair<std::string, std::string>>> kv;  // sec,(key,val)
+    bool load(const char *path) {
+        FILE *f = fopen(path, "rb");
+        if (!f) return false;
+        std::string sec;
+        char line[4096];
+        while (fgets(line, sizeof line, f)) {
+            std::string l = trim(line);
+            if (l.empty() || l[0] == ';') continue;
+            if (l[0] == '[') {
+                size_t e = l.find(']');
+                sec = lower(trim(l.substr(1, e == std::string::npos ? std::string::npos : e - 1)));
+                continue;
+            }
+            size_t eq = l.find('=');
+            if (eq == std::string::npos) continue;
+            std::string k = lower(trim(l.substr(0, eq)));
+            std::string v = trim(l.substr(eq + 1));
+            if (v.size() >= 2 && ((v.front() == '"' && v.back() == '"') || (v.front() == '\'' && v.back() == '\'')))
+                v = v.substr(1, v.size() - 2);
+            kv.push_back({sec, {k, v}});
+        }
+        fclose(f);
+        return true;
+    }
+    const std::string *find(const char *sec, const char *key) const {
+        std::string s = lower(sec), k = lower(key);
+        for (auto &e : kv)
+            if (e.first == s && e.second.first == k) return &e.second.second;
+        return nullptr;
+    }
+    int get_int(const char *sec, const char *key, int def) const {
+        const std::string *v = find(sec, key);
+        return v ? atoi(v->c_str()) : def;
+    }
+    std::string get_str(const char *sec, const char *key, const char *def) const {
+        const std::string *v = find(sec, key);
+        return v ? *v : std::string(def);
+    }
+};
+
+}  // namespace
+
+extern "C" void rfba_srrf_default_conf(SRRFConf *c) {
+    c->RingRadius = 0.5f;
+    c->Magnification = 5;
+    c->AxesinRing = 6;
+    c->MainRange = 1;
+    c->XRange = 0;
+    c->YRange = 0;
+    c->ZRange = 0;
+    c->Method = 1;
+    c->ITC = 1;
+    c->TRACOrder = 2;
+    c->RemoveCon = 0;
+    c->Renormalize = 0;
+    c->GradientSmooth = 0;
+    c->IntensityW = 1;
+    c->blockSizeX = 16;
+    c->blockSizeY = 16;
+    c->blockSizeZ = 16;
+}
+
+extern "C" int rfba_srrf_load_ini(const char *path, SRRFConf *c) {
+    if (!c) return fail(RFBA_EINVAL, "conf is NULL");
+    Ini ini;
+    ini.load(path ? path : "./SRRF.ini");
+    c->RingRadius = (float)atof(ini.get_str("SRRF", "RingRadius", "0.5").c_str());
+    c->Magnification = ini.get_int("SRRF", "Magnification", 5);
+    c->AxesinRing = ini.get_int("SRRF", "AxesinRing", 6);
+    c->MainRange = ini.get_int("SRRF", "MainRange", 1);
+    c->XRange = ini.get_int("SRRF", "XRange", 0);
+    c->YRange = ini.get_int("SRRF", "YRange", 0);
+    c->ZRange = ini.get_int("SRRF", "ZRange", 0);
+    c->Method = ini.get_int("Temproal", "Method", 1);
+    c->ITC = ini.get_int("Temproal", "Integrate_Temporal_Correlations", 1);
+    c->TRACOrder = ini.get_int("Temproal", "TRAC_ORDER", 2);
+    c->RemoveCon = ini.get_int("Radiality", "Remove_Positivity_Constraint", 0);
+    c->Renormalize = ini.get_int("Radiality", "Renormalize", 0);
+    c->GradientSmooth = ini.get_int("Radiality", "Gradient_Smoothing", 0);
+    c->IntensityW = ini.get_int("Weighting", "Intensity_Weighting", 1);
+    c->blockSizeX = ini.get_int("Split", "BlockSizeX", 16);
+    c->blockSizeY = ini.get_int("Split", "BlockSizeY", 16);
+    c->blockSizeZ = ini.get_int("Split", "BlockSizeZ", 16);
+    return RFBA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// sampling sets
+
+extern "C" int rfba_srrf_build_ring(int axes, float radius, float *xyz) {
+    // buildRing, kernel.cu:23-33: float angle, float sin/cos.
+    const int n = 2 * axes;
+    if (axes <= 0 || n * n > RFBA_SRRF_MAX_SAMPLES)
+        return fail(RFBA_EINVAL, "AxesinRing %d out of range", axes);
+    if (!xyz) return fail(RFBA_EINVAL, "xyz is NULL");
+    const float PI = 3.141592654f;
+    const float step = (PI * 2.0f) / (float)n;
+    for (int a1 = 0; a1 < n; a1++)
+        for (int a2 = 0; a2 < n; a2++) {
+            const int i = a1 * n + a2;
+            xyz[3 * i + 0] = radius * std::sin(step * a1) * std::cos(step * a2);
+            xyz[3 * i + 1] = radius * std::sin(step * a1) * std::sin(step * a2);
+            xyz[3 * i + 2] = radius * std::cos(step * a1);
+        }
+    return n * n;
+}
+
+extern "C" int rfba_srrf_build_dirs(int kind, float radius, float *xyz) {
+    if (!xyz) return fail(RFBA_EINVAL, "xyz is NULL");
+    if (kind != 6 && kind != 26) return fail(RFBA_EINVAL, "direction kind must be 6 or 26");
+    struct P3 { float x, y, z; };
+    std::vector<P3> v;
+    for (int dx = -1; dx <= 1; dx++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dz = -1; dz <= 1; dz++) {
+                const int nz = (dx != 0) + (dy != 0) + (dz != 0);
+                if (nz == 0 || (kind == 6 && nz != 1)) continue;
+                const double len = std::sqrt((double)(dx * dx + dy * dy + dz * dz));
+                v.push_back({(float)(radius * dx / len), (float)(radius * dy / len), (float)(radius * dz / len)});
+            }
+    std::stable_sort(v.begin(), v.end(), [](const P3 &a, const P3 &b) {
+        if (a.x != b.x) return a.x < b.x;
+        if (a.y != b.y) return a.y < b.y;
+        return a.z < b.z;
+    });
+    for (size_t i = 0; i < v.size(); i++) {
+        xyz[3 * i] = v[i].x;
+        xyz[3 * i + 1] = v[i].y;
+        xyz[3 * i + 2] = v[i].z;
+    }
+    return (int)v.size();
+}
+
+namespace {
+
+int find_or_add(float *tab, int &n, float v) {
+    for (int i = 0; i < n; i++)
+        if (tab[i] == v) return i;  // +0 == -0: same coordinate
+    if (n >= kMaxOffsets) return -1;
+    tab[n] = v;
+    return n++;
+}
+
+int analyse_samples(const float *xyz, int n, float divisor, float R, DevSamples &s) {
+    if (n <= 0 || n > kMaxSamples) return fail(RFBA_EINVAL, "sample count %d out of range", n);
+    if (!(divisor == divisor) || divisor == 0.0f) return fail(RFBA_EINVAL, "invalid divisor");
+    memset(&s, 0, sizeof(s));
+    s.n = n;
+    s.divisor = divisor;
+    s.R = R;
+    int ex = 0;
+    const float m = std::frexp(R, &ex);
+    s.r_pow2 = (R > 0.0f && std::isnormal(R) && m == 0.5f) ? 1 : 0;
+    s.invR = s.r_pow2 ? 1.0f / R : 0.0f;
+    s.oxmin = s.oymin = s.ozmin = 0.0f;
+    s.oxmax = s.oymax = s.ozmax = 0.0f;
+    int prev_ix = -1, prev_iy = -1;
+    for (int i = 0; i < n; i++) {
+        const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+        if (!std::isfinite(x) || !std::isfinite(y) || !std::isfinite(z))
+            return fail(RFBA_EINVAL, "non-finite sample offset");
+        const int ix = find_or_add(s.ox, s.nox, x);
+        const int iy = find_or_add(s.oy, s.noy, y);
+        const int iz = find_or_add(s.oz, s.noz, z);
+        if (ix < 0 || iy < 0 || iz < 0) return fail(RFBA_EINVAL, "too many distinct offsets per axis");
+        s.rx[i] = x;
+        s.ry[i] = y;
+        s.rz[i] = z;
+        s.siz[i] = iz;
+        s.oxmin = std::min(s.oxmin, x); s.oxmax = std::max(s.oxmax, x);
+        s.oymin = std::min(s.oymin, y); s.oymax = std::max(s.oymax, y);
+        s.ozmin = std::min(s.ozmin, z); s.ozmax = std::max(s.ozmax, z);
+        if (ix == prev_ix && iy == prev_iy) {
+            s.run_len[s.nruns - 1]++;
+        } else {
+            s.run_start[s.nruns] = i;
+            s.run_len[s.nruns] = 1;
+            s.run_iy[s.nruns] = iy;
+            if (ix == prev_ix) {
+                s.grp_nrun[s.ngroups - 1]++;
+            } else {
+                s.grp_ix[s.ngroups] = ix;
+                s.grp_run0[s.ngroups] = s.nruns;
+                s.grp_nrun[s.ngroups] = 1;
+                s.ngroups++;
+            }
+            s.nruns++;
+        }
+        prev_ix = ix;
+        prev_iy = iy;
+    }
+    return RFBA_OK;
+}
+
+int samples_from(const SRRFConf *conf, const rfba_srrf_samples *smp, DevSamples &s) {
+    if (smp) {
+        if (!smp->xyz) return fail(RFBA_EINVAL, "samples->xyz is NULL");
+        return analyse_samples(smp->xyz, smp->n, smp->divisor, conf->RingRadius, s);
+    }
+    const int nr = 2 * conf->AxesinRing;
+    if (conf->AxesinRing <= 0 || nr * nr > kMaxSamples)
+        return fail(RFBA_EINVAL, "AxesinRing %d out of range", conf->AxesinRing);
+    std::vector<float> xyz(3 * nr * nr);
+    rfba_srrf_build_ring(conf->AxesinRing, conf->RingRadius, xyz.data());
+    return analyse_samples(xyz.data(), nr * nr, (float)nr, conf->RingRadius, s);
+}
+
+int check_conf(const SRRFConf *c) {
+    if (!c) return fail(RFBA_EINVAL, "conf is NULL");
+    if (c->Magnification <= 0 || c->Magnification > 64)
+        return fail(RFBA_EINVAL, "Magnification %d out of range", c->Magnification);
+    if (c->Method < 0 || c->Method > 3) return fail(RFBA_EINVAL, "unknown Method %d", c->Method);
+    if (c->Method == 3 && (c->TRACOrder < 1 || c->TRACOrder > 4))
+        return fail(RFBA_EINVAL, "TRACOrder %d out of range", c->TRACOrder);
+    if (!(c->RingRadius == c->RingRadius)) return fail(RFBA_EINVAL, "RingRadius is NaN");
+    if (c->GradientSmooth == 1 &&
+        (c->MainRange < 0 || c->XRange < 0 || c->YRange < 0 || c->ZRange < 0 || c->MainRange > 8 ||
+         c->XRange > 8 || c->YRange > 8 || c->ZRange > 8))
+        return fail(RFBA_EINVAL, "gradient ranges out of range");
+    return RFBA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device context: cached buffers, one device, calls serialised.
+
+struct Ctx {
+    std::mutex mu;
+    bool init = false;
+    int dev = 0;
+    int ncu = 0;
+    DevSamples *d_smp = nullptr;
+    int *d_ctl = nullptr;  // [0] counter, [1] err
+    DevItem *d_items = nullptr;
+    size_t items_cap = 0;
+    DevTile *d_tiles = nullptr;
+    size_t tiles_cap = 0;
+    float *d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+    void *d_in = nullptr;
+    size_t in_bytes = 0;
+    float *d_out = nullptr;
+    size_t out_bytes = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+Ctx g_ctx;
+
+int ctx_init() {
+    if (g_ctx.init) return RFBA_OK;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(RFBA_ENODEV, "no HIP device");
+    HIP_TRY(hipGetDevice(&g_ctx.dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, g_ctx.dev));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(RFBA_ENODEV, "device %s is not gfx950", prop.gcnArchName);
+    g_ctx.ncu = prop.multiProcessorCount;
+    HIP_TRY(hipMalloc(&g_ctx.d_smp, sizeof(DevSamples)));
+    HIP_TRY(hipMalloc(&g_ctx.d_ctl, 2 * sizeof(int)));
+    HIP_TRY(hipEventCreate(&g_ctx.ev0));
+    HIP_TRY(hipEventCreate(&g_ctx.ev1));
+    g_ctx.init = true;
+    return RFBA_OK;
+}
+
+template <typename T> int ensure(T *&p, size_t &cap, size_t bytes) {
+    if (bytes <= cap && p) return RFBA_OK;
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    HIP_TRY(hipMalloc((void **)&p, bytes));
+    cap = bytes;
+    return RFBA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// planning
+
+int first_tap_h(float c, int M) { return (int)std::floor(c / (float)M - 0.5f) - 1; }
+
+struct Plan {
+    LaunchGeom g;
+    int TZ;
+    size_t lds_bytes;
+    int Tmax;
+    std::vector<DevTile> tiles;             // queue order
+    std::vector<std::pair<int, int>> item_tiles;  // per item [begin, count) in tiles_by_item
+    std::vector<DevTile> tiles_by_item;
+};
+
+void axis_extent(int b0, int b1, int tw, float omin, float omax, int M, int &need) {
+    for (int X0 = b0; X0 < b1; X0 += tw) {
+        const int lo = first_tap_h(((float)X0 + 0.5f) + omin, M);
+        const int hi = first_tap_h(((float)(X0 + tw - 1) + 0.5f) + omax, M) + 3;
+        need = std::max(need, hi - lo + 1);
+    }
+}
+
+size_t lds_layout(LaunchGeom &g, int TZ, int noz) {
+    g.RU = g.NU + 2 * g.hx;
+    g.RV = g.NV + 2 * g.hy;
+    g.RW = g.NW + 2 * g.hz;
+    // X stride of the x-interpolated planes: rows of different v shifted by
+    // half the banks so two y rows of a half-wave do not collide.
+    g.TXP = kTX + 1;
+    for (int p = 0; p < 32; p++)
+        if ((g.NW * (kTX + p)) % 32 == 16) { g.TXP = kTX + p; break; }
+    g.noz = noz;
+    int off = 0;
+    g.off_raw = off; off += g.RU * g.RV * g.RW;
+    g.off_g = off; off += 3 * g.NU * g.NV * g.NW;
+    g.off_px = off; off += 3 * g.NV * g.NW * g.TXP;
+    g.off_py = off; off += 3 * g.NW * kThreads;
+    g.off_xtab = off; off += kTX * 6;
+    g.off_ztab = off; off += TZ * noz * 8;
+    g.off_zc = off; off += TZ;
+    g.off_ctab = off; off += TZ * 6;
+    g.off_dsum = off; off += TZ * kThreads;
+    g.off_total = off;
+    return (size_t)off * sizeof(float);
+}
+
+int plan_geometry(const std::vector<DevItem> &items, const DevSamples &s, const SRRFConf &c, int TZ,
+                  Plan &pl) {
+    LaunchGeom g;
+    memset(&g, 0, sizeof g);
+    if (c.GradientSmooth == 0) {
+        g.hx = g.hy = g.hz = 1;
+    } else {
+        g.hx = std::max(c.MainRange, c.XRange);
+        g.hy = std::max(c.MainRange, c.YRange);
+        g.hz = std::max(c.MainRange, c.ZRange);
+    }
+    int NU = 4, NV = 4, NW = 4, Tmax = 1;
+    for (const auto &it : items) {
+        axis_extent(it.bx0, it.bx1, kTX, s.oxmin, s.oxmax, it.M, NU);
+        axis_extent(it.by0, it.by1, kTY, s.oymin, s.oymax, it.M, NV);
+        axis_extent(it.bz0, it.bz1, TZ, s.ozmin, s.ozmax, it.M, NW);
+        Tmax = std::max(Tmax, it.T);
+    }
+    g.NU = NU; g.NV = NV; g.NW = NW;
+    pl.lds_bytes = lds_layout(g, TZ, s.noz) + 64;
+    pl.g = g;
+    pl.TZ = TZ;
+    pl.Tmax = Tmax;
+    const int NR = g.RU * g.RV * g.RW;
+    if (NR > kRawPerThread * kThreads)
+        return fail(RFBA_EINVAL, "raw halo tile %d exceeds %d values", NR, kRawPerThread * kThreads);
+    if (pl.lds_bytes > 160 * 1024) return fail(RFBA_EINVAL, "LDS plan %zu bytes exceeds 160 KiB", pl.lds_bytes);
+    if (3 * g.NW < TZ) return fail(RFBA_EINVAL, "internal: y-column buffer smaller than TZ rows");
+    return RFBA_OK;
+}
+
+void plan_tiles(const std::vector<DevItem> &items, int TZ, Plan &pl) {
+    pl.tiles_by_item.clear();
+    pl.item_tiles.clear();
+    for (size_t i = 0; i < items.size(); i++) {
+        const auto &it = items[i];
+        const int begin = (int)pl.tiles_by_item.size();
+        for (int z = it.bz0; z < it.bz1; z += TZ)
+            for (int y = it.by0; y < it.by1; y += kTY)
+                for (int x = it.bx0; x < it.bx1; x += kTX) pl.tiles_by_item.push_back({(int)i, x, y, z});
+        pl.item_tiles.push_back({begin, (int)pl.tiles_by_item.size() - begin});
+    }
+    // Queue order: most expensive first (frames per tile), then spatial
+    // order so that consecutively dequeued tiles share halo data in L2.
+    pl.tiles = pl.tiles_by_item;
+    std::stable_sort(pl.tiles.begin(), pl.tiles.end(), [&](const DevTile &a, const DevTile &b) {
+        const long ca = (long)items[a.item].T, cb = (long)items[b.item].T;
+        return ca > cb;
+    });
+}
+
+int choose_tz(const std::vector<DevItem> &items, const DevSamples &s, const SRRFConf &c, Plan &pl,
+              int force_tz) {
+    if (force_tz == 8 || force_tz == 16) return plan_geometry(items, s, c, force_tz, pl);
+    Plan p16;
+    const int rc = plan_geometry(items, s, c, 16, p16);
+    if (rc == RFBA_OK && p16.lds_bytes <= 80 * 1024) {
+        pl = p16;
+        return RFBA_OK;
+    }
+    return plan_geometry(items, s, c, 8, pl);
+}
+
+typedef void (*KernelFn)(LaunchParams);
+
+KernelFn pick_kernel(int dtype, int TZ) {
+    if (dtype == RFBA_U16) return TZ == 16 ? srrf_fused<uint16_t, 16> : srrf_fused<uint16_t, 8>;
+    return TZ == 16 ? srrf_fused<float, 16> : srrf_fused<float, 8>;
+}
+
+int occupancy_grid(KernelFn fn, size_t lds, int ntiles, int &grid) {
+    if (lds > 64 * 1024)
+        HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, kThreads, lds));
+    if (per_cu <= 0) return fail(RFBA_EINVAL, "kernel does not fit on a CU (LDS %zu)", lds);
+    grid = std::max(1, std::min(ntiles, per_cu * g_ctx.ncu));
+    return RFBA_OK;
+}
+
+LaunchParams make_params(const SRRFConf &c, const Plan &pl) {
+    LaunchParams P;
+    memset(&P, 0, sizeof P);
+    P.method = c.Method;
+    P.itc = c.ITC;
+    P.order = c.TRACOrder;
+    P.remove_con = c.RemoveCon;
+    P.renorm = c.Renormalize;
+    P.smooth = c.GradientSmooth == 0 ? 0 : 1;
+    P.iw = c.IntensityW;
+    P.mR = c.MainRange;
+    P.xR = c.XRange;
+    P.yR = c.YRange;
+    P.zR = c.ZRange;
+    P.g = pl.g;
+    P.Tmax = pl.Tmax;
+    return P;
+}
+
+bool needs_series(const SRRFConf &c) { return c.Method == 2 || c.Method == 3; }
+
+// Run items through the fused kernel (single queue).  Synchronous.
+int run_items(const std::vector<DevItem> &items, int dtype, const SRRFConf &c, const DevSamples &s,
+              hipStream_t stream, float *ms) {
+    if (items.empty()) return RFBA_OK;
+    Plan pl;
+    const char *tz_env = getenv("RFBA_SRRF_TZ");  // tuning override: 8 or 16
+    int rc = choose_tz(items, s, c, pl, tz_env ? atoi(tz_env) : 0);
+    if (rc) return rc;
+    plan_tiles(items, pl.TZ, pl);
+    if (pl.tiles.empty()) return RFBA_OK;
+    KernelFn fn = pick_kernel(dtype, pl.TZ);
+    int grid = 0;
+    if ((rc = occupancy_grid(fn, pl.lds_bytes, (int)pl.tiles.size(), grid))) return rc;
+    if ((rc = ensure(g_ctx.d_items, g_ctx.items_cap, items.size() * sizeof(DevItem)))) return rc;
+    if ((rc = ensure(g_ctx.d_tiles, g_ctx.tiles_cap, pl.tiles.size() * sizeof(DevTile)))) return rc;
+    LaunchParams P = make_params(c, pl);
+    if (needs_series(c)) {
+        const size_t stride = (size_t)pl.Tmax * pl.TZ * kThreads;
+        if ((rc = ensure(g_ctx.d_scratch, g_ctx.scratch_bytes, stride * grid * sizeof(float)))) return rc;
+        P.scratch = g_ctx.d_scratch;
+        P.scratch_stride = (long long)stride;
+    }
+    HIP_TRY(hipMemcpyAsync(g_ctx.d_smp, &s, sizeof(DevSamples), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(g_ctx.d_items, items.data(), items.size() * sizeof(DevItem), hipMemcpyHostToDevice,
+                           stream));
+    HIP_TRY(hipMemcpyAsync(g_ctx.d_tiles, pl.tiles.data(), pl.tiles.size() * sizeof(DevTile),
+                           hipMemcpyHostToDevice, stream));
+    P.smp = g_ctx.d_smp;
+    P.items = g_ctx.d_items;
+    P.tiles = g_ctx.d_tiles;
+    P.ntiles = (int)pl.tiles.size();
+    P.counter = g_ctx.d_ctl;
+    P.err = g_ctx.d_ctl + 1;
+    HIP_TRY(hipEventRecord(g_ctx.ev0, stream));
+    hipLaunchKernelGGL(srrf_reset, dim3(1), dim3(64), 0, stream, P.counter, P.err);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), pl.lds_bytes, stream, P);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(g_ctx.ev1, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    int ctl[2] = {0, 0};
+    HIP_TRY(hipMemcpy(ctl, g_ctx.d_ctl, sizeof ctl, hipMemcpyDeviceToHost));
+    if (ctl[1]) return fail(RFBA_EDEVICE, "tile geometry exceeded the planned LDS extents");
+    if (ms) HIP_TRY(hipEventElapsedTime(ms, g_ctx.ev0, g_ctx.ev1));
+    return RFBA_OK;
+}
+
+DevItem stack_item(const void *d_in, const int dims[4], int M, float *d_out) {
+    DevItem it;
+    memset(&it, 0, sizeof it);
+    it.src = d_in;
+    it.sy = dims[0];
+    it.sz = (long long)dims[0] * dims[1];
+    it.st = (long long)dims[0] * dims[1] * dims[2];
+    it.W = dims[0]; it.H = dims[1]; it.D = dims[2]; it.T = dims[3];
+    it.M = M;
+    it.out = d_out;
+    it.osy = (long long)dims[0] * M;
+    it.osz = (long long)dims[0] * M * dims[1] * M;
+    it.bx0 = it.by0 = it.bz0 = 0;
+    it.bx1 = dims[0] * M; it.by1 = dims[1] * M; it.bz1 = dims[2] * M;
+    return it;
+}
+
+int check_dims(const int *dims) {
+    if (!dims) return fail(RFBA_EINVAL, "dims is NULL");
+    for (int i = 0; i < 4; i++)
+        if (dims[i] <= 0) return fail(RFBA_EINVAL, "dimension %d is %d", i, dims[i]);
+    return RFBA_OK;
+}
+
+size_t dtype_size(int dtype) { return dtype == RFBA_U16 ? 2 : (dtype == RFBA_F32 ? 4 : 0); }
+
+int stack_device_locked(const void *d_in, int dtype, const int dims[4], const SRRFConf *conf,
+                        const rfba_srrf_samples *smp, float *d_out, hipStream_t stream, float *ms) {
+    int rc;
+    if ((rc = check_conf(conf)) || (rc = check_dims(dims))) return rc;
+    if (!dtype_size(dtype)) return fail(RFBA_EINVAL, "unknown dtype %d", dtype);
+    if (!d_in || !d_out) return fail(RFBA_EINVAL, "NULL buffer");
+    if ((rc = ctx_init())) return rc;
+    DevSamples s;
+    if ((rc = samples_from(conf, smp, s))) return rc;
+    std::vector<DevItem> items{stack_item(d_in, dims, conf->Magnification, d_out)};
+    return run_items(items, dtype, *conf, s, stream, ms);
+}
+
+int stack_host_locked(const void *in, int dtype, const int dims[4], const SRRFConf *conf,
+                      const rfba_srrf_samples *smp, float *out) {
+    int rc;
+    if ((rc = check_conf(conf)) || (rc = check_dims(dims))) return rc;
+    if (!dtype_size(dtype)) return fail(RFBA_EINVAL, "unknown dtype %d", dtype);
+    if (!in || !out) return fail(RFBA_EINVAL, "NULL buffer");
+    if ((rc = ctx_init())) return rc;
+    const size_t M = conf->Magnification;
+    const size_t nin = (size_t)dims[0] * dims[1] * dims[2] * dims[3] * dtype_size(dtype);
+    const size_t nout = (size_t)dims[0] * dims[1] * dims[2] * M * M * M * sizeof(float);
+    if ((rc = ensure(g_ctx.d_in, g_ctx.in_bytes, nin))) return rc;
+    if ((rc = ensure(g_ctx.d_out, g_ctx.out_bytes, nout))) return rc;
+    HIP_TRY(hipMemcpy(g_ctx.d_in, in, nin, hipMemcpyHostToDevice));
+    if ((rc = stack_device_locked(g_ctx.d_in, dtype, dims, conf, smp, g_ctx.d_out, nullptr, nullptr))) return rc;
+    HIP_TRY(hipMemcpy(out, g_ctx.d_out, nout, hipMemcpyDeviceToHost));
+    return RFBA_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" int rfba_srrf_stack(const void *in, int dtype, const int dims[4], const SRRFConf *conf,
+                               const rfba_srrf_samples *samples, float *out) {
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    return stack_host_locked(in, dtype, dims, conf, samples, out);
+}
+
+extern "C" int rfba_srrf_stack_device(const void *d_in, int dtype, const int dims[4], const SRRFConf *conf,
+                                      const rfba_srrf_samples *samples, float *d_out, void *stream,
+                                      float *kernel_ms) {
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    return stack_device_locked(d_in, dtype, dims, conf, samples, d_out, (hipStream_t)stream, kernel_ms);
+}
+
+extern "C" int SRRFGradientCalcGPU(float *inImage, int *imageDim, float *outImage, SRRFConf *conf) {
+    // kernel.cu:560-657: one stack, reference ring sampling, host buffers.
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    return stack_host_locked(inImage, RFBA_F32, imageDim, conf, nullptr, outImage);
+}
+
+extern "C" int SRRFGetMag(void) {
+    // GradientCalc.cpp:176-178
+    Ini ini;
+    ini.load("SRRF.ini");
+    return ini.get_int("SRRF", "Magnification", 5);
+}
+
+// Tiles of SRRFGradientCalc (GradientCalc.cpp:47-116) as batched items.  Each
+// tile is its own stack (own clamp-to-edge borders and NaN bands).  The
+// reference copies tile voxels [2M, res*M) per axis into the output in loop
+// order k, j, i, later tiles overwriting earlier ones, so the surviving part
+// of tile p along an axis is [2M, (B-2)M) (last tile: [2M, res*M)); only that
+// box is computed and written, straight into the output.
+static int tiled_items(const int *imageDim, const SRRFConf &c, const void *d_in, float *d_out,
+                       std::vector<DevItem> &items) {
+    const int border = 2, M = c.Magnification;
+    const int maxDim[3] = {c.blockSizeX, c.blockSizeY, c.blockSizeZ};
+    int splitN[3];
+    for (int i = 0; i < 3; i++) {
+        if (maxDim[i] - 2 * border <= 0) return fail(RFBA_EINVAL, "blockSize %d <= 4", maxDim[i]);
+        splitN[i] = (imageDim[i] - 2 * border) / (maxDim[i] - 2 * border);
+        if ((imageDim[i] - 2 * border) % (maxDim[i] - 2 * border) != 0) splitN[i] = splitN[i] + 1;
+    }
+    const long long OX = (long long)imageDim[0] * M, OY = (long long)imageDim[1] * M;
+    for (int k = 0; k < splitN[2]; k++)
+        for (int j = 0; j < splitN[1]; j++)
+            for (int i = 0; i < splitN[0]; i++) {
+                const int posN[3] = {i, j, k};
+                int res[3], start[3], hi[3];
+                for (int a = 0; a < 3; a++) {
+                    const bool last = posN[a] == splitN[a] - 1;
+                    res[a] = last ? imageDim[a] - (splitN[a] - 1) * (maxDim[a] - 2 * border) : maxDim[a];
+                    start[a] = posN[a] == 0 ? 0 : posN[a] * (maxDim[a] - 2 * border);
+                    hi[a] = last ? res[a] * M : (maxDim[a] - 2) * M;
+                    hi[a] = std::min(hi[a], res[a] * M);
+                }
+                DevItem it;
+                memset(&it, 0, sizeof it);
+                it.src = d_in;
+                it.sy = imageDim[0];
+                it.sz = (long long)imageDim[0] * imageDim[1];
+                it.st = (long long)imageDim[0] * imageDim[1] * imageDim[2];
+                it.ox = start[0]; it.oy = start[1]; it.oz = start[2];
+                it.W = res[0]; it.H = res[1]; it.D = res[2]; it.T = imageDim[3];
+                it.M = M;
+                it.osy = OX;
+                it.osz = OX * OY;
+                it.out = d_out + (long long)start[2] * M * it.osz + (long long)start[1] * M * it.osy +
+                         (long long)start[0] * M;
+                it.bx0 = it.by0 = it.bz0 = 2 * M;
+                it.bx1 = hi[0]; it.by1 = hi[1]; it.bz1 = hi[2];
+                if (it.bx1 > it.bx0 && it.by1 > it.by0 && it.bz1 > it.bz0) items.push_back(it);
+            }
+    return RFBA_OK;
+}
+
+extern "C" int SRRFGradientCalc(float *inImage, int *imageDim, float *outImage) {
+    // GradientCalc.cpp:169-173 -> doSRRFGradientCalc :120-163
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    SRRFConf c;
+    int rc;
+    if ((rc = rfba_srrf_load_ini("./SRRF.ini", &c))) return rc;
+    if ((rc = check_conf(&c)) || (rc = check_dims(imageDim))) return rc;
+    if (!inImage || !outImage) return fail(RFBA_EINVAL, "NULL buffer");
+    if ((rc = ctx_init())) return rc;
+    const size_t M = c.Magnification;
+    const size_t nin = (size_t)imageDim[0] * imageDim[1] * imageDim[2] * imageDim[3] * sizeof(float);
+    const size_t nout = (size_t)imageDim[0] * imageDim[1] * imageDim[2] * M * M * M * sizeof(float);
+    if ((rc = ensure(g_ctx.d_in, g_ctx.in_bytes, nin))) return rc;
+    if ((rc = ensure(g_ctx.d_out, g_ctx.out_bytes, nout))) return rc;
+    HIP_TRY(hipMemcpy(g_ctx.d_in, inImage, nin, hipMemcpyHostToDevice));
+    // Voxels the reference never writes keep the caller's values.
+    HIP_TRY(hipMemcpy(g_ctx.d_out, outImage, nout, hipMemcpyHostToDevice));
+    std::vector<DevItem> items;
+    if ((rc = tiled_items(imageDim, c, g_ctx.d_in, g_ctx.d_out, items))) return rc;
+    DevSamples s;
+    if ((rc = samples_from(&c, nullptr, s))) return rc;
+    if ((rc = run_items(items, RFBA_F32, c, s, nullptr, nullptr))) return rc;
+    HIP_TRY(hipMemcpy(outImage, g_ctx.d_out, nout, hipMemcpyDeviceToHost));
+    return RFBA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched form
+
+struct rfba_srrf_batch {
+    SRRFConf conf;
+    int dtype = 0;
+    Plan pl;
+    KernelFn fn = nullptr;
+    int grid = 0;
+    std::vector<DevItem> items;
+    DevSamples *d_smp = nullptr;
+    DevItem *d_items = nullptr;
+    DevTile *d_tiles = nullptr;   // queue order
+    DevTile *d_tiles_by_item = nullptr;
+    int *d_ctl = nullptr;
+    float *d_scratch = nullptr;
+    long long scratch_stride = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+static void batch_free(rfba_srrf_batch *b) {
+    if (!b) return;
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->d_smp) hipFree(b->d_smp);
+    if (b->d_items) hipFree(b->d_items);
+    if (b->d_tiles) hipFree(b->d_tiles);
+    if (b->d_tiles_by_item) hipFree(b->d_tiles_by_item);
+    if (b->d_ctl) hipFree(b->d_ctl);
+    if (b->d_scratch) hipFree(b->d_scratch);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    delete b;
+}
+
+extern "C" int rfba_srrf_batch_create(const rfba_srrf_block *blocks, int n, int dtype, const SRRFConf *conf,
+                                      const rfba_srrf_samples *samples, rfba_srrf_batch **out) {
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    int rc;
+    if (!out) return fail(RFBA_EINVAL, "plan pointer is NULL");
+    *out = nullptr;
+    if (!blocks || n <= 0) return fail(RFBA_EINVAL, "no blocks");
+    if ((rc = check_conf(conf))) return rc;
+    if (!dtype_size(dtype)) return fail(RFBA_EINVAL, "unknown dtype %d", dtype);
+    if ((rc = ctx_init())) return rc;
+    DevSamples s;
+    if ((rc = samples_from(conf, samples, s))) return rc;
+    rfba_srrf_batch *b = new rfba_srrf_batch();
+    b->conf = *conf;
+    b->dtype = dtype;
+    for (int i = 0; i < n; i++) {
+        const rfba_srrf_block &bl = blocks[i];
+        for (int a = 0; a < 3; a++)
+            if (bl.dims[a] <= 0 || bl.origin[a] < 0 || bl.origin[a] + bl.dims[a] > bl.src_dims[a]) {
+                batch_free(b);
+                return fail(RFBA_EINVAL, "block %d axis %d outside its source", i, a);
+            }
+        if (bl.frames <= 0 || bl.frames > bl.src_dims[3] || bl.mag <= 0 || bl.mag > 64 || !bl.d_src || !bl.d_out) {
+            batch_free(b);
+            return fail(RFBA_EINVAL, "block %d: invalid frames/mag/buffers", i);
+        }
+        DevItem it;
+        memset(&it, 0, sizeof it);
+        it.src = bl.d_src;
+        it.sy = bl.src_dims[0];
+        it.sz = (long long)bl.src_dims[0] * bl.src_dims[1];
+        it.st = (long long)bl.src_dims[0] * bl.src_dims[1] * bl.src_dims[2];
+        it.ox = bl.origin[0]; it.oy = bl.origin[1]; it.oz = bl.origin[2];
+        it.W = bl.dims[0]; it.H = bl.dims[1]; it.D = bl.dims[2]; it.T = bl.frames;
+        it.M = bl.mag;
+        it.out = bl.d_out;
+        it.osy = (long long)bl.dims[0] * bl.mag;
+        it.osz = it.osy * bl.dims[1] * bl.mag;
+        it.bx1 = bl.dims[0] * bl.mag; it.by1 = bl.dims[1] * bl.mag; it.bz1 = bl.dims[2] * bl.mag;
+        b->items.push_back(it);
+    }
+    if ((rc = choose_tz(b->items, s, *conf, b->pl, 0))) {
+        batch_free(b);
+        return rc;
+    }
+    plan_tiles(b->items, b->pl.TZ, b->pl);
+    b->fn = pick_kernel(dtype, b->pl.TZ);
+    if ((rc = occupancy_grid(b->fn, b->pl.lds_bytes, (int)b->pl.tiles.size(), b->grid))) {
+        batch_free(b);
+        return rc;
+    }
+    auto alloc = [&](void **p, size_t bytes) -> int {
+        HIP_TRY(hipMalloc(p, bytes));
+        return RFBA_OK;
+    };
+    const size_t nt = b->pl.tiles.size();
+    if ((rc = alloc((void **)&b->d_smp, sizeof(DevSamples))) ||
+        (rc = alloc((void **)&b->d_items, b->items.size() * sizeof(DevItem))) ||
+        (rc = alloc((void **)&b->d_tiles, nt * sizeof(DevTile))) ||
+        (rc = alloc((void **)&b->d_tiles_by_item, nt * sizeof(DevTile))) ||
+        (rc = alloc((void **)&b->d_ctl, 2 * sizeof(int)))) {
+        batch_free(b);
+        return rc;
+    }
+    if (needs_series(*conf)) {
+        b->scratch_stride = (long long)b->pl.Tmax * b->pl.TZ * kThreads;
+        if ((rc = alloc((void **)&b->d_scratch, (size_t)b->scratch_stride * b->grid * sizeof(float)))) {
+            batch_free(b);
+            return rc;
+        }
+    }
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = hipMemcpy(b->d_smp, &s, sizeof s, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_items, b->items.data(), b->items.size() * sizeof(DevItem), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_tiles, b->pl.tiles.data(), nt * sizeof(DevTile), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_tiles_by_item, b->pl.tiles_by_item.data(), nt * sizeof(DevTile), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e != hipSuccess) {
+        batch_free(b);
+        return fail(RFBA_EDEVICE, "batch setup: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return RFBA_OK;
+}
+
+static LaunchParams batch_params(rfba_srrf_batch *b, const DevTile *tiles, int ntiles) {
+    LaunchParams P = make_params(b->conf, b->pl);
+    P.smp = b->d_smp;
+    P.items = b->d_items;
+    P.tiles = tiles;
+    P.ntiles = ntiles;
+    P.counter = b->d_ctl;
+    P.err = b->d_ctl + 1;
+    P.scratch = b->d_scratch;
+    P.scratch_stride = b->scratch_stride;
+    return P;
+}
+
+static int batch_enqueue(rfba_srrf_batch *b, hipStream_t st) {
+    LaunchParams P = batch_params(b, b->d_tiles, (int)b->pl.tiles.size());
+    hipLaunchKernelGGL(srrf_reset, dim3(1), dim3(64), 0, st, P.counter, P.err);
+    HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(b->fn, dim3(b->grid), dim3(kThreads), b->pl.lds_bytes, st, P);
+    HIP_TRY(hipGetLastError());
+    return RFBA_OK;
+}
+
+static int batch_check(rfba_srrf_batch *b) {
+    int ctl[2] = {0, 0};
+    HIP_TRY(hipMemcpy(ctl, b->d_ctl, sizeof ctl, hipMemcpyDeviceToHost));
+    if (ctl[1]) return fail(RFBA_EDEVICE, "tile geometry exceeded the planned LDS extents");
+    return RFBA_OK;
+}
+
+extern "C" int rfba_srrf_batch_run(rfba_srrf_batch *b, void *stream, int use_graph, float *kernel_ms) {
+    if (!b) return fail(RFBA_EINVAL, "plan is NULL");
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if (use_graph) {
+        if (!b->exec) {
+            hipStream_t cs;
+            HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+            HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            rc = batch_enqueue(b, cs);
+            hipGraph_t gr = nullptr;
+            hipError_t e = hipStreamEndCapture(cs, &gr);
+            hipStreamDestroy(cs);
+            if (rc) return rc;
+            if (e != hipSuccess) return fail(RFBA_EDEVICE, "graph capture: %s", hipGetErrorString(e));
+            b->graph = gr;
+            HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        }
+        HIP_TRY(hipEventRecord(b->ev0, st));
+        HIP_TRY(hipGraphLaunch(b->exec, st));
+        HIP_TRY(hipEventRecord(b->ev1, st));
+    } else {
+        HIP_TRY(hipEventRecord(b->ev0, st));
+        if ((rc = batch_enqueue(b, st))) return rc;
+        HIP_TRY(hipEventRecord(b->ev1, st));
+    }
+    if (kernel_ms) {
+        HIP_TRY(hipEventSynchronize(b->ev1));
+        HIP_TRY(hipEventElapsedTime(kernel_ms, b->ev0, b->ev1));
+        return batch_check(b);
+    }
+    return RFBA_OK;
+}
+
+extern "C" int rfba_srrf_batch_run_per_block(rfba_srrf_batch *b, void *stream, float *kernel_ms) {
+    if (!b) return fail(RFBA_EINVAL, "plan is NULL");
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipEventRecord(b->ev0, st));
+    for (size_t i = 0; i < b->items.size(); i++) {
+        const auto &r = b->pl.item_tiles[i];
+        if (r.second == 0) continue;
+        LaunchParams P = batch_params(b, b->d_tiles_by_item + r.first, r.second);
+        const int grid = std::min(b->grid, r.second);
+        hipLaunchKernelGGL(srrf_reset, dim3(1), dim3(64), 0, st, P.counter, P.err);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(b->fn, dim3(grid), dim3(kThreads), b->pl.lds_bytes, st, P);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipEventRecord(b->ev1, st));
+    if (kernel_ms) {
+        HIP_TRY(hipEventSynchronize(b->ev1));
+        HIP_TRY(hipEventElapsedTime(kernel_ms, b->ev0, b->ev1));
+        return batch_check(b);
+    }
+    return RFBA_OK;
+}
+
+extern "C" void rfba_srrf_batch_destroy(rfba_srrf_batch *b) {
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    batch_free(b);
+}
+
+// ---------------------------------------------------------------------------
+// runtime helpers
+
+extern "C" int rfba_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+extern "C" int rfba_malloc_device(void **p, size_t bytes) {
+    if (!p) return fail(RFBA_EINVAL, "NULL");
+    HIP_TRY(hipMalloc(p, bytes));
+    return RFBA_OK;
+}
+extern "C" int rfba_free_device(void *p) {
+    HIP_TRY(hipFree(p));
+    return RFBA_OK;
+}
+extern "C" int rfba_memcpy_h2d(void *d, const void *s, size_t n) {
+    HIP_TRY(hipMemcpy(d, s, n, hipMemcpyHostToDevice));
+    return RFBA_OK;
+}
+extern "C" int rfba_memcpy_d2h(void *d, const void *s, size_t n) {
+    HIP_TRY(hipMemcpy(d, s, n, hipMemcpyDeviceToHost));
+    return RFBA_OK;
+}
+extern "C" int rfba_memset_device(void *d, int v, size_t n) {
+    HIP_TRY(hipMemset(d, v, n));
+    return RFBA_OK;
+}
+extern "C" int rfba_device_synchronize(void) {
+    HIP_TRY(hipDeviceSynchronize());
+    return RFBA_OK;
+}
+extern "C" void rfba_release_cache(void) {
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    auto rel = [](void *&p, size_t &cap) {
+        if (p) hipFree(p);
+        p = nullptr;
+        cap = 0;
+    };
+    rel((void *&)g_ctx.d_scratch, g_ctx.scratch_bytes);
+    rel(g_ctx.d_in, g_ctx.in_bytes);
+    rel((void *&)g_ctx.d_out, g_ctx.out_bytes);
+    rel((void *&)g_ctx.d_items, g_ctx.items_cap);
+    rel((void *&)g_ctx.d_tiles, g_ctx.tiles_cap);
+}

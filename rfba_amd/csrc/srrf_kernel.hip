@@ -1,0 +1,549 @@
+// srrf_kernel.hip -- fused 3D SRRF radiality kernel for gfx950 (MI355X).
+//
+// One persistent workgroup of 256 threads takes SR tiles (16 x 16 columns x
+// TZ voxels) from a device work queue and, for every frame of the tile's
+// stack, runs
+//   raw halo (uint16 or float, coalesced loads, clamp-to-edge)
+//     -> LDS -> xyz gradient in LDS                (gradKernel, kernel.cu:44-125)
+//   -> per x-offset group: x-interpolated planes in LDS
+//   -> per (x,y)-offset run: y-interpolated columns (thread-private LDS)
+//   -> per sample: z-pass, radiality term         (interpolateG :141-169,
+//                                                   calculateRadiality :186-253)
+//   -> temporal accumulation in registers          (:291-555)
+// The per-voxel signed sample sums live in thread-private LDS rows so that
+// the voxel loop stays rolled (register pressure).
+// The tricubic Catmull-Rom interpolation of the reference evaluates
+// p = sum_i G*wx, q = sum_j p*wy, s = sum_k q*wz in exactly this order, so
+// the partial sums p (depending on the sample's x only) and q (x and y only)
+// are shared between samples and voxels without changing a single rounding.
+// Every voxel still accumulates its sample terms in list order.  The
+// translation unit is compiled with -ffp-contract=off and IEEE division and
+// square root, so each voxel-frame value is the reference arithmetic.
+//
+// TRAC and TRPPM need the whole per-voxel series after the temporal mean is
+// known (two-pass in the reference, :458-507); the series goes to a
+// per-workgroup HBM slot and is reduced per thread after the frame loop.
+#include <hip/hip_runtime.h>
+
+#include "srrf_internal.h"
+
+namespace rfba {
+
+__device__ __forceinline__ float cubic_w(float xx) {
+    // cubic, kernel.cu:128-138 (Catmull-Rom, a = 0.5)
+    const float a = 0.5f;
+    float x = xx;
+    if (x < 0.0f) x = -x;
+    float z = 0.0f;
+    if (x < 1.0f)
+        z = x * x * (x * (-a + 2.0f) + (a - 3.0f)) + 1.0f;
+    else if (x < 2.0f)
+        z = -a * x * x * x + 5.0f * a * x * x - 8.0f * a * x + 4.0f * a;
+    return z;
+}
+
+// Sample coordinate c (SR pixels) -> first tap index b = u0 - 1, the four
+// Catmull-Rom weights and the in-range test of interpolateG (:143-151).
+__device__ __forceinline__ void axis_weights(float c, int M, int dim, int &b, float &w0,
+                                             float &w1, float &w2, float &w3, int &valid) {
+    float x = c / (float)M;
+    valid = !(x < 1.5f || x > (float)dim - 1.5f);
+    int u0 = (int)floorf(x - 0.5f);
+    w0 = cubic_w(x - ((float)(u0 - 1) + 0.5f));
+    w1 = cubic_w(x - ((float)(u0) + 0.5f));
+    w2 = cubic_w(x - ((float)(u0 + 1) + 0.5f));
+    w3 = cubic_w(x - ((float)(u0 + 2) + 0.5f));
+    b = u0 - 1;
+}
+
+__device__ __forceinline__ int first_tap(float c, int M) {
+    return (int)floorf(c / (float)M - 0.5f) - 1;
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// (q > 0 || q != q) for q = num / den rounded to nearest, without the
+// division on the common path.  For finite den > 0: num <= 0 gives q <= 0;
+// num >= den * 2^-64 gives q >= 2^-65 (> 0 after rounding); only the
+// remaining tiny-positive band and non-finite/zero den divide.
+__device__ __forceinline__ bool diverging(float num, float den) {
+    if (den > 0.0f && den < __builtin_inff()) {
+        if (num > 0.0f) {
+            if (num >= den * 0x1p-64f) return true;
+            float q = num / den;
+            return q > 0.0f;
+        }
+        return !(num <= 0.0f);
+    }
+    float q = num / den;
+    return (q > 0.0f) || (q != q);
+}
+
+template <typename Tin> __device__ __forceinline__ float load_px(const Tin *p, long long i);
+template <> __device__ __forceinline__ float load_px<uint16_t>(const uint16_t *p, long long i) {
+    return (float)p[i];
+}
+template <> __device__ __forceinline__ float load_px<float>(const float *p, long long i) {
+    return p[i];
+}
+
+__global__ void srrf_reset(int *counter, int *err) {
+    if (threadIdx.x == 0) {
+        *counter = 0;
+        *err = 0;
+    }
+}
+
+template <typename Tin, int TZ>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void srrf_fused(LaunchParams P) {
+    extern __shared__ float lds[];
+    __shared__ int s_tile;
+    const LaunchGeom &g = P.g;
+    float *raw = lds + g.off_raw;
+    float *G = lds + g.off_g;
+    float *Px = lds + g.off_px;
+    float *Py = lds + g.off_py;
+    float *xtab = lds + g.off_xtab;  // kTX x {b, w0..w3}
+    float *ztab = lds + g.off_ztab;  // TZ x noz x {wrel, wz0..3, valid, z0, dz}
+    float *zcen = lds + g.off_zc;    // TZ: Zc
+    float *ctab = lds + g.off_ctab;  // TZ x {wrel, wz0..3, valid} (voxel centre)
+    float *dsum = lds + g.off_dsum;  // TZ x 256: per-voxel signed sums (thread-private)
+
+    const int tid = threadIdx.x;
+    const int xl = tid % kTX, yl = tid / kTX;
+    const DevSamples *__restrict__ S = P.smp;
+    float *scr = P.scratch + (long long)blockIdx.x * P.scratch_stride;
+    const int NU = g.NU, NV = g.NV, NW = g.NW, TXP = g.TXP, noz = g.noz;
+    const int RU = g.RU, RV = g.RV, RW = g.RW;
+    const int NR = RU * RV * RW;
+
+    for (;;) {
+        __syncthreads();
+        if (tid == 0) s_tile = atomicAdd(P.counter, 1);
+        __syncthreads();
+        const int ti = __builtin_amdgcn_readfirstlane(s_tile);
+        if (ti >= P.ntiles) break;
+        const DevTile tl = P.tiles[ti];
+        const DevItem it = P.items[tl.item];
+        const int M = it.M, W = it.W, H = it.H, D = it.D, T = it.T;
+        const int X0 = tl.X0, Y0 = tl.Y0, Z0 = tl.Z0;
+
+        // Low-res extents of all taps of this tile (monotone in coordinate).
+        const int ulo = first_tap(((float)X0 + 0.5f) + S->oxmin, M);
+        const int vlo = first_tap(((float)Y0 + 0.5f) + S->oymin, M);
+        const int wlo = first_tap(((float)Z0 + 0.5f) + S->ozmin, M);
+        const int uhi = first_tap(((float)(X0 + kTX - 1) + 0.5f) + S->oxmax, M) + 3;
+        const int vhi = first_tap(((float)(Y0 + kTY - 1) + 0.5f) + S->oymax, M) + 3;
+        const int whi = first_tap(((float)(Z0 + TZ - 1) + 0.5f) + S->ozmax, M) + 3;
+        if (uhi - ulo + 1 > NU || vhi - vlo + 1 > NV || whi - wlo + 1 > NW) {
+            if (tid == 0) atomicOr(P.err, 1);
+            continue;
+        }
+
+        // z tables (uniform per tile, frame independent).
+        for (int e = tid; e < TZ * noz; e += kThreads) {
+            const int Z = e / noz, iz = e % noz;
+            const float Zc = (float)(Z0 + Z) + 0.5f;
+            const float z0 = Zc + S->oz[iz];
+            int b, valid;
+            float w0, w1, w2, w3;
+            axis_weights(z0, M, D, b, w0, w1, w2, w3, valid);
+            float *zt = ztab + e * 8;
+            zt[0] = __int_as_float(clampi(b - wlo, 0, NW - 4));
+            zt[1] = w0; zt[2] = w1; zt[3] = w2; zt[4] = w3;
+            zt[5] = __int_as_float(valid);
+            zt[6] = z0;
+            zt[7] = z0 - Zc;
+        }
+        if (tid < TZ) {
+            const float Zc = (float)(Z0 + tid) + 0.5f;
+            zcen[tid] = Zc;
+            int b, valid;
+            float w0, w1, w2, w3;
+            axis_weights(Zc, M, D, b, w0, w1, w2, w3, valid);
+            float *ct = ctab + tid * 6;
+            ct[0] = __int_as_float(clampi(b - wlo, 0, NW - 4));
+            ct[1] = w0; ct[2] = w1; ct[3] = w2; ct[4] = w3;
+            ct[5] = __int_as_float(valid);
+        }
+
+        const int Xs = X0 + xl, Ys = Y0 + yl;
+        const float Xc = (float)Xs + 0.5f, Yc = (float)Ys + 0.5f;
+
+        float acc0[TZ], acc1[TZ];
+#pragma unroll
+        for (int z = 0; z < TZ; z++) { acc0[z] = 0.0f; acc1[z] = 0.0f; }
+
+        // raw halo (clamp-to-edge inside the stack, kernel.cu:59-80)
+        const Tin *src = (const Tin *)it.src;
+        const int rx0 = ulo - g.hx, ry0 = vlo - g.hy, rz0 = wlo - g.hz;
+
+        for (int t = 0; t < T; t++) {
+            __syncthreads();
+            {
+                const Tin *srct = src + (long long)t * it.st;
+                for (int e = tid; e < NR; e += kThreads) {
+                    const int a = e % RU, bb = (e / RU) % RV, c = e / (RU * RV);
+                    const int x = clampi(rx0 + a, 0, W - 1) + it.ox;
+                    const int y = clampi(ry0 + bb, 0, H - 1) + it.oy;
+                    const int z = clampi(rz0 + c, 0, D - 1) + it.oz;
+                    raw[e] = load_px<Tin>(srct, (long long)x + y * it.sy + z * it.sz);
+                }
+            }
+            __syncthreads();
+
+            // ---- gradient planes G[c][w][v][u] ------------------------------
+            for (int e = tid; e < NU * NV * NW; e += kThreads) {
+                const int u = e % NU, v = (e / NU) % NV, w = e / (NU * NV);
+                const int cu = u + g.hx, cv = v + g.hy, cw = w + g.hz;
+                auto R3 = [&](int du, int dv, int dw) {
+                    return raw[((cw + dw) * RV + (cv + dv)) * RU + (cu + du)];
+                };
+                float gx, gy, gz;
+                if (P.smooth == 0) {
+                    gx = -R3(-1, 0, 0) + R3(1, 0, 0);
+                    gy = -R3(0, -1, 0) + R3(0, 1, 0);
+                    gz = -R3(0, 0, -1) + R3(0, 0, 1);
+                } else {
+                    const int mR = P.mR, xR = P.xR, yR = P.yR, zR = P.zR;
+                    float s = 0.0f;
+                    for (int i = -mR; i <= mR; i++)
+                        for (int j = -yR; j <= yR; j++)
+                            for (int k = -zR; k <= zR; k++) {
+                                if (i < 0) s -= R3(i, j, k);
+                                else if (i > 0) s += R3(i, j, k);
+                            }
+                    gx = s;
+                    s = 0.0f;
+                    for (int j = -mR; j <= mR; j++)
+                        for (int i = -xR; i <= xR; i++)
+                            for (int k = -zR; k <= zR; k++) {
+                                if (j < 0) s -= R3(i, j, k);
+                                else if (j > 0) s += R3(i, j, k);
+                            }
+                    gy = s;
+                    s = 0.0f;
+                    for (int k = -mR; k <= mR; k++)
+                        for (int i = -xR; i <= xR; i++)
+                            for (int j = -yR; j <= yR; j++) {
+                                if (k < 0) s -= R3(i, j, k);
+                                else if (k > 0) s += R3(i, j, k);
+                            }
+                    gz = s;
+                }
+                const int NP = NU * NV * NW;
+                G[e] = gx;
+                G[NP + e] = gy;
+                G[2 * NP + e] = gz;
+            }
+
+            for (int z = 0; z < TZ; z++) dsum[z * kThreads + tid] = 0.0f;
+
+            const int ngroups = S->ngroups;
+            for (int gi = 0; gi < ngroups; gi++) {
+                const int ix = S->grp_ix[gi];
+                const float oxv = S->ox[ix];
+                if (tid < kTX) {
+                    int b, valid;
+                    float w0, w1, w2, w3;
+                    axis_weights(((float)(X0 + tid) + 0.5f) + oxv, M, W, b, w0, w1, w2, w3, valid);
+                    float *xt = xtab + tid * 6;
+                    xt[0] = __int_as_float(clampi(b - ulo, 0, NU - 4));
+                    xt[1] = w0; xt[2] = w1; xt[3] = w2; xt[4] = w3;
+                }
+                __syncthreads();
+                // ---- x pass: Px[c][v][w][X] = sum_i G[c][w][v][b+i] * wx_i ----
+                {
+                    const int NP = NU * NV * NW;
+                    for (int e = tid; e < NV * NW * kTX; e += kThreads) {
+                        const int X = e % kTX, w = (e / kTX) % NW, v = e / (kTX * NW);
+                        const float *xt = xtab + X * 6;
+                        const int ub = __float_as_int(xt[0]);
+                        const float w0 = xt[1], w1 = xt[2], w2 = xt[3], w3 = xt[4];
+                        const int gb = (w * NV + v) * NU + ub;
+#pragma unroll
+                        for (int c = 0; c < 3; c++) {
+                            const float *gg = G + c * NP + gb;
+                            float p = 0.0f;
+                            p = p + gg[0] * w0;
+                            p = p + gg[1] * w1;
+                            p = p + gg[2] * w2;
+                            p = p + gg[3] * w3;
+                            Px[((c * NV + v) * NW + w) * TXP + X] = p;
+                        }
+                    }
+                }
+                __syncthreads();
+
+                const float x0 = Xc + oxv;
+                const float dxs = x0 - Xc;
+                const int validx = !((x0 / (float)M) < 1.5f || (x0 / (float)M) > (float)W - 1.5f);
+                const int r0 = S->grp_run0[gi], nr = S->grp_nrun[gi];
+                for (int r = r0; r < r0 + nr; r++) {
+                    const int iy = S->run_iy[r];
+                    const float y0 = Yc + S->oy[iy];
+                    const float dys = y0 - Yc;
+                    int vb, validy;
+                    float wy0, wy1, wy2, wy3;
+                    axis_weights(y0, M, H, vb, wy0, wy1, wy2, wy3, validy);
+                    const int vrel = clampi(vb - vlo, 0, NV - 4);
+                    // ---- y pass: Py[c][w][tid] = sum_j Px[c][vrel+j][w][X] * wy_j
+                    for (int w = 0; w < NW; w++) {
+#pragma unroll
+                        for (int c = 0; c < 3; c++) {
+                            const float *pp = Px + ((c * NV + vrel) * NW + w) * TXP + xl;
+                            const int vs = NW * TXP;
+                            float q = 0.0f;
+                            q = q + pp[0] * wy0;
+                            q = q + pp[vs] * wy1;
+                            q = q + pp[2 * vs] * wy2;
+                            q = q + pp[3 * vs] * wy3;
+                            Py[(c * NW + w) * kThreads + tid] = q;
+                        }
+                    }
+                    const int validxy = validx & validy;
+                    const int s0 = S->run_start[r], ns = S->run_len[r];
+                    const float rxs = S->rx[s0], rys = S->ry[s0];
+                    for (int Z = 0; Z < TZ; Z++) {
+                        const float Zc = zcen[Z];
+                        float dd = dsum[Z * kThreads + tid];
+                        for (int s = s0; s < s0 + ns; s++) {
+                            const int iz = S->siz[s];
+                            const float rzs = S->rz[s];
+                            const float *zt = ztab + (Z * noz + iz) * 8;
+                            const int wrel = __float_as_int(zt[0]);
+                            const float wz0 = zt[1], wz1 = zt[2], wz2 = zt[3], wz3 = zt[4];
+                            const int valid = validxy & __float_as_int(zt[5]);
+                            const float z0 = zt[6], dzs = zt[7];
+                            // ---- z pass
+                            float Gv[3];
+#pragma unroll
+                            for (int c = 0; c < 3; c++) {
+                                const float *qq = Py + (c * NW + wrel) * kThreads + tid;
+                                float sv = 0.0f;
+                                sv = sv + qq[0] * wz0;
+                                sv = sv + qq[kThreads] * wz1;
+                                sv = sv + qq[2 * kThreads] * wz2;
+                                sv = sv + qq[3 * kThreads] * wz3;
+                                Gv[c] = valid ? sv : 0.0f;
+                            }
+                            const float Gx = Gv[0], Gy = Gv[1], Gz = Gv[2];
+                            // ---- radiality term (kernel.cu:227-240, :174-183)
+                            float g2 = Gx * Gx + Gy * Gy;
+                            g2 = g2 + Gz * Gz;
+                            const float GMag = sqrtf(g2);
+                            float num = Gx * rxs + Gy * rys;
+                            num = num + Gz * rzs;
+                            const bool dv = diverging(num, GMag * S->R);
+                            float tn = Gx * dxs + Gy * dys;
+                            tn = tn + Gz * dzs;
+                            const float tt = tn / (GMag * GMag);
+                            const float xc = Gx * tt + Xc;
+                            const float yc = Gy * tt + Yc;
+                            const float zc = Gz * tt + Zc;
+                            const float ex = x0 - xc, ey = y0 - yc, ez = z0 - zc;
+                            float d2 = ex * ex + ey * ey;
+                            d2 = d2 + ez * ez;
+                            float Dk = sqrtf(d2);
+                            Dk = 1.0f - (S->r_pow2 ? Dk * S->invR : Dk / S->R);
+                            Dk = Dk * Dk;
+                            dd = dv ? dd - Dk : dd + Dk;
+                        }
+                        dsum[Z * kThreads + tid] = dd;
+                    }
+                }
+                __syncthreads();
+            }
+
+            // ---- per-frame radiality value (kernel.cu:244-251)
+            float rv[TZ];
+#pragma unroll
+            for (int Z = 0; Z < TZ; Z++) {
+                float dd = dsum[Z * kThreads + tid] / S->divisor;
+                if (P.renorm == 1) dd = 0.5f + (dd / 2.0f);
+                rv[Z] = (P.remove_con == 1) ? fmaxf(dd, 0.0f) : dd;
+            }
+
+            // ---- intensity at the voxel centre (interpolateIntensity :257-287)
+            float Iv[TZ];
+#pragma unroll
+            for (int Z = 0; Z < TZ; Z++) Iv[Z] = 0.0f;
+            if (P.iw == 1) {
+                if (tid < kTX) {
+                    int b, valid;
+                    float w0, w1, w2, w3;
+                    axis_weights((float)(X0 + tid) + 0.5f, M, W, b, w0, w1, w2, w3, valid);
+                    float *xt = xtab + tid * 6;
+                    xt[0] = __int_as_float(clampi(b - ulo, 0, NU - 4));
+                    xt[1] = w0; xt[2] = w1; xt[3] = w2; xt[4] = w3;
+                }
+                __syncthreads();
+                for (int e = tid; e < NV * NW * kTX; e += kThreads) {
+                    const int X = e % kTX, w = (e / kTX) % NW, v = e / (kTX * NW);
+                    const float *xt = xtab + X * 6;
+                    const int ub = __float_as_int(xt[0]);
+                    const float *rr = raw + ((w + g.hz) * RV + (v + g.hy)) * RU + ub + g.hx;
+                    float p = 0.0f;
+                    p = p + rr[0] * xt[1];
+                    p = p + rr[1] * xt[2];
+                    p = p + rr[2] * xt[3];
+                    p = p + rr[3] * xt[4];
+                    Px[(v * NW + w) * TXP + X] = p;
+                }
+                __syncthreads();
+                int vb, validy;
+                float wy0, wy1, wy2, wy3;
+                axis_weights(Yc, M, H, vb, wy0, wy1, wy2, wy3, validy);
+                const int vrel = clampi(vb - vlo, 0, NV - 4);
+                const int validx = !((Xc / (float)M) < 1.5f || (Xc / (float)M) > (float)W - 1.5f);
+                for (int w = 0; w < NW; w++) {
+                    const float *pp = Px + (vrel * NW + w) * TXP + xl;
+                    const int vs = NW * TXP;
+                    float q = 0.0f;
+                    q = q + pp[0] * wy0;
+                    q = q + pp[vs] * wy1;
+                    q = q + pp[2 * vs] * wy2;
+                    q = q + pp[3 * vs] * wy3;
+                    Py[w * kThreads + tid] = q;
+                }
+#pragma unroll
+                for (int Z = 0; Z < TZ; Z++) {
+                    const float *ct = ctab + Z * 6;
+                    const int wrel = __float_as_int(ct[0]);
+                    const float *qq = Py + wrel * kThreads + tid;
+                    float sv = 0.0f;
+                    sv = sv + qq[0] * ct[1];
+                    sv = sv + qq[kThreads] * ct[2];
+                    sv = sv + qq[2 * kThreads] * ct[3];
+                    sv = sv + qq[3 * kThreads] * ct[4];
+                    const int valid = validx & validy & __float_as_int(ct[5]);
+                    Iv[Z] = valid ? fmaxf(sv, 0.0f) : 0.0f;
+                }
+            }
+
+            // ---- temporal accumulation (same order as the reference)
+            const float Tf = (float)T;
+#pragma unroll
+            for (int Z = 0; Z < TZ; Z++) {
+                const float r = rv[Z];
+                if (P.method == 0) {
+                    acc0[Z] = fmaxf(P.iw == 1 ? r * Iv[Z] : r, acc0[Z]);
+                } else if (P.method == 1) {
+                    acc0[Z] += (P.iw == 1) ? r * Iv[Z] : r;
+                } else if (P.method == 2) {
+                    if (P.iw == 1) acc0[Z] += Iv[Z] / Tf;
+                    scr[((long long)t * TZ + Z) * kThreads + tid] = r;
+                } else {
+                    acc0[Z] += r * Iv[Z];
+                    acc1[Z] += r;
+                    scr[((long long)t * TZ + Z) * kThreads + tid] = r;
+                }
+            }
+        }  // frames
+
+        // ---- final temporal reduction and output
+        const bool colok = Xs >= it.bx0 && Xs < it.bx1 && Ys >= it.by0 && Ys < it.by1;
+        const float Tf = (float)T;
+        // park the register accumulators in the (now free) private LDS rows
+        float *acc_l = dsum;
+        float *acc1_l = Py;  // 3*NW*256 >= 2*TZ*256 is checked by the planner
+#pragma unroll
+        for (int Z = 0; Z < TZ; Z++) {
+            acc_l[Z * kThreads + tid] = acc0[Z];
+            acc1_l[Z * kThreads + tid] = acc1[Z];
+        }
+        for (int Z = 0; Z < TZ; Z++) {
+            const int Zs = Z0 + Z;
+            const float a0 = acc_l[Z * kThreads + tid], a1 = acc1_l[Z * kThreads + tid];
+            float out = 0.0f;
+            auto ser = [&](int tt) -> float {
+                return scr[((long long)tt * TZ + Z) * kThreads + tid];
+            };
+            if (P.method == 0) {
+                out = a0;
+            } else if (P.method == 1) {
+                out = a0 / Tf;
+            } else if (P.method == 2) {
+                // calculatePairwiseProductSum :394-416
+                float pps = 0.0f;
+                int counter = 0;
+                for (int t0 = 0; t0 < T; t0++) {
+                    const float r0 = fmaxf(ser(t0), 0.0f);
+                    if (r0 > 0) {
+                        for (int t1 = t0; t1 < T; t1++) {
+                            const float r1 = fmaxf(ser(t1), 0.0f);
+                            pps += r0 * r1;
+                            counter++;
+                        }
+                    } else
+                        counter += T - t0;
+                }
+                pps /= fmaxf((float)counter, 1.0f);
+                if (P.iw == 1) pps *= a0;
+                out = pps;
+            } else {
+                // calculateACRF :458-554
+                const int ord = P.order;
+                const float mean = a1 / Tf;
+                const float IW = a0 / Tf;
+                auto rr = [&](int tt) -> float { return tt < T ? ser(tt) : 0.0f; };
+                float ABCD = 0, ABC = 0, AB = 0, CD = 0, AC = 0, BD = 0, AD = 0, BC = 0;
+                float acc = 0.0f;
+                if (P.itc != 1) {
+                    for (int tt = 0; tt < T - ord; tt++) {
+                        const float A = rr(tt) - mean, B = rr(tt + 1) - mean;
+                        AB += (A * B);
+                        if (ord == 3) ABC += (A * B * (rr(tt + 2) - mean));
+                        if (ord == 4) {
+                            const float C = rr(tt + 2) - mean, Dd = rr(tt + 3) - mean;
+                            ABCD += A * B * C * Dd;
+                            CD += C * Dd; AC += A * C; BD += B * Dd; AD += A * Dd; BC += B * C;
+                        }
+                    }
+                    if (ord == 3) acc = fabsf(ABC) / Tf;
+                    else if (ord == 4) acc = fabsf(ABCD - AB * CD - AC * BD - AD * BC) / Tf;
+                    else acc = fabsf(AB) / Tf;
+                } else {
+                    int nB = T;
+                    while (nB > ord) {
+                        AB = 0.0f;
+                        for (int tt = 0; tt < nB - ord; tt++) {
+                            const int tBin = tt * ord;
+                            const float A = rr(tt) - mean, B = rr(tt + 1) - mean;
+                            AB += (A * B);
+                            if (ord == 3) ABC += (A * B * (rr(tt + 2) - mean));
+                            if (ord == 4) {
+                                const float C = rr(tt + 2) - mean, Dd = rr(tt + 3) - mean;
+                                ABCD += A * B * C * Dd;
+                                CD += C * Dd; AC += A * C; BD += B * Dd; AD += A * Dd; BC += B * C;
+                            }
+                            float nv = 0.0f;
+                            scr[((long long)tt * TZ + Z) * kThreads + tid] = 0.0f;
+                            if (tBin < nB)
+                                for (int q = 0; q < ord; q++) {
+                                    nv += rr(tBin + q) / (float)ord;
+                                    scr[((long long)tt * TZ + Z) * kThreads + tid] = nv;
+                                }
+                        }
+                        if (ord == 3) acc += fabsf(ABC) / (float)nB;
+                        else if (ord == 4) acc += fabsf((ABCD - AB * CD - AC * BD - AD * BC)) / (float)nB;
+                        else acc += fabsf(AB) / (float)nB;
+                        nB = nB / ord;
+                    }
+                }
+                if (P.iw == 1) acc = IW * acc;
+                out = acc;
+            }
+            if (colok && Zs >= it.bz0 && Zs < it.bz1)
+                it.out[(long long)Zs * it.osz + (long long)Ys * it.osy + Xs] = out;
+        }
+    }
+}
+
+// Explicit instantiations used by the host launcher.
+template __global__ void srrf_fused<uint16_t, 8>(LaunchParams);
+template __global__ void srrf_fused<uint16_t, 16>(LaunchParams);
+template __global__ void srrf_fused<float, 8>(LaunchParams);
+template __global__ void srrf_fused<float, 16>(LaunchParams);
+
+}  // namespace rfba

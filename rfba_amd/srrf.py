@@ -1,0 +1,347 @@
+"""Python mirror of the SRRF stage interface (librfba_srrf.so via ctypes).
+
+The names follow the reference DLL: SRRFGradientCalc / SRRFGetMag
+(RFBA Main/GradientCalc/GradientCalc.h:4-5), SRRFGradientCalcGPU and the
+SRRFConf struct (RFBA Main/GradientCalc/kernel.cuh:3-26).  Arrays are numpy,
+x-fastest: series [T, Z, Y, X], SR outputs [Z*M, Y*M, X*M].  Errors raise
+SRRFError with the library's message; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_float, c_int, c_void_p, c_size_t, c_char_p
+from dataclasses import dataclass, fields
+
+import numpy as np
+
+from . import _lib
+
+RFBA_U16 = 1
+RFBA_F32 = 2
+MAX_SAMPLES = 1024
+
+
+class SRRFError(RuntimeError):
+    pass
+
+
+class SRRFConf(ctypes.Structure):
+    """ABI-identical to kernel.cuh:3-24."""
+
+    _fields_ = [
+        ("RingRadius", c_float),
+        ("Magnification", c_int),
+        ("AxesinRing", c_int),
+        ("MainRange", c_int),
+        ("XRange", c_int),
+        ("YRange", c_int),
+        ("ZRange", c_int),
+        ("Method", c_int),
+        ("ITC", c_int),
+        ("TRACOrder", c_int),
+        ("RemoveCon", c_int),
+        ("Renormalize", c_int),
+        ("GradientSmooth", c_int),
+        ("IntensityW", c_int),
+        ("blockSizeX", c_int),
+        ("blockSizeY", c_int),
+        ("blockSizeZ", c_int),
+    ]
+
+    def replace(self, **kw) -> "SRRFConf":
+        c = SRRFConf()
+        ctypes.memmove(ctypes.addressof(c), ctypes.addressof(self), ctypes.sizeof(self))
+        for k, v in kw.items():
+            if not hasattr(c, k):
+                raise AttributeError(k)
+            setattr(c, k, v)
+        return c
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class _Samples(ctypes.Structure):
+    _fields_ = [("n", c_int), ("divisor", c_float), ("xyz", POINTER(c_float))]
+
+
+class _Block(ctypes.Structure):
+    _fields_ = [
+        ("d_src", c_void_p),
+        ("src_dims", c_int * 4),
+        ("origin", c_int * 3),
+        ("dims", c_int * 3),
+        ("frames", c_int),
+        ("mag", c_int),
+        ("d_out", c_void_p),
+    ]
+
+
+_sig_done = False
+
+
+def lib() -> ctypes.CDLL:
+    global _sig_done
+    L = _lib.load("rfba_srrf")
+    if not _sig_done:
+        L.SRRFGradientCalc.argtypes = [POINTER(c_float), POINTER(c_int), POINTER(c_float)]
+        L.SRRFGradientCalc.restype = c_int
+        L.SRRFGetMag.argtypes = []
+        L.SRRFGetMag.restype = c_int
+        L.SRRFGradientCalcGPU.argtypes = [POINTER(c_float), POINTER(c_int), POINTER(c_float), POINTER(SRRFConf)]
+        L.SRRFGradientCalcGPU.restype = c_int
+        L.rfba_srrf_load_ini.argtypes = [c_char_p, POINTER(SRRFConf)]
+        L.rfba_srrf_load_ini.restype = c_int
+        L.rfba_srrf_default_conf.argtypes = [POINTER(SRRFConf)]
+        L.rfba_srrf_default_conf.restype = None
+        L.rfba_srrf_build_ring.argtypes = [c_int, c_float, POINTER(c_float)]
+        L.rfba_srrf_build_ring.restype = c_int
+        L.rfba_srrf_build_dirs.argtypes = [c_int, c_float, POINTER(c_float)]
+        L.rfba_srrf_build_dirs.restype = c_int
+        L.rfba_srrf_stack.argtypes = [c_void_p, c_int, POINTER(c_int), POINTER(SRRFConf), POINTER(_Samples),
+                                      POINTER(c_float)]
+        L.rfba_srrf_stack.restype = c_int
+        L.rfba_srrf_stack_device.argtypes = [c_void_p, c_int, POINTER(c_int), POINTER(SRRFConf),
+                                             POINTER(_Samples), c_void_p, c_void_p, POINTER(c_float)]
+        L.rfba_srrf_stack_device.restype = c_int
+        L.rfba_srrf_batch_create.argtypes = [POINTER(_Block), c_int, c_int, POINTER(SRRFConf), POINTER(_Samples),
+                                             POINTER(c_void_p)]
+        L.rfba_srrf_batch_create.restype = c_int
+        L.rfba_srrf_batch_run.argtypes = [c_void_p, c_void_p, c_int, POINTER(c_float)]
+        L.rfba_srrf_batch_run.restype = c_int
+        L.rfba_srrf_batch_run_per_block.argtypes = [c_void_p, c_void_p, POINTER(c_float)]
+        L.rfba_srrf_batch_run_per_block.restype = c_int
+        L.rfba_srrf_batch_destroy.argtypes = [c_void_p]
+        L.rfba_srrf_batch_destroy.restype = None
+        L.rfba_last_error.argtypes = []
+        L.rfba_last_error.restype = c_char_p
+        L.rfba_device_count.argtypes = []
+        L.rfba_device_count.restype = c_int
+        L.rfba_malloc_device.argtypes = [POINTER(c_void_p), c_size_t]
+        L.rfba_malloc_device.restype = c_int
+        L.rfba_free_device.argtypes = [c_void_p]
+        L.rfba_free_device.restype = c_int
+        L.rfba_memcpy_h2d.argtypes = [c_void_p, c_void_p, c_size_t]
+        L.rfba_memcpy_h2d.restype = c_int
+        L.rfba_memcpy_d2h.argtypes = [c_void_p, c_void_p, c_size_t]
+        L.rfba_memcpy_d2h.restype = c_int
+        L.rfba_memset_device.argtypes = [c_void_p, c_int, c_size_t]
+        L.rfba_memset_device.restype = c_int
+        L.rfba_device_synchronize.argtypes = []
+        L.rfba_device_synchronize.restype = c_int
+        L.rfba_release_cache.argtypes = []
+        L.rfba_release_cache.restype = None
+        _sig_done = True
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise SRRFError("rfba error %d: %s" % (rc, lib().rfba_last_error().decode(errors="replace")))
+
+
+# ---------------------------------------------------------------- config
+def default_conf() -> SRRFConf:
+    c = SRRFConf()
+    lib().rfba_srrf_default_conf(ctypes.byref(c))
+    return c
+
+
+def load_ini(path: str | None = None) -> SRRFConf:
+    c = SRRFConf()
+    _check(lib().rfba_srrf_load_ini(path.encode() if path else None, ctypes.byref(c)))
+    return c
+
+
+def SRRFGetMag() -> int:
+    return lib().SRRFGetMag()
+
+
+# ---------------------------------------------------------------- samples
+@dataclass
+class Samples:
+    """Ordered sample offsets (SR pixels) and the divisor of their signed sum."""
+
+    xyz: np.ndarray  # float32 [n, 3]
+    divisor: float
+
+    @property
+    def n(self) -> int:
+        return int(self.xyz.shape[0])
+
+    def _c(self):
+        self._keep = np.ascontiguousarray(self.xyz, dtype=np.float32)
+        return _Samples(self.n, float(self.divisor), self._keep.ctypes.data_as(POINTER(c_float)))
+
+
+def ring(axes_in_ring: int, radius: float) -> Samples:
+    """Reference ring (kernel.cu:23-33): (2*axes)^2 samples, divisor 2*axes."""
+    n = 2 * axes_in_ring
+    buf = np.zeros((max(n * n, 1), 3), np.float32)
+    rc = lib().rfba_srrf_build_ring(axes_in_ring, radius, buf.ctypes.data_as(POINTER(c_float)))
+    if rc < 0:
+        _check(rc)
+    return Samples(buf[:rc], float(n))
+
+
+def directions(kind: int, radius: float, divisor: float | None = None) -> Samples:
+    """6 axis or 26 neighbourhood directions at `radius`; divisor defaults to n."""
+    buf = np.zeros((26, 3), np.float32)
+    rc = lib().rfba_srrf_build_dirs(kind, radius, buf.ctypes.data_as(POINTER(c_float)))
+    if rc < 0:
+        _check(rc)
+    return Samples(buf[:rc], float(rc if divisor is None else divisor))
+
+
+# ---------------------------------------------------------------- compute
+def _dims(a: np.ndarray):
+    T, Z, Y, X = a.shape
+    return (c_int * 4)(X, Y, Z, T)
+
+
+def srrf_stack(series: np.ndarray, conf: SRRFConf, samples: Samples | None = None) -> np.ndarray:
+    """One stack through the fused kernel (host arrays in and out).
+
+    series: uint16 or float32 [T, Z, Y, X]."""
+    if series.dtype == np.uint16:
+        dt = RFBA_U16
+    elif series.dtype == np.float32:
+        dt = RFBA_F32
+    else:
+        raise TypeError("series must be uint16 or float32")
+    a = np.ascontiguousarray(series)
+    T, Z, Y, X = a.shape
+    M = conf.Magnification
+    out = np.empty((Z * M, Y * M, X * M), np.float32)
+    smp = samples._c() if samples is not None else None
+    _check(lib().rfba_srrf_stack(a.ctypes.data_as(c_void_p), dt, _dims(a), ctypes.byref(conf),
+                                 ctypes.byref(smp) if smp is not None else None,
+                                 out.ctypes.data_as(POINTER(c_float))))
+    return out
+
+
+def SRRFGradientCalcGPU(series: np.ndarray, conf: SRRFConf) -> np.ndarray:
+    """kernel.cu:560: one float stack, reference ring sampling."""
+    a = np.ascontiguousarray(series, dtype=np.float32)
+    T, Z, Y, X = a.shape
+    M = conf.Magnification
+    out = np.empty((Z * M, Y * M, X * M), np.float32)
+    _check(lib().SRRFGradientCalcGPU(a.ctypes.data_as(POINTER(c_float)), _dims(a),
+                                     out.ctypes.data_as(POINTER(c_float)), ctypes.byref(conf)))
+    return out
+
+
+def SRRFGradientCalc(series: np.ndarray, out: np.ndarray | None = None, fill: float = 0.0) -> np.ndarray:
+    """GradientCalc.cpp:169: ./SRRF.ini + 16^3 tiling.  Voxels the reference
+    never writes (the low band [0, 2M) of every axis) keep `out`'s values."""
+    a = np.ascontiguousarray(series, dtype=np.float32)
+    T, Z, Y, X = a.shape
+    M = SRRFGetMag()
+    if out is None:
+        out = np.full((Z * M, Y * M, X * M), fill, np.float32)
+    _check(lib().SRRFGradientCalc(a.ctypes.data_as(POINTER(c_float)), _dims(a), out.ctypes.data_as(POINTER(c_float))))
+    return out
+
+
+# ---------------------------------------------------------------- device buffers
+class DeviceBuffer:
+    """A raw device allocation owned by Python (freed on close/del)."""
+
+    def __init__(self, nbytes: int):
+        self.ptr = c_void_p()
+        self.nbytes = int(nbytes)
+        _check(lib().rfba_malloc_device(ctypes.byref(self.ptr), c_size_t(self.nbytes)))
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        _check(lib().rfba_memcpy_h2d(b.ptr, a.ctypes.data_as(c_void_p), c_size_t(a.nbytes)))
+        return b
+
+    def to_array(self, shape, dtype) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib().rfba_memcpy_d2h(out.ctypes.data_as(c_void_p), self.ptr, c_size_t(out.nbytes)))
+        return out
+
+    def zero(self):
+        _check(lib().rfba_memset_device(self.ptr, 0, c_size_t(self.nbytes)))
+
+    def close(self):
+        if self.ptr:
+            lib().rfba_free_device(self.ptr)
+            self.ptr = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def srrf_stack_device(d_in: DeviceBuffer, dtype: int, shape_tzyx, conf: SRRFConf, samples: Samples | None,
+                      d_out: DeviceBuffer) -> float:
+    """Device-resident stack; returns the kernel time (ms) from HIP events."""
+    T, Z, Y, X = shape_tzyx
+    dims = (c_int * 4)(X, Y, Z, T)
+    ms = c_float(0)
+    smp = samples._c() if samples is not None else None
+    _check(lib().rfba_srrf_stack_device(d_in.ptr, dtype, dims, ctypes.byref(conf),
+                                        ctypes.byref(smp) if smp is not None else None, d_out.ptr, None,
+                                        ctypes.byref(ms)))
+    return ms.value
+
+
+# ---------------------------------------------------------------- batched
+@dataclass
+class Block:
+    src: DeviceBuffer
+    src_shape_tzyx: tuple
+    origin_xyz: tuple
+    dims_xyz: tuple
+    frames: int
+    mag: int
+    out: DeviceBuffer
+
+
+class Batch:
+    """Persistent work-queue plan over uneven blocks (rfba_srrf_batch_*)."""
+
+    def __init__(self, blocks: list[Block], dtype: int, conf: SRRFConf, samples: Samples | None):
+        arr = (_Block * len(blocks))()
+        for i, b in enumerate(blocks):
+            T, Z, Y, X = b.src_shape_tzyx
+            arr[i].d_src = b.src.ptr
+            arr[i].src_dims[:] = [X, Y, Z, T]
+            arr[i].origin[:] = list(b.origin_xyz)
+            arr[i].dims[:] = list(b.dims_xyz)
+            arr[i].frames = b.frames
+            arr[i].mag = b.mag
+            arr[i].d_out = b.out.ptr
+        self._blocks = blocks
+        self.plan = c_void_p()
+        smp = samples._c() if samples is not None else None
+        _check(lib().rfba_srrf_batch_create(arr, len(blocks), dtype, ctypes.byref(conf),
+                                            ctypes.byref(smp) if smp is not None else None, ctypes.byref(self.plan)))
+
+    def run(self, use_graph: bool = True) -> float:
+        ms = c_float(0)
+        _check(lib().rfba_srrf_batch_run(self.plan, None, int(use_graph), ctypes.byref(ms)))
+        return ms.value
+
+    def run_per_block(self) -> float:
+        ms = c_float(0)
+        _check(lib().rfba_srrf_batch_run_per_block(self.plan, None, ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.plan:
+            lib().rfba_srrf_batch_destroy(self.plan)
+            self.plan = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

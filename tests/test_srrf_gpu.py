@@ -1,0 +1,196 @@
+"""GPU parity of the fused SRRF kernel (librfba_srrf.so) against the FP32
+oracle (oracle/srrf_oracle.cpp) on the same seeded inputs.
+
+Bar: NaN masks identical and |gpu - oracle| <= 1e-5 * |oracle| + 1e-7 * max|oracle|
+elementwise (the kernel reproduces the reference arithmetic operation by
+operation, so in practice the outputs are bit-identical; the number of
+differing voxels is printed).  The FP64 re-evaluation error of both is
+printed per case.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from rfba_amd import srrf, synth
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+ATOL_FRAC = 1e-7
+
+
+def conf(**kw):
+    c = srrf.SRRFConf()
+    base = dict(RingRadius=0.5, Magnification=2, AxesinRing=2, MainRange=1, XRange=1, YRange=1, ZRange=1,
+                Method=1, ITC=0, TRACOrder=2, RemoveCon=0, Renormalize=0, GradientSmooth=1, IntensityW=0,
+                blockSizeX=16, blockSizeY=16, blockSizeZ=16)
+    base.update(kw)
+    for k, v in base.items():
+        setattr(c, k, v)
+    return c
+
+
+def parity(gpu, ref, label=""):
+    assert gpu.shape == ref.shape
+    nan_g, nan_r = np.isnan(gpu), np.isnan(ref)
+    assert np.array_equal(nan_g, nan_r), "%s: NaN masks differ (%d vs %d)" % (label, nan_g.sum(), nan_r.sum())
+    m = ~nan_r
+    if not m.any():
+        return 0.0
+    g, r = gpu[m].astype(np.float64), ref[m].astype(np.float64)
+    scale = np.abs(r).max()
+    diff = np.abs(g - r)
+    ndiff = int((g != r).sum())
+    tol = RTOL * np.abs(r) + ATOL_FRAC * scale
+    worst = float((diff / (np.abs(r) + ATOL_FRAC / RTOL * scale + 1e-300)).max())
+    print("%s: %d/%d voxels differ, max |d| %.3g, scale %.3g" % (label, ndiff, m.sum(), diff.max(), scale))
+    assert np.all(diff <= tol), "%s: max diff %g" % (label, diff.max())
+    return worst
+
+
+def series(shape_tzyx, n_em, seed):
+    T, Z, Y, X = shape_tzyx
+    return synth.srrf_series((X, Y, Z, T), n_em, seed)
+
+
+def test_config1_six_directions_tra(gpu):
+    # 48x48x24, T=50, 12 emitters, M=2, R=0.5, 6 axis directions, TRA
+    s = series((50, 24, 48, 48), 12, 1)
+    smp = srrf.directions(6, 0.5)
+    for iw in (0, 1):
+        c = conf(Method=1, IntensityW=iw, Magnification=2)
+        g = srrf.srrf_stack(s, c, smp)
+        o = oracle.srrf_stack(s.astype(np.float32), c, smp.xyz, smp.divisor)
+        parity(g, o, "C1 6dir TRA iw=%d" % iw)
+        o64 = oracle.srrf_stack(s.astype(np.float64), c, smp.xyz.astype(np.float64), smp.divisor, f64=True)
+        m = np.isfinite(o64)
+        print("   err vs FP64: gpu %.3g  oracle %.3g (max-normalised)" % (
+            np.abs(g[m] - o64[m]).max() / np.abs(o64[m]).max(), np.abs(o[m] - o64[m]).max() / np.abs(o64[m]).max()))
+
+
+def test_config1_reference_ring(gpu):
+    s = series((12, 24, 48, 48), 12, 1)
+    c = conf(Method=1, IntensityW=0, Magnification=2, AxesinRing=2)
+    g = srrf.srrf_stack(s, c, None)
+    ring = oracle.build_ring(2, 0.5)
+    o = oracle.srrf_stack(s.astype(np.float32), c, ring, 4.0)
+    parity(g, o, "C1 ring2 TRA")
+
+
+@pytest.mark.parametrize("method,itc,order,iw", [
+    (0, 0, 2, 0), (0, 0, 2, 1), (1, 0, 2, 1), (2, 0, 2, 0), (2, 0, 2, 1),
+    (3, 0, 2, 0), (3, 0, 3, 1), (3, 0, 4, 0), (3, 1, 2, 1), (3, 1, 3, 0), (3, 1, 4, 1)])
+def test_temporal_modes(gpu, method, itc, order, iw):
+    s = series((9, 10, 18, 20), 4, 7)   # odd T exercises the ITC read past the end
+    smp = srrf.directions(26, 0.5)
+    c = conf(Method=method, ITC=itc, TRACOrder=order, IntensityW=iw, Magnification=2)
+    g = srrf.srrf_stack(s, c, smp)
+    o = oracle.srrf_stack(s.astype(np.float32), c, smp.xyz, smp.divisor)
+    parity(g, o, "method %d itc %d ord %d iw %d" % (method, itc, order, iw))
+
+
+@pytest.mark.parametrize("mag,dirs", [(1, "d6"), (3, "d26"), (4, "d26"), (5, "ring6"), (2, "ring3")])
+def test_magnifications_and_sets(gpu, mag, dirs):
+    s = series((4, 9, 13, 15), 4, 11)
+    if dirs.startswith("ring"):
+        axes = int(dirs[4:])
+        smp = None
+        xyz, div = oracle.build_ring(axes, 0.5), 2.0 * axes
+    else:
+        smp = srrf.directions(int(dirs[1:]), 0.5)
+        xyz, div = smp.xyz, smp.divisor
+    c = conf(Method=1, IntensityW=1, Magnification=mag, AxesinRing=int(dirs[4:]) if smp is None else 2)
+    g = srrf.srrf_stack(s, c, smp)
+    o = oracle.srrf_stack(s.astype(np.float32), c, xyz, div)
+    parity(g, o, "M=%d %s" % (mag, dirs))
+
+
+@pytest.mark.parametrize("smooth,renorm,poscon,ranges", [
+    (0, 0, 0, (1, 0, 0, 0)), (1, 1, 0, (1, 1, 1, 1)), (1, 0, 1, (2, 0, 1, 0)), (0, 1, 1, (1, 0, 0, 0))])
+def test_gradient_and_radiality_options(gpu, smooth, renorm, poscon, ranges):
+    s = series((3, 11, 12, 14), 3, 13)
+    smp = srrf.directions(26, 0.5)
+    mR, xR, yR, zR = ranges
+    c = conf(Method=1, IntensityW=1, GradientSmooth=smooth, Renormalize=renorm, RemoveCon=poscon,
+             MainRange=mR, XRange=xR, YRange=yR, ZRange=zR, Magnification=3)
+    g = srrf.srrf_stack(s, c, smp)
+    o = oracle.srrf_stack(s.astype(np.float32), c, smp.xyz, smp.divisor)
+    parity(g, o, "smooth %d renorm %d poscon %d ranges %s" % (smooth, renorm, poscon, ranges))
+
+
+def test_float_input_and_drop_in_gpu_entry(gpu):
+    # normalised float frames, as the reference caller passes them
+    s = series((5, 10, 14, 16), 4, 17).astype(np.float32)
+    s = (s - s.mean()) / s.std()
+    c = conf(Method=3, ITC=1, TRACOrder=2, IntensityW=1, Magnification=2, AxesinRing=2)
+    g = srrf.SRRFGradientCalcGPU(s, c)
+    o = oracle.srrf_stack(s, c, oracle.build_ring(2, 0.5), 4.0)
+    parity(g, o, "SRRFGradientCalcGPU float TRAC ITC")
+
+
+def test_drop_in_tiled_entry(gpu, tmp_path, monkeypatch):
+    # SRRFGradientCalc reads ./SRRF.ini and tiles 16^3 with border 2
+    (tmp_path / "SRRF.ini").write_text(
+        "[SRRF]\nRingRadius=0.5\nMagnification=2\nAxesinRing=2\nMainRange=1\nXRange=1\nYRange=1\nZRange=1\n"
+        "[Temproal]\nMethod=1\n[Radiality]\nGradient_Smoothing=1\n[Weighting]\nIntensity_Weighting=1\n")
+    monkeypatch.chdir(tmp_path)
+    s = series((4, 20, 29, 33), 6, 19).astype(np.float32)
+    g = srrf.SRRFGradientCalc(s, fill=np.nan)
+    c = srrf.load_ini("SRRF.ini")
+    o = oracle.srrf_tiled(s, c, fill=np.nan)
+    parity(g, o, "SRRFGradientCalc tiled")
+
+
+def test_batched_matches_single_stacks(gpu):
+    rng = np.random.default_rng(5)
+    src = series((11, 14, 30, 33), 10, 23)
+    d_src = srrf.DeviceBuffer.from_array(src)
+    smp = srrf.directions(6, 0.5)
+    c = conf(Method=3, ITC=1, IntensityW=1)
+    blocks, refs = [], []
+    for i in range(6):
+        dims = tuple(int(v) for v in (rng.integers(6, 16), rng.integers(6, 14), rng.integers(5, 9)))
+        org = tuple(int(rng.integers(0, lim - d + 1)) for lim, d in zip((33, 30, 14), dims))
+        frames = int(rng.integers(3, 12))
+        mag = int(rng.integers(2, 5))
+        out = srrf.DeviceBuffer(dims[0] * dims[1] * dims[2] * mag ** 3 * 4)
+        blocks.append(srrf.Block(d_src, src.shape, org, dims, frames, mag, out))
+        sub = src[:frames, org[2]:org[2] + dims[2], org[1]:org[1] + dims[1], org[0]:org[0] + dims[0]]
+        refs.append(srrf.srrf_stack(np.ascontiguousarray(sub), c.replace(Magnification=mag), smp))
+    b = srrf.Batch(blocks, srrf.RFBA_U16, c, smp)
+    for mode in ("graph", "plain", "per_block", "graph"):
+        for bl in blocks:
+            bl.out.zero()
+        if mode == "per_block":
+            b.run_per_block()
+        else:
+            b.run(use_graph=(mode == "graph"))
+        for i, bl in enumerate(blocks):
+            X, Y, Z = bl.dims_xyz
+            M = bl.mag
+            got = bl.out.to_array((Z * M, Y * M, X * M), np.float32)
+            assert np.array_equal(got, refs[i], equal_nan=True), (mode, i)
+    b.close()
+
+
+def test_full_size_crop_consistency(gpu):
+    # A voxel depends on raw data within a few low-res voxels; on a crop whose
+    # border is farther than that, the crop's result equals the full result
+    # bit for bit.  Checks the large-volume tiling and queue at real size.
+    s = series((20, 40, 96, 96), 150, 29)
+    smp = srrf.directions(26, 0.5)
+    c = conf(Method=3, ITC=0, TRACOrder=2, IntensityW=1, Magnification=4)
+    full = srrf.srrf_stack(s, c, smp)
+    x0, y0, z0, n = 40, 37, 15, 12
+    crop = np.ascontiguousarray(s[:, z0:z0 + n, y0:y0 + n, x0:x0 + n])
+    o = oracle.srrf_stack(crop.astype(np.float32), c, smp.xyz, smp.divisor)
+    M, m = 4, 4
+    core_o = o[m * M:(n - m) * M, m * M:(n - m) * M, m * M:(n - m) * M]
+    core_f = full[(z0 + m) * M:(z0 + n - m) * M, (y0 + m) * M:(y0 + n - m) * M, (x0 + m) * M:(x0 + n - m) * M]
+    parity(core_f, core_o, "full-size vs crop oracle")
+    # power-of-two scaling of the input leaves radiality unchanged exactly;
+    # with IW the output scales by the same power of two
+    full2 = srrf.srrf_stack(s.astype(np.float32) * 2, c, smp)
+    assert np.array_equal(full2, full * 2, equal_nan=True)
