@@ -37,6 +37,7 @@ def lib():
         L.oracle_interp_f64.argtypes = [dp, ip, c_int, c_int, dp, c_int, dp]
         L.oracle_srrf_stack_f32.argtypes = [fp, ip, cp, fp, c_int, c_float, fp, fp, c_int]
         L.oracle_srrf_stack_f64.argtypes = [dp, ip, cp, dp, c_int, c_double, dp, dp, c_int]
+        L.oracle_srrf_stack_box_f32.argtypes = [fp, ip, cp, fp, c_int, c_float, ip, fp, c_int]
         L.oracle_temporal_f32.argtypes = [fp, fp, ip, cp, fp]
         L.oracle_temporal_f64.argtypes = [dp, dp, ip, cp, dp]
         L.oracle_srrf_tiled_f32.argtypes = [fp, ip, cp, fp, c_int]
@@ -113,6 +114,21 @@ def srrf_stack(series, conf, xyz, divisor, f64=False, with_rad=False, nthreads=N
     if rc:
         raise ValueError("oracle rejected the input")
     return (out, rad) if with_rad else out
+
+
+def srrf_stack_box(series, conf, xyz, divisor, box_xyz0, box_shape_xyz, nthreads=None):
+    """FP32 stack oracle evaluated only on an SR box (absolute coordinates);
+    returns [nz, ny, nx]."""
+    a = np.ascontiguousarray(series, np.float32)
+    ring = np.ascontiguousarray(xyz, np.float32)
+    nx, ny, nz = box_shape_xyz
+    box = (c_int * 6)(*box_xyz0, nx, ny, nz)
+    out = np.empty((nz, ny, nx), np.float32)
+    rc = lib().oracle_srrf_stack_box_f32(_p(a, c_float), _dims(a.shape), ctypes.byref(conf), _p(ring, c_float),
+                                         len(ring), divisor, box, _p(out, c_float), nthreads or threads())
+    if rc:
+        raise ValueError("oracle rejected the input")
+    return out
 
 
 def temporal(series, rad, conf, f64=False):

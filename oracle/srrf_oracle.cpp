@@ -207,14 +207,25 @@ template <typename R> struct Job {
     R divisor;
     int W, H, D, T, M;
     SRRFConf c;
+    // SR voxels evaluated: box origin and extent (default: the whole grid).
+    // rad is [T][box] and out is [box], box voxels in x-fastest order.
+    int bx0 = 0, by0 = 0, bz0 = 0, bnx = 0, bny = 0, bnz = 0;
+    size_t nbox() const { return (size_t)bnx * bny * bnz; }
+    void box_xyz(size_t i, int &X, int &Y, int &Z) const {
+        X = bx0 + (int)(i % bnx);
+        Y = by0 + (int)((i / bnx) % bny);
+        Z = bz0 + (int)(i / ((size_t)bnx * bny));
+    }
 };
 
 // calculateRadiality, kernel.cu:186-253, for SR voxels [v0, v1).
 template <typename R> void radiality_range(const Job<R> &j, size_t v0, size_t v1) {
     const int WM = j.W * j.M, HM = j.H * j.M, DM = j.D * j.M;
     const R radius = (R)j.c.RingRadius;
+    const size_t NB = j.nbox();
     for (size_t pix = v0; pix < v1; pix++) {
-        int X = (int)(pix % WM), Y = (int)((pix / WM) % HM), Z = (int)((pix / ((size_t)WM * HM)) % DM);
+        int X, Y, Z;
+        j.box_xyz(pix, X, Y, Z);
         for (int t = 0; t < j.T; t++) {
             R Xc = (R)X + (R)0.5, Yc = (R)Y + (R)0.5, Zc = (R)Z + (R)0.5;
             R DivD = 0;
@@ -235,7 +246,7 @@ template <typename R> void radiality_range(const Job<R> &j, size_t v0, size_t v1
             DivD = DivD / j.divisor;
             if (j.c.Renormalize == 1) DivD = (R)0.5 + (DivD / (R)2);
             R CGH = (j.c.RemoveCon == 1) ? std::fmax(DivD, (R)0) : DivD;
-            j.rad[(size_t)t * WM * HM * DM + pix] = CGH;
+            j.rad[(size_t)t * NB + pix] = CGH;
         }
     }
 }
@@ -244,13 +255,13 @@ template <typename R> void radiality_range(const Job<R> &j, size_t v0, size_t v1
 // calculateAVE :291-327, calculatePairwiseProductSum :372-419,
 // calculateACRF :423-555.
 template <typename R> void temporal_range(const Job<R> &j, size_t v0, size_t v1) {
-    const int WM = j.W * j.M, HM = j.H * j.M, DM = j.D * j.M;
-    const size_t N = (size_t)WM * HM * DM;
+    const size_t N = j.nbox();
     const int T = j.T, M = j.M;
     const int W = j.W, H = j.H, D = j.D;
     std::vector<R> ser(T);
     for (size_t pix = v0; pix < v1; pix++) {
-        int x = (int)(pix % WM), y = (int)((pix / WM) % HM), z = (int)((pix / ((size_t)WM * HM)) % DM);
+        int x, y, z;
+        j.box_xyz(pix, x, y, z);
         R _x = (R)x + (R)0.5, _y = (R)y + (R)0.5, _z = (R)z + (R)0.5;
         auto I = [&](int t) { return interp_intensity(_x, _y, _z, t, M, W, H, D, j.img); };
         auto rad = [&](int t) { return j.rad[(size_t)t * N + pix]; };
@@ -364,10 +375,18 @@ template <typename F> void parallel_for(size_t n, int threads, F f) {
 
 template <typename R>
 int stack_impl(const R *img, const int *dims, const SRRFConf *conf, const R *ring, int n,
-               R divisor, R *out, R *rad_out, int threads) {
+               R divisor, R *out, R *rad_out, int threads, const int *box = nullptr) {
     Job<R> j;
     j.W = dims[0]; j.H = dims[1]; j.D = dims[2]; j.T = dims[3];
     j.M = conf->Magnification;
+    if (box) {
+        j.bx0 = box[0]; j.by0 = box[1]; j.bz0 = box[2]; j.bnx = box[3]; j.bny = box[4]; j.bnz = box[5];
+        if (j.bx0 < 0 || j.by0 < 0 || j.bz0 < 0 || j.bnx <= 0 || j.bny <= 0 || j.bnz <= 0 ||
+            j.bx0 + j.bnx > j.W * j.M || j.by0 + j.bny > j.H * j.M || j.bz0 + j.bnz > j.D * j.M)
+            return -1;
+    } else {
+        j.bnx = j.W * j.M; j.bny = j.H * j.M; j.bnz = j.D * j.M;
+    }
     j.c = *conf;
     j.n = n;
     j.divisor = divisor;
@@ -385,7 +404,7 @@ int stack_impl(const R *img, const int *dims, const SRRFConf *conf, const R *rin
         }
     });
     j.gx = gx.data(); j.gy = gy.data(); j.gz = gz.data();
-    const size_t NSR = N * (size_t)j.M * j.M * j.M;
+    const size_t NSR = j.nbox();
     std::vector<R> radbuf;
     if (!rad_out) {
         radbuf.resize(NSR * j.T);
@@ -459,6 +478,13 @@ int oracle_srrf_stack_f64(const double *img, const int *dims, const SRRFConf *co
     return stack_impl<double>(img, dims, conf, ring, n, divisor, out, rad_out, threads);
 }
 
+// The same for the SR voxels of a box {x0, y0, z0, nx, ny, nz} only, with the
+// whole stack's gradients and absolute SR coordinates; out is [nz][ny][nx].
+int oracle_srrf_stack_box_f32(const float *img, const int *dims, const SRRFConf *conf, const float *ring,
+                              int n, float divisor, const int *box, float *out, int threads) {
+    return stack_impl<float>(img, dims, conf, ring, n, divisor, out, nullptr, threads, box);
+}
+
 // Temporal reduction alone on a given radiality series rad[T][N] with
 // intensities from img (kernel.cu:291-555); dims are the low-res dims.
 int oracle_temporal_f32(const float *img, const float *rad, const int *dims, const SRRFConf *conf,
@@ -470,7 +496,8 @@ int oracle_temporal_f32(const float *img, const float *rad, const int *dims, con
     j.img = img;
     j.rad = const_cast<float *>(rad);
     j.out = out;
-    size_t NSR = (size_t)j.W * j.H * j.D * j.M * j.M * j.M;
+    j.bnx = j.W * j.M; j.bny = j.H * j.M; j.bnz = j.D * j.M;
+    size_t NSR = j.nbox();
     temporal_range(j, 0, NSR);
     return 0;
 }
@@ -483,7 +510,8 @@ int oracle_temporal_f64(const double *img, const double *rad, const int *dims, c
     j.img = img;
     j.rad = const_cast<double *>(rad);
     j.out = out;
-    size_t NSR = (size_t)j.W * j.H * j.D * j.M * j.M * j.M;
+    j.bnx = j.W * j.M; j.bny = j.H * j.M; j.bnz = j.D * j.M;
+    size_t NSR = j.nbox();
     temporal_range(j, 0, NSR);
     return 0;
 }

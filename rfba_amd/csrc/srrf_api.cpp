@@ -409,9 +409,6 @@ int plan_geometry(const std::vector<DevItem> &items, const DevSamples &s, const 
     pl.g = g;
     pl.TZ = TZ;
     pl.Tmax = Tmax;
-    const int NR = g.RU * g.RV * g.RW;
-    if (NR > kRawPerThread * kThreads)
-        return fail(RFBA_EINVAL, "raw halo tile %d exceeds %d values", NR, kRawPerThread * kThreads);
     if (pl.lds_bytes > 160 * 1024) return fail(RFBA_EINVAL, "LDS plan %zu bytes exceeds 160 KiB", pl.lds_bytes);
     if (3 * g.NW < TZ) return fail(RFBA_EINVAL, "internal: y-column buffer smaller than TZ rows");
     return RFBA_OK;
@@ -439,21 +436,22 @@ void plan_tiles(const std::vector<DevItem> &items, int TZ, Plan &pl) {
 
 int choose_tz(const std::vector<DevItem> &items, const DevSamples &s, const SRRFConf &c, Plan &pl,
               int force_tz) {
-    if (force_tz == 8 || force_tz == 16) return plan_geometry(items, s, c, force_tz, pl);
-    Plan p16;
-    const int rc = plan_geometry(items, s, c, 16, p16);
-    if (rc == RFBA_OK && p16.lds_bytes <= 80 * 1024) {
-        pl = p16;
-        return RFBA_OK;
-    }
-    return plan_geometry(items, s, c, 8, pl);
+    if (force_tz == 4 || force_tz == 8 || force_tz == 16) return plan_geometry(items, s, c, force_tz, pl);
+    // Largest voxel column that leaves room for two workgroups per CU, else
+    // the largest that fits at all (small magnifications need wide halos).
+    Plan p;
+    if (plan_geometry(items, s, c, 16, p) == RFBA_OK && p.lds_bytes <= 80 * 1024) { pl = p; return RFBA_OK; }
+    if (plan_geometry(items, s, c, 8, p) == RFBA_OK && p.lds_bytes <= 80 * 1024) { pl = p; return RFBA_OK; }
+    if (plan_geometry(items, s, c, 8, p) == RFBA_OK) { pl = p; return RFBA_OK; }
+    return plan_geometry(items, s, c, 4, pl);
 }
 
 typedef void (*KernelFn)(LaunchParams);
 
 KernelFn pick_kernel(int dtype, int TZ) {
-    if (dtype == RFBA_U16) return TZ == 16 ? srrf_fused<uint16_t, 16> : srrf_fused<uint16_t, 8>;
-    return TZ == 16 ? srrf_fused<float, 16> : srrf_fused<float, 8>;
+    if (dtype == RFBA_U16)
+        return TZ == 16 ? srrf_fused<uint16_t, 16> : (TZ == 8 ? srrf_fused<uint16_t, 8> : srrf_fused<uint16_t, 4>);
+    return TZ == 16 ? srrf_fused<float, 16> : (TZ == 8 ? srrf_fused<float, 8> : srrf_fused<float, 4>);
 }
 
 int occupancy_grid(KernelFn fn, size_t lds, int ntiles, int &grid) {

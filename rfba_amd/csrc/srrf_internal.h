@@ -57,7 +57,6 @@ struct DevTile {
 constexpr int kTX = 16;
 constexpr int kTY = 16;
 constexpr int kThreads = kTX * kTY;
-constexpr int kRawPerThread = 16;  // prefetch registers per thread
 
 struct LaunchGeom {
     int NU, NV, NW;      // gradient-plane extents held in LDS

@@ -541,8 +541,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
 }
 
 // Explicit instantiations used by the host launcher.
+template __global__ void srrf_fused<uint16_t, 4>(LaunchParams);
 template __global__ void srrf_fused<uint16_t, 8>(LaunchParams);
 template __global__ void srrf_fused<uint16_t, 16>(LaunchParams);
+template __global__ void srrf_fused<float, 4>(LaunchParams);
 template __global__ void srrf_fused<float, 8>(LaunchParams);
 template __global__ void srrf_fused<float, 16>(LaunchParams);
 
