@@ -67,7 +67,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-crop", type=str, default="12,12,8")
     ap.add_argument("--frames", type=int, default=200)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run the plain copy kernel over this many MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        from rfba_amd import srrf
+        nbytes = args.copy_probe_mib << 20
+        ms = srrf.copy_probe(nbytes)
+        print(json.dumps({"copy_probe_bytes": nbytes, "ms": ms, "GB/s": 2 * nbytes / ms / 1e6}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only; multi-GPU runs are not part of this build "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)

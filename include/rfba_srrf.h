@@ -175,6 +175,9 @@ int rfba_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int rfba_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int rfba_memset_device(void *dst, int value, size_t bytes);
 int rfba_device_synchronize(void);
+/* Plain device-to-device copy kernel of `bytes` (multiple of 16): the
+ * reference point used to calibrate HBM byte counters. */
+int rfba_copy_probe(void *d_dst, const void *d_src, size_t bytes, float *ms);
 /* Release the library's cached device scratch (kept between calls). */
 void rfba_release_cache(void);
 

@@ -19,6 +19,7 @@
 namespace rfba {
 template <typename Tin, int TZ> __global__ void srrf_fused(LaunchParams P);
 __global__ void srrf_reset(int *counter, int *err);
+__global__ void copy_probe(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n);
 }  // namespace rfba
 
 using namespace rfba;
@@ -362,7 +363,8 @@ void axis_extent(int b0, int b1, int tw, float omin, float omax, int M, int &nee
     }
 }
 
-size_t lds_layout(LaunchGeom &g, int TZ, int noz) {
+size_t lds_layout(LaunchGeom &g, int TZ, const DevSamples &s) {
+    const int noz = s.noz;
     g.RU = g.NU + 2 * g.hx;
     g.RV = g.NV + 2 * g.hy;
     g.RW = g.NW + 2 * g.hz;
@@ -382,6 +384,16 @@ size_t lds_layout(LaunchGeom &g, int TZ, int noz) {
     g.off_zc = off; off += TZ;
     g.off_ctab = off; off += TZ * 6;
     g.off_dsum = off; off += TZ * kThreads;
+    g.off_stab = off;
+    {
+        int w = s.nox;
+        g.st_oy = w; w += s.noy;
+        g.st_oz = w; w += s.noz;
+        g.st_grp = w; w += 3 * s.ngroups;
+        g.st_run = w; w += 5 * s.nruns;
+        g.st_smp = w; w += 2 * s.n;
+        off += w;
+    }
     g.off_total = off;
     return (size_t)off * sizeof(float);
 }
@@ -405,7 +417,7 @@ int plan_geometry(const std::vector<DevItem> &items, const DevSamples &s, const 
         Tmax = std::max(Tmax, it.T);
     }
     g.NU = NU; g.NV = NV; g.NW = NW;
-    pl.lds_bytes = lds_layout(g, TZ, s.noz) + 64;
+    pl.lds_bytes = lds_layout(g, TZ, s) + 64;
     pl.g = g;
     pl.TZ = TZ;
     pl.Tmax = Tmax;
@@ -439,10 +451,16 @@ int choose_tz(const std::vector<DevItem> &items, const DevSamples &s, const SRRF
     if (force_tz == 4 || force_tz == 8 || force_tz == 16) return plan_geometry(items, s, c, force_tz, pl);
     // Largest voxel column that leaves room for two workgroups per CU, else
     // the largest that fits at all (small magnifications need wide halos).
+    // Measured on MI355X: occupancy (workgroups per CU, set by LDS) matters
+    // more than the shared y-pass work a taller column saves, so the first
+    // shape that fits three workgroups per CU wins, then two, then one.
     Plan p;
-    if (plan_geometry(items, s, c, 16, p) == RFBA_OK && p.lds_bytes <= 80 * 1024) { pl = p; return RFBA_OK; }
-    if (plan_geometry(items, s, c, 8, p) == RFBA_OK && p.lds_bytes <= 80 * 1024) { pl = p; return RFBA_OK; }
-    if (plan_geometry(items, s, c, 8, p) == RFBA_OK) { pl = p; return RFBA_OK; }
+    for (int budget : {53 * 1024, 80 * 1024, 160 * 1024})
+        for (int tz : {8, 16, 4})
+            if (plan_geometry(items, s, c, tz, p) == RFBA_OK && p.lds_bytes <= (size_t)budget) {
+                pl = p;
+                return RFBA_OK;
+            }
     return plan_geometry(items, s, c, 4, pl);
 }
 
@@ -958,4 +976,19 @@ extern "C" void rfba_release_cache(void) {
     rel((void *&)g_ctx.d_out, g_ctx.out_bytes);
     rel((void *&)g_ctx.d_items, g_ctx.items_cap);
     rel((void *&)g_ctx.d_tiles, g_ctx.tiles_cap);
+}
+
+extern "C" int rfba_copy_probe(void *d_dst, const void *d_src, size_t bytes, float *ms) {
+    if (!d_dst || !d_src || bytes % 16) return fail(RFBA_EINVAL, "copy probe needs 16-byte multiples");
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    int rc;
+    if ((rc = ctx_init())) return rc;
+    HIP_TRY(hipEventRecord(g_ctx.ev0, nullptr));
+    hipLaunchKernelGGL(copy_probe, dim3(g_ctx.ncu * 8), dim3(256), 0, nullptr, (const float4 *)d_src,
+                       (float4 *)d_dst, bytes / 16);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(g_ctx.ev1, nullptr));
+    HIP_TRY(hipEventSynchronize(g_ctx.ev1));
+    if (ms) HIP_TRY(hipEventElapsedTime(ms, g_ctx.ev0, g_ctx.ev1));
+    return RFBA_OK;
 }

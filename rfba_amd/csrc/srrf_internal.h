@@ -65,7 +65,9 @@ struct LaunchGeom {
     int TXP;             // padded X stride of the x-interpolated planes
     int noz;
     // float offsets into dynamic LDS
-    int off_raw, off_g, off_px, off_py, off_xtab, off_ztab, off_zc, off_ctab, off_dsum, off_total;
+    int off_raw, off_g, off_px, off_py, off_xtab, off_ztab, off_zc, off_ctab, off_dsum, off_stab, off_total;
+    // sample tables inside the off_stab region (words): ox at 0
+    int st_oy, st_oz, st_grp, st_run, st_smp;
 };
 
 struct LaunchParams {

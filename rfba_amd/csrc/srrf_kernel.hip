@@ -62,21 +62,72 @@ __device__ __forceinline__ int first_tap(float c, int M) {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-// (q > 0 || q != q) for q = num / den rounded to nearest, without the
-// division on the common path.  For finite den > 0: num <= 0 gives q <= 0;
-// num >= den * 2^-64 gives q >= 2^-65 (> 0 after rounding); only the
-// remaining tiny-positive band and non-finite/zero den divide.
-__device__ __forceinline__ bool diverging(float num, float den) {
-    if (den > 0.0f && den < __builtin_inff()) {
-        if (num > 0.0f) {
-            if (num >= den * 0x1p-64f) return true;
-            float q = num / den;
-            return q > 0.0f;
-        }
-        return !(num <= 0.0f);
-    }
-    float q = num / den;
-    return (q > 0.0f) || (q != q);
+// Signed radiality term of one sample (kernel.cu:227-240 with _calculateDk
+// :174-183): -Dk when diverging, +Dk otherwise (dd - Dk == dd + (-Dk)).
+struct TermIn {
+    float rx, ry, rz;     // ring offset of the sample
+    float dx, dy, dz;     // x0 - Xc, y0 - Yc, z0 - Zc
+    float x0, y0, z0;     // sample position
+    float Xc, Yc, Zc;     // voxel centre
+};
+
+// General form: IEEE sqrtf and division for every input.
+__device__ __forceinline__ float term_general(float Gx, float Gy, float Gz, const TermIn &a, float R,
+                                              float invR, int pow2) {
+    float g2 = Gx * Gx + Gy * Gy;
+    g2 = g2 + Gz * Gz;
+    const float GMag = sqrtf(g2);
+    float num = Gx * a.rx + Gy * a.ry;
+    num = num + Gz * a.rz;
+    const float q = num / (GMag * R);
+    const bool dv = (q > 0.0f) || (q != q);
+    float tn = Gx * a.dx + Gy * a.dy;
+    tn = tn + Gz * a.dz;
+    const float tt = tn / (GMag * GMag);
+    const float ex = a.x0 - (Gx * tt + a.Xc), ey = a.y0 - (Gy * tt + a.Yc), ez = a.z0 - (Gz * tt + a.Zc);
+    float d2 = ex * ex + ey * ey;
+    d2 = d2 + ez * ez;
+    float Dk = sqrtf(d2);
+    Dk = 1.0f - (pow2 ? Dk * invR : Dk / R);
+    Dk = Dk * Dk;
+    return dv ? -Dk : Dk;
+}
+
+// Branch-free form, bit-identical to term_general whenever `bad` stays
+// false: both sqrt arguments lie in [2^-96, inf) (there the compiler's sqrtf
+// reduces to v_sqrt_f32 plus the FMA-residual choice between neighbours) and
+// the sign of num / (GMag * R) is decided without dividing (finite positive
+// denominator, numerator <= 0 or >= den * 2^-64, or NaN).
+__device__ __forceinline__ float sqrt_core(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __int_as_float(__float_as_int(s) - 1);
+    const float su = __int_as_float(__float_as_int(s) + 1);
+    float r = (fmaf(-sd, s, x) <= 0.0f) ? sd : s;
+    r = (fmaf(-su, s, x) > 0.0f) ? su : r;
+    return r;
+}
+
+__device__ __forceinline__ float term_fast(float Gx, float Gy, float Gz, const TermIn &a, float R, float invR,
+                                           int pow2, bool &bad) {
+    float g2 = Gx * Gx + Gy * Gy;
+    g2 = g2 + Gz * Gz;
+    const float GMag = sqrt_core(g2);
+    float num = Gx * a.rx + Gy * a.ry;
+    num = num + Gz * a.rz;
+    const float den = GMag * R;
+    const bool dv = !(num <= 0.0f);
+    float tn = Gx * a.dx + Gy * a.dy;
+    tn = tn + Gz * a.dz;
+    const float tt = tn / (GMag * GMag);
+    const float ex = a.x0 - (Gx * tt + a.Xc), ey = a.y0 - (Gy * tt + a.Yc), ez = a.z0 - (Gz * tt + a.Zc);
+    float d2 = ex * ex + ey * ey;
+    d2 = d2 + ez * ez;
+    float Dk = sqrt_core(d2);
+    Dk = 1.0f - (pow2 ? Dk * invR : Dk / R);
+    Dk = Dk * Dk;
+    bad = !(g2 >= 0x1p-96f && g2 < __builtin_inff()) || !(d2 >= 0x1p-96f && d2 < __builtin_inff()) ||
+          !(den > 0.0f && den < __builtin_inff()) || (num > 0.0f && num < den * 0x1p-64f);
+    return dv ? -Dk : Dk;
 }
 
 template <typename Tin> __device__ __forceinline__ float load_px(const Tin *p, long long i);
@@ -92,6 +143,13 @@ __global__ void srrf_reset(int *counter, int *err) {
         *counter = 0;
         *err = 0;
     }
+}
+
+// Plain streaming copy of n 16-byte vectors: the reference point for HBM
+// byte counters (each byte read once and written once).
+__global__ void copy_probe(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
 }
 
 template <typename Tin, int TZ>
@@ -112,6 +170,43 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
     const int tid = threadIdx.x;
     const int xl = tid % kTX, yl = tid / kTX;
     const DevSamples *__restrict__ S = P.smp;
+    // Sample tables are read in the innermost loops with uniform indices:
+    // keep them in LDS (global reads there would wait on memory every
+    // sample) and the scalars in registers.
+    float *st_ox = lds + g.off_stab;
+    float *st_oy = st_ox + g.st_oy;
+    float *st_oz = st_ox + g.st_oz;
+    int *st_grp = (int *)(st_ox + g.st_grp);  // ix, run0, nrun
+    int *st_run = (int *)(st_ox + g.st_run);  // start, len, iy, rx, ry
+    int *st_smp = (int *)(st_ox + g.st_smp);  // iz, rz
+    const float sR = S->R, sInvR = S->invR, sDiv = S->divisor;
+    const int sPow2 = S->r_pow2, ngroups = S->ngroups;
+    const float oxmin = S->oxmin, oxmax = S->oxmax, oymin = S->oymin, oymax = S->oymax;
+    const float ozmin = S->ozmin, ozmax = S->ozmax;
+    {
+        const int nox = S->nox, noy = S->noy, nzz = S->noz, nrun = S->nruns, ns = S->n;
+        for (int e = tid; e < nox; e += kThreads) st_ox[e] = S->ox[e];
+        for (int e = tid; e < noy; e += kThreads) st_oy[e] = S->oy[e];
+        for (int e = tid; e < nzz; e += kThreads) st_oz[e] = S->oz[e];
+        for (int e = tid; e < ngroups; e += kThreads) {
+            st_grp[3 * e] = S->grp_ix[e];
+            st_grp[3 * e + 1] = S->grp_run0[e];
+            st_grp[3 * e + 2] = S->grp_nrun[e];
+        }
+        for (int e = tid; e < nrun; e += kThreads) {
+            const int s0 = S->run_start[e];
+            st_run[5 * e] = s0;
+            st_run[5 * e + 1] = S->run_len[e];
+            st_run[5 * e + 2] = S->run_iy[e];
+            st_run[5 * e + 3] = __float_as_int(S->rx[s0]);
+            st_run[5 * e + 4] = __float_as_int(S->ry[s0]);
+        }
+        for (int e = tid; e < ns; e += kThreads) {
+            st_smp[2 * e] = S->siz[e];
+            st_smp[2 * e + 1] = __float_as_int(S->rz[e]);
+        }
+    }
+    // end of sample-table setup
     float *scr = P.scratch + (long long)blockIdx.x * P.scratch_stride;
     const int NU = g.NU, NV = g.NV, NW = g.NW, TXP = g.TXP, noz = g.noz;
     const int RU = g.RU, RV = g.RV, RW = g.RW;
@@ -129,12 +224,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
         const int X0 = tl.X0, Y0 = tl.Y0, Z0 = tl.Z0;
 
         // Low-res extents of all taps of this tile (monotone in coordinate).
-        const int ulo = first_tap(((float)X0 + 0.5f) + S->oxmin, M);
-        const int vlo = first_tap(((float)Y0 + 0.5f) + S->oymin, M);
-        const int wlo = first_tap(((float)Z0 + 0.5f) + S->ozmin, M);
-        const int uhi = first_tap(((float)(X0 + kTX - 1) + 0.5f) + S->oxmax, M) + 3;
-        const int vhi = first_tap(((float)(Y0 + kTY - 1) + 0.5f) + S->oymax, M) + 3;
-        const int whi = first_tap(((float)(Z0 + TZ - 1) + 0.5f) + S->ozmax, M) + 3;
+        const int ulo = first_tap(((float)X0 + 0.5f) + oxmin, M);
+        const int vlo = first_tap(((float)Y0 + 0.5f) + oymin, M);
+        const int wlo = first_tap(((float)Z0 + 0.5f) + ozmin, M);
+        const int uhi = first_tap(((float)(X0 + kTX - 1) + 0.5f) + oxmax, M) + 3;
+        const int vhi = first_tap(((float)(Y0 + kTY - 1) + 0.5f) + oymax, M) + 3;
+        const int whi = first_tap(((float)(Z0 + TZ - 1) + 0.5f) + ozmax, M) + 3;
         if (uhi - ulo + 1 > NU || vhi - vlo + 1 > NV || whi - wlo + 1 > NW) {
             if (tid == 0) atomicOr(P.err, 1);
             continue;
@@ -144,7 +239,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
         for (int e = tid; e < TZ * noz; e += kThreads) {
             const int Z = e / noz, iz = e % noz;
             const float Zc = (float)(Z0 + Z) + 0.5f;
-            const float z0 = Zc + S->oz[iz];
+            const float z0 = Zc + st_oz[iz];
             int b, valid;
             float w0, w1, w2, w3;
             axis_weights(z0, M, D, b, w0, w1, w2, w3, valid);
@@ -239,10 +334,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
 
             for (int z = 0; z < TZ; z++) dsum[z * kThreads + tid] = 0.0f;
 
-            const int ngroups = S->ngroups;
             for (int gi = 0; gi < ngroups; gi++) {
-                const int ix = S->grp_ix[gi];
-                const float oxv = S->ox[ix];
+                const int ix = st_grp[3 * gi];
+                const float oxv = st_ox[ix];
                 if (tid < kTX) {
                     int b, valid;
                     float w0, w1, w2, w3;
@@ -264,8 +358,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
 #pragma unroll
                         for (int c = 0; c < 3; c++) {
                             const float *gg = G + c * NP + gb;
-                            float p = 0.0f;
-                            p = p + gg[0] * w0;
+                            float p = gg[0] * w0;
                             p = p + gg[1] * w1;
                             p = p + gg[2] * w2;
                             p = p + gg[3] * w3;
@@ -278,10 +371,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                 const float x0 = Xc + oxv;
                 const float dxs = x0 - Xc;
                 const int validx = !((x0 / (float)M) < 1.5f || (x0 / (float)M) > (float)W - 1.5f);
-                const int r0 = S->grp_run0[gi], nr = S->grp_nrun[gi];
+                const int r0 = st_grp[3 * gi + 1], nr = st_grp[3 * gi + 2];
                 for (int r = r0; r < r0 + nr; r++) {
-                    const int iy = S->run_iy[r];
-                    const float y0 = Yc + S->oy[iy];
+                    const int iy = st_run[5 * r + 2];
+                    const float y0 = Yc + st_oy[iy];
                     const float dys = y0 - Yc;
                     int vb, validy;
                     float wy0, wy1, wy2, wy3;
@@ -293,8 +386,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                         for (int c = 0; c < 3; c++) {
                             const float *pp = Px + ((c * NV + vrel) * NW + w) * TXP + xl;
                             const int vs = NW * TXP;
-                            float q = 0.0f;
-                            q = q + pp[0] * wy0;
+                            float q = pp[0] * wy0;
                             q = q + pp[vs] * wy1;
                             q = q + pp[2 * vs] * wy2;
                             q = q + pp[3 * vs] * wy3;
@@ -302,54 +394,57 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                         }
                     }
                     const int validxy = validx & validy;
-                    const int s0 = S->run_start[r], ns = S->run_len[r];
-                    const float rxs = S->rx[s0], rys = S->ry[s0];
-                    for (int Z = 0; Z < TZ; Z++) {
-                        const float Zc = zcen[Z];
-                        float dd = dsum[Z * kThreads + tid];
+                    const int s0 = st_run[5 * r], ns = st_run[5 * r + 1];
+                    const float rxs = __int_as_float(st_run[5 * r + 3]), rys = __int_as_float(st_run[5 * r + 4]);
+                    // Voxel pairs (Z, Z+1): two independent chains per sample.
+                    for (int Z = 0; Z < TZ; Z += 2) {
+                        float dd0 = dsum[Z * kThreads + tid], dd1 = dsum[(Z + 1) * kThreads + tid];
+                        TermIn a0, a1;
+                        a0.rx = a1.rx = rxs; a0.ry = a1.ry = rys;
+                        a0.dx = a1.dx = dxs; a0.dy = a1.dy = dys;
+                        a0.x0 = a1.x0 = x0; a0.y0 = a1.y0 = y0;
+                        a0.Xc = a1.Xc = Xc; a0.Yc = a1.Yc = Yc;
+                        a0.Zc = zcen[Z];
+                        a1.Zc = zcen[Z + 1];
                         for (int s = s0; s < s0 + ns; s++) {
-                            const int iz = S->siz[s];
-                            const float rzs = S->rz[s];
-                            const float *zt = ztab + (Z * noz + iz) * 8;
-                            const int wrel = __float_as_int(zt[0]);
-                            const float wz0 = zt[1], wz1 = zt[2], wz2 = zt[3], wz3 = zt[4];
-                            const int valid = validxy & __float_as_int(zt[5]);
-                            const float z0 = zt[6], dzs = zt[7];
-                            // ---- z pass
-                            float Gv[3];
+                            const int iz = st_smp[2 * s];
+                            a0.rz = a1.rz = __int_as_float(st_smp[2 * s + 1]);
+                            const float *zt0 = ztab + (Z * noz + iz) * 8;
+                            const float *zt1 = ztab + ((Z + 1) * noz + iz) * 8;
+                            const int wr0 = __float_as_int(zt0[0]), wr1 = __float_as_int(zt1[0]);
+                            const int v0 = validxy & __float_as_int(zt0[5]);
+                            const int v1 = validxy & __float_as_int(zt1[5]);
+                            a0.z0 = zt0[6]; a0.dz = zt0[7];
+                            a1.z0 = zt1[6]; a1.dz = zt1[7];
+                            // ---- z pass (interpolateG :154-167, innermost sum last)
+                            float G0[3], G1[3];
 #pragma unroll
                             for (int c = 0; c < 3; c++) {
-                                const float *qq = Py + (c * NW + wrel) * kThreads + tid;
-                                float sv = 0.0f;
-                                sv = sv + qq[0] * wz0;
-                                sv = sv + qq[kThreads] * wz1;
-                                sv = sv + qq[2 * kThreads] * wz2;
-                                sv = sv + qq[3 * kThreads] * wz3;
-                                Gv[c] = valid ? sv : 0.0f;
+                                const float *q0 = Py + (c * NW + wr0) * kThreads + tid;
+                                const float *q1 = Py + (c * NW + wr1) * kThreads + tid;
+                                float s0v = q0[0] * zt0[1];
+                                float s1v = q1[0] * zt1[1];
+                                s0v = s0v + q0[kThreads] * zt0[2];
+                                s1v = s1v + q1[kThreads] * zt1[2];
+                                s0v = s0v + q0[2 * kThreads] * zt0[3];
+                                s1v = s1v + q1[2 * kThreads] * zt1[3];
+                                s0v = s0v + q0[3 * kThreads] * zt0[4];
+                                s1v = s1v + q1[3 * kThreads] * zt1[4];
+                                G0[c] = v0 ? s0v : 0.0f;
+                                G1[c] = v1 ? s1v : 0.0f;
                             }
-                            const float Gx = Gv[0], Gy = Gv[1], Gz = Gv[2];
-                            // ---- radiality term (kernel.cu:227-240, :174-183)
-                            float g2 = Gx * Gx + Gy * Gy;
-                            g2 = g2 + Gz * Gz;
-                            const float GMag = sqrtf(g2);
-                            float num = Gx * rxs + Gy * rys;
-                            num = num + Gz * rzs;
-                            const bool dv = diverging(num, GMag * S->R);
-                            float tn = Gx * dxs + Gy * dys;
-                            tn = tn + Gz * dzs;
-                            const float tt = tn / (GMag * GMag);
-                            const float xc = Gx * tt + Xc;
-                            const float yc = Gy * tt + Yc;
-                            const float zc = Gz * tt + Zc;
-                            const float ex = x0 - xc, ey = y0 - yc, ez = z0 - zc;
-                            float d2 = ex * ex + ey * ey;
-                            d2 = d2 + ez * ez;
-                            float Dk = sqrtf(d2);
-                            Dk = 1.0f - (S->r_pow2 ? Dk * S->invR : Dk / S->R);
-                            Dk = Dk * Dk;
-                            dd = dv ? dd - Dk : dd + Dk;
+                            bool b0, b1;
+                            float t0 = term_fast(G0[0], G0[1], G0[2], a0, sR, sInvR, sPow2, b0);
+                            float t1 = term_fast(G1[0], G1[1], G1[2], a1, sR, sInvR, sPow2, b1);
+                            if (b0 || b1) {
+                                if (b0) t0 = term_general(G0[0], G0[1], G0[2], a0, sR, sInvR, sPow2);
+                                if (b1) t1 = term_general(G1[0], G1[1], G1[2], a1, sR, sInvR, sPow2);
+                            }
+                            dd0 = dd0 + t0;
+                            dd1 = dd1 + t1;
                         }
-                        dsum[Z * kThreads + tid] = dd;
+                        dsum[Z * kThreads + tid] = dd0;
+                        dsum[(Z + 1) * kThreads + tid] = dd1;
                     }
                 }
                 __syncthreads();
@@ -359,7 +454,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
             float rv[TZ];
 #pragma unroll
             for (int Z = 0; Z < TZ; Z++) {
-                float dd = dsum[Z * kThreads + tid] / S->divisor;
+                float dd = dsum[Z * kThreads + tid] / sDiv;
                 if (P.renorm == 1) dd = 0.5f + (dd / 2.0f);
                 rv[Z] = (P.remove_con == 1) ? fmaxf(dd, 0.0f) : dd;
             }

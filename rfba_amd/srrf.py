@@ -129,6 +129,8 @@ def lib() -> ctypes.CDLL:
         L.rfba_memset_device.restype = c_int
         L.rfba_device_synchronize.argtypes = []
         L.rfba_device_synchronize.restype = c_int
+        L.rfba_copy_probe.argtypes = [c_void_p, c_void_p, c_size_t, POINTER(c_float)]
+        L.rfba_copy_probe.restype = c_int
         L.rfba_release_cache.argtypes = []
         L.rfba_release_cache.restype = None
         _sig_done = True
@@ -290,6 +292,17 @@ def srrf_stack_device(d_in: DeviceBuffer, dtype: int, shape_tzyx, conf: SRRFConf
     _check(lib().rfba_srrf_stack_device(d_in.ptr, dtype, dims, ctypes.byref(conf),
                                         ctypes.byref(smp) if smp is not None else None, d_out.ptr, None,
                                         ctypes.byref(ms)))
+    return ms.value
+
+
+def copy_probe(nbytes: int) -> float:
+    """Run the plain copy kernel over nbytes (device to device); returns ms."""
+    a, b = DeviceBuffer(nbytes), DeviceBuffer(nbytes)
+    a.zero()
+    ms = c_float(0)
+    _check(lib().rfba_copy_probe(b.ptr, a.ptr, c_size_t(nbytes), ctypes.byref(ms)))
+    a.close()
+    b.close()
     return ms.value
 
 
