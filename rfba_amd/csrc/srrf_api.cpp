@@ -17,7 +17,7 @@
 #include "srrf_internal.h"
 
 namespace rfba {
-template <typename Tin, int TZ> __global__ void srrf_fused(LaunchParams P);
+template <typename Tin, int TZ, bool RP2> __global__ void srrf_fused(LaunchParams P);
 __global__ void srrf_reset(int *counter, int *err);
 __global__ void copy_probe(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n);
 }  // namespace rfba
@@ -220,6 +220,7 @@ int analyse_samples(const float *xyz, int n, float divisor, float R, DevSamples 
     const float m = std::frexp(R, &ex);
     s.r_pow2 = (R > 0.0f && std::isnormal(R) && m == 0.5f) ? 1 : 0;
     s.invR = s.r_pow2 ? 1.0f / R : 0.0f;
+    s.fast_ok = (R >= 0x1p-20f && R <= 0x1p20f) ? 1 : 0;
     s.oxmin = s.oymin = s.ozmin = 0.0f;
     s.oxmax = s.oymax = s.ozmax = 0.0f;
     int prev_ix = -1, prev_iy = -1;
@@ -466,10 +467,15 @@ int choose_tz(const std::vector<DevItem> &items, const DevSamples &s, const SRRF
 
 typedef void (*KernelFn)(LaunchParams);
 
-KernelFn pick_kernel(int dtype, int TZ) {
+template <bool RP2> KernelFn pick_kernel_r(int dtype, int TZ) {
     if (dtype == RFBA_U16)
-        return TZ == 16 ? srrf_fused<uint16_t, 16> : (TZ == 8 ? srrf_fused<uint16_t, 8> : srrf_fused<uint16_t, 4>);
-    return TZ == 16 ? srrf_fused<float, 16> : (TZ == 8 ? srrf_fused<float, 8> : srrf_fused<float, 4>);
+        return TZ == 16 ? srrf_fused<uint16_t, 16, RP2>
+                        : (TZ == 8 ? srrf_fused<uint16_t, 8, RP2> : srrf_fused<uint16_t, 4, RP2>);
+    return TZ == 16 ? srrf_fused<float, 16, RP2> : (TZ == 8 ? srrf_fused<float, 8, RP2> : srrf_fused<float, 4, RP2>);
+}
+
+KernelFn pick_kernel(int dtype, int TZ, const DevSamples &s) {
+    return s.r_pow2 ? pick_kernel_r<true>(dtype, TZ) : pick_kernel_r<false>(dtype, TZ);
 }
 
 int occupancy_grid(KernelFn fn, size_t lds, int ntiles, int &grid) {
@@ -513,7 +519,7 @@ int run_items(const std::vector<DevItem> &items, int dtype, const SRRFConf &c, c
     if (rc) return rc;
     plan_tiles(items, pl.TZ, pl);
     if (pl.tiles.empty()) return RFBA_OK;
-    KernelFn fn = pick_kernel(dtype, pl.TZ);
+    KernelFn fn = pick_kernel(dtype, pl.TZ, s);
     int grid = 0;
     if ((rc = occupancy_grid(fn, pl.lds_bytes, (int)pl.tiles.size(), grid))) return rc;
     if ((rc = ensure(g_ctx.d_items, g_ctx.items_cap, items.size() * sizeof(DevItem)))) return rc;
@@ -797,7 +803,7 @@ extern "C" int rfba_srrf_batch_create(const rfba_srrf_block *blocks, int n, int 
         return rc;
     }
     plan_tiles(b->items, b->pl.TZ, b->pl);
-    b->fn = pick_kernel(dtype, b->pl.TZ);
+    b->fn = pick_kernel(dtype, b->pl.TZ, s);
     if ((rc = occupancy_grid(b->fn, b->pl.lds_bytes, (int)b->pl.tiles.size(), b->grid))) {
         batch_free(b);
         return rc;

@@ -21,6 +21,7 @@ struct DevSamples {
     float R;        // ring radius (SR pixels)
     float invR;     // 1/R when R is a power of two (then x/R == x*invR)
     int r_pow2;
+    int fast_ok;    // R in [2^-20, 2^20]: the kernel's branch-free path applies
     int nox, noy, noz;
     int nruns, ngroups;
     float oxmin, oxmax, oymin, oymax, ozmin, ozmax;  // include 0 (voxel centre)

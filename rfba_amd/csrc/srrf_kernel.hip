@@ -72,8 +72,8 @@ struct TermIn {
 };
 
 // General form: IEEE sqrtf and division for every input.
-__device__ __forceinline__ float term_general(float Gx, float Gy, float Gz, const TermIn &a, float R,
-                                              float invR, int pow2) {
+template <bool RP2>
+__device__ __forceinline__ float term_general(float Gx, float Gy, float Gz, const TermIn &a, float R, float invR) {
     float g2 = Gx * Gx + Gy * Gy;
     g2 = g2 + Gz * Gz;
     const float GMag = sqrtf(g2);
@@ -88,7 +88,7 @@ __device__ __forceinline__ float term_general(float Gx, float Gy, float Gz, cons
     float d2 = ex * ex + ey * ey;
     d2 = d2 + ez * ez;
     float Dk = sqrtf(d2);
-    Dk = 1.0f - (pow2 ? Dk * invR : Dk / R);
+    Dk = 1.0f - (RP2 ? Dk * invR : Dk / R);
     Dk = Dk * Dk;
     return dv ? -Dk : Dk;
 }
@@ -107,8 +107,9 @@ __device__ __forceinline__ float sqrt_core(float x) {
     return r;
 }
 
+template <bool RP2>
 __device__ __forceinline__ float term_fast(float Gx, float Gy, float Gz, const TermIn &a, float R, float invR,
-                                           int pow2, bool &bad) {
+                                           bool &bad) {
     float g2 = Gx * Gx + Gy * Gy;
     g2 = g2 + Gz * Gz;
     const float GMag = sqrt_core(g2);
@@ -123,10 +124,13 @@ __device__ __forceinline__ float term_fast(float Gx, float Gy, float Gz, const T
     float d2 = ex * ex + ey * ey;
     d2 = d2 + ez * ez;
     float Dk = sqrt_core(d2);
-    Dk = 1.0f - (pow2 ? Dk * invR : Dk / R);
+    Dk = 1.0f - (RP2 ? Dk * invR : Dk / R);
     Dk = Dk * Dk;
-    bad = !(g2 >= 0x1p-96f && g2 < __builtin_inff()) || !(d2 >= 0x1p-96f && d2 < __builtin_inff()) ||
-          !(den > 0.0f && den < __builtin_inff()) || (num > 0.0f && num < den * 0x1p-64f);
+    // With R in [2^-20, 2^20] (checked on the host, else every sample takes
+    // the general form) a finite g2 >= 2^-96 makes den finite and positive;
+    // infinite or NaN d2 come out of sqrt_core as sqrtf gives them.
+    bad = !(g2 >= 0x1p-96f && g2 < __builtin_inff()) || !(d2 >= 0x1p-96f) ||
+          (num > 0.0f && num < den * 0x1p-64f);
     return dv ? -Dk : Dk;
 }
 
@@ -152,7 +156,7 @@ __global__ void copy_probe(const float4 *__restrict__ src, float4 *__restrict__ 
         dst[i] = src[i];
 }
 
-template <typename Tin, int TZ>
+template <typename Tin, int TZ, bool RP2>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void srrf_fused(LaunchParams P) {
     extern __shared__ float lds[];
     __shared__ int s_tile;
@@ -180,7 +184,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
     int *st_run = (int *)(st_ox + g.st_run);  // start, len, iy, rx, ry
     int *st_smp = (int *)(st_ox + g.st_smp);  // iz, rz
     const float sR = S->R, sInvR = S->invR, sDiv = S->divisor;
-    const int sPow2 = S->r_pow2, ngroups = S->ngroups;
+    const int ngroups = S->ngroups;
+    const bool fast_ok = S->fast_ok != 0;
     const float oxmin = S->oxmin, oxmax = S->oxmax, oymin = S->oymin, oymax = S->oymax;
     const float ozmin = S->ozmin, ozmax = S->ozmax;
     {
@@ -380,7 +385,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                     float wy0, wy1, wy2, wy3;
                     axis_weights(y0, M, H, vb, wy0, wy1, wy2, wy3, validy);
                     const int vrel = clampi(vb - vlo, 0, NV - 4);
-                    // ---- y pass: Py[c][w][tid] = sum_j Px[c][vrel+j][w][X] * wy_j
+                    // ---- y pass: Py[w][c][tid] = sum_j Px[c][vrel+j][w][X] * wy_j
                     for (int w = 0; w < NW; w++) {
 #pragma unroll
                         for (int c = 0; c < 3; c++) {
@@ -390,7 +395,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                             q = q + pp[vs] * wy1;
                             q = q + pp[2 * vs] * wy2;
                             q = q + pp[3 * vs] * wy3;
-                            Py[(c * NW + w) * kThreads + tid] = q;
+                            Py[(w * 3 + c) * kThreads + tid] = q;
                         }
                     }
                     const int validxy = validx & validy;
@@ -420,25 +425,27 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                             float G0[3], G1[3];
 #pragma unroll
                             for (int c = 0; c < 3; c++) {
-                                const float *q0 = Py + (c * NW + wr0) * kThreads + tid;
-                                const float *q1 = Py + (c * NW + wr1) * kThreads + tid;
+                                const float *q0 = Py + (wr0 * 3 + c) * kThreads + tid;
+                                const float *q1 = Py + (wr1 * 3 + c) * kThreads + tid;
                                 float s0v = q0[0] * zt0[1];
                                 float s1v = q1[0] * zt1[1];
-                                s0v = s0v + q0[kThreads] * zt0[2];
-                                s1v = s1v + q1[kThreads] * zt1[2];
-                                s0v = s0v + q0[2 * kThreads] * zt0[3];
-                                s1v = s1v + q1[2 * kThreads] * zt1[3];
-                                s0v = s0v + q0[3 * kThreads] * zt0[4];
-                                s1v = s1v + q1[3 * kThreads] * zt1[4];
+                                s0v = s0v + q0[3 * kThreads] * zt0[2];
+                                s1v = s1v + q1[3 * kThreads] * zt1[2];
+                                s0v = s0v + q0[6 * kThreads] * zt0[3];
+                                s1v = s1v + q1[6 * kThreads] * zt1[3];
+                                s0v = s0v + q0[9 * kThreads] * zt0[4];
+                                s1v = s1v + q1[9 * kThreads] * zt1[4];
                                 G0[c] = v0 ? s0v : 0.0f;
                                 G1[c] = v1 ? s1v : 0.0f;
                             }
                             bool b0, b1;
-                            float t0 = term_fast(G0[0], G0[1], G0[2], a0, sR, sInvR, sPow2, b0);
-                            float t1 = term_fast(G1[0], G1[1], G1[2], a1, sR, sInvR, sPow2, b1);
+                            float t0 = term_fast<RP2>(G0[0], G0[1], G0[2], a0, sR, sInvR, b0);
+                            float t1 = term_fast<RP2>(G1[0], G1[1], G1[2], a1, sR, sInvR, b1);
+                            b0 = b0 || !fast_ok;
+                            b1 = b1 || !fast_ok;
                             if (b0 || b1) {
-                                if (b0) t0 = term_general(G0[0], G0[1], G0[2], a0, sR, sInvR, sPow2);
-                                if (b1) t1 = term_general(G1[0], G1[1], G1[2], a1, sR, sInvR, sPow2);
+                                if (b0) t0 = term_general<RP2>(G0[0], G0[1], G0[2], a0, sR, sInvR);
+                                if (b1) t1 = term_general<RP2>(G1[0], G1[1], G1[2], a1, sR, sInvR);
                             }
                             dd0 = dd0 + t0;
                             dd1 = dd1 + t1;
@@ -636,11 +643,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
 }
 
 // Explicit instantiations used by the host launcher.
-template __global__ void srrf_fused<uint16_t, 4>(LaunchParams);
-template __global__ void srrf_fused<uint16_t, 8>(LaunchParams);
-template __global__ void srrf_fused<uint16_t, 16>(LaunchParams);
-template __global__ void srrf_fused<float, 4>(LaunchParams);
-template __global__ void srrf_fused<float, 8>(LaunchParams);
-template __global__ void srrf_fused<float, 16>(LaunchParams);
+#define RFBA_INST(T, Z)                                                  \
+    template __global__ void srrf_fused<T, Z, false>(LaunchParams);     \
+    template __global__ void srrf_fused<T, Z, true>(LaunchParams);
+RFBA_INST(uint16_t, 4)
+RFBA_INST(uint16_t, 8)
+RFBA_INST(uint16_t, 16)
+RFBA_INST(float, 4)
+RFBA_INST(float, 8)
+RFBA_INST(float, 16)
 
 }  // namespace rfba

@@ -191,3 +191,15 @@ def test_full_size_boxes(gpu):
     # with intensity weighting the output scales by the same power of two
     full2 = srrf.srrf_stack(s.astype(np.float32) * 2, c, smp)
     assert np.array_equal(full2, full * 2, equal_nan=True)
+
+
+@pytest.mark.parametrize("radius", [0.6, 0.75, 1.0, 5e-7])
+def test_radius_variants(gpu, radius):
+    # non power-of-two radius (division path), a larger radius (wider halo)
+    # and one below the fast path's range (every sample takes the general form)
+    s = series((4, 10, 13, 15), 4, 31)
+    smp = srrf.directions(26, radius)
+    c = conf(Method=3, ITC=1, IntensityW=1, Magnification=3, RingRadius=radius)
+    g = srrf.srrf_stack(s, c, smp)
+    o = oracle.srrf_stack(s.astype(np.float32), c, smp.xyz, smp.divisor)
+    parity(g, o, "R=%g" % radius)
