@@ -1,0 +1,102 @@
+/*
+ * rfba_3b.h -- C ABI of the MI355X (gfx950) 3B likelihood library
+ * (librfba_3b.so): the per-block spot log-likelihood of RFBA's 3B stage,
+ * marginalised over on/off/bleached blink states, with its gradient.
+ *
+ * Model (reference): spot shape and its derivative
+ *   RFBA Main/storm.h:51-80 (spot_shape_s, spot_shape_diff_position),
+ * per-frame log-probabilities of a spot over a background
+ *   RFBA Main/spot_with_background.hh:106-151,
+ * HMM forward recursion in log space with ln(0) = -1e100
+ *   RFBA Main/forward_algorithm.h:34-40, :157-256, :344-382,
+ * and the conjugate-gradient functor over sampled backgrounds
+ *   RFBA Main/multispot5.cc:579-669.
+ * The reference has no C ABI for this path (an in-process C++ functor); the
+ * entry points below are what its callers (ConjugateGradientOnly<5>::
+ * find_next_point, conjugate_gradient_only.h:106-232; GibbsSampler2::next,
+ * sampled_multispot.h:681-709) would bind.
+ *
+ * Precision: spot images and residuals in FP32; per-frame sums, the
+ * forward/backward recursions and all reductions in FP64.  The gradient is
+ * the adjoint (reverse mode): forward messages are kept only every
+ * `checkpoint` frames and recomputed segment by segment during the
+ * backward pass.
+ *
+ * Conventions: parameters phi = (I, sigma, x, y, z) (storm.h:71); states
+ * 0 = on, 1 = off, 2 = bleached; pixel coordinates are integer voxel
+ * positions (sampled_multispot.h:170-177); frames x pixels arrays are
+ * frame-major [T][P].  Return 0 on success, negative on error
+ * (rfba3b_last_error).
+ */
+#ifndef RFBA_3B_H
+#define RFBA_3B_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rfba3b_model {
+    double A[9];          /* A[3*i + j] = P(state i -> state j) (multispot5.cfg:15) */
+    double pi[3];         /* initial state probabilities (multispot5.cfg:16) */
+    double variance;      /* Gaussian pixel-noise variance (multispot5.cc:1440) */
+    double zfactor;       /* z scale of the spot shape (storm.h:55) */
+    int z_exact;          /* 0: the reference's d/dz (storm.h:78, no 1/zF^2); 1: exact */
+    int checkpoint;       /* forward-message checkpoint stride; 0 = ceil(sqrt(T)) */
+    double mu_brightness, sigma_brightness;  /* log-normal prior of I (multispot5.cc:660-661) */
+    double mu_blur, sigma_blur;              /* log-normal prior of sigma (:662) */
+} rfba3b_model;
+
+/* Reference functor, SpotNegProbabilityDiffWithSampledBackground::operator()
+ * (multispot5.cc:616-669), for n candidate parameter vectors of one spot:
+ *   g = -( (1/S) sum_s d/dphi logL(phi; bg_s) + [dlnLN(I), dlnLN(sigma), 0, 0, 0] )
+ * NaN rows for I <= 0 or sigma <= 0 (multispot5.cc:618-619).
+ * Host buffers: xyz [P][3] pixel coordinates, data [T][P], bg [S][T][P]
+ * (sampled intensities of the other spots), phi [n][5].
+ * Outputs: grad [n][5]; logL [n][S] may be NULL. */
+int rfba3b_neg_grad_sampled(const int *xyz, int P, int T, const float *data, const float *bg, int S,
+                            const double *phi, int n, const rfba3b_model *model, double *grad,
+                            double *logL);
+
+/* Forward messages delta_t(i) (forward_algorithm.h:344-382) of one spot over
+ * one background, for forward filtering / backward sampling.
+ * Host buffers; delta [T][3]; returns logL in *logL (may be NULL). */
+int rfba3b_forward_messages(const int *xyz, int P, int T, const float *data, const float *bg,
+                            const double *phi, const rfba3b_model *model, double *delta, double *logL);
+
+/* ---- batched model evaluation over blocks -------------------------------
+ * A block holds a dense X x Y x Z voxel box with T frames of data, a
+ * constant background b0, K spots, Q parameter sets for all K spots and S
+ * sampled state paths of every spot.  Work item (q, k, s): spot k at
+ * params[q][k] over the background b0 + sum_{j != k} [state_s,j(t) == on]
+ * mu(params[q][j]) (the other spots' sampled intensities, as remove_spot in
+ * multispot5.cc:1602-1603), giving logL[q][k][s] and grad[q][k][s][5]. */
+typedef struct rfba3b_block {
+    int dims[3];              /* X, Y, Z (pixel (x,y,z) has coordinates x, y, z) */
+    int T, K, Q, S;
+    double b0;                /* constant background */
+    const float *d_data;      /* device [T][Z][Y][X] */
+    const double *d_params;   /* device [Q][K][5] */
+    const uint8_t *d_states;  /* device [S][K][T], 0 = on */
+    double *d_logL;           /* device [Q][K][S] */
+    double *d_grad;           /* device [Q][K][S][5], NULL for logL only */
+} rfba3b_block;
+
+typedef struct rfba3b_batch rfba3b_batch;  /* opaque plan */
+
+int rfba3b_batch_create(const rfba3b_block *blocks, int n, const rfba3b_model *model, rfba3b_batch **plan);
+/* All blocks: background planes then one persistent work-queue launch over
+ * every (block, q, k, s) item, optionally captured into a hipGraph. */
+int rfba3b_batch_run(rfba3b_batch *plan, void *stream, int use_graph, float *kernel_ms);
+/* Reference form for comparison: one launch pair per (block, q). */
+int rfba3b_batch_run_per_block(rfba3b_batch *plan, void *stream, float *kernel_ms);
+void rfba3b_batch_destroy(rfba3b_batch *plan);
+
+const char *rfba3b_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RFBA_3B_H */

@@ -1,0 +1,244 @@
+// b3_oracle.cpp -- TEST INFRASTRUCTURE ONLY (the oracle).
+//
+// FP64 host restatement of the 3B likelihood path, used by tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker.
+// No product code path links or calls it.  Each function follows the cited
+// reference lines in operation order (the gradient in forward mode, as the
+// reference computes it; the GPU computes it in reverse mode).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <limits>
+#include <thread>
+#include <vector>
+
+struct rfba3b_model {
+    double A[9];
+    double pi[3];
+    double variance;
+    double zfactor;
+    int z_exact;
+    int checkpoint;
+    double mu_brightness, sigma_brightness;
+    double mu_blur, sigma_blur;
+};
+
+namespace {
+
+// ln, forward_algorithm.h:34-40
+inline double ln(double x) { return x == 0 ? -1e100 : std::log(x); }
+inline double sq(double x) { return x * x; }
+
+// spot_shape_s + spot_shape_diff_position, storm.h:51-80.  With z_exact the
+// z derivative carries the 1/zF^2 the reference omits (storm.h:78).
+void spot_diff(const int *x, const double *phi, double zF, int z_exact, double &prob, double *deriv) {
+    double d0 = x[0] - phi[2], d1 = x[1] - phi[3], d2 = x[2] - phi[4];
+    d2 = d2 / zF;
+    const double s = -(d0 * d0 + d1 * d1 + d2 * d2) / (2 * phi[1] * phi[1]);
+    const double r_2_pi = std::sqrt(2 * M_PI);
+    prob = std::exp(s) * phi[0] / (phi[1] * r_2_pi);
+    const double f = std::exp(s) / (phi[1] * r_2_pi);
+    deriv[0] = f * 1;
+    deriv[1] = f * (-phi[0] * (1 + 2 * s) / phi[1]);
+    deriv[2] = f * ((x[0] - phi[2]) * (phi[0] / sq(phi[1])));
+    deriv[3] = f * ((x[1] - phi[3]) * (phi[0] / sq(phi[1])));
+    double dz = (x[2] - phi[4]) * (phi[0] / sq(phi[1]));
+    if (z_exact) dz = dz / (zF * zF);
+    deriv[4] = f * dz;
+}
+
+// SpotWithBackground constructor, spot_with_background.hh:106-151.
+void emissions(const double *px, const double *bg, int T, int P, const double *mu, const double *dmu, double var,
+               double *on, double *off, double *diff) {
+    for (int t = 0; t < T; t++) {
+        double lo = 0, lf = 0, d[5] = {0, 0, 0, 0, 0};
+        const double *pt = px + (size_t)t * P, *bt = bg + (size_t)t * P;
+        for (int p = 0; p < P; p++) {
+            const double e = pt[p] - (bt[p] + mu[p]);
+            lf -= sq(pt[p] - bt[p]);
+            lo -= sq(e);
+            for (int i = 0; i < 5; i++) d[i] += dmu[5 * p + i] * e;
+        }
+        lo = lo / (2 * var) - P * std::log(2 * M_PI * var) / 2;
+        lf = lf / (2 * var) - P * std::log(2 * M_PI * var) / 2;
+        on[t] = lo;
+        off[t] = lf;
+        for (int i = 0; i < 5; i++) diff[5 * t + i] = d[i] / var;
+    }
+}
+
+inline double B(int state, int t, const double *on, const double *off) { return state == 0 ? on[t] : off[t]; }
+
+// forward_algorithm_hessian with compute_deriv, forward_algorithm.h:157-256.
+void forward_deriv(const double *A, const double *pi, const double *on, const double *off, const double *diff, int T,
+                   double &logL, double *grad) {
+    std::vector<double> delta(3 * T), dd(15 * T);
+    for (int i = 0; i < 3; i++) {
+        delta[i] = ln(pi[i]) + B(i, 0, on, off);
+        for (int n = 0; n < 5; n++) dd[5 * i + n] = (i == 0) ? diff[n] : 0.0;
+    }
+    for (int t = 1; t < T; t++)
+        for (int j = 0; j < 3; j++) {
+            double Z = -HUGE_VAL;
+            for (int i = 0; i < 3; i++) Z = std::max(Z, delta[3 * (t - 1) + i] + ln(A[3 * i + j]));
+            double sum = 0;
+            for (int i = 0; i < 3; i++) sum += std::exp(delta[3 * (t - 1) + i] + ln(A[3 * i + j]) - Z);
+            delta[3 * t + j] = B(j, t, on, off) + Z + ln(sum);
+            double top[5] = {0, 0, 0, 0, 0};
+            for (int i = 0; i < 3; i++) {
+                const double w = std::exp(delta[3 * (t - 1) + i] + ln(A[3 * i + j]) - Z);
+                for (int n = 0; n < 5; n++) top[n] += dd[15 * (t - 1) + 5 * i + n] * w;
+            }
+            for (int n = 0; n < 5; n++) dd[15 * t + 5 * j + n] = ((j == 0) ? diff[5 * t + n] : 0.0) + top[n] / sum;
+        }
+    double Z = -HUGE_VAL;
+    for (int i = 0; i < 3; i++) Z = std::max(Z, delta[3 * (T - 1) + i]);
+    double sum = 0;
+    for (int i = 0; i < 3; i++) sum += std::exp(delta[3 * (T - 1) + i] - Z);
+    logL = Z + ln(sum);
+    for (int n = 0; n < 5; n++) grad[n] = 0;
+    for (int i = 0; i < 3; i++) {
+        const double w = std::exp(delta[3 * (T - 1) + i] - logL);
+        for (int n = 0; n < 5; n++) grad[n] += w * dd[15 * (T - 1) + 5 * i + n];
+    }
+}
+
+// forward_algorithm_delta, forward_algorithm.h:344-382 (+ the final log
+// probability of forward_algorithm :301-310).
+double forward_delta(const double *A, const double *pi, const double *on, const double *off, int T, double *delta) {
+    for (int i = 0; i < 3; i++) delta[i] = ln(pi[i]) + B(i, 0, on, off);
+    for (int t = 1; t < T; t++)
+        for (int j = 0; j < 3; j++) {
+            double Z = -HUGE_VAL;
+            for (int i = 0; i < 3; i++) Z = std::max(Z, delta[3 * (t - 1) + i] + ln(A[3 * i + j]));
+            double sum = 0;
+            for (int i = 0; i < 3; i++) sum += std::exp(delta[3 * (t - 1) + i] + ln(A[3 * i + j]) - Z);
+            delta[3 * t + j] = B(j, t, on, off) + Z + ln(sum);
+        }
+    double Z = -HUGE_VAL;
+    for (int i = 0; i < 3; i++) Z = std::max(Z, delta[3 * (T - 1) + i]);
+    double sum = 0;
+    for (int i = 0; i < 3; i++) sum += std::exp(delta[3 * (T - 1) + i] - Z);
+    return Z + ln(sum);
+}
+
+// diff_log_log_normal, storm.h:434-437
+inline double diff_log_log_normal(double x, double mu, double sigma) { return -(1 + (ln(x) - mu) / sq(sigma)) / x; }
+
+void spot_tables(const int *xyz, int P, const double *phi, const rfba3b_model &m, std::vector<double> &mu,
+                 std::vector<double> &dmu) {
+    mu.resize(P);
+    dmu.resize(5 * (size_t)P);
+    for (int p = 0; p < P; p++) spot_diff(xyz + 3 * p, phi, m.zfactor, m.z_exact, mu[p], &dmu[5 * (size_t)p]);
+}
+
+template <typename F> void parallel_items(int n, int threads, F f) {
+    threads = std::max(1, std::min(threads, n));
+    std::vector<std::thread> th;
+    for (int i = 0; i < threads; i++)
+        th.emplace_back([=] {
+            for (int j = i; j < n; j += threads) f(j);
+        });
+    for (auto &t : th) t.join();
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_b3_spot(const int *xyz, int P, const double *phi, double zF, int z_exact, double *mu, double *dmu) {
+    for (int p = 0; p < P; p++) spot_diff(xyz + 3 * p, phi, zF, z_exact, mu[p], dmu + 5 * (size_t)p);
+    return 0;
+}
+
+int oracle_b3_emissions(const double *px, const double *bg, int T, int P, const double *mu, const double *dmu,
+                        double var, double *on, double *off, double *diff) {
+    emissions(px, bg, T, P, mu, dmu, var, on, off, diff);
+    return 0;
+}
+
+int oracle_b3_forward_deriv(const double *A, const double *pi, const double *on, const double *off,
+                            const double *diff, int T, double *logL, double *grad) {
+    forward_deriv(A, pi, on, off, diff, T, *logL, grad);
+    return 0;
+}
+
+double oracle_b3_forward_delta(const double *A, const double *pi, const double *on, const double *off, int T,
+                               double *delta) {
+    return forward_delta(A, pi, on, off, T, delta);
+}
+
+double oracle_b3_diff_log_log_normal(double x, double mu, double sigma) { return diff_log_log_normal(x, mu, sigma); }
+
+// logL and d logL/dphi of one spot over one background (data, bg [T][P]).
+int oracle_b3_logl_grad(const int *xyz, int P, int T, const double *data, const double *bg, const double *phi,
+                        const rfba3b_model *m, double *logL, double *grad) {
+    std::vector<double> mu, dmu, on(T), off(T), diff(5 * (size_t)T);
+    spot_tables(xyz, P, phi, *m, mu, dmu);
+    emissions(data, bg, T, P, mu.data(), dmu.data(), m->variance, on.data(), off.data(), diff.data());
+    forward_deriv(m->A, m->pi, on.data(), off.data(), diff.data(), T, *logL, grad);
+    return 0;
+}
+
+// SpotNegProbabilityDiffWithSampledBackground::operator(), multispot5.cc:616-669
+// (samples summed in index order).
+int oracle_b3_neg_grad(const int *xyz, int P, int T, const double *data, const double *bg, int S, const double *phi,
+                       const rfba3b_model *m, double *g, double *logL_s) {
+    if (phi[0] <= 0 || phi[1] <= 0) {
+        for (int n = 0; n < 5; n++) g[n] = std::numeric_limits<double>::quiet_NaN();
+        return 0;
+    }
+    double sum[5] = {0, 0, 0, 0, 0};
+    for (int s = 0; s < S; s++) {
+        double L, gr[5];
+        oracle_b3_logl_grad(xyz, P, T, data, bg + (size_t)s * T * P, phi, m, &L, gr);
+        if (logL_s) logL_s[s] = L;
+        for (int n = 0; n < 5; n++) sum[n] += gr[n];
+    }
+    double d[5];
+    for (int n = 0; n < 5; n++) d[n] = sum[n] / S;
+    const double prior[5] = {diff_log_log_normal(phi[0], m->mu_brightness, m->sigma_brightness),
+                             diff_log_log_normal(phi[1], m->mu_blur, m->sigma_blur), 0, 0, 0};
+    for (int n = 0; n < 5; n++) g[n] = -(d[n] + prior[n]);
+    return 0;
+}
+
+// Batched model evaluation of one block: item (q, k, s) is spot k at
+// params[q][k] over b0 + the sampled on-intensities of the other spots.
+int oracle_b3_model_eval(const int *dims, int T, const double *data, double b0, const rfba3b_model *m, int K,
+                         const double *params, int Q, const uint8_t *states, int S, double *logL, double *grad,
+                         int threads) {
+    const int X = dims[0], Y = dims[1], Z = dims[2];
+    const int P = X * Y * Z;
+    std::vector<int> xyz(3 * (size_t)P);
+    for (int p = 0; p < P; p++) {
+        xyz[3 * p] = p % X;
+        xyz[3 * p + 1] = (p / X) % Y;
+        xyz[3 * p + 2] = p / (X * Y);
+    }
+    parallel_items(Q * S, threads, [&](int qs) {
+        const int q = qs / S, s = qs % S;
+        std::vector<std::vector<double>> mus(K);
+        std::vector<double> dmu;
+        for (int j = 0; j < K; j++) spot_tables(xyz.data(), P, params + (size_t)(q * K + j) * 5, *m, mus[j], dmu);
+        std::vector<double> bg((size_t)T * P);
+        for (int k = 0; k < K; k++) {
+            for (int t = 0; t < T; t++)
+                for (int p = 0; p < P; p++) {
+                    double v = b0;
+                    for (int j = 0; j < K; j++)
+                        if (j != k && states[((size_t)s * K + j) * T + t] == 0) v += mus[j][p];
+                    bg[(size_t)t * P + p] = v;
+                }
+            double L, gr[5];
+            oracle_b3_logl_grad(xyz.data(), P, T, data, bg.data(), params + (size_t)(q * K + k) * 5, m, &L, gr);
+            const size_t item = ((size_t)q * K + k) * S + s;
+            logL[item] = L;
+            if (grad)
+                for (int n = 0; n < 5; n++) grad[item * 5 + n] = gr[n];
+        }
+    });
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,262 @@
+"""Python mirror of the 3B likelihood interface (librfba_3b.so via ctypes).
+
+Reference: the gradient functor SpotNegProbabilityDiffWithSampledBackground
+(RFBA Main/multispot5.cc:579-669) over SpotWithBackground
+(spot_with_background.hh:57-152) and forward_algorithm_deriv
+(forward_algorithm.h:157-332); forward messages for FFBS as
+forward_algorithm_delta (forward_algorithm.h:344-382).  Parameters are
+(I, sigma, x, y, z); states 0 = on, 1 = off, 2 = bleached.  Errors raise
+B3Error; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_double, c_float, c_int, c_uint8, c_uint64, c_void_p, c_char_p
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .srrf import DeviceBuffer
+
+
+class B3Error(RuntimeError):
+    pass
+
+
+class Model(ctypes.Structure):
+    """ABI of rfba3b_model."""
+
+    _fields_ = [
+        ("A", c_double * 9),
+        ("pi", c_double * 3),
+        ("variance", c_double),
+        ("zfactor", c_double),
+        ("z_exact", c_int),
+        ("checkpoint", c_int),
+        ("mu_brightness", c_double),
+        ("sigma_brightness", c_double),
+        ("mu_blur", c_double),
+        ("sigma_blur", c_double),
+    ]
+
+
+def make_model(A, pi, variance, zfactor=1.0, z_exact=0, checkpoint=0, mu_brightness=0.0, sigma_brightness=1.0,
+               mu_blur=0.0, sigma_blur=1.0) -> Model:
+    m = Model()
+    m.A[:] = [float(v) for v in np.asarray(A, np.float64).ravel()]
+    m.pi[:] = [float(v) for v in np.asarray(pi, np.float64).ravel()]
+    m.variance = variance
+    m.zfactor = zfactor
+    m.z_exact = z_exact
+    m.checkpoint = checkpoint
+    m.mu_brightness = mu_brightness
+    m.sigma_brightness = sigma_brightness
+    m.mu_blur = mu_blur
+    m.sigma_blur = sigma_blur
+    return m
+
+
+def blink_matrix(p_on_off=0.2, p_off_on=0.1, p_bleach=0.01):
+    """3-state transition matrix (on, off, bleached), A[i][j] = P(i -> j)."""
+    return np.array([[1 - p_on_off - p_bleach, p_on_off, p_bleach],
+                     [p_off_on, 1 - p_off_on - p_bleach, p_bleach],
+                     [0.0, 0.0, 1.0]])
+
+
+class _Block(ctypes.Structure):
+    _fields_ = [
+        ("dims", c_int * 3),
+        ("T", c_int), ("K", c_int), ("Q", c_int), ("S", c_int),
+        ("b0", c_double),
+        ("d_data", c_void_p), ("d_params", c_void_p), ("d_states", c_void_p),
+        ("d_logL", c_void_p), ("d_grad", c_void_p),
+    ]
+
+
+_sig = False
+
+
+def lib():
+    global _sig
+    L = _lib.load("rfba_3b")
+    if not _sig:
+        ip, fp, dp = POINTER(c_int), POINTER(c_float), POINTER(c_double)
+        L.rfba3b_neg_grad_sampled.argtypes = [ip, c_int, c_int, fp, fp, c_int, dp, c_int, POINTER(Model), dp, dp]
+        L.rfba3b_neg_grad_sampled.restype = c_int
+        L.rfba3b_forward_messages.argtypes = [ip, c_int, c_int, fp, fp, dp, POINTER(Model), dp, dp]
+        L.rfba3b_forward_messages.restype = c_int
+        L.rfba3b_batch_create.argtypes = [POINTER(_Block), c_int, POINTER(Model), POINTER(c_void_p)]
+        L.rfba3b_batch_create.restype = c_int
+        L.rfba3b_batch_run.argtypes = [c_void_p, c_void_p, c_int, POINTER(c_float)]
+        L.rfba3b_batch_run.restype = c_int
+        L.rfba3b_batch_run_per_block.argtypes = [c_void_p, c_void_p, POINTER(c_float)]
+        L.rfba3b_batch_run_per_block.restype = c_int
+        L.rfba3b_batch_destroy.argtypes = [c_void_p]
+        L.rfba3b_batch_destroy.restype = None
+        L.rfba3b_last_error.argtypes = []
+        L.rfba3b_last_error.restype = c_char_p
+        _sig = True
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise B3Error("rfba3b error %d: %s" % (rc, lib().rfba3b_last_error().decode(errors="replace")))
+
+
+def _p(a, t):
+    return a.ctypes.data_as(POINTER(t))
+
+
+def neg_grad_sampled(xyz, data, bg, phi, model: Model):
+    """The CG functor: -(mean_s grad logL(phi; bg_s) + grad log-prior).
+
+    xyz [P,3] int, data [T,P], bg [S,T,P], phi [n,5] -> (grad [n,5], logL [n,S])."""
+    xyz = np.ascontiguousarray(xyz, np.int32)
+    data = np.ascontiguousarray(data, np.float32)
+    bg = np.ascontiguousarray(bg, np.float32)
+    phi = np.ascontiguousarray(np.atleast_2d(phi), np.float64)
+    S, T, P = bg.shape
+    n = phi.shape[0]
+    g = np.empty((n, 5), np.float64)
+    L = np.empty((n, S), np.float64)
+    _check(lib().rfba3b_neg_grad_sampled(_p(xyz, c_int), P, T, _p(data, c_float), _p(bg, c_float), S,
+                                         _p(phi, c_double), n, ctypes.byref(model), _p(g, c_double), _p(L, c_double)))
+    return g, L
+
+
+def forward_messages(xyz, data, bg, phi, model: Model):
+    """delta [T,3] (ln forward messages) and logL of one spot over one background."""
+    xyz = np.ascontiguousarray(xyz, np.int32)
+    data = np.ascontiguousarray(data, np.float32)
+    bg = np.ascontiguousarray(bg, np.float32)
+    phi = np.ascontiguousarray(phi, np.float64).ravel()
+    T, P = data.shape
+    delta = np.empty((T, 3), np.float64)
+    L = c_double(0)
+    _check(lib().rfba3b_forward_messages(_p(xyz, c_int), P, T, _p(data, c_float), _p(bg, c_float),
+                                         _p(phi, c_double), ctypes.byref(model), _p(delta, c_double), ctypes.byref(L)))
+    return delta, L.value
+
+
+@dataclass
+class BlockData:
+    data: np.ndarray     # float32 [T, Z, Y, X]
+    params: np.ndarray   # float64 [Q, K, 5]
+    states: np.ndarray   # uint8 [S, K, T]
+    b0: float
+
+
+class Batch:
+    """Device plan over blocks: logL [Q,K,S] and grad [Q,K,S,5] per block."""
+
+    def __init__(self, blocks: list[BlockData], model: Model, with_grad=True):
+        self.model = model
+        self.blocks = blocks
+        self._bufs = []
+        arr = (_Block * len(blocks))()
+        self.out = []
+        for i, b in enumerate(blocks):
+            T, Z, Y, X = b.data.shape
+            Q, K, _ = b.params.shape
+            S = b.states.shape[0]
+            dd = DeviceBuffer.from_array(np.ascontiguousarray(b.data, np.float32))
+            dp = DeviceBuffer.from_array(np.ascontiguousarray(b.params, np.float64))
+            ds = DeviceBuffer.from_array(np.ascontiguousarray(b.states, np.uint8))
+            dl = DeviceBuffer(Q * K * S * 8)
+            dg = DeviceBuffer(Q * K * S * 5 * 8) if with_grad else None
+            self._bufs += [dd, dp, ds, dl] + ([dg] if dg else [])
+            self.out.append((dl, dg, (Q, K, S)))
+            a = arr[i]
+            a.dims[:] = [X, Y, Z]
+            a.T, a.K, a.Q, a.S = T, K, Q, S
+            a.b0 = b.b0
+            a.d_data, a.d_params, a.d_states = dd.ptr, dp.ptr, ds.ptr
+            a.d_logL = dl.ptr
+            a.d_grad = dg.ptr if dg else None
+        self.plan = c_void_p()
+        _check(lib().rfba3b_batch_create(arr, len(blocks), ctypes.byref(model), ctypes.byref(self.plan)))
+
+    def run(self, use_graph=True) -> float:
+        ms = c_float(0)
+        _check(lib().rfba3b_batch_run(self.plan, None, int(use_graph), ctypes.byref(ms)))
+        return ms.value
+
+    def run_per_block(self) -> float:
+        ms = c_float(0)
+        _check(lib().rfba3b_batch_run_per_block(self.plan, None, ctypes.byref(ms)))
+        return ms.value
+
+    def results(self, i):
+        dl, dg, (Q, K, S) = self.out[i]
+        L = dl.to_array((Q, K, S), np.float64)
+        G = dg.to_array((Q, K, S, 5), np.float64) if dg else None
+        return L, G
+
+    def close(self):
+        if self.plan:
+            lib().rfba3b_batch_destroy(self.plan)
+            self.plan = c_void_p()
+        for b in self._bufs:
+            b.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- workload
+def synth_block(dims_xyz, T, K, seed, I=1304.0, sigma=1.3, zfactor=2.0, b0=100.0, variance=100.0, A=None,
+                pi=(0.5, 0.5, 0.0)):
+    """Seeded block: (data [T,Z,Y,X] float32, true params [K,5], states [K,T])."""
+    from ctypes import c_uint16  # noqa: F401
+    L = _lib.load("rfba_synth")
+    L.rfba_synth_3b_block.argtypes = [c_int, c_int, c_int, c_int, c_int, c_uint64, c_double, c_double, c_double,
+                                      c_double, c_double, POINTER(c_double), POINTER(c_double), POINTER(c_float),
+                                      POINTER(c_double), POINTER(c_uint8)]
+    L.rfba_synth_3b_block.restype = c_int
+    X, Y, Z = dims_xyz
+    A = blink_matrix() if A is None else np.asarray(A)
+    A = np.ascontiguousarray(A, np.float64)
+    pi = np.ascontiguousarray(pi, np.float64)
+    data = np.empty((T, Z, Y, X), np.float32)
+    params = np.empty((K, 5), np.float64)
+    states = np.empty((K, T), np.uint8)
+    rc = L.rfba_synth_3b_block(X, Y, Z, T, K, seed, I, sigma, zfactor, b0, variance, _p(A, c_double),
+                               _p(pi, c_double), _p(data, c_float), _p(params, c_double), _p(states, c_uint8))
+    if rc:
+        raise B3Error("generator failed")
+    return data, params, states
+
+
+def perturb(params, Q, seed, pos=0.5, rel_int=0.2):
+    """Q seeded perturbations of all spots: positions +-pos px, intensity +-rel_int."""
+    rng = np.random.default_rng(seed)
+    K = params.shape[0]
+    out = np.repeat(params[None], Q, 0).copy()
+    out[:, :, 2:5] += rng.uniform(-pos, pos, (Q, K, 3))
+    out[:, :, 0] *= 1 + rng.uniform(-rel_int, rel_int, (Q, K))
+    return out
+
+
+def smoke_check():
+    """One small block through the batched adjoint kernel vs the FP64 oracle."""
+    from oracle import b3oracle
+    A = blink_matrix()
+    m = make_model(A, (0.5, 0.5, 0.0), 100.0, zfactor=2.0, checkpoint=4)
+    data, params, states = synth_block((9, 8, 7), 12, 3, 5)
+    P = perturb(params, 2, 6)
+    st = states[None]
+    b = Batch([BlockData(data, P, st, 100.0)], m)
+    b.run(use_graph=False)
+    L, G = b.results(0)
+    b.close()
+    Lo, Go = b3oracle.model_eval(data, 100.0, m, P, st)
+    errL = float(np.max(np.abs(L - Lo) / np.abs(Lo)))
+    errG = float(np.max(np.abs(G - Go)) / np.max(np.abs(Go)))
+    if not (errL < 1e-6 and errG < 1e-4):
+        raise AssertionError("3B smoke: logL rel err %g, grad err %g" % (errL, errG))
+    print("smoke 3b: logL rel err %.3g, grad err %.3g (max-normalised)" % (errL, errG), flush=True)

@@ -1,0 +1,544 @@
+// b3_api.cpp -- host side of librfba_3b.so: the C ABI of include/rfba_3b.h,
+// work planning (residual planes, item queue, memory chunks), launches and
+// hipGraph capture.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <vector>
+
+#include "b3_internal.h"
+
+namespace rfba3b {
+__global__ void b3_reset(int *counter);
+__global__ void b3_resid(const DevBlock *__restrict__ blocks, const PlaneTask *__restrict__ tasks, int ntasks,
+                         float *__restrict__ R, const DevModel *__restrict__ mdl);
+template <int MODE> __global__ void b3_eval(EvalParams E);
+}  // namespace rfba3b
+
+using namespace rfba3b;
+
+static thread_local char g_err[512] = "";
+
+static int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(e_ == hipErrorOutOfMemory ? -2 : -3, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" const char *rfba3b_last_error(void) { return g_err; }
+
+namespace {
+
+std::mutex g_mu;
+int g_ncu = 0;
+
+int dev_init() {
+    if (g_ncu) return 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(-4, "no HIP device");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0) return fail(-4, "device %s is not gfx950", prop.gcnArchName);
+    g_ncu = prop.multiProcessorCount;
+    return 0;
+}
+
+int check_model(const rfba3b_model *m) {
+    if (!m) return fail(-1, "model is NULL");
+    if (!(m->variance > 0) || !std::isfinite(m->variance)) return fail(-1, "variance must be positive");
+    if (!(m->zfactor > 0) || !std::isfinite(m->zfactor)) return fail(-1, "zfactor must be positive");
+    for (int i = 0; i < 9; i++)
+        if (!(m->A[i] >= 0)) return fail(-1, "transition matrix has a negative or NaN entry");
+    for (int i = 0; i < 3; i++)
+        if (!(m->pi[i] >= 0)) return fail(-1, "initial distribution has a negative or NaN entry");
+    if (m->checkpoint < 0) return fail(-1, "checkpoint stride must be >= 0");
+    return 0;
+}
+
+DevModel dev_model(const rfba3b_model *m) {
+    DevModel d;
+    memset(&d, 0, sizeof d);
+    for (int i = 0; i < 9; i++) d.A[i] = m->A[i];
+    for (int i = 0; i < 3; i++) d.pi[i] = m->pi[i];
+    d.variance = m->variance;
+    d.zfactor = m->zfactor;
+    d.z_exact = m->z_exact;
+    return d;
+}
+
+int ckpt_of(const rfba3b_model *m, int T) {
+    return m->checkpoint > 0 ? std::min(m->checkpoint, T) : std::max(1, (int)std::ceil(std::sqrt((double)T)));
+}
+
+size_t eval_lds(int Tmax, int XYmax, int Zmax, int ckptmax) {
+    const int nseg = (Tmax + 0) / 1 + 1;  // bound: at most Tmax segments (ckpt >= 1)
+    return 8 * 8 * (size_t)Tmax + 16 * (size_t)XYmax + 16 * (size_t)Zmax + 16 +
+           8 * (3 * (size_t)nseg + 4 * (size_t)ckptmax);
+}
+
+typedef void (*EvalFn)(EvalParams);
+EvalFn eval_fn(int mode) {
+    return mode == MODE_GRAD ? b3_eval<MODE_GRAD> : (mode == MODE_DELTA ? b3_eval<MODE_DELTA> : b3_eval<MODE_LOGL>);
+}
+
+int eval_grid(EvalFn fn, size_t lds, int nitems, int &grid) {
+    if (lds > 64 * 1024)
+        HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, 256, lds));
+    if (per_cu <= 0) return fail(-1, "likelihood kernel does not fit on a CU (LDS %zu bytes)", lds);
+    grid = std::max(1, std::min(nitems, per_cu * g_ncu));
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// batched plan
+
+struct rfba3b_batch {
+    rfba3b_model model;
+    std::vector<rfba3b_block> ublocks;
+    std::vector<DevBlock> blocks;
+    // chunks of blocks whose residual planes fit the plane buffer together
+    struct Chunk {
+        int task0, ntask, item0, nitem;
+        std::vector<std::pair<int, int>> bq;  // (block, q) ranges: task/item sub-ranges per (b, q)
+        std::vector<int> bq_task0, bq_ntask, bq_item0, bq_nitem;
+    };
+    std::vector<Chunk> chunks;
+    int mode = MODE_GRAD;
+    int Tmax = 1, XYmax = 1, Zmax = 1, ckpt = 1;
+    size_t lds_eval = 0, lds_resid = 0;
+    int grid_eval = 0;
+    DevModel *d_model = nullptr;
+    DevBlock *d_blocks = nullptr;
+    PlaneTask *d_tasks = nullptr;
+    Item *d_items = nullptr;
+    int *d_counter = nullptr;
+    float *d_R = nullptr;
+    size_t R_floats = 0;
+    std::vector<PlaneTask> tasks;
+    std::vector<Item> items;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+static void batch_free(rfba3b_batch *b) {
+    if (!b) return;
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    hipFree(b->d_model);
+    hipFree(b->d_blocks);
+    hipFree(b->d_tasks);
+    hipFree(b->d_items);
+    hipFree(b->d_counter);
+    hipFree(b->d_R);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    delete b;
+}
+
+extern "C" int rfba3b_batch_create(const rfba3b_block *blocks, int n, const rfba3b_model *model,
+                                   rfba3b_batch **plan) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if (!plan) return fail(-1, "plan pointer is NULL");
+    *plan = nullptr;
+    if (!blocks || n <= 0) return fail(-1, "no blocks");
+    if ((rc = check_model(model)) || (rc = dev_init())) return rc;
+    rfba3b_batch *b = new rfba3b_batch();
+    b->model = *model;
+    bool grad = true;
+    for (int i = 0; i < n; i++) {
+        const rfba3b_block &u = blocks[i];
+        if (u.dims[0] <= 0 || u.dims[1] <= 0 || u.dims[2] <= 0 || u.T <= 0 || u.K <= 0 || u.Q <= 0 || u.S <= 0 ||
+            !u.d_data || !u.d_params || !u.d_states || !u.d_logL) {
+            batch_free(b);
+            return fail(-1, "block %d: invalid dimensions or buffers", i);
+        }
+        if (u.dims[0] * u.dims[1] > 4096 || u.dims[2] > 1024 || u.T > 4096) {
+            batch_free(b);
+            return fail(-1, "block %d: box or frame count too large", i);
+        }
+        grad = grad && u.d_grad;
+        DevBlock d;
+        memset(&d, 0, sizeof d);
+        d.X = u.dims[0]; d.Y = u.dims[1]; d.Z = u.dims[2];
+        d.T = u.T; d.K = u.K; d.Q = u.Q; d.S = u.S;
+        d.b0 = u.b0;
+        d.npix = u.dims[0] * u.dims[1] * u.dims[2];
+        d.data = u.d_data; d.params = u.d_params; d.states = u.d_states;
+        d.logL = u.d_logL; d.grad = u.d_grad;
+        b->blocks.push_back(d);
+        b->ublocks.push_back(u);
+        b->Tmax = std::max(b->Tmax, u.T);
+        b->XYmax = std::max(b->XYmax, u.dims[0] * u.dims[1]);
+        b->Zmax = std::max(b->Zmax, u.dims[2]);
+        const size_t lr = sizeof(float) * ((size_t)u.K * (u.dims[0] + u.dims[1] + u.dims[2]) + 2 * (size_t)u.K);
+        b->lds_resid = std::max(b->lds_resid, lr);
+    }
+    if (b->lds_resid > 64 * 1024) {
+        batch_free(b);
+        return fail(-1, "too many spots per block for the residual kernel");
+    }
+    b->mode = grad ? MODE_GRAD : MODE_LOGL;
+    b->ckpt = 0;
+    int ckmax = 1;
+    for (auto &d : b->blocks) ckmax = std::max(ckmax, ckpt_of(model, d.T));
+    b->ckpt = model->checkpoint;  // 0 -> per-item ceil(sqrt(T)) in the kernel
+    b->lds_eval = eval_lds(b->Tmax, b->XYmax, b->Zmax, ckmax);
+
+    // chunks: residual planes of all (q, s) of a block stay together; blocks
+    // are added to a chunk while the planes fit the budget.
+    const char *bud = getenv("RFBA3B_PLANE_BUDGET_MB");
+    const size_t budget = (size_t)(bud ? atol(bud) : 16384) << 20;
+    size_t cur = 0;
+    rfba3b_batch::Chunk ch;
+    ch.task0 = ch.item0 = 0;
+    ch.ntask = ch.nitem = 0;
+    size_t maxR = 0;
+    for (int bi = 0; bi < n; bi++) {
+        const DevBlock &d = b->blocks[bi];
+        const size_t P = (size_t)d.X * d.Y * d.Z;
+        const size_t bytes = (size_t)d.Q * d.S * d.T * P * sizeof(float);
+        if (cur > 0 && cur + bytes > budget) {
+            b->chunks.push_back(ch);
+            ch = rfba3b_batch::Chunk();
+            ch.task0 = (int)b->tasks.size();
+            ch.item0 = (int)b->items.size();
+            ch.ntask = ch.nitem = 0;
+            cur = 0;
+        }
+        for (int q = 0; q < d.Q; q++) {
+            ch.bq_task0.push_back((int)b->tasks.size());
+            ch.bq_item0.push_back((int)b->items.size());
+            for (int s = 0; s < d.S; s++) {
+                const long long off = (long long)(cur / sizeof(float));
+                for (int t = 0; t < d.T; t++) b->tasks.push_back({bi, q, s, t, off});
+                for (int k = 0; k < d.K; k++) b->items.push_back({bi, q, k, s, off});
+                cur += (size_t)d.T * P * sizeof(float);
+            }
+            ch.bq_ntask.push_back((int)b->tasks.size() - ch.bq_task0.back());
+            ch.bq_nitem.push_back((int)b->items.size() - ch.bq_item0.back());
+        }
+        ch.ntask = (int)b->tasks.size() - ch.task0;
+        ch.nitem = (int)b->items.size() - ch.item0;
+        maxR = std::max(maxR, cur);
+    }
+    b->chunks.push_back(ch);
+    b->R_floats = maxR / sizeof(float);
+
+    if ((rc = eval_grid(eval_fn(b->mode), b->lds_eval, (int)b->items.size(), b->grid_eval))) {
+        batch_free(b);
+        return rc;
+    }
+    hipError_t e = hipSuccess;
+    DevModel dm = dev_model(model);
+    if (e == hipSuccess) e = hipMalloc(&b->d_model, sizeof dm);
+    if (e == hipSuccess) e = hipMalloc(&b->d_blocks, b->blocks.size() * sizeof(DevBlock));
+    if (e == hipSuccess) e = hipMalloc(&b->d_tasks, b->tasks.size() * sizeof(PlaneTask));
+    if (e == hipSuccess) e = hipMalloc(&b->d_items, b->items.size() * sizeof(Item));
+    if (e == hipSuccess) e = hipMalloc(&b->d_counter, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&b->d_R, std::max<size_t>(1, b->R_floats) * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(b->d_model, &dm, sizeof dm, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_blocks, b->blocks.data(), b->blocks.size() * sizeof(DevBlock), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_tasks, b->tasks.data(), b->tasks.size() * sizeof(PlaneTask), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_items, b->items.data(), b->items.size() * sizeof(Item), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e != hipSuccess) {
+        int code = e == hipErrorOutOfMemory ? -2 : -3;
+        batch_free(b);
+        return fail(code, "3B plan setup: %s", hipGetErrorString(e));
+    }
+    *plan = b;
+    return 0;
+}
+
+static EvalParams eval_params(rfba3b_batch *b, int item0, int nitem) {
+    EvalParams E;
+    memset(&E, 0, sizeof E);
+    E.blocks = b->d_blocks;
+    E.items = b->d_items + item0;
+    E.nitems = nitem;
+    E.counter = b->d_counter;
+    E.R = b->d_R;
+    E.model = b->d_model;
+    E.rmode = 0;
+    E.ckpt = b->ckpt;
+    E.Tmax = b->Tmax;
+    E.XYmax = b->XYmax;
+    E.Zmax = b->Zmax;
+    return E;
+}
+
+static int enqueue_range(rfba3b_batch *b, hipStream_t st, int task0, int ntask, int item0, int nitem, int grid_cap) {
+    if (ntask > 0) {
+        const int gr = std::min(ntask, g_ncu * 8);
+        hipLaunchKernelGGL(b3_resid, dim3(gr), dim3(256), b->lds_resid, st, (const DevBlock *)b->d_blocks,
+                           (const PlaneTask *)(b->d_tasks + task0), ntask, b->d_R, (const DevModel *)b->d_model);
+        HIP_TRY(hipGetLastError());
+    }
+    if (nitem > 0) {
+        hipLaunchKernelGGL(b3_reset, dim3(1), dim3(64), 0, st, b->d_counter);
+        HIP_TRY(hipGetLastError());
+        EvalParams E = eval_params(b, item0, nitem);
+        hipLaunchKernelGGL(eval_fn(b->mode), dim3(std::min(nitem, grid_cap)), dim3(256), b->lds_eval, st, E);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+static int enqueue_all(rfba3b_batch *b, hipStream_t st) {
+    int rc;
+    for (auto &c : b->chunks)
+        if ((rc = enqueue_range(b, st, c.task0, c.ntask, c.item0, c.nitem, b->grid_eval))) return rc;
+    return 0;
+}
+
+extern "C" int rfba3b_batch_run(rfba3b_batch *b, void *stream, int use_graph, float *kernel_ms) {
+    if (!b) return fail(-1, "plan is NULL");
+    std::lock_guard<std::mutex> lk(g_mu);
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    if (use_graph) {
+        if (!b->exec) {
+            hipStream_t cs;
+            HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+            HIP_TRY(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            rc = enqueue_all(b, cs);
+            hipGraph_t gr = nullptr;
+            hipError_t e = hipStreamEndCapture(cs, &gr);
+            hipStreamDestroy(cs);
+            if (rc) return rc;
+            if (e != hipSuccess) return fail(-3, "graph capture: %s", hipGetErrorString(e));
+            b->graph = gr;
+            HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        }
+        HIP_TRY(hipEventRecord(b->ev0, st));
+        HIP_TRY(hipGraphLaunch(b->exec, st));
+        HIP_TRY(hipEventRecord(b->ev1, st));
+    } else {
+        HIP_TRY(hipEventRecord(b->ev0, st));
+        if ((rc = enqueue_all(b, st))) return rc;
+        HIP_TRY(hipEventRecord(b->ev1, st));
+    }
+    if (kernel_ms) {
+        HIP_TRY(hipEventSynchronize(b->ev1));
+        HIP_TRY(hipEventElapsedTime(kernel_ms, b->ev0, b->ev1));
+    }
+    return 0;
+}
+
+extern "C" int rfba3b_batch_run_per_block(rfba3b_batch *b, void *stream, float *kernel_ms) {
+    if (!b) return fail(-1, "plan is NULL");
+    std::lock_guard<std::mutex> lk(g_mu);
+    hipStream_t st = (hipStream_t)stream;
+    int rc;
+    HIP_TRY(hipEventRecord(b->ev0, st));
+    for (auto &c : b->chunks)
+        for (size_t i = 0; i < c.bq_task0.size(); i++)
+            if ((rc = enqueue_range(b, st, c.bq_task0[i], c.bq_ntask[i], c.bq_item0[i], c.bq_nitem[i],
+                                    c.bq_nitem[i])))
+                return rc;
+    HIP_TRY(hipEventRecord(b->ev1, st));
+    if (kernel_ms) {
+        HIP_TRY(hipEventSynchronize(b->ev1));
+        HIP_TRY(hipEventElapsedTime(kernel_ms, b->ev0, b->ev1));
+    }
+    return 0;
+}
+
+extern "C" void rfba3b_batch_destroy(rfba3b_batch *b) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    batch_free(b);
+}
+
+// ---------------------------------------------------------------------------
+// functor and forward messages over explicit backgrounds
+
+namespace {
+
+inline double ln0(double x) { return x == 0 ? -1e100 : std::log(x); }
+inline double dlln(double x, double mu, double sigma) { return -(1 + (ln0(x) - mu) / (sigma * sigma)) / x; }
+
+// Explicit-background evaluation: candidates phi[n][5] of one spot over S
+// backgrounds.  Returns logL [n][S] and grad [n][S][5] (mode GRAD) or the
+// forward messages [n*S][T][3] (mode DELTA).
+int eval_explicit(const int *xyz, int P, int T, const float *data, const float *bg, int S, const double *phi, int n,
+                  const rfba3b_model *m, int mode, std::vector<double> &logL, std::vector<double> &grad,
+                  std::vector<double> *delta) {
+    int rc;
+    if ((rc = check_model(m)) || (rc = dev_init())) return rc;
+    if (!xyz || !data || !bg || !phi || P <= 0 || T <= 0 || S <= 0 || n <= 0) return fail(-1, "invalid arguments");
+    int lo[3] = {xyz[0], xyz[1], xyz[2]}, hi[3] = {xyz[0], xyz[1], xyz[2]};
+    for (int p = 0; p < P; p++)
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], xyz[3 * p + a]);
+            hi[a] = std::max(hi[a], xyz[3 * p + a]);
+        }
+    const int X = hi[0] - lo[0] + 1, Y = hi[1] - lo[1] + 1, Z = hi[2] - lo[2] + 1;
+    if ((long long)X * Y > 4096 || Z > 1024 || T > 4096) return fail(-1, "pixel region too large");
+    const size_t Pd = (size_t)X * Y * Z;
+    std::vector<uint8_t> mask(Pd, 0);
+    std::vector<int> map(P);
+    for (int p = 0; p < P; p++) {
+        const size_t d = ((size_t)(xyz[3 * p + 2] - lo[2]) * Y + (xyz[3 * p + 1] - lo[1])) * X + (xyz[3 * p] - lo[0]);
+        if (mask[d]) return fail(-1, "duplicate pixel in the region");
+        mask[d] = 1;
+        map[p] = (int)d;
+    }
+    std::vector<float> R((size_t)S * T * Pd, 0.0f);
+    for (int s = 0; s < S; s++)
+        for (int t = 0; t < T; t++)
+            for (int p = 0; p < P; p++)
+                R[((size_t)s * T + t) * Pd + map[p]] = data[(size_t)t * P + p] - bg[((size_t)s * T + t) * P + p];
+    DevBlock d;
+    memset(&d, 0, sizeof d);
+    d.X = X; d.Y = Y; d.Z = Z; d.T = T; d.K = n; d.Q = 1; d.S = S;
+    d.ox = lo[0]; d.oy = lo[1]; d.oz = lo[2];
+    d.npix = P;
+    std::vector<Item> items;
+    for (int k = 0; k < n; k++)
+        for (int s = 0; s < S; s++) items.push_back({0, 0, k, s, (long long)s * T * (long long)Pd});
+    const int ckmax = ckpt_of(m, T);
+    const size_t lds = eval_lds(T, X * Y, Z, ckmax);
+    EvalFn fn = eval_fn(mode);
+    int grid = 0;
+    if ((rc = eval_grid(fn, lds, (int)items.size(), grid))) return rc;
+
+    float *dR = nullptr;
+    uint8_t *dmask = nullptr;
+    double *dphi = nullptr, *dL = nullptr, *dG = nullptr, *dD = nullptr;
+    DevModel *dm = nullptr;
+    DevBlock *db = nullptr;
+    Item *di = nullptr;
+    int *dc = nullptr;
+    const DevModel hm = dev_model(m);
+    auto cleanup = [&] {
+        hipFree(dR); hipFree(dmask); hipFree(dphi); hipFree(dL); hipFree(dG); hipFree(dD);
+        hipFree(dm); hipFree(db); hipFree(di); hipFree(dc);
+    };
+    hipError_t e = hipSuccess;
+    const size_t nitem = items.size();
+    if (e == hipSuccess) e = hipMalloc(&dR, R.size() * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(&dmask, Pd);
+    if (e == hipSuccess) e = hipMalloc(&dphi, (size_t)n * 5 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&dL, nitem * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&dG, nitem * 5 * sizeof(double));
+    if (e == hipSuccess && mode == MODE_DELTA) e = hipMalloc(&dD, nitem * 3 * (size_t)T * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&dm, sizeof(DevModel));
+    if (e == hipSuccess) e = hipMalloc(&db, sizeof(DevBlock));
+    if (e == hipSuccess) e = hipMalloc(&di, nitem * sizeof(Item));
+    if (e == hipSuccess) e = hipMalloc(&dc, sizeof(int));
+    d.params = dphi;
+    d.mask = dmask;
+    d.logL = dL;
+    d.grad = dG;
+    if (e == hipSuccess) e = hipMemcpy(dR, R.data(), R.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dmask, mask.data(), Pd, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dphi, phi, (size_t)n * 5 * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dm, &hm, sizeof hm, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(db, &d, sizeof d, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(di, items.data(), nitem * sizeof(Item), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(e == hipErrorOutOfMemory ? -2 : -3, "3B setup: %s", hipGetErrorString(e));
+    }
+    EvalParams E;
+    memset(&E, 0, sizeof E);
+    E.blocks = db; E.items = di; E.nitems = (int)nitem; E.counter = dc; E.R = dR; E.model = dm;
+    E.rmode = 1; E.ckpt = m->checkpoint; E.Tmax = T; E.XYmax = X * Y; E.Zmax = Z; E.delta_out = dD;
+    hipLaunchKernelGGL(b3_reset, dim3(1), dim3(64), 0, nullptr, dc);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(256), lds, nullptr, E);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    logL.resize(nitem);
+    grad.resize(nitem * 5);
+    if (e == hipSuccess) e = hipMemcpy(logL.data(), dL, nitem * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && mode == MODE_GRAD)
+        e = hipMemcpy(grad.data(), dG, nitem * 5 * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && mode == MODE_DELTA) {
+        delta->resize(nitem * 3 * (size_t)T);
+        e = hipMemcpy(delta->data(), dD, delta->size() * sizeof(double), hipMemcpyDeviceToHost);
+    }
+    cleanup();
+    if (e != hipSuccess) return fail(-3, "3B evaluation: %s", hipGetErrorString(e));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int rfba3b_neg_grad_sampled(const int *xyz, int P, int T, const float *data, const float *bg, int S,
+                                       const double *phi, int n, const rfba3b_model *model, double *grad,
+                                       double *logL) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!grad || !phi || n <= 0) return fail(-1, "invalid arguments");
+    // multispot5.cc:618-619: invalid parameters give a NaN vector.
+    std::vector<double> good;
+    std::vector<int> idx;
+    for (int i = 0; i < n; i++)
+        if (phi[5 * i] > 0 && phi[5 * i + 1] > 0) {
+            idx.push_back(i);
+            good.insert(good.end(), phi + 5 * i, phi + 5 * i + 5);
+        }
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    for (int i = 0; i < n; i++) {
+        for (int c = 0; c < 5; c++) grad[5 * i + c] = nan;
+        if (logL)
+            for (int s = 0; s < S; s++) logL[(size_t)i * S + s] = nan;
+    }
+    if (idx.empty()) return 0;
+    std::vector<double> L, G;
+    int rc = eval_explicit(xyz, P, T, data, bg, S, good.data(), (int)idx.size(), model, MODE_GRAD, L, G, nullptr);
+    if (rc) return rc;
+    for (size_t c = 0; c < idx.size(); c++) {
+        const int i = idx[c];
+        double sum[5] = {0, 0, 0, 0, 0};
+        for (int s = 0; s < S; s++) {
+            for (int k = 0; k < 5; k++) sum[k] += G[(c * S + s) * 5 + k];
+            if (logL) logL[(size_t)i * S + s] = L[c * S + s];
+        }
+        const double *p = phi + 5 * i;
+        const double prior[5] = {dlln(p[0], model->mu_brightness, model->sigma_brightness),
+                                 dlln(p[1], model->mu_blur, model->sigma_blur), 0, 0, 0};
+        for (int k = 0; k < 5; k++) grad[5 * i + k] = -(sum[k] / S + prior[k]);
+    }
+    return 0;
+}
+
+extern "C" int rfba3b_forward_messages(const int *xyz, int P, int T, const float *data, const float *bg,
+                                       const double *phi, const rfba3b_model *model, double *delta, double *logL) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!delta || !phi) return fail(-1, "invalid arguments");
+    if (!(phi[0] > 0 && phi[1] > 0)) return fail(-1, "invalid spot parameters");
+    std::vector<double> L, G, D;
+    int rc = eval_explicit(xyz, P, T, data, bg, 1, phi, 1, model, MODE_DELTA, L, G, &D);
+    if (rc) return rc;
+    memcpy(delta, D.data(), 3 * (size_t)T * sizeof(double));
+    if (logL) *logL = L[0];
+    return 0;
+}

@@ -192,6 +192,55 @@ int rfba_synth_lambda_f64(int W, int H, int D, int T, int n, uint64_t seed, doub
     return 0;
 }
 
+// One 3B block: K spots with the reference spot shape (storm.h:229-235)
+// I/(sigma sqrt(2 pi)) exp(-(dx^2 + dy^2 + (dz/zF)^2)/(2 sigma^2)) at integer
+// voxel coordinates, states sampled from the 3-state HMM (A row-major,
+// 0 = on), data = b0 + sum of on spots + N(0, var).  data [T][Z][Y][X],
+// params [K][5] = (I, sigma, x, y, z), states [K][T].
+int rfba_synth_3b_block(int X, int Y, int Z, int T, int K, uint64_t seed, double I, double sigma, double zF,
+                        double b0, double var, const double *A, const double *pi, float *data, double *params,
+                        uint8_t *states) {
+    if (X <= 0 || Y <= 0 || Z <= 0 || T <= 0 || K < 0) return -1;
+    Rng r(seed);
+    for (int k = 0; k < K; k++) {
+        params[5 * k] = I;
+        params[5 * k + 1] = sigma;
+        params[5 * k + 2] = r.uniform() * (X - 1);
+        params[5 * k + 3] = r.uniform() * (Y - 1);
+        params[5 * k + 4] = r.uniform() * (Z - 1);
+    }
+    for (int k = 0; k < K; k++) {
+        double u = r.uniform();
+        int st = u < pi[0] ? 0 : (u < pi[0] + pi[1] ? 1 : 2);
+        states[(size_t)k * T] = (uint8_t)st;
+        for (int t = 1; t < T; t++) {
+            u = r.uniform();
+            const double *row = A + 3 * st;
+            st = u < row[0] ? 0 : (u < row[0] + row[1] ? 1 : 2);
+            states[(size_t)k * T + t] = (uint8_t)st;
+        }
+    }
+    const size_t P = (size_t)X * Y * Z;
+    std::vector<double> mu(P);
+    std::vector<double> img(P);
+    const double r2pi = std::sqrt(2 * M_PI);
+    for (int t = 0; t < T; t++) {
+        std::fill(img.begin(), img.end(), b0);
+        for (int k = 0; k < K; k++) {
+            if (states[(size_t)k * T + t] != 0) continue;
+            const double *ph = params + 5 * k;
+            for (size_t p = 0; p < P; p++) {
+                const int x = (int)(p % X), y = (int)((p / X) % Y), z = (int)(p / ((size_t)X * Y));
+                const double dx = x - ph[2], dy = y - ph[3], dz = (z - ph[4]) / zF;
+                img[p] += std::exp(-(dx * dx + dy * dy + dz * dz) / (2 * ph[1] * ph[1])) * ph[0] / (ph[1] * r2pi);
+            }
+        }
+        const double sd = std::sqrt(var);
+        for (size_t p = 0; p < P; p++) data[(size_t)t * P + p] = (float)(img[p] + sd * r.normal());
+    }
+    return 0;
+}
+
 // Poisson draws for a given mean (sampler self-test).
 int rfba_synth_poisson(double lam, uint64_t seed, int n, long *out) {
     Rng r(seed);

@@ -1,0 +1,169 @@
+"""GPU parity of the 3B likelihood kernels (librfba_3b.so) against the FP64
+oracle (oracle/b3_oracle.cpp).
+
+Bars (FP32 pixel terms, FP64 sums and recursions):
+  logL: |gpu - oracle| <= 1e-6 * |oracle| (the FP32 spot images carry ~3e-7
+        relative rounding, which moves logL by ~1e-7 relative; measured
+        values are printed)
+  gradient: ||gpu - oracle|| <= 1e-5 * ||oracle|| per item, and
+            ||gpu - central FD (h = 1e-3) of the FP64 oracle|| <= 1e-4 * ||FD||.
+Errors are printed per case.
+"""
+import numpy as np
+import pytest
+
+from oracle import b3oracle as bo
+from rfba_amd import b3
+
+pytestmark = pytest.mark.gpu
+
+LOGL_RTOL = 1e-6
+GRAD_RTOL = 1e-5
+
+
+def model(**kw):
+    base = dict(A=b3.blink_matrix(), pi=(0.5, 0.5, 0.0), variance=100.0, zfactor=2.0, z_exact=0, checkpoint=0,
+                mu_brightness=7.0, sigma_brightness=1.0, mu_blur=0.25, sigma_blur=0.1)
+    base.update(kw)
+    return b3.make_model(**base)
+
+
+def grid(X, Y, Z):
+    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    return np.stack([x.ravel(), y.ravel(), z.ravel()], 1).astype(np.int32)
+
+
+def compare(L, G, Lo, Go, label):
+    el = np.abs(L - Lo) / np.abs(Lo)
+    ng = np.linalg.norm(Go, axis=-1)
+    eg = np.linalg.norm(G - Go, axis=-1) / np.maximum(ng, 1e-300)
+    print("%s: logL max rel err %.3g, grad max rel err %.3g (items %d)" % (label, el.max(), eg.max(), el.size))
+    assert el.max() <= LOGL_RTOL
+    assert eg.max() <= GRAD_RTOL
+
+
+def run_batch(blocks, m, graph=False):
+    bt = b3.Batch(blocks, m)
+    bt.run(use_graph=graph)
+    res = [bt.results(i) for i in range(len(blocks))]
+    bt.close()
+    return res
+
+
+@pytest.mark.parametrize("ckpt", [0, 1, 3, 11])
+def test_batched_vs_oracle(gpu, ckpt):
+    m = model(checkpoint=ckpt)
+    data, params, states = b3.synth_block((12, 11, 10), 23, 5, 21)
+    P = b3.perturb(params, 3, 22)
+    st = np.stack([states, np.roll(states, 3, axis=1)])
+    (L, G), = run_batch([b3.BlockData(data, P, st, 100.0)], m)
+    Lo, Go = bo.model_eval(data, 100.0, m, P, st)
+    compare(L, G, Lo, Go, "ckpt=%d" % ckpt)
+
+
+def test_checkpoint_stride_does_not_change_results(gpu):
+    data, params, states = b3.synth_block((10, 9, 8), 40, 4, 23)
+    P = b3.perturb(params, 2, 24)
+    outs = [run_batch([b3.BlockData(data, P, states[None], 100.0)], model(checkpoint=c))[0] for c in (1, 5, 7, 40)]
+    for L, G in outs[1:]:
+        np.testing.assert_allclose(L, outs[0][0], rtol=1e-13)
+        np.testing.assert_allclose(G, outs[0][1], rtol=1e-9, atol=1e-12 * np.abs(outs[0][1]).max())
+
+
+def test_uneven_blocks_plain_graph_and_per_block_agree(gpu):
+    m = model(checkpoint=11)
+    rng = np.random.default_rng(7)
+    blocks = []
+    for i in range(5):
+        dims = tuple(int(v) for v in rng.integers(8, 15, 3))
+        T = int(rng.integers(12, 30))
+        K = int(rng.integers(2, 6))
+        data, params, states = b3.synth_block(dims, T, K, 100 + i)
+        blocks.append(b3.BlockData(data, b3.perturb(params, 2, 200 + i), states[None], 100.0))
+    bt = b3.Batch(blocks, m)
+    res = {}
+    for mode in ("plain", "graph", "per_block", "graph2"):
+        if mode == "per_block":
+            bt.run_per_block()
+        else:
+            bt.run(use_graph=mode.startswith("graph"))
+        res[mode] = [bt.results(i) for i in range(len(blocks))]
+    bt.close()
+    for mode in ("graph", "per_block", "graph2"):
+        for (L, G), (L0, G0) in zip(res[mode], res["plain"]):
+            assert np.array_equal(L, L0) and np.array_equal(G, G0), mode
+    for b, (L, G) in zip(blocks, res["plain"]):
+        Lo, Go = bo.model_eval(b.data, 100.0, m, b.params, b.states)
+        compare(L, G, Lo, Go, "uneven block %s" % (b.data.shape,))
+
+
+def test_gradient_vs_central_fd(gpu):
+    # exact z derivative so that finite differences apply with zF = 2
+    m = model(z_exact=1, checkpoint=4)
+    data, params, states = b3.synth_block((11, 10, 9), 16, 3, 31)
+    P = b3.perturb(params, 1, 32)
+    (L, G), = run_batch([b3.BlockData(data, P, states[None], 100.0)], m)
+    xyz = grid(11, 10, 9)
+    d2 = data.reshape(16, -1).astype(np.float64)
+    for k in range(3):
+        bg = np.full_like(d2, 100.0)
+        for j in range(3):
+            if j != k:
+                bg += (states[j] == 0)[:, None] * bo.spot(xyz, P[0, j], 2.0)[0][None]
+        phi = P[0, k]
+        fd = np.empty(5)
+        for n in range(5):
+            h = 1e-3 * (abs(phi[n]) if n < 2 else 1.0)
+            e = np.zeros(5)
+            e[n] = h
+            fd[n] = (bo.logl_grad(xyz, d2, bg, phi + e, m)[0] - bo.logl_grad(xyz, d2, bg, phi - e, m)[0]) / (2 * h)
+        err = np.linalg.norm(G[0, k, 0] - fd) / np.linalg.norm(fd)
+        print("spot %d: |grad - FD| / |FD| = %.3g" % (k, err))
+        assert err <= 1e-4
+
+
+def test_functor_and_forward_messages(gpu):
+    m = model()
+    data, params, states = b3.synth_block((9, 8, 7), 14, 3, 41)
+    xyz_all = grid(9, 8, 7)
+    # an irregular region: a ball of pixels around spot 0 (masked list)
+    c = params[0, 2:5]
+    keep = np.linalg.norm((xyz_all - c) * [1, 1, 0.5], axis=1) < 4.5
+    xyz = xyz_all[keep]
+    d2 = data.reshape(14, -1)[:, keep]
+    bgs = []
+    for s in range(3):
+        bg = np.full(d2.shape, 100.0)
+        for j in (1, 2):
+            bg += (np.roll(states[j], s) == 0)[:, None] * bo.spot(xyz, params[j], 2.0)[0][None]
+        bgs.append(bg)
+    bgs = np.stack(bgs)
+    phis = np.stack([params[0], params[0] + [50, 0.1, 0.3, -0.2, 0.4], [0.0, 1.3, 3, 3, 3], [900, -1, 3, 3, 3]])
+    g, Ls = b3.neg_grad_sampled(xyz, d2.astype(np.float32), bgs.astype(np.float32), phis, m)
+    assert np.isnan(g[2:]).all()
+    for i in range(2):
+        go, Lo = bo.neg_grad(xyz, d2.astype(np.float32).astype(np.float64), bgs.astype(np.float32).astype(np.float64),
+                             phis[i], m)
+        eg = np.linalg.norm(g[i] - go) / np.linalg.norm(go)
+        el = np.abs(Ls[i] - Lo) / np.abs(Lo)
+        print("functor %d: grad rel err %.3g, logL rel err %.3g" % (i, eg, el.max()))
+        assert eg <= GRAD_RTOL and el.max() <= LOGL_RTOL
+    delta, L = b3.forward_messages(xyz, d2.astype(np.float32), bgs[0].astype(np.float32), params[0], m)
+    mu, dmu = bo.spot(xyz, params[0], 2.0)
+    on, off, _ = bo.emissions(d2.astype(np.float32).astype(np.float64), bgs[0].astype(np.float32).astype(np.float64),
+                              mu, dmu, 100.0)
+    do, Lo = bo.forward_delta(b3.blink_matrix(), (0.5, 0.5, 0.0), on, off)
+    fin = do > -1e99
+    assert np.array_equal(fin, delta > -1e99)
+    np.testing.assert_allclose(delta[fin], do[fin], rtol=LOGL_RTOL)
+    assert L == pytest.approx(Lo, rel=LOGL_RTOL)
+
+
+def test_config4_shape_subset(gpu):
+    # 29^3 block, T = 100, 40 spots; two of the perturbed parameter sets
+    m = model(checkpoint=0)
+    data, params, states = b3.synth_block((29, 29, 29), 100, 40, 6)
+    P = b3.perturb(params, 2, 7)
+    (L, G), = run_batch([b3.BlockData(data, P, states[None], 100.0)], m, graph=True)
+    Lo, Go = bo.model_eval(data, 100.0, m, P, states[None])
+    compare(L, G, Lo, Go, "29^3 T=100 K=40")

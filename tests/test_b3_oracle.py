@@ -1,0 +1,226 @@
+"""CPU tests that pin the FP64 3B oracle (oracle/b3_oracle.cpp).
+
+The reference's own 3B headers cannot be built here (they need MSVC's
+std::tr1 aliases), and the reference ships no tests or vectors for this
+path, so each function is pinned by mathematics a transcription slip would
+break: the spot's value and derivatives against central finite differences
+(including the reference's z-derivative without 1/zF^2), the per-frame log
+probabilities against the textbook Gaussian log density, the forward
+recursion against brute-force enumeration of all 3^T state paths and the
+T = 1 closed form, its derivative against finite differences, and the
+functor's composition and NaN guard.
+"""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+from oracle import b3oracle as bo
+from rfba_amd import b3
+
+
+def model(**kw):
+    base = dict(A=b3.blink_matrix(), pi=(0.5, 0.5, 0.0), variance=100.0, zfactor=1.0, z_exact=0, checkpoint=0,
+                mu_brightness=7.0, sigma_brightness=1.0, mu_blur=0.25, sigma_blur=0.1)
+    base.update(kw)
+    return b3.make_model(**base)
+
+
+def grid(X, Y, Z):
+    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    return np.stack([x.ravel(), y.ravel(), z.ravel()], 1).astype(np.int32)
+
+
+# ------------------------------------------------------------------ spot shape
+def test_spot_value_closed_form():
+    xyz = grid(7, 6, 5)
+    phi = np.array([1300.0, 1.3, 3.0, 2.0, 2.0])
+    mu, _ = bo.spot(xyz, phi, zfactor=2.0)
+    centre = np.where((xyz == [3, 2, 2]).all(1))[0][0]
+    assert mu[centre] == pytest.approx(1300 / (1.3 * math.sqrt(2 * math.pi)), rel=1e-15)
+    d = xyz - phi[2:]
+    expect = mu[centre] * np.exp(-(d[:, 0] ** 2 + d[:, 1] ** 2 + (d[:, 2] / 2.0) ** 2) / (2 * 1.3 ** 2))
+    np.testing.assert_allclose(mu, expect, rtol=1e-14)
+
+
+@pytest.mark.parametrize("zf,z_exact", [(1.0, 0), (2.0, 1), (2.0, 0)])
+def test_spot_derivatives_vs_fd(zf, z_exact):
+    xyz = grid(6, 5, 7)
+    phi = np.array([900.0, 1.4, 2.3, 2.7, 3.1])
+    mu, dmu = bo.spot(xyz, phi, zf, z_exact)
+    h = 1e-6
+    for n in range(5):
+        e = np.zeros(5)
+        e[n] = h * max(1.0, abs(phi[n]))
+        fd = (bo.spot(xyz, phi + e, zf, z_exact)[0] - bo.spot(xyz, phi - e, zf, z_exact)[0]) / (2 * e[n])
+        if n == 4 and zf != 1.0 and not z_exact:
+            # the reference's d/dz omits 1/zF^2 (storm.h:78)
+            np.testing.assert_allclose(dmu[:, n], fd * zf * zf, rtol=1e-6, atol=1e-9 * np.abs(fd).max())
+        else:
+            np.testing.assert_allclose(dmu[:, n], fd, rtol=1e-6, atol=1e-9 * np.abs(fd).max())
+
+
+# ------------------------------------------------------------------ emissions
+def test_emissions_are_gaussian_log_densities():
+    rng = np.random.default_rng(0)
+    T, P, var = 4, 30, 100.0
+    px = rng.normal(120, 10, (T, P))
+    bg = rng.normal(100, 3, (T, P))
+    mu = rng.uniform(0, 50, P)
+    dmu = rng.normal(size=(P, 5))
+    on, off, diff = bo.emissions(px, bg, mu, dmu, var)
+
+    def logpdf(x, m):
+        return -0.5 * np.log(2 * np.pi * var) - (x - m) ** 2 / (2 * var)
+
+    np.testing.assert_allclose(on, logpdf(px, bg + mu).sum(1), rtol=1e-13)
+    np.testing.assert_allclose(off, logpdf(px, bg).sum(1), rtol=1e-13)
+    np.testing.assert_allclose(diff, ((px - bg - mu)[:, :, None] * dmu[None]).sum(1) / var, rtol=1e-12)
+
+
+# ------------------------------------------------------------------ forward algorithm
+def brute_force(A, pi, on, off):
+    T = len(on)
+    A = np.asarray(A)
+    tot = 0.0
+    mx = -np.inf
+    terms = []
+    for path in itertools.product(range(3), repeat=T):
+        lp = np.log(pi[path[0]]) if pi[path[0]] > 0 else -np.inf
+        for t in range(1, T):
+            a = A[path[t - 1], path[t]]
+            lp += np.log(a) if a > 0 else -np.inf
+        lp += sum(on[t] if s == 0 else off[t] for t, s in enumerate(path))
+        terms.append(lp)
+    terms = np.array(terms)
+    m = terms.max()
+    return m + np.log(np.exp(terms - m).sum())
+
+
+@pytest.mark.parametrize("T", [1, 2, 5, 7])
+def test_forward_vs_brute_force(T):
+    rng = np.random.default_rng(T)
+    A = b3.blink_matrix()
+    pi = np.array([0.5, 0.5, 0.0])
+    on, off = rng.normal(-50, 5, T), rng.normal(-50, 5, T)
+    L, _ = bo.forward_deriv(A, pi, on, off, np.zeros((T, 5)))
+    assert L == pytest.approx(brute_force(A, pi, on, off), rel=1e-13, abs=1e-12)
+    if T == 1:
+        lse = np.logaddexp(np.log(0.5) + on[0], np.log(0.5) + off[0])
+        assert L == pytest.approx(lse, rel=1e-15)
+    delta, L2 = bo.forward_delta(A, pi, on, off)
+    assert L2 == pytest.approx(L, rel=1e-14)
+    # delta_t(j) = log of the summed probability of all prefixes ending in j
+    for t in range(T):
+        for j in range(3):
+            ref = -np.inf
+            for path in itertools.product(range(3), repeat=t + 1):
+                if path[-1] != j or pi[path[0]] == 0:
+                    continue
+                lp = np.log(pi[path[0]])
+                ok = True
+                for u in range(1, t + 1):
+                    a = A[path[u - 1], path[u]]
+                    if a == 0:
+                        ok = False
+                        break
+                    lp += np.log(a)
+                if ok:
+                    lp += sum(on[u] if s == 0 else off[u] for u, s in enumerate(path))
+                    ref = np.logaddexp(ref, lp)
+            if ref == -np.inf:
+                assert delta[t, j] < -1e99  # ln(0) = -1e100 sentinel
+            else:
+                assert delta[t, j] == pytest.approx(ref, rel=1e-12, abs=1e-10)
+
+
+def test_forward_derivative_vs_fd():
+    rng = np.random.default_rng(3)
+    T = 9
+    A, pi = b3.blink_matrix(), np.array([0.5, 0.5, 0.0])
+    on, off = rng.normal(-50, 3, T), rng.normal(-50, 3, T)
+    diff = rng.normal(size=(T, 5))
+    _, g = bo.forward_deriv(A, pi, on, off, diff)
+    h = 1e-4  # |logL| ~ 1e3: rounding noise eps*|L|/h stays ~1e-9
+    for n in range(5):
+        Lp = bo.forward_deriv(A, pi, on + h * diff[:, n], off, diff)[0]
+        Lm = bo.forward_deriv(A, pi, on - h * diff[:, n], off, diff)[0]
+        assert g[n] == pytest.approx((Lp - Lm) / (2 * h), rel=1e-6, abs=1e-8)
+
+
+# ------------------------------------------------------------------ end to end
+def small_case(seed=1, zf=2.0, T=10):
+    data, params, states = b3.synth_block((8, 7, 6), T, 3, seed, zfactor=zf)
+    xyz = grid(8, 7, 6)
+    data2 = data.reshape(T, -1).astype(np.float64)
+    # background for spot 0: b0 plus the other spots when on
+    bg = np.full_like(data2, 100.0)
+    for j in (1, 2):
+        mu, _ = bo.spot(xyz, params[j], zf)
+        bg += (states[j] == 0)[:, None] * mu[None]
+    return xyz, data2, bg, params, states
+
+
+@pytest.mark.parametrize("zf", [1.0, 2.0])
+def test_logl_gradient_vs_central_fd(zf):
+    xyz, data, bg, params, _ = small_case(zf=zf)
+    m = model(zfactor=zf, z_exact=1)
+    phi = params[0] * [1.1, 1.0, 1, 1, 1] + [0, 0.05, 0.2, -0.3, 0.25]
+    L, g = bo.logl_grad(xyz, data, bg, phi, m)
+    fd = np.empty(5)
+    for n in range(5):
+        h = 1e-3 * (abs(phi[n]) if n < 2 else 1.0)
+        e = np.zeros(5)
+        e[n] = h
+        fd[n] = (bo.logl_grad(xyz, data, bg, phi + e, m)[0] - bo.logl_grad(xyz, data, bg, phi - e, m)[0]) / (2 * h)
+    assert np.linalg.norm(g - fd) / np.linalg.norm(fd) < 1e-4
+    if zf != 1.0:
+        # reference derivative: the z component is off by zF^2
+        _, g_ref = bo.logl_grad(xyz, data, bg, phi, model(zfactor=zf, z_exact=0))
+        np.testing.assert_allclose(g_ref[:4], g[:4], rtol=1e-12)
+        assert g_ref[4] != pytest.approx(g[4], rel=1e-3)
+
+
+def test_neg_grad_composition_and_nan_guard():
+    xyz, data, bg, params, _ = small_case()
+    m = model(zfactor=2.0)
+    bgs = np.stack([bg, bg + 1.0])
+    phi = params[0]
+    g, Ls = bo.neg_grad(xyz, data, bgs, phi, m)
+    g0 = bo.logl_grad(xyz, data, bgs[0], phi, m)
+    g1 = bo.logl_grad(xyz, data, bgs[1], phi, m)
+    assert Ls[0] == g0[0] and Ls[1] == g1[0]
+
+    def dlln(x, mu, s):
+        # derivative of -(ln x - mu)^2/(2 s^2) - ln x - ln(s sqrt(2 pi)) by finite differences
+        f = lambda v: -(np.log(v) - mu) ** 2 / (2 * s * s) - np.log(v) - np.log(s * np.sqrt(2 * np.pi))
+        h = 1e-6 * x
+        return (f(x + h) - f(x - h)) / (2 * h)
+
+    prior = np.array([dlln(phi[0], 7.0, 1.0), dlln(phi[1], 0.25, 0.1), 0, 0, 0])
+    np.testing.assert_allclose(g, -((g0[1] + g1[1]) / 2 + prior), rtol=1e-7)
+    for bad in ([0.0, 1.3, 3, 3, 3], [100.0, -1.0, 3, 3, 3]):
+        gb, _ = bo.neg_grad(xyz, data, bgs, np.array(bad), m)
+        assert np.isnan(gb).all()
+
+
+def test_model_eval_matches_composition():
+    data, params, states = b3.synth_block((6, 5, 4), 7, 3, 11, zfactor=2.0)
+    m = model(zfactor=2.0)
+    P = b3.perturb(params, 2, 12)
+    st = np.stack([states, np.roll(states, 1, axis=1)])
+    L, G = bo.model_eval(data, 100.0, m, P, st, threads=2)
+    xyz = grid(6, 5, 4)
+    d2 = data.reshape(7, -1).astype(np.float64)
+    for q in range(2):
+        mus = [bo.spot(xyz, P[q, j], 2.0)[0] for j in range(3)]
+        for k in range(3):
+            for s in range(2):
+                bg = np.full_like(d2, 100.0)
+                for j in range(3):
+                    if j != k:
+                        bg += (st[s, j] == 0)[:, None] * mus[j][None]
+                Lr, Gr = bo.logl_grad(xyz, d2, bg, P[q, k], m)
+                assert L[q, k, s] == pytest.approx(Lr, rel=1e-13)
+                np.testing.assert_allclose(G[q, k, s], Gr, rtol=1e-10, atol=1e-12)
