@@ -1,23 +1,38 @@
-"""Benchmark: 3D SRRF radiality on one MI355X (full-volume workload).
+"""Benchmark: RFBA's 3D SRRF radiality stage and 3B likelihood core on one MI355X.
 
-Workload: T=200 volumes of 256x256x96 uint16 (2000 emitters, seed 2, PSF
-sigma_xy 1.3 / sigma_z 2.6, 400 photons peak, background 100, Markov
-blinking 0.3/0.1, Poisson noise), magnification 4 -> 1024x1024x384 FP32 SR
-grid, ring radius 0.5 SR px, 26-direction sample sphere, TRAC order 2
-(ITC 0), gradient smoothing and intensity weighting as in the shipped
-SRRF.ini.  One step = one pass of the fused kernel over the whole series,
-input resident in HBM.
+Headline workload (value, roofline, cpu_baseline): plain 3D SRRF over one
+full-volume stack -- T=200 volumes of 256x256x96 uint16 (2000 emitters,
+seed 2, PSF sigma_xy 1.3 / sigma_z 2.6, 400 photons peak, background 100,
+Markov blinking 0.3/0.1, Poisson noise), magnification 4 -> 1024x1024x384
+FP32 SR grid, ring radius 0.5 SR px, 26-direction sample sphere, TRAC
+order 2 (ITC 0), gradient smoothing and intensity weighting on.  One step =
+one pass of the fused kernel over the whole series, input resident in HBM.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Beside it, in "workloads" of the same JSON line:
+  srrf_batched  512x512x128 T=120 series (8000 emitters, seed 3) split into
+                29^3 blocks with 3-voxel overlap (doSplit.m), plus 64 extra
+                stacks (edges 8-29, T 30-200, seed 4), magnification per block
+                from {2,3,4} (seed 5): one persistent launch in a hipGraph
+                with autotuned tile depth vs one launch per block;
+  b3_single     29^3 block, T=100, 40 spots (seed 6), 256 perturbed
+                parameter sets (seed 7): logL + adjoint for every spot,
+                batched vs one launch per parameter set;
+  b3_batched    1200 blocks of the same split (seed 8), K ~ Poisson(40)
+                clipped to 5-120, T=120, 32 parameter sets per block,
+                checkpoint stride 11: batched vs one launch per (block,
+                parameter set), the latter on the first 100 blocks.
 
-Prints one JSON line.  value = SR voxels / s; roofline = algorithmic HBM
-bytes per launch / measured kernel time against 8 TB/s (the metric asks
-for %HBM), with the FP32-VALU roof beside it; cpu_baseline = the
-multithreaded oracle on a bounded crop of the same series.
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only c2,c3,c4,c5]
+
+Prints one JSON line.  value = SR voxels / s of the headline workload;
+roofline = algorithmic HBM bytes per launch / measured kernel time against
+8 TB/s, with the FP32-VALU roof beside it; cpu_baseline = the multithreaded
+oracle on a bounded crop of the same series.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -43,6 +58,12 @@ def workload_conf(srrf):
         XRange=1, YRange=1, ZRange=1, RemoveCon=0, Renormalize=0, IntensityW=1)
 
 
+def hbm_roof(alg_bytes, kernel_ms):
+    gbs = alg_bytes / (kernel_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+
+
+# ---------------------------------------------------------------- headline: one full-volume stack
 def cpu_baseline(series, conf, smp, crop, threads):
     """Time the oracle on a crop [T, cz, cy, cx] of the same series."""
     from oracle import oracle
@@ -59,30 +80,8 @@ def cpu_baseline(series, conf, smp, crop, threads):
                       "(%d SR voxels, %.1f s); rate per SR voxel at full T" % (threads, cx, cy, cz, T, n_sr, dt)}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-crop", type=str, default="12,12,8")
-    ap.add_argument("--frames", type=int, default=200)
-    ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="only run the plain copy kernel over this many MiB (HBM counter calibration)")
-    args = ap.parse_args()
-    if args.copy_probe_mib:
-        from rfba_amd import srrf
-        nbytes = args.copy_probe_mib << 20
-        ms = srrf.copy_probe(nbytes)
-        print(json.dumps({"copy_probe_bytes": nbytes, "ms": ms, "GB/s": 2 * nbytes / ms / 1e6}), flush=True)
-        return
-    if args.gpus != 1:
-        print("bench.py measures one GPU only; multi-GPU runs are not part of this build "
-              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
-        sys.exit(2)
-
+def bench_stack(args, rec):
     from rfba_amd import srrf, synth
-
     X, Y, Z, T = 256, 256, 96, args.frames
     n_em, seed = 2000, 2
     t0 = time.perf_counter()
@@ -110,45 +109,337 @@ def main():
     wall = time.perf_counter() - t0
     ms_per_step = wall * 1e3 / args.steps
     kernel_ms = float(np.mean(kms))
-
     out = d_out.to_array((Z * M, Y * M, X * M), np.float32)
     nan_frac = float(np.isnan(out).mean())
+    d_in.close()
+    d_out.close()
+    srrf.lib().rfba_release_cache()
 
     alg_bytes = 2.0 * n_lr * T + 4.0 * n_sr
-    achieved = alg_bytes / (kernel_ms * 1e-3) / 1e9
     K = smp.n
     alg_flops = float(T) * n_sr * (K * 430 + 128 * conf.IntensityW + 6)
     tflops = alg_flops / (kernel_ms * 1e-3) / 1e12
-
-    rec = {
-        "metric": METRIC,
+    roof = hbm_roof(alg_bytes, kernel_ms)
+    roof.update({"traffic": args.traffic, "kernel": "srrf_fused (uint16 input)", "kernel_ms": kernel_ms,
+                 "algorithmic_bytes_per_launch": alg_bytes,
+                 "valu_fp32": {"achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                               "frac": tflops / FP32_PEAK_TFLOPS, "algorithmic_flops_per_launch": alg_flops}})
+    rec.update({
         "value": n_sr / (ms_per_step * 1e-3),
-        "unit": "SR-voxels/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
         "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded emitters, Markov blinking, Poisson noise; generated on the host)",
         "config": {"workload": "plain 3D SRRF, one stack: %dx%dx%d uint16, T=%d, %d emitters, M=%d -> %dx%dx%d SR, "
                                "R=0.5, 26 directions, TRAC order 2 (ITC 0), smoothing 1, IW 1"
                                % (X, Y, Z, T, n_em, M, X * M, Y * M, Z * M),
                    "sr_voxels": n_sr, "frames": T, "nan_fraction": nan_frac},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "srrf_fused<uint16,16>", "kernel_ms": kernel_ms,
-                     "algorithmic_bytes_per_launch": alg_bytes,
-                     "valu_fp32": {"achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                                   "frac": tflops / FP32_PEAK_TFLOPS, "algorithmic_flops_per_launch": alg_flops}},
-        "cpu_baseline": None,
-    }
+        "roofline": roof,
+    })
     if not args.no_cpu:
         crop = tuple(int(v) for v in args.cpu_crop.split(","))
-        threads = synth.default_threads()
-        rec["cpu_baseline"] = cpu_baseline(series, conf, smp, crop, threads)
+        rec["cpu_baseline"] = cpu_baseline(series, conf, smp, crop, synth.default_threads())
+
+
+# ---------------------------------------------------------------- batched SRRF over uneven blocks
+class _View:
+    """A pointer into a larger device buffer (what srrf.Block reads .ptr of)."""
+
+    def __init__(self, base, offset):
+        self.ptr = ctypes.c_void_p(base.ptr.value + offset)
+
+
+def bench_srrf_batched(args):
+    from rfba_amd import srrf, synth
+    from rfba_amd.blocks import volume_blocks
+    vol, T, n_em = (512, 512, 128), 120, 8000
+    t0 = time.perf_counter()
+    series = synth.srrf_series(vol + (T,), n_em, 3)
+    log("c3: generated %s series in %.1f s" % (series.shape, time.perf_counter() - t0))
+    split = volume_blocks(vol, 29, 3)
+    rng4 = np.random.default_rng(4)
+    extra_dims = rng4.integers(8, 30, (64, 3))
+    extra_T = rng4.integers(30, 201, 64)
+    mags = np.random.default_rng(5).integers(2, 5, len(split) + 64)
+    density = n_em / float(np.prod(vol))
+    extras = [synth.srrf_series((int(d[0]), int(d[1]), int(d[2]), int(t)),
+                                max(1, int(round(density * float(np.prod(d))))), 400 + i)
+              for i, (d, t) in enumerate(zip(extra_dims, extra_T))]
+    conf = srrf.default_conf().replace(RingRadius=0.5, Method=1, GradientSmooth=1, MainRange=1, XRange=1, YRange=1,
+                                       ZRange=1, RemoveCon=0, Renormalize=0, IntensityW=1)
+    smp = srrf.directions(6, conf.RingRadius)
+
+    d_vol = srrf.DeviceBuffer.from_array(series)
+    d_extra = [srrf.DeviceBuffer.from_array(e) for e in extras]
+    geo = [(d_vol, series.shape, o, d, T) for (o, d) in split]
+    geo += [(d_extra[i], e.shape, (0, 0, 0), tuple(int(v) for v in e.shape[:0:-1]), e.shape[0])
+            for i, e in enumerate(extras)]
+    sizes = [int(np.prod(d)) * int(m) ** 3 for (_, _, _, d, _), m in zip(geo, mags)]
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]]) * 4
+    d_out = srrf.DeviceBuffer(int(sum(sizes)) * 4)
+    blocks = [srrf.Block(src, shp, o, d, t, int(m), _View(d_out, int(off)))
+              for (src, shp, o, d, t), m, off in zip(geo, mags, offs)]
+    n_sr = int(sum(sizes))
+    sr_frames = float(sum(s * g[4] for s, g in zip(sizes, geo)))
+    alg_bytes = float(sum(2 * np.prod(g[3]) * g[4] for g in geo)) + 4.0 * n_sr
+    edges = sorted({int(v) for g in geo for v in g[3]})
+
+    bt = srrf.Batch(blocks, srrf.RFBA_U16, conf, smp)
+    t0 = time.perf_counter()
+    tz, tune = bt.autotune()
+    log("c3: autotune %.1f s -> tile depth %d, sample ms %s" % (time.perf_counter() - t0, tz, tune))
+    for i in range(args.warmup):
+        log("c3: warmup batched %.1f ms" % bt.run(use_graph=True))
+    srrf.lib().rfba_device_synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        kms.append(bt.run(use_graph=True))
+        log("c3: batched step %d: %.1f ms" % (i, kms[-1]))
+    srrf.lib().rfba_device_synchronize()
+    wall_b = (time.perf_counter() - t0) * 1e3 / args.steps
+    probe = [0, len(split) // 2, len(split) - 1, len(blocks) - 1]
+
+    def snap():
+        out = []
+        for i in probe:
+            b = blocks[i]
+            n = sizes[i]
+            buf = np.empty(n, np.float32)
+            srrf._check(srrf.lib().rfba_memcpy_d2h(buf.ctypes.data_as(ctypes.c_void_p), b.out.ptr, n * 4))
+            out.append(buf)
+        return out
+
+    ref = snap()
+    t0 = time.perf_counter()
+    ms_pb = bt.run_per_block()
+    srrf.lib().rfba_device_synchronize()
+    wall_pb = (time.perf_counter() - t0) * 1e3
+    log("c3: per-block %.1f ms" % ms_pb)
+    same = all(np.array_equal(a, b, equal_nan=True) for a, b in zip(ref, snap()))
+    bt.close()
+    for b in [d_vol, d_out] + d_extra:
+        b.close()
+    kb = float(np.mean(kms))
+    roof = hbm_roof(alg_bytes, kb)
+    roof.update({"kernel": "srrf_fused persistent queue (uint16 input)", "kernel_ms": kb,
+                 "algorithmic_bytes_per_launch": alg_bytes})
+    res = {
+        "workload": "batched 3D SRRF: 512x512x128 T=120 uint16 (8000 emitters, seed 3) split by doSplit.m into "
+                    "%d blocks of 29^3 with 3-voxel overlap (z edge row 13 deep) + 64 extra stacks (edges 8-29, "
+                    "T 30-200, seed 4); M per block from {2,3,4} (seed 5); R=0.5, 6 axis directions, TRA, "
+                    "smoothing 1, IW 1" % len(split),
+        "blocks": len(blocks), "edge_sizes": edges, "sr_voxels": n_sr, "sr_voxel_frames": sr_frames,
+        "tile_depth": tz, "autotune_sample_ms": {str(k): v for k, v in tune.items()},
+        "batched": {"ms": wall_b, "kernel_ms": kb, "sr_voxels_per_s": n_sr / (wall_b * 1e-3)},
+        "per_block": {"ms": wall_pb, "kernel_ms": ms_pb, "sr_voxels_per_s": n_sr / (wall_pb * 1e-3)},
+        "speedup_batched_vs_per_block": wall_pb / wall_b,
+        "batched_equals_per_block": bool(same),
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        from oracle import oracle
+        i = 0
+        b = blocks[i]
+        o, d = geo[i][2], geo[i][3]
+        sub = np.ascontiguousarray(series[:, o[2]:o[2] + d[2], o[1]:o[1] + d[1], o[0]:o[0] + 12]).astype(np.float32)
+        c = conf.replace(Magnification=int(mags[i]))
+        th = synth.default_threads()
+        t0 = time.perf_counter()
+        out = oracle.srrf_stack(sub[:, :, :12, :], c, smp.xyz, smp.divisor, nthreads=th)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": out.size / dt, "unit": "SR-voxels/s", "cores": th, "kind": "port",
+                               "sample": "oracle on a 12x12x29 x T=120 part of block 0 (M=%d), %.1f s"
+                                         % (int(mags[i]), dt)}
+    return res
+
+
+# ---------------------------------------------------------------- 3B
+def b3_model(b3, ckpt):
+    return b3.make_model(b3.blink_matrix(0.2, 0.1, 0.01), (0.5, 0.5, 0.0), 100.0, zfactor=2.0, z_exact=0,
+                         checkpoint=ckpt, mu_brightness=7.0, sigma_brightness=1.0, mu_blur=0.25, sigma_blur=0.1)
+
+
+def b3_alg(blocks):
+    """(bytes, flops) per batched launch: data + residual planes read and
+    written once, R read once per spot; 13 FP32 flops per pixel-frame per
+    (q, spot) item and 30 per pixel of the spot tables."""
+    by, fl = 0.0, 0.0
+    for b in blocks:
+        T, Z, Y, X = b.data.shape
+        Q, K, _ = b.params.shape
+        S = b.states.shape[0]
+        P = X * Y * Z
+        by += 4.0 * T * P * (1 + Q * S * (2 + K))
+        fl += float(Q) * K * S * (13.0 * T * P + 30.0 * P)
+    return by, fl
+
+
+def bench_b3_single(args):
+    from rfba_amd import b3
+    from rfba_amd.synth import default_threads
+    m = b3_model(b3, 0)
+    data, params, states = b3.synth_block((29, 29, 29), 100, 40, 6)
+    P = b3.perturb(params, 256, 7)
+    blk = b3.BlockData(data, P, states[None], 100.0)
+    bt = b3.Batch([blk], m)
+    for i in range(max(1, args.warmup)):
+        log("c4: warmup batched %.2f ms" % bt.run(use_graph=True))
+    kms = []
+    t0 = time.perf_counter()
+    for i in range(max(3, args.steps)):
+        kms.append(bt.run(use_graph=True))
+    wall_b = (time.perf_counter() - t0) * 1e3 / len(kms)
+    L, G = bt.results(0)
+    bt.run_per_block()
+    t0 = time.perf_counter()
+    ms_pb = bt.run_per_block()
+    wall_pb = (time.perf_counter() - t0) * 1e3
+    L2, G2 = bt.results(0)
+    bt.close()
+    log("c4: batched %.2f ms, per-parameter-set %.2f ms" % (wall_b, wall_pb))
+    by, fl = b3_alg([blk])
+    kb = float(np.mean(kms))
+    roof = hbm_roof(by, kb)
+    roof.update({"kernel": "b3_resid + b3_eval<grad>", "kernel_ms": kb, "algorithmic_bytes_per_launch": by,
+                 "valu_fp32": {"achieved": fl / (kb * 1e-3) / 1e12, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                               "frac": fl / (kb * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, "algorithmic_flops_per_launch": fl}})
+    res = {
+        "workload": "3B logL + adjoint, one 29^3 block, T=100, 40 spots (seed 6), 256 parameter sets (+-0.5 px, "
+                    "+-20%% intensity, seed 7), S=1, checkpoints every ceil(sqrt(T)) frames",
+        "block_evals": 256, "spot_evals": 256 * 40,
+        "batched": {"ms": wall_b, "kernel_ms": kb, "block_evals_per_s": 256 / (wall_b * 1e-3)},
+        "per_param_set": {"ms": wall_pb, "kernel_ms": ms_pb, "block_evals_per_s": 256 / (wall_pb * 1e-3)},
+        "speedup_batched_vs_plain": wall_pb / wall_b,
+        "plain_equals_batched": bool(np.array_equal(L, L2) and np.array_equal(G, G2)),
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        from oracle import b3oracle as bo
+        th = default_threads()
+        q = 4
+        t0 = time.perf_counter()
+        Lo, Go = bo.model_eval(data, 100.0, m, P[:q], states[None], threads=th)
+        dt = time.perf_counter() - t0
+        el = float(np.max(np.abs(L[:q] - Lo) / np.abs(Lo)))
+        ng = np.linalg.norm(Go, axis=-1)
+        nz = ng > 0  # spots whose posterior on-probability underflows have a zero gradient
+        eg = float(np.max(np.linalg.norm(G[:q] - Go, axis=-1)[nz] / ng[nz]))
+        res["error_vs_fp64_oracle"] = {"logL_max_rel": el, "grad_max_rel": eg, "parameter_sets": q,
+                                       "zero_gradient_items": int((~nz).sum()),
+                                       "zero_gradient_items_exact": bool(np.all(G[:q][~nz] == 0))}
+        res["cpu_baseline"] = {"value": q / dt, "unit": "block-evals/s", "cores": th, "kind": "port",
+                               "sample": "FP64 oracle, %d parameter sets x 40 spots, %.1f s" % (q, dt)}
+    return res
+
+
+def bench_b3_batched(args):
+    from rfba_amd import b3
+    from rfba_amd.blocks import volume_blocks
+    from rfba_amd.synth import default_threads
+    m = b3_model(b3, 11)
+    split = volume_blocks((512, 512, 128), 29, 3)
+    rng = np.random.default_rng(8)
+    idx = np.sort(rng.choice(len(split), 1200, replace=False))
+    K = np.clip(rng.poisson(40, 1200), 5, 120)
+    T, Q = 120, 32
+    dims = [split[i][1] for i in idx]
+    t0 = time.perf_counter()
+    gen = b3.synth_blocks(dims, T, K, 8000)
+    blocks = [b3.BlockData(d, b3.perturb(p, Q, 80000 + i), s[None], 100.0) for i, (d, p, s) in enumerate(gen)]
+    log("c5: generated %d blocks in %.1f s" % (len(blocks), time.perf_counter() - t0))
+    bt = b3.Batch(blocks, m)
+    log("c5: warmup batched %.1f ms" % bt.run(use_graph=True))
+    kms = []
+    t0 = time.perf_counter()
+    for i in range(max(1, args.steps)):
+        kms.append(bt.run(use_graph=True))
+        log("c5: batched step %d %.1f ms" % (i, kms[-1]))
+    wall_b = (time.perf_counter() - t0) * 1e3 / len(kms)
+    bt.close()
+    nsub = min(args.c5_plain_blocks, len(blocks))
+    sub = b3.Batch(blocks[:nsub], m)
+    sub.run(use_graph=True)
+    t0 = time.perf_counter()
+    ms_sub = sub.run(use_graph=True)
+    wall_sub = (time.perf_counter() - t0) * 1e3
+    sub.run_per_block()
+    t0 = time.perf_counter()
+    ms_pb = sub.run_per_block()
+    wall_pb = (time.perf_counter() - t0) * 1e3
+    sub.close()
+    log("c5: batched %.1f ms (all), subset batched %.1f ms, subset plain %.1f ms" % (wall_b, wall_sub, wall_pb))
+    by, fl = b3_alg(blocks)
+    kb = float(np.mean(kms))
+    roof = hbm_roof(by, kb)
+    roof.update({"kernel": "b3_resid + b3_eval<grad>", "kernel_ms": kb, "algorithmic_bytes_per_launch": by,
+                 "valu_fp32": {"achieved": fl / (kb * 1e-3) / 1e12, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                               "frac": fl / (kb * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, "algorithmic_flops_per_launch": fl}})
+    n_ev = len(blocks) * Q
+    res = {
+        "workload": "batched 3B logL + adjoint: 1200 blocks of the 512x512x128 doSplit (29^3, 3-voxel overlap; "
+                    "seed 8), K ~ Poisson(40) clipped 5-120, T=120, 32 parameter sets per block, S=1, "
+                    "checkpoint stride 11",
+        "blocks": len(blocks), "block_evals": n_ev, "spot_evals": int(K.sum()) * Q,
+        "batched": {"ms": wall_b, "kernel_ms": kb, "block_evals_per_s": n_ev / (wall_b * 1e-3)},
+        "subset": {"blocks": nsub, "batched_ms": wall_sub, "batched_kernel_ms": ms_sub, "plain_ms": wall_pb,
+                   "plain_kernel_ms": ms_pb, "plain_block_evals_per_s": nsub * Q / (wall_pb * 1e-3),
+                   "batched_block_evals_per_s": nsub * Q / (wall_sub * 1e-3)},
+        "speedup_batched_vs_plain": (n_ev / wall_b) / (nsub * Q / wall_pb),
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        from oracle import b3oracle as bo
+        th = default_threads()
+        t0 = time.perf_counter()
+        b = blocks[0]
+        bo.model_eval(b.data, 100.0, m, b.params[:2], b.states, threads=th)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": 2 / dt, "unit": "block-evals/s", "cores": th, "kind": "port",
+                               "sample": "FP64 oracle, block 0 (K=%d), 2 parameter sets, %.1f s"
+                                         % (b.params.shape[1], dt)}
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-crop", type=str, default="12,12,8")
+    ap.add_argument("--frames", type=int, default=200)
+    ap.add_argument("--only", type=str, default="c2,c3,c4,c5",
+                    help="workloads to run: c2 (headline stack), c3 (batched SRRF), c4, c5 (3B)")
+    ap.add_argument("--c5-plain-blocks", type=int, default=100)
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="measured HBM bytes per launch of the headline kernel (rocprofv3 FETCH_SIZE+WRITE_SIZE)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run the plain copy kernel over this many MiB (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.copy_probe_mib:
+        from rfba_amd import srrf
+        nbytes = args.copy_probe_mib << 20
+        ms = srrf.copy_probe(nbytes)
+        print(json.dumps({"copy_probe_bytes": nbytes, "ms": ms, "GB/s": 2 * nbytes / ms / 1e6}), flush=True)
+        return
+    if args.gpus != 1:
+        print("bench.py measures one GPU only; multi-GPU runs are not part of this build "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    only = set(args.only.split(","))
+    rec = {
+        "metric": METRIC, "value": None, "unit": "SR-voxels/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded emitters, Markov blinking, Poisson / Gaussian noise; generated on the host)",
+        "config": None, "roofline": None, "cpu_baseline": None, "workloads": {},
+    }
+    if "c2" in only:
+        bench_stack(args, rec)
+    if "c3" in only:
+        rec["workloads"]["srrf_batched"] = bench_srrf_batched(args)
+    if "c4" in only:
+        rec["workloads"]["b3_single"] = bench_b3_single(args)
+    if "c5" in only:
+        rec["workloads"]["b3_batched"] = bench_b3_batched(args)
     print(json.dumps(rec), flush=True)
 
 

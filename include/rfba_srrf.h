@@ -164,6 +164,15 @@ int rfba_srrf_batch_run(rfba_srrf_batch *plan, void *stream, int use_graph,
 /* Reference form for comparison: one launch per block on the same stream. */
 int rfba_srrf_batch_run_per_block(rfba_srrf_batch *plan, void *stream,
                                   float *kernel_ms);
+/* Autotuner (no reference counterpart): times every feasible tile depth
+ * (4, 8, 16 SR voxels along z) on a sub-plan of every sample_stride-th block
+ * (0: about 64 blocks), keeps the fastest and rebuilds the plan for it.
+ * *tz gets the depth, ms_by_tz[3] the sample times (0 = not feasible);
+ * either may be NULL.  Synchronous on stream. */
+int rfba_srrf_batch_autotune(rfba_srrf_batch *plan, void *stream,
+                             int sample_stride, int *tz, float *ms_by_tz);
+/* Tile depth the plan currently uses. */
+int rfba_srrf_batch_tile_depth(const rfba_srrf_batch *plan);
 void rfba_srrf_batch_destroy(rfba_srrf_batch *plan);
 
 /* ---- runtime helpers ---------------------------------------------------- */

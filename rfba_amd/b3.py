@@ -232,6 +232,44 @@ def synth_block(dims_xyz, T, K, seed, I=1304.0, sigma=1.3, zfactor=2.0, b0=100.0
     return data, params, states
 
 
+def synth_blocks(dims_list, T, K_list, seed0, I=1304.0, sigma=1.3, zfactor=2.0, b0=100.0, variance=100.0, A=None,
+                 pi=(0.5, 0.5, 0.0), threads=None):
+    """Many seeded blocks at once (host threads over blocks; block i uses
+    seed seed0 + i and spots truncated at 5 sigma).  Returns a list of
+    (data [T,Z,Y,X] float32, true params [K,5], states [K,T]) views."""
+    from .synth import default_threads
+    L = _lib.load("rfba_synth")
+    lp = POINTER(ctypes.c_longlong)
+    L.rfba_synth_3b_blocks.argtypes = [c_int, POINTER(c_int), c_int, POINTER(c_int), c_uint64, c_double,
+                                       c_double, c_double, c_double, c_double, POINTER(c_double), POINTER(c_double),
+                                       POINTER(c_float), lp, POINTER(c_double), lp, POINTER(c_uint8), lp, c_int]
+    L.rfba_synth_3b_blocks.restype = c_int
+    n = len(dims_list)
+    dims = np.ascontiguousarray(np.asarray(dims_list, np.int32).reshape(n, 3))
+    K = np.ascontiguousarray(np.asarray(K_list, np.int32))
+    P = dims.prod(1).astype(np.int64)
+    doff = np.concatenate([[0], np.cumsum(P * T)[:-1]]).astype(np.int64)
+    soff = np.concatenate([[0], np.cumsum(K)[:-1]]).astype(np.int64)
+    toff = soff * T
+    data = np.empty(int((P * T).sum()), np.float32)
+    params = np.empty((int(K.sum()), 5), np.float64)
+    states = np.empty(int(K.sum()) * T, np.uint8)
+    A = np.ascontiguousarray(blink_matrix() if A is None else A, np.float64)
+    pi = np.ascontiguousarray(pi, np.float64)
+    rc = L.rfba_synth_3b_blocks(n, _p(dims, c_int), T, _p(K, c_int), seed0, I, sigma, zfactor, b0, variance,
+                                _p(A, c_double), _p(pi, c_double), _p(data, c_float), _p(doff, ctypes.c_longlong),
+                                _p(params, c_double), _p(soff, ctypes.c_longlong), _p(states, c_uint8),
+                                _p(toff, ctypes.c_longlong), threads or default_threads())
+    if rc:
+        raise B3Error("generator failed")
+    out = []
+    for i in range(n):
+        X, Y, Z = (int(v) for v in dims[i])
+        d = data[doff[i]:doff[i] + P[i] * T].reshape(T, Z, Y, X)
+        out.append((d, params[soff[i]:soff[i] + K[i]], states[toff[i]:toff[i] + K[i] * T].reshape(int(K[i]), T)))
+    return out
+
+
 def perturb(params, Q, seed, pos=0.5, rel_int=0.2):
     """Q seeded perturbations of all spots: positions +-pos px, intensity +-rel_int."""
     rng = np.random.default_rng(seed)

@@ -44,6 +44,11 @@ static int fail(int code, const char *fmt, ...) {
                         #expr, hipGetErrorString(e_));                                    \
     } while (0)
 
+static int hip_status(hipError_t e) {
+    if (e == hipSuccess) return RFBA_OK;
+    return fail(e == hipErrorOutOfMemory ? RFBA_ENOMEM : RFBA_EDEVICE, "%s", hipGetErrorString(e));
+}
+
 extern "C" const char *rfba_last_error(void) { return g_err; }
 
 // ---------------------------------------------------------------------------
@@ -740,6 +745,9 @@ struct rfba_srrf_batch {
     hipGraphExec_t exec = nullptr;
     hipStream_t graph_stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevSamples smp;
+    int tuned_tz = 0;
+    float tune_ms[3] = {0, 0, 0};  // sample time at TZ 4, 8, 16 (0: not feasible)
 };
 
 static void batch_free(rfba_srrf_batch *b) {
@@ -755,6 +763,46 @@ static void batch_free(rfba_srrf_batch *b) {
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);
     delete b;
+}
+
+// (Re)build the device side of a plan for the tile depth in pl: tile
+// queue, kernel, grid, TRAC/TRPPM series scratch.  Drops a captured graph.
+static int batch_build(rfba_srrf_batch *b, const DevSamples &s, const Plan &geom) {
+    int rc;
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->smp = s;
+    b->pl = geom;
+    plan_tiles(b->items, b->pl.TZ, b->pl);
+    b->fn = pick_kernel(b->dtype, b->pl.TZ, s);
+    if ((rc = occupancy_grid(b->fn, b->pl.lds_bytes, (int)b->pl.tiles.size(), b->grid))) return rc;
+    auto realloc_dev = [&](void **p, size_t bytes) -> int {
+        if (*p) hipFree(*p);
+        *p = nullptr;
+        HIP_TRY(hipMalloc(p, std::max<size_t>(bytes, 16)));
+        return RFBA_OK;
+    };
+    const size_t nt = b->pl.tiles.size();
+    if ((rc = realloc_dev((void **)&b->d_smp, sizeof(DevSamples))) ||
+        (rc = realloc_dev((void **)&b->d_items, b->items.size() * sizeof(DevItem))) ||
+        (rc = realloc_dev((void **)&b->d_tiles, nt * sizeof(DevTile))) ||
+        (rc = realloc_dev((void **)&b->d_tiles_by_item, nt * sizeof(DevTile))) ||
+        (rc = realloc_dev((void **)&b->d_ctl, 2 * sizeof(int))))
+        return rc;
+    if (b->d_scratch) hipFree(b->d_scratch);
+    b->d_scratch = nullptr;
+    b->scratch_stride = 0;
+    if (needs_series(b->conf)) {
+        b->scratch_stride = (long long)b->pl.Tmax * b->pl.TZ * kThreads;
+        if ((rc = realloc_dev((void **)&b->d_scratch, (size_t)b->scratch_stride * b->grid * sizeof(float)))) return rc;
+    }
+    HIP_TRY(hipMemcpy(b->d_smp, &s, sizeof s, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->d_items, b->items.data(), b->items.size() * sizeof(DevItem), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->d_tiles, b->pl.tiles.data(), nt * sizeof(DevTile), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->d_tiles_by_item, b->pl.tiles_by_item.data(), nt * sizeof(DevTile), hipMemcpyHostToDevice));
+    return RFBA_OK;
 }
 
 extern "C" int rfba_srrf_batch_create(const rfba_srrf_block *blocks, int n, int dtype, const SRRFConf *conf,
@@ -798,42 +846,11 @@ extern "C" int rfba_srrf_batch_create(const rfba_srrf_block *blocks, int n, int 
         it.bx1 = bl.dims[0] * bl.mag; it.by1 = bl.dims[1] * bl.mag; it.bz1 = bl.dims[2] * bl.mag;
         b->items.push_back(it);
     }
-    if ((rc = choose_tz(b->items, s, *conf, b->pl, 0))) {
+    if ((rc = choose_tz(b->items, s, *conf, b->pl, 0)) || (rc = batch_build(b, s, b->pl))) {
         batch_free(b);
         return rc;
-    }
-    plan_tiles(b->items, b->pl.TZ, b->pl);
-    b->fn = pick_kernel(dtype, b->pl.TZ, s);
-    if ((rc = occupancy_grid(b->fn, b->pl.lds_bytes, (int)b->pl.tiles.size(), b->grid))) {
-        batch_free(b);
-        return rc;
-    }
-    auto alloc = [&](void **p, size_t bytes) -> int {
-        HIP_TRY(hipMalloc(p, bytes));
-        return RFBA_OK;
-    };
-    const size_t nt = b->pl.tiles.size();
-    if ((rc = alloc((void **)&b->d_smp, sizeof(DevSamples))) ||
-        (rc = alloc((void **)&b->d_items, b->items.size() * sizeof(DevItem))) ||
-        (rc = alloc((void **)&b->d_tiles, nt * sizeof(DevTile))) ||
-        (rc = alloc((void **)&b->d_tiles_by_item, nt * sizeof(DevTile))) ||
-        (rc = alloc((void **)&b->d_ctl, 2 * sizeof(int)))) {
-        batch_free(b);
-        return rc;
-    }
-    if (needs_series(*conf)) {
-        b->scratch_stride = (long long)b->pl.Tmax * b->pl.TZ * kThreads;
-        if ((rc = alloc((void **)&b->d_scratch, (size_t)b->scratch_stride * b->grid * sizeof(float)))) {
-            batch_free(b);
-            return rc;
-        }
     }
     hipError_t e = hipSuccess;
-    if (e == hipSuccess) e = hipMemcpy(b->d_smp, &s, sizeof s, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(b->d_items, b->items.data(), b->items.size() * sizeof(DevItem), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(b->d_tiles, b->pl.tiles.data(), nt * sizeof(DevTile), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(b->d_tiles_by_item, b->pl.tiles_by_item.data(), nt * sizeof(DevTile), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     if (e != hipSuccess) {
@@ -931,6 +948,59 @@ extern "C" int rfba_srrf_batch_run_per_block(rfba_srrf_batch *b, void *stream, f
     }
     return RFBA_OK;
 }
+
+// Autotuner: each feasible tile depth (geometry planned over all blocks, so
+// that the LDS layout is the one the full run would use) is timed on a
+// sub-plan of every sample_stride-th block, one warm run then one timed run;
+// the fastest depth is kept and the plan rebuilt for it.
+extern "C" int rfba_srrf_batch_autotune(rfba_srrf_batch *b, void *stream, int sample_stride, int *tz,
+                                        float *ms_by_tz) {
+    if (!b) return fail(RFBA_EINVAL, "plan is NULL");
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    hipStream_t st = (hipStream_t)stream;
+    const int n = (int)b->items.size();
+    const int stride = sample_stride > 0 ? sample_stride : std::max(1, n / 64);
+    rfba_srrf_batch *sub = new rfba_srrf_batch();
+    sub->conf = b->conf;
+    sub->dtype = b->dtype;
+    for (int i = 0; i < n; i += stride) sub->items.push_back(b->items[i]);
+    const int cand[3] = {4, 8, 16};
+    Plan best;
+    float best_ms = 0.0f;
+    int rc = RFBA_OK;
+    for (int c = 0; c < 3; c++) {
+        b->tune_ms[c] = 0.0f;
+        Plan p;
+        if (plan_geometry(b->items, b->smp, b->conf, cand[c], p) != RFBA_OK) continue;
+        if ((rc = batch_build(sub, b->smp, p))) break;
+        float ms = 0.0f;
+        for (int rep = 0; rep < 2 && rc == RFBA_OK; rep++) {
+            if ((rc = hip_status(hipEventRecord(b->ev0, st)))) break;
+            if ((rc = batch_enqueue(sub, st))) break;
+            if ((rc = hip_status(hipEventRecord(b->ev1, st)))) break;
+            if ((rc = hip_status(hipEventSynchronize(b->ev1)))) break;
+            if ((rc = hip_status(hipEventElapsedTime(&ms, b->ev0, b->ev1)))) break;
+            rc = batch_check(sub);
+        }
+        if (rc) break;
+        b->tune_ms[c] = ms;
+        if (best_ms == 0.0f || ms < best_ms) {
+            best_ms = ms;
+            best = p;
+        }
+    }
+    batch_free(sub);
+    if (rc) return rc;
+    if (best_ms == 0.0f) return fail(RFBA_EINVAL, "no feasible tile depth");
+    if ((rc = batch_build(b, b->smp, best))) return rc;
+    b->tuned_tz = best.TZ;
+    if (tz) *tz = best.TZ;
+    if (ms_by_tz)
+        for (int c = 0; c < 3; c++) ms_by_tz[c] = b->tune_ms[c];
+    return RFBA_OK;
+}
+
+extern "C" int rfba_srrf_batch_tile_depth(const rfba_srrf_batch *b) { return b ? b->pl.TZ : 0; }
 
 extern "C" void rfba_srrf_batch_destroy(rfba_srrf_batch *b) {
     std::lock_guard<std::mutex> lk(g_ctx.mu);

@@ -241,6 +241,81 @@ int rfba_synth_3b_block(int X, int Y, int Z, int T, int K, uint64_t seed, double
     return 0;
 }
 
+// Many 3B blocks at once (threads over blocks), each as rfba_synth_3b_block
+// but with the spot truncated to |dx|,|dy| <= 5 sigma, |dz| <= 5 sigma zF
+// and evaluated from separable per-axis factors.  Block i: dims[3*i..],
+// K[i] spots, seed seed0 + i, outputs at data + data_off[i],
+// params + 5*spot_off[i], states + state_off[i].
+int rfba_synth_3b_blocks(int n, const int *dims, int T, const int *K, uint64_t seed0, double I, double sigma,
+                         double zF, double b0, double var, const double *A, const double *pi, float *data,
+                         const long long *data_off, double *params, const long long *spot_off, uint8_t *states,
+                         const long long *state_off, int threads) {
+    parallel(n, threads, [&](int i) {
+        const int X = dims[3 * i], Y = dims[3 * i + 1], Z = dims[3 * i + 2], Kb = K[i];
+        Rng r(seed0 + (uint64_t)i * 0x9E3779B97F4A7C15ULL);
+        double *pp = params + 5 * spot_off[i];
+        uint8_t *ss = states + state_off[i];
+        for (int k = 0; k < Kb; k++) {
+            pp[5 * k] = I;
+            pp[5 * k + 1] = sigma;
+            pp[5 * k + 2] = r.uniform() * (X - 1);
+            pp[5 * k + 3] = r.uniform() * (Y - 1);
+            pp[5 * k + 4] = r.uniform() * (Z - 1);
+        }
+        for (int k = 0; k < Kb; k++) {
+            double u = r.uniform();
+            int st = u < pi[0] ? 0 : (u < pi[0] + pi[1] ? 1 : 2);
+            ss[(size_t)k * T] = (uint8_t)st;
+            for (int t = 1; t < T; t++) {
+                u = r.uniform();
+                const double *row = A + 3 * st;
+                st = u < row[0] ? 0 : (u < row[0] + row[1] ? 1 : 2);
+                ss[(size_t)k * T + t] = (uint8_t)st;
+            }
+        }
+        const size_t P = (size_t)X * Y * Z;
+        float *dd = data + data_off[i];
+        const double r2pi = std::sqrt(2 * M_PI), sd = std::sqrt(var);
+        const int rxy = (int)std::ceil(5 * sigma), rz = (int)std::ceil(5 * sigma * zF);
+        std::vector<double> img(P), gx, gy, gz;
+        for (int t = 0; t < T; t++) {
+            std::fill(img.begin(), img.end(), b0);
+            for (int k = 0; k < Kb; k++) {
+                if (ss[(size_t)k * T + t] != 0) continue;
+                const double *ph = pp + 5 * k;
+                const int x0 = std::max(0, (int)std::floor(ph[2]) - rxy), x1 = std::min(X - 1, (int)std::floor(ph[2]) + rxy + 1);
+                const int y0 = std::max(0, (int)std::floor(ph[3]) - rxy), y1 = std::min(Y - 1, (int)std::floor(ph[3]) + rxy + 1);
+                const int z0 = std::max(0, (int)std::floor(ph[4]) - rz), z1 = std::min(Z - 1, (int)std::floor(ph[4]) + rz + 1);
+                const double amp = ph[0] / (ph[1] * r2pi), i2s = 1.0 / (2 * ph[1] * ph[1]);
+                gx.assign(x1 - x0 + 1, 0);
+                gy.assign(y1 - y0 + 1, 0);
+                gz.assign(z1 - z0 + 1, 0);
+                for (int x = x0; x <= x1; x++) gx[x - x0] = std::exp(-(x - ph[2]) * (x - ph[2]) * i2s);
+                for (int y = y0; y <= y1; y++) gy[y - y0] = std::exp(-(y - ph[3]) * (y - ph[3]) * i2s);
+                for (int z = z0; z <= z1; z++) {
+                    const double dz = (z - ph[4]) / zF;
+                    gz[z - z0] = amp * std::exp(-dz * dz * i2s);
+                }
+                for (int z = z0; z <= z1; z++)
+                    for (int y = y0; y <= y1; y++) {
+                        const double f = gz[z - z0] * gy[y - y0];
+                        double *row = img.data() + ((size_t)z * Y + y) * X;
+                        for (int x = x0; x <= x1; x++) row[x] += f * gx[x - x0];
+                    }
+            }
+            float *dt = dd + (size_t)t * P;
+            for (size_t p = 0; p < P; p += 2) {
+                double u1 = r.uniform(), u2 = r.uniform();
+                if (u1 < 1e-300) u1 = 1e-300;
+                const double rad = std::sqrt(-2.0 * std::log(u1)), th = 2.0 * M_PI * u2;
+                dt[p] = (float)(img[p] + sd * rad * std::cos(th));
+                if (p + 1 < P) dt[p + 1] = (float)(img[p + 1] + sd * rad * std::sin(th));
+            }
+        }
+    });
+    return 0;
+}
+
 // Poisson draws for a given mean (sampler self-test).
 int rfba_synth_poisson(double lam, uint64_t seed, int n, long *out) {
     Rng r(seed);

@@ -111,6 +111,10 @@ def lib() -> ctypes.CDLL:
         L.rfba_srrf_batch_run.restype = c_int
         L.rfba_srrf_batch_run_per_block.argtypes = [c_void_p, c_void_p, POINTER(c_float)]
         L.rfba_srrf_batch_run_per_block.restype = c_int
+        L.rfba_srrf_batch_autotune.argtypes = [c_void_p, c_void_p, c_int, POINTER(c_int), POINTER(c_float)]
+        L.rfba_srrf_batch_autotune.restype = c_int
+        L.rfba_srrf_batch_tile_depth.argtypes = [c_void_p]
+        L.rfba_srrf_batch_tile_depth.restype = c_int
         L.rfba_srrf_batch_destroy.argtypes = [c_void_p]
         L.rfba_srrf_batch_destroy.restype = None
         L.rfba_last_error.argtypes = []
@@ -347,6 +351,18 @@ class Batch:
         ms = c_float(0)
         _check(lib().rfba_srrf_batch_run_per_block(self.plan, None, ctypes.byref(ms)))
         return ms.value
+
+    def autotune(self, sample_stride: int = 0):
+        """Time tile depths 4/8/16 on a sample of blocks, keep the fastest.
+        Returns (depth, {depth: sample ms})."""
+        tz = c_int(0)
+        ms = (c_float * 3)()
+        _check(lib().rfba_srrf_batch_autotune(self.plan, None, sample_stride, ctypes.byref(tz), ms))
+        return tz.value, {d: ms[i] for i, d in enumerate((4, 8, 16)) if ms[i] > 0}
+
+    @property
+    def tile_depth(self) -> int:
+        return lib().rfba_srrf_batch_tile_depth(self.plan)
 
     def close(self):
         if self.plan:

@@ -175,6 +175,47 @@ def test_batched_matches_single_stacks(gpu):
     b.close()
 
 
+def test_autotuned_batch_matches_single_stacks(gpu):
+    # uneven blocks and frame counts (TRAC keeps a per-voxel series in
+    # scratch, whose size depends on the tile depth): after autotuning, and
+    # at every tile depth the tuner times, results stay bit-identical.
+    rng = np.random.default_rng(11)
+    src = series((40, 12, 26, 28), 8, 31)
+    d_src = srrf.DeviceBuffer.from_array(src)
+    smp = srrf.directions(6, 0.5)
+    c = conf(Method=3, ITC=0, TRACOrder=2, IntensityW=1)
+    blocks, refs = [], []
+    for i in range(7):
+        dims = tuple(int(v) for v in (rng.integers(5, 14), rng.integers(5, 14), rng.integers(4, 12)))
+        org = tuple(int(rng.integers(0, lim - d + 1)) for lim, d in zip((28, 26, 12), dims))
+        frames = int(rng.integers(4, 41))
+        mag = int(rng.integers(2, 5))
+        out = srrf.DeviceBuffer(dims[0] * dims[1] * dims[2] * mag ** 3 * 4)
+        blocks.append(srrf.Block(d_src, src.shape, org, dims, frames, mag, out))
+        sub = src[:frames, org[2]:org[2] + dims[2], org[1]:org[1] + dims[1], org[0]:org[0] + dims[0]]
+        refs.append(srrf.srrf_stack(np.ascontiguousarray(sub), c.replace(Magnification=mag), smp))
+    b = srrf.Batch(blocks, srrf.RFBA_U16, c, smp)
+    b.run(use_graph=True)  # a graph captured before tuning must be dropped
+    tz, ms = b.autotune(sample_stride=2)
+    print("autotune: depth %d, sample ms %s" % (tz, ms))
+    assert tz in ms and ms[tz] == min(ms.values()) and b.tile_depth == tz
+    assert set(ms) <= {4, 8, 16} and 8 in ms
+    for mode in ("graph", "per_block"):
+        for bl in blocks:
+            bl.out.zero()
+        if mode == "per_block":
+            b.run_per_block()
+        else:
+            b.run(use_graph=True)
+        for i, bl in enumerate(blocks):
+            X, Y, Z = bl.dims_xyz
+            M = bl.mag
+            got = bl.out.to_array((Z * M, Y * M, X * M), np.float32)
+            assert np.array_equal(got, refs[i], equal_nan=True), (mode, i, tz)
+    b.close()
+    d_src.close()
+
+
 def test_full_size_boxes(gpu):
     # Full-width volume (256 x 256 low-res, M=4 -> 1024 x 1024 SR) against the
     # oracle evaluated on SR boxes with the same absolute coordinates: the
