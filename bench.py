@@ -288,9 +288,9 @@ def bench_b3_single(args):
         kms.append(bt.run(use_graph=True))
     wall_b = (time.perf_counter() - t0) * 1e3 / len(kms)
     L, G = bt.results(0)
-    bt.run_per_block()
+    bt.run_per_block(per_param_set=True)
     t0 = time.perf_counter()
-    ms_pb = bt.run_per_block()
+    ms_pb = bt.run_per_block(per_param_set=True)
     wall_pb = (time.perf_counter() - t0) * 1e3
     L2, G2 = bt.results(0)
     bt.close()
@@ -307,7 +307,7 @@ def bench_b3_single(args):
         "block_evals": 256, "spot_evals": 256 * 40,
         "batched": {"ms": wall_b, "kernel_ms": kb, "block_evals_per_s": 256 / (wall_b * 1e-3)},
         "per_param_set": {"ms": wall_pb, "kernel_ms": ms_pb, "block_evals_per_s": 256 / (wall_pb * 1e-3)},
-        "speedup_batched_vs_plain": wall_pb / wall_b,
+        "speedup_batched_vs_per_param_set": wall_pb / wall_b,
         "plain_equals_batched": bool(np.array_equal(L, L2) and np.array_equal(G, G2)),
         "roofline": roof,
     }
@@ -365,7 +365,15 @@ def bench_b3_batched(args):
     ms_pb = sub.run_per_block()
     wall_pb = (time.perf_counter() - t0) * 1e3
     sub.close()
-    log("c5: batched %.1f ms (all), subset batched %.1f ms, subset plain %.1f ms" % (wall_b, wall_sub, wall_pb))
+    nq = min(10, nsub)  # one launch sequence per (block, parameter set): the CG-call granularity
+    sq = b3.Batch(blocks[:nq], m)
+    sq.run_per_block(per_param_set=True)
+    t0 = time.perf_counter()
+    ms_pq = sq.run_per_block(per_param_set=True)
+    wall_pq = (time.perf_counter() - t0) * 1e3
+    sq.close()
+    log("c5: batched %.1f ms (all), subset batched %.1f ms, subset per block %.1f ms, %d blocks per (block, q) %.1f ms"
+        % (wall_b, wall_sub, wall_pb, nq, wall_pq))
     by, fl = b3_alg(blocks)
     kb = float(np.mean(kms))
     roof = hbm_roof(by, kb)
@@ -379,10 +387,14 @@ def bench_b3_batched(args):
                     "checkpoint stride 11",
         "blocks": len(blocks), "block_evals": n_ev, "spot_evals": int(K.sum()) * Q,
         "batched": {"ms": wall_b, "kernel_ms": kb, "block_evals_per_s": n_ev / (wall_b * 1e-3)},
-        "subset": {"blocks": nsub, "batched_ms": wall_sub, "batched_kernel_ms": ms_sub, "plain_ms": wall_pb,
-                   "plain_kernel_ms": ms_pb, "plain_block_evals_per_s": nsub * Q / (wall_pb * 1e-3),
-                   "batched_block_evals_per_s": nsub * Q / (wall_sub * 1e-3)},
+        "plain_per_block": {"blocks": nsub, "ms": wall_pb, "kernel_ms": ms_pb,
+                            "block_evals_per_s": nsub * Q / (wall_pb * 1e-3),
+                            "batched_ms_same_blocks": wall_sub,
+                            "batched_block_evals_per_s_same_blocks": nsub * Q / (wall_sub * 1e-3)},
+        "plain_per_param_set": {"blocks": nq, "ms": wall_pq, "kernel_ms": ms_pq,
+                                "block_evals_per_s": nq * Q / (wall_pq * 1e-3)},
         "speedup_batched_vs_plain": (n_ev / wall_b) / (nsub * Q / wall_pb),
+        "speedup_batched_vs_per_param_set": (n_ev / wall_b) / (nq * Q / wall_pq),
         "roofline": roof,
     }
     if not args.no_cpu:

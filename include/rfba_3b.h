@@ -87,11 +87,13 @@ typedef struct rfba3b_block {
 typedef struct rfba3b_batch rfba3b_batch;  /* opaque plan */
 
 int rfba3b_batch_create(const rfba3b_block *blocks, int n, const rfba3b_model *model, rfba3b_batch **plan);
-/* All blocks: background planes then one persistent work-queue launch over
- * every (block, q, k, s) item, optionally captured into a hipGraph. */
+/* All blocks at once: per memory chunk, three launches (residual planes,
+ * spot correlations, HMM forward + adjoint) over every block, parameter set
+ * and background sample; optionally captured into a hipGraph. */
 int rfba3b_batch_run(rfba3b_batch *plan, void *stream, int use_graph, float *kernel_ms);
-/* Reference form for comparison: one launch pair per (block, q). */
-int rfba3b_batch_run_per_block(rfba3b_batch *plan, void *stream, float *kernel_ms);
+/* Plain form for comparison, same kernels and results: one launch sequence
+ * per block (per_param_set = 0) or per (block, parameter set) (1). */
+int rfba3b_batch_run_per_block(rfba3b_batch *plan, void *stream, int per_param_set, float *kernel_ms);
 void rfba3b_batch_destroy(rfba3b_batch *plan);
 
 const char *rfba3b_last_error(void);

@@ -90,7 +90,7 @@ def lib():
         L.rfba3b_batch_create.restype = c_int
         L.rfba3b_batch_run.argtypes = [c_void_p, c_void_p, c_int, POINTER(c_float)]
         L.rfba3b_batch_run.restype = c_int
-        L.rfba3b_batch_run_per_block.argtypes = [c_void_p, c_void_p, POINTER(c_float)]
+        L.rfba3b_batch_run_per_block.argtypes = [c_void_p, c_void_p, c_int, POINTER(c_float)]
         L.rfba3b_batch_run_per_block.restype = c_int
         L.rfba3b_batch_destroy.argtypes = [c_void_p]
         L.rfba3b_batch_destroy.restype = None
@@ -183,9 +183,10 @@ class Batch:
         _check(lib().rfba3b_batch_run(self.plan, None, int(use_graph), ctypes.byref(ms)))
         return ms.value
 
-    def run_per_block(self) -> float:
+    def run_per_block(self, per_param_set=False) -> float:
+        """Plain form: one launch sequence per block (or per (block, q))."""
         ms = c_float(0)
-        _check(lib().rfba3b_batch_run_per_block(self.plan, None, ctypes.byref(ms)))
+        _check(lib().rfba3b_batch_run_per_block(self.plan, None, int(per_param_set), ctypes.byref(ms)))
         return ms.value
 
     def results(self, i):

@@ -18,8 +18,11 @@
 namespace rfba3b {
 __global__ void b3_reset(int *counter);
 __global__ void b3_resid(const DevBlock *__restrict__ blocks, const PlaneTask *__restrict__ tasks, int ntasks,
-                         float *__restrict__ R, const DevModel *__restrict__ mdl);
+                         float *__restrict__ R, double *__restrict__ S0, int s0_stride,
+                         const DevModel *__restrict__ mdl);
 template <int MODE> __global__ void b3_eval(EvalParams E);
+template <bool GRAD> __global__ void b3_corr(CorrParams C);
+template <bool GRAD> __global__ void b3_hmm(HmmParams H);
 }  // namespace rfba3b
 
 using namespace rfba3b;
@@ -118,25 +121,31 @@ struct rfba3b_batch {
     rfba3b_model model;
     std::vector<rfba3b_block> ublocks;
     std::vector<DevBlock> blocks;
-    // chunks of blocks whose residual planes fit the plane buffer together
+    // Chunks of blocks whose residual planes fit the plane buffer together.
+    // Planes, items and correlation tasks are indexed chunk-locally on the
+    // device (the chunk's first entry is passed as the array base).
     struct Chunk {
-        int task0, ntask, item0, nitem;
-        std::vector<std::pair<int, int>> bq;  // (block, q) ranges: task/item sub-ranges per (b, q)
-        std::vector<int> bq_task0, bq_ntask, bq_item0, bq_nitem;
+        int task0 = 0, ntask = 0, item0 = 0, nitem = 0, plane0 = 0, nplane = 0, corr0 = 0, ncorr = 0;
+        // per (block, q): residual tasks, correlation tasks (global index),
+        // items (chunk-local index)
+        std::vector<int> bq_task0, bq_ntask, bq_corr0, bq_ncorr, bq_item0, bq_nitem, bq_block;
     };
     std::vector<Chunk> chunks;
-    int mode = MODE_GRAD;
-    int Tmax = 1, XYmax = 1, Zmax = 1, ckpt = 1;
-    size_t lds_eval = 0, lds_resid = 0;
-    int grid_eval = 0;
+    bool grad = true;
+    int Tmax = 1, naxmax = 1, ckslots = 1;
+    size_t lds_resid = 0, lds_corr = 0;
     DevModel *d_model = nullptr;
     DevBlock *d_blocks = nullptr;
     PlaneTask *d_tasks = nullptr;
+    PlaneRef *d_planes = nullptr;
+    CorrTask *d_corr = nullptr;
     Item *d_items = nullptr;
-    int *d_counter = nullptr;
     float *d_R = nullptr;
-    size_t R_floats = 0;
+    double *d_S0 = nullptr, *d_em = nullptr, *d_ck = nullptr;
+    size_t R_floats = 0, S0_doubles = 0, em_doubles = 0, ck_doubles = 0;
     std::vector<PlaneTask> tasks;
+    std::vector<PlaneRef> planes;
+    std::vector<CorrTask> corr;
     std::vector<Item> items;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -150,12 +159,28 @@ static void batch_free(rfba3b_batch *b) {
     hipFree(b->d_model);
     hipFree(b->d_blocks);
     hipFree(b->d_tasks);
+    hipFree(b->d_planes);
+    hipFree(b->d_corr);
     hipFree(b->d_items);
-    hipFree(b->d_counter);
     hipFree(b->d_R);
+    hipFree(b->d_S0);
+    hipFree(b->d_em);
+    hipFree(b->d_ck);
     if (b->ev0) hipEventDestroy(b->ev0);
     if (b->ev1) hipEventDestroy(b->ev1);
     delete b;
+}
+
+typedef void (*CorrFn)(CorrParams);
+typedef void (*HmmFn)(HmmParams);
+static CorrFn corr_fn(bool grad) { return grad ? b3_corr<true> : b3_corr<false>; }
+static HmmFn hmm_fn(bool grad) { return grad ? b3_hmm<true> : b3_hmm<false>; }
+
+static size_t corr_lds(bool grad, int naxmax) {
+    const int nb = grad ? 8 : 4;  // padded basis stride per pixel and spot
+    return sizeof(double) * kCorrSpots * 6 + sizeof(float) * ((size_t)kCorrFrames * kCorrRow +
+                                                              (size_t)kCorrPix * kCorrSpots * nb +
+                                                              (size_t)kCorrSpots * naxmax * 4);
 }
 
 extern "C" int rfba3b_batch_create(const rfba3b_block *blocks, int n, const rfba3b_model *model,
@@ -176,7 +201,8 @@ extern "C" int rfba3b_batch_create(const rfba3b_block *blocks, int n, const rfba
             batch_free(b);
             return fail(-1, "block %d: invalid dimensions or buffers", i);
         }
-        if (u.dims[0] * u.dims[1] > 4096 || u.dims[2] > 1024 || u.T > 4096) {
+        if (u.dims[0] + u.dims[1] + u.dims[2] > 1024 || (long long)u.dims[0] * u.dims[1] * u.dims[2] > (1 << 24) ||
+            u.T > 4096) {
             batch_free(b);
             return fail(-1, "block %d: box or frame count too large", i);
         }
@@ -192,21 +218,22 @@ extern "C" int rfba3b_batch_create(const rfba3b_block *blocks, int n, const rfba
         b->blocks.push_back(d);
         b->ublocks.push_back(u);
         b->Tmax = std::max(b->Tmax, u.T);
-        b->XYmax = std::max(b->XYmax, u.dims[0] * u.dims[1]);
-        b->Zmax = std::max(b->Zmax, u.dims[2]);
-        const size_t lr = sizeof(float) * ((size_t)u.K * (u.dims[0] + u.dims[1] + u.dims[2]) + 2 * (size_t)u.K);
+        b->naxmax = std::max(b->naxmax, u.dims[0] + u.dims[1] + u.dims[2]);
+        const size_t lr = sizeof(float) * ((size_t)u.K * (u.dims[0] + u.dims[1] + u.dims[2]) + 2 * (size_t)u.K + 8);
         b->lds_resid = std::max(b->lds_resid, lr);
+        const int ck = ckpt_of(model, u.T);
+        b->ckslots = std::max(b->ckslots, 3 * ((u.T + ck - 1) / ck) + 2 * ck);
     }
     if (b->lds_resid > 64 * 1024) {
         batch_free(b);
         return fail(-1, "too many spots per block for the residual kernel");
     }
-    b->mode = grad ? MODE_GRAD : MODE_LOGL;
-    b->ckpt = 0;
-    int ckmax = 1;
-    for (auto &d : b->blocks) ckmax = std::max(ckmax, ckpt_of(model, d.T));
-    b->ckpt = model->checkpoint;  // 0 -> per-item ceil(sqrt(T)) in the kernel
-    b->lds_eval = eval_lds(b->Tmax, b->XYmax, b->Zmax, ckmax);
+    b->grad = grad;
+    b->lds_corr = corr_lds(grad, b->naxmax);
+    if (b->lds_corr > 160 * 1024) {
+        batch_free(b);
+        return fail(-1, "block edges too long for the correlation kernel");
+    }
 
     // chunks: residual planes of all (q, s) of a block stay together; blocks
     // are added to a chunk while the planes fit the budget.
@@ -214,59 +241,87 @@ extern "C" int rfba3b_batch_create(const rfba3b_block *blocks, int n, const rfba
     const size_t budget = (size_t)(bud ? atol(bud) : 16384) << 20;
     size_t cur = 0;
     rfba3b_batch::Chunk ch;
-    ch.task0 = ch.item0 = 0;
-    ch.ntask = ch.nitem = 0;
-    size_t maxR = 0;
+    size_t maxR = 0, maxPlanes = 0, maxItems = 0;
+    auto close_chunk = [&]() {
+        ch.ntask = (int)b->tasks.size() - ch.task0;
+        ch.nitem = (int)b->items.size() - ch.item0;
+        ch.nplane = (int)b->planes.size() - ch.plane0;
+        ch.ncorr = (int)b->corr.size() - ch.corr0;
+        maxR = std::max(maxR, cur);
+        maxPlanes = std::max(maxPlanes, (size_t)ch.nplane);
+        maxItems = std::max(maxItems, (size_t)ch.nitem);
+        b->chunks.push_back(ch);
+        ch = rfba3b_batch::Chunk();
+        ch.task0 = (int)b->tasks.size();
+        ch.item0 = (int)b->items.size();
+        ch.plane0 = (int)b->planes.size();
+        ch.corr0 = (int)b->corr.size();
+        cur = 0;
+    };
     for (int bi = 0; bi < n; bi++) {
         const DevBlock &d = b->blocks[bi];
         const size_t P = (size_t)d.X * d.Y * d.Z;
         const size_t bytes = (size_t)d.Q * d.S * d.T * P * sizeof(float);
-        if (cur > 0 && cur + bytes > budget) {
-            b->chunks.push_back(ch);
-            ch = rfba3b_batch::Chunk();
-            ch.task0 = (int)b->tasks.size();
-            ch.item0 = (int)b->items.size();
-            ch.ntask = ch.nitem = 0;
-            cur = 0;
-        }
+        if (cur > 0 && cur + bytes > budget) close_chunk();
         for (int q = 0; q < d.Q; q++) {
+            ch.bq_block.push_back(bi);
             ch.bq_task0.push_back((int)b->tasks.size());
-            ch.bq_item0.push_back((int)b->items.size());
+            ch.bq_corr0.push_back((int)b->corr.size());
+            ch.bq_item0.push_back((int)b->items.size() - ch.item0);
             for (int s = 0; s < d.S; s++) {
                 const long long off = (long long)(cur / sizeof(float));
-                for (int t = 0; t < d.T; t++) b->tasks.push_back({bi, q, s, t, off});
+                const int plane = (int)b->planes.size() - ch.plane0;
+                const int item0 = (int)b->items.size() - ch.item0;
+                b->planes.push_back({bi, q, s, 0, off});
+                for (int t = 0; t < d.T; t++) b->tasks.push_back({bi, q, s, t, off, plane, 0});
                 for (int k = 0; k < d.K; k++) b->items.push_back({bi, q, k, s, off});
+                for (int k0 = 0; k0 < d.K; k0 += kCorrSpots)
+                    b->corr.push_back({plane, k0, std::min(kCorrSpots, d.K - k0), item0 + k0});
                 cur += (size_t)d.T * P * sizeof(float);
             }
             ch.bq_ntask.push_back((int)b->tasks.size() - ch.bq_task0.back());
-            ch.bq_nitem.push_back((int)b->items.size() - ch.bq_item0.back());
+            ch.bq_ncorr.push_back((int)b->corr.size() - ch.bq_corr0.back());
+            ch.bq_nitem.push_back((int)b->items.size() - ch.item0 - ch.bq_item0.back());
         }
-        ch.ntask = (int)b->tasks.size() - ch.task0;
-        ch.nitem = (int)b->items.size() - ch.item0;
-        maxR = std::max(maxR, cur);
     }
-    b->chunks.push_back(ch);
+    close_chunk();
     b->R_floats = maxR / sizeof(float);
+    b->S0_doubles = maxPlanes * (size_t)b->Tmax;
+    b->em_doubles = maxItems * (size_t)b->Tmax * 8;
+    b->ck_doubles = maxItems * (size_t)b->ckslots;
 
-    if ((rc = eval_grid(eval_fn(b->mode), b->lds_eval, (int)b->items.size(), b->grid_eval))) {
-        batch_free(b);
-        return rc;
+    if (b->lds_corr > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)corr_fn(grad), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)b->lds_corr);
+        if (e != hipSuccess) {
+            batch_free(b);
+            return fail(-3, "correlation kernel LDS: %s", hipGetErrorString(e));
+        }
     }
     hipError_t e = hipSuccess;
     DevModel dm = dev_model(model);
-    if (e == hipSuccess) e = hipMalloc(&b->d_model, sizeof dm);
-    if (e == hipSuccess) e = hipMalloc(&b->d_blocks, b->blocks.size() * sizeof(DevBlock));
-    if (e == hipSuccess) e = hipMalloc(&b->d_tasks, b->tasks.size() * sizeof(PlaneTask));
-    if (e == hipSuccess) e = hipMalloc(&b->d_items, b->items.size() * sizeof(Item));
-    if (e == hipSuccess) e = hipMalloc(&b->d_counter, sizeof(int));
-    if (e == hipSuccess) e = hipMalloc(&b->d_R, std::max<size_t>(1, b->R_floats) * sizeof(float));
-    if (e == hipSuccess) e = hipMemcpy(b->d_model, &dm, sizeof dm, hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(b->d_blocks, b->blocks.data(), b->blocks.size() * sizeof(DevBlock), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(b->d_tasks, b->tasks.data(), b->tasks.size() * sizeof(PlaneTask), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(b->d_items, b->items.data(), b->items.size() * sizeof(Item), hipMemcpyHostToDevice);
+    auto al = [&](void **p, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(p, std::max<size_t>(bytes, 16));
+    };
+    al((void **)&b->d_model, sizeof dm);
+    al((void **)&b->d_blocks, b->blocks.size() * sizeof(DevBlock));
+    al((void **)&b->d_tasks, b->tasks.size() * sizeof(PlaneTask));
+    al((void **)&b->d_planes, b->planes.size() * sizeof(PlaneRef));
+    al((void **)&b->d_corr, b->corr.size() * sizeof(CorrTask));
+    al((void **)&b->d_items, b->items.size() * sizeof(Item));
+    al((void **)&b->d_R, b->R_floats * sizeof(float));
+    al((void **)&b->d_S0, b->S0_doubles * sizeof(double));
+    al((void **)&b->d_em, b->em_doubles * sizeof(double));
+    al((void **)&b->d_ck, b->ck_doubles * sizeof(double));
+    auto up = [&](void *d, const void *h, size_t bytes) {
+        if (e == hipSuccess && bytes) e = hipMemcpy(d, h, bytes, hipMemcpyHostToDevice);
+    };
+    up(b->d_model, &dm, sizeof dm);
+    up(b->d_blocks, b->blocks.data(), b->blocks.size() * sizeof(DevBlock));
+    up(b->d_tasks, b->tasks.data(), b->tasks.size() * sizeof(PlaneTask));
+    up(b->d_planes, b->planes.data(), b->planes.size() * sizeof(PlaneRef));
+    up(b->d_corr, b->corr.data(), b->corr.size() * sizeof(CorrTask));
+    up(b->d_items, b->items.data(), b->items.size() * sizeof(Item));
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     if (e != hipSuccess) {
@@ -278,35 +333,48 @@ extern "C" int rfba3b_batch_create(const rfba3b_block *blocks, int n, const rfba
     return 0;
 }
 
-static EvalParams eval_params(rfba3b_batch *b, int item0, int nitem) {
-    EvalParams E;
-    memset(&E, 0, sizeof E);
-    E.blocks = b->d_blocks;
-    E.items = b->d_items + item0;
-    E.nitems = nitem;
-    E.counter = b->d_counter;
-    E.R = b->d_R;
-    E.model = b->d_model;
-    E.rmode = 0;
-    E.ckpt = b->ckpt;
-    E.Tmax = b->Tmax;
-    E.XYmax = b->XYmax;
-    E.Zmax = b->Zmax;
-    return E;
-}
-
-static int enqueue_range(rfba3b_batch *b, hipStream_t st, int task0, int ntask, int item0, int nitem, int grid_cap) {
+// One chunk's work, or the part of it that belongs to one (block, q):
+// residual planes (+ per-frame sums of squares), correlations, HMMs.
+static int enqueue_range(rfba3b_batch *b, hipStream_t st, const rfba3b_batch::Chunk &c, int task0, int ntask,
+                         int corr0, int ncorr, int ilo, int nitem) {
     if (ntask > 0) {
         const int gr = std::min(ntask, g_ncu * 8);
         hipLaunchKernelGGL(b3_resid, dim3(gr), dim3(256), b->lds_resid, st, (const DevBlock *)b->d_blocks,
-                           (const PlaneTask *)(b->d_tasks + task0), ntask, b->d_R, (const DevModel *)b->d_model);
+                           (const PlaneTask *)(b->d_tasks + task0), ntask, b->d_R, b->d_S0, b->Tmax,
+                           (const DevModel *)b->d_model);
+        HIP_TRY(hipGetLastError());
+    }
+    if (ncorr > 0) {
+        CorrParams C;
+        memset(&C, 0, sizeof C);
+        C.blocks = b->d_blocks;
+        C.planes = b->d_planes + c.plane0;
+        C.tasks = b->d_corr + corr0;
+        C.ntasks = ncorr;
+        C.naxmax = b->naxmax;
+        C.Tmax = b->Tmax;
+        C.NI = c.nitem;
+        C.R = b->d_R;
+        C.S0 = b->d_S0;
+        C.model = b->d_model;
+        C.em = b->d_em;
+        hipLaunchKernelGGL(corr_fn(b->grad), dim3(ncorr), dim3(256), b->lds_corr, st, C);
         HIP_TRY(hipGetLastError());
     }
     if (nitem > 0) {
-        hipLaunchKernelGGL(b3_reset, dim3(1), dim3(64), 0, st, b->d_counter);
-        HIP_TRY(hipGetLastError());
-        EvalParams E = eval_params(b, item0, nitem);
-        hipLaunchKernelGGL(eval_fn(b->mode), dim3(std::min(nitem, grid_cap)), dim3(256), b->lds_eval, st, E);
+        HmmParams H;
+        memset(&H, 0, sizeof H);
+        H.blocks = b->d_blocks;
+        H.items = b->d_items + c.item0;
+        H.lo = ilo;
+        H.n = nitem;
+        H.NI = c.nitem;
+        H.ckpt = b->model.checkpoint;
+        H.ckslots = b->ckslots;
+        H.em = b->d_em;
+        H.ck = b->d_ck;
+        H.model = b->d_model;
+        hipLaunchKernelGGL(hmm_fn(b->grad), dim3((nitem + 255) / 256), dim3(256), 0, st, H);
         HIP_TRY(hipGetLastError());
     }
     return 0;
@@ -315,7 +383,7 @@ static int enqueue_range(rfba3b_batch *b, hipStream_t st, int task0, int ntask, 
 static int enqueue_all(rfba3b_batch *b, hipStream_t st) {
     int rc;
     for (auto &c : b->chunks)
-        if ((rc = enqueue_range(b, st, c.task0, c.ntask, c.item0, c.nitem, b->grid_eval))) return rc;
+        if ((rc = enqueue_range(b, st, c, c.task0, c.ntask, c.corr0, c.ncorr, 0, c.nitem))) return rc;
     return 0;
 }
 
@@ -353,17 +421,24 @@ extern "C" int rfba3b_batch_run(rfba3b_batch *b, void *stream, int use_graph, fl
     return 0;
 }
 
-extern "C" int rfba3b_batch_run_per_block(rfba3b_batch *b, void *stream, float *kernel_ms) {
+extern "C" int rfba3b_batch_run_per_block(rfba3b_batch *b, void *stream, int per_param_set, float *kernel_ms) {
     if (!b) return fail(-1, "plan is NULL");
     std::lock_guard<std::mutex> lk(g_mu);
     hipStream_t st = (hipStream_t)stream;
     int rc;
     HIP_TRY(hipEventRecord(b->ev0, st));
     for (auto &c : b->chunks)
-        for (size_t i = 0; i < c.bq_task0.size(); i++)
-            if ((rc = enqueue_range(b, st, c.bq_task0[i], c.bq_ntask[i], c.bq_item0[i], c.bq_nitem[i],
-                                    c.bq_nitem[i])))
-                return rc;
+        for (size_t i = 0; i < c.bq_task0.size();) {
+            // ranges of one (block, q), or of all q of one block (consecutive)
+            size_t j = i + 1;
+            if (!per_param_set)
+                while (j < c.bq_task0.size() && c.bq_block[j] == c.bq_block[i]) j++;
+            const int nt = c.bq_task0[j - 1] + c.bq_ntask[j - 1] - c.bq_task0[i];
+            const int nc = c.bq_corr0[j - 1] + c.bq_ncorr[j - 1] - c.bq_corr0[i];
+            const int ni = c.bq_item0[j - 1] + c.bq_nitem[j - 1] - c.bq_item0[i];
+            if ((rc = enqueue_range(b, st, c, c.bq_task0[i], nt, c.bq_corr0[i], nc, c.bq_item0[i], ni))) return rc;
+            i = j;
+        }
     HIP_TRY(hipEventRecord(b->ev1, st));
     if (kernel_ms) {
         HIP_TRY(hipEventSynchronize(b->ev1));

@@ -35,11 +35,59 @@ struct DevBlock {
 struct PlaneTask {
     int block, q, s, t;
     long long rslot;  // residual plane slot
+    int plane;        // chunk-local plane index (row of the per-frame sum of squares)
+    int pad;
 };
+
+// One residual plane (block, q, s) of a chunk.
+struct PlaneRef {
+    int block, q, s, pad;
+    long long rslot;
+};
+
+// Correlation task: up to kCorrSpots spots of one plane.
+struct CorrTask {
+    int plane;  // chunk-local plane index
+    int k0;     // first spot
+    int nk;     // number of spots (<= kCorrSpots)
+    int item0;  // chunk-local item index of spot k0 of this plane
+};
+
+constexpr int kCorrSpots = 8;   // spots per correlation workgroup
+constexpr int kCorrPix = 32;    // pixels per LDS chunk
+constexpr int kCorrRow = 36;    // LDS row stride of the residual chunk (floats)
+constexpr int kCorrFrames = 128;  // frames per pass (4 per thread)
+
+struct CorrParams {
+    const DevBlock *blocks;
+    const PlaneRef *planes;
+    const CorrTask *tasks;
+    int ntasks;
+    int naxmax;             // max X + Y + Z over blocks
+    int Tmax;               // row stride of S0
+    int NI;                 // items in the chunk (em row stride)
+    const float *R;
+    const double *S0;       // [plane][Tmax]: sum over pixels of R^2
+    const DevModel *model;
+    double *em;             // [t][NI][8]: Soff, D, diff[5]
+};
+
 
 struct Item {
     int block, q, k, s;
     long long rslot;
+};
+
+struct HmmParams {
+    const DevBlock *blocks;
+    const Item *items;      // chunk's items (chunk-local index 0..NI)
+    int lo, n;              // items lo .. lo + n - 1 of the chunk
+    int NI;
+    int ckpt;               // 0: ceil(sqrt(T)) per block
+    int ckslots;            // doubles per item in ck: 3 ceil(T/ckpt) + 2 ckpt
+    const double *em;
+    double *ck;             // [slot][NI]
+    const DevModel *model;
 };
 
 struct EvalParams {

@@ -36,8 +36,10 @@ __global__ void b3_reset(int *counter) {
 // kernel A: residual planes
 __global__ __launch_bounds__(256) void b3_resid(const DevBlock *__restrict__ blocks,
                                                 const PlaneTask *__restrict__ tasks, int ntasks,
-                                                float *__restrict__ R, const DevModel *__restrict__ mdl) {
+                                                float *__restrict__ R, double *__restrict__ S0, int s0_stride,
+                                                const DevModel *__restrict__ mdl) {
     extern __shared__ float lds[];
+    __shared__ double red[4];
     const int tid = threadIdx.x;
     for (int task = blockIdx.x; task < ntasks; task += gridDim.x) {
         const PlaneTask tk = tasks[task];
@@ -72,13 +74,41 @@ __global__ __launch_bounds__(256) void b3_resid(const DevBlock *__restrict__ blo
         // FP32 spot images, FP64 accumulation, one rounding of the residual
         // (summing b0 + images in FP32 would drop the far tails of every
         // spot below half an ulp of b0: a one-sided bias).
-        for (int p = tid; p < P; p += 256) {
-            const int z = p / (X * Y), r = p - z * X * Y;
-            const int y = r / X, x = r - y * X;
-            double v = (double)dt[p] - b.b0;
-            for (int j = 0; j < K; j++)
-                if (on[j]) v -= (double)(amp[j] * g[j * n_ax + x] * g[j * n_ax + X + y] * g[j * n_ax + X + Y + z]);
-            Rt[p] = (float)v;
+        // threads over (x,y) column x 8-deep z runs: the spot's xy factor is
+        // formed once per run; the product order ((amp gx) gy) gz is kept.
+        double ss = 0.0;
+        constexpr int RZ = 8;
+        const int XY = X * Y, nzc = (Z + RZ - 1) / RZ;
+        for (int e = tid; e < XY * nzc; e += 256) {
+            const int zc = e / XY, col = e - zc * XY;
+            const int y = col / X, x = col - y * X;
+            const int z0 = zc * RZ;
+            double v[RZ];
+#pragma unroll
+            for (int u = 0; u < RZ; u++) v[u] = z0 + u < Z ? (double)dt[(size_t)(z0 + u) * XY + col] - b.b0 : 0.0;
+            for (int j = 0; j < K; j++) {
+                if (!on[j]) continue;
+                const float *gj = g + j * n_ax;
+                const float a = amp[j] * gj[x] * gj[X + y];
+                const float *gz = gj + X + Y + z0;  // may run past spot j's z entries: padded, never stored
+#pragma unroll
+                for (int u = 0; u < RZ; u++) v[u] -= (double)(a * gz[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < RZ; u++)
+                if (z0 + u < Z) {
+                    const float rf = (float)v[u];
+                    Rt[(size_t)(z0 + u) * XY + col] = rf;
+                    ss += (double)rf * (double)rf;
+                }
+        }
+        // per-frame sum of squares of the stored residual (exact products)
+        if (S0) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o);
+            if ((tid & 63) == 0) red[tid >> 6] = ss;
+            __syncthreads();
+            if (tid == 0) S0[(size_t)tk.plane * s0_stride + tk.t] = (red[0] + red[1]) + (red[2] + red[3]);
         }
     }
 }
@@ -337,6 +367,279 @@ __global__ __launch_bounds__(256) void b3_eval(EvalParams E) {
         }
     }
 }
+
+// ---------------------------------------------------------------------------
+// Kernel C (b3_corr): the per-frame sums of kernel B for all spots of a
+// residual plane at once.  With rb = R + [on_k(t)] mu_k,
+//     Soff = S0 + [on] (2 C + Q),   D = 2 C + (on ? Q : -Q),
+//     diff_n = G_n - [off] H_n,
+// where S0(t) = sum R^2 (from kernel A), C(t) = sum mu R, G_n(t) = sum dmu_n R
+// and the frame-independent Q = sum mu^2, H_n = sum dmu_n mu.  C and G are a
+// product of the plane [T x P] with the spots' basis [P x 6 nk]: the
+// workgroup streams 32-pixel chunks of the plane (all frames of the pass)
+// and of the basis through LDS; thread (spot j = tid & 7, frame f = tid >> 3)
+// accumulates frames f, f+32, f+64, f+96 x 6 basis columns in FP32 over a
+// chunk and in FP64 across chunks.  Rows of the chunk are 36 floats apart, so
+// the 8 distinct frame rows a wave reads hit disjoint 4-bank groups.
+template <bool GRAD>
+__global__ __launch_bounds__(256) void b3_corr(CorrParams C) {
+    extern __shared__ double ldsc[];
+    constexpr int NB = GRAD ? 6 : 1;  // basis columns: mu, then dmu_0..4
+    constexpr int NBP = GRAD ? 8 : 4;  // padded LDS stride of a pixel's basis (float4 reads)
+    const int tid = threadIdx.x;
+    const CorrTask tk = C.tasks[blockIdx.x];
+    const PlaneRef pr = C.planes[tk.plane];
+    const DevBlock &b = C.blocks[pr.block];
+    const int X = b.X, Y = b.Y, Z = b.Z, T = b.T, K = b.K;
+    const int XY = X * Y, P = XY * Z, nax = X + Y + Z;
+    const int nk = tk.nk;
+    const DevModel &md = *C.model;
+
+    double *qh = ldsc;                                   // [8][6] Q, H_0..4
+    float *Rs = (float *)(qh + kCorrSpots * 6);          // [kCorrFrames][kCorrRow]
+    float *Bs = Rs + kCorrFrames * kCorrRow;             // [kCorrPix][8][NBP]
+    float *ax = Bs + kCorrPix * kCorrSpots * NBP;        // [8][nax][4]: exp(s), s, d I/sigma^2
+
+    // per-axis tables of the nk spots (same expressions as kernel B)
+    const float zF = (float)md.zfactor;
+    const float zdiv = md.z_exact ? zF * zF : 1.0f;
+    for (int e = tid; e < nk * nax; e += 256) {
+        const int j = e / nax, a = e - j * nax;
+        const double *ph = b.params + ((size_t)pr.q * K + tk.k0 + j) * 5;
+        const double sg = ph[1];
+        const float i2s2 = (float)(1.0 / (2.0 * sg * sg));
+        const float Iis2 = (float)(ph[0] / (sg * sg));
+        float sv, dI;
+        if (a < X) {
+            const float d = (float)(a + b.ox) - (float)ph[2];
+            sv = -(d * d) * i2s2;
+            dI = d * Iis2;
+        } else if (a < X + Y) {
+            const float d = (float)(a - X + b.oy) - (float)ph[3];
+            sv = -(d * d) * i2s2;
+            dI = d * Iis2;
+        } else {
+            const float dz = (float)(a - X - Y + b.oz) - (float)ph[4];
+            const float dzs = dz / zF;
+            sv = -(dzs * dzs) * i2s2;
+            dI = dz * Iis2 / zdiv;
+        }
+        float *q = ax + 4 * (j * nax + a);
+        q[0] = expf(sv);
+        q[1] = sv;
+        q[2] = dI;
+    }
+
+    // basis builder role: spot jb = tid >> 5, pixel lane cb = tid & 31
+    const int jb = tid >> 5, cb = tid & 31;
+    float Isb = 0.0f, invb = 0.0f, msigb = 0.0f;
+    if (jb < nk) {
+        const double *ph = b.params + ((size_t)pr.q * K + tk.k0 + jb) * 5;
+        Isb = (float)ph[0];
+        invb = (float)(1.0 / (ph[1] * 2.5066282746310002));
+        msigb = -Isb / (float)ph[1];
+    }
+    double qacc[6] = {0, 0, 0, 0, 0, 0};
+
+    // compute role: spot j = tid & 7, frame f = tid >> 3
+    const int j = tid & 7, f = tid >> 3;
+    const float *Rp = C.R + pr.rslot;
+    for (int t0 = 0; t0 < T; t0 += kCorrFrames) {
+        const int tn = min(kCorrFrames, T - t0);
+        double acc[4][NB];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int n = 0; n < NB; n++) acc[r][n] = 0.0;
+        for (int p0 = 0; p0 < P; p0 += kCorrPix) {
+            __syncthreads();
+            // residual chunk: rows = frames t0 .. t0 + TP - 1, zero-padded
+            for (int e = tid; e < kCorrFrames * kCorrPix; e += 256) {
+                const int r = e >> 5, c = e & 31;
+                const int p = p0 + c;
+                Rs[r * kCorrRow + c] = (r < tn && p < P) ? Rp[(size_t)(t0 + r) * P + p] : 0.0f;
+            }
+            // basis chunk (and, on the first pass, Q and H)
+            {
+                const int p = p0 + cb;
+                float v[NB];
+#pragma unroll
+                for (int n = 0; n < NB; n++) v[n] = 0.0f;
+                if (jb < nk && p < P) {
+                    const int z = p / XY, rr = p - z * XY;
+                    const int y = rr / X, x = rr - y * X;
+                    const float *qx = ax + 4 * (jb * nax + x);
+                    const float *qy = ax + 4 * (jb * nax + X + y);
+                    const float *qz = ax + 4 * (jb * nax + X + Y + z);
+                    const float E = (qx[0] * qy[0] * invb) * qz[0];
+                    const float mu = Isb * E;
+                    v[0] = mu;
+                    if (GRAD) {
+                        const float sv = (qx[1] + qy[1]) + qz[1];
+                        v[1] = E;
+                        v[2] = E * msigb * (1.0f + 2.0f * sv);
+                        v[3] = E * qx[2];
+                        v[4] = E * qy[2];
+                        v[5] = E * qz[2];
+                    }
+                    if (t0 == 0) {
+#pragma unroll
+                        for (int n = 0; n < NB; n++) qacc[n] += (double)v[n] * (double)mu;
+                    }
+                }
+                float *bq = Bs + (cb * kCorrSpots + jb) * NBP;
+                *(float4 *)bq = make_float4(v[0], GRAD ? v[1] : 0.0f, GRAD ? v[2] : 0.0f, GRAD ? v[3] : 0.0f);
+                if (GRAD) *(float4 *)(bq + 4) = make_float4(v[NB > 4 ? 4 : 0], v[NB > 5 ? 5 : 0], 0.0f, 0.0f);
+            }
+            __syncthreads();
+            float a32[4][NB];
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int n = 0; n < NB; n++) a32[r][n] = 0.0f;
+#pragma unroll 2
+            for (int c = 0; c < kCorrPix; c += 4) {
+                float4 rv[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) rv[r] = *(const float4 *)(Rs + (f + 32 * r) * kCorrRow + c);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    float bv[NBP];
+                    const float *bq = Bs + ((c + i) * kCorrSpots + j) * NBP;
+                    const float4 b0 = *(const float4 *)bq;
+                    bv[0] = b0.x; bv[1] = b0.y; bv[2] = b0.z; bv[3] = b0.w;
+                    if (GRAD) {
+                        const float4 b1 = *(const float4 *)(bq + 4);
+                        bv[4 % NBP] = b1.x; bv[5 % NBP] = b1.y;
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const float x = i == 0 ? rv[r].x : (i == 1 ? rv[r].y : (i == 2 ? rv[r].z : rv[r].w));
+#pragma unroll
+                        for (int n = 0; n < NB; n++) a32[r][n] = fmaf(x, bv[n], a32[r][n]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int n = 0; n < NB; n++) acc[r][n] += (double)a32[r][n];
+        }
+        if (t0 == 0) {
+            // Q, H of spot jb: reduce over the 32 pixel lanes that share it
+#pragma unroll
+            for (int n = 0; n < NB; n++) {
+                double v = qacc[n];
+#pragma unroll
+                for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o);
+                if (cb == 0 && jb < kCorrSpots) qh[jb * 6 + n] = v;
+            }
+        }
+        __syncthreads();
+        if (j < nk) {
+            const int k = tk.k0 + j;
+            const int item = tk.item0 + j;
+            const uint8_t *stk = b.states + ((size_t)pr.s * K + k) * T;
+            const double Q = qh[j * 6];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int t = t0 + f + 32 * r;
+                if (f + 32 * r < tn) {
+                    const bool on = stk[t] == 0;
+                    const double S0 = C.S0[(size_t)tk.plane * C.Tmax + t];
+                    const double c2 = 2.0 * acc[r][0];
+                    double *m = C.em + ((size_t)t * C.NI + item) * 8;
+                    m[0] = on ? S0 + (c2 + Q) : S0;
+                    m[1] = on ? c2 + Q : c2 - Q;
+                    if (GRAD) {
+#pragma unroll
+                        for (int n = 1; n < NB; n++) m[1 + n] = on ? acc[r][n] : acc[r][n] - qh[j * 6 + n];
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel D (b3_hmm): one lane per item runs the FP64 HMM of kernel B from
+// the per-frame sums em (forward with checkpoints, logL; for GRAD the
+// adjoint over segments recomputed from the checkpoints, held in ck).
+template <bool GRAD>
+__global__ __launch_bounds__(256) void b3_hmm(HmmParams H) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= H.n) return;
+    const int li = H.lo + i;
+    const Item it = H.items[li];
+    const DevBlock &b = H.blocks[it.block];
+    const int T = b.T;
+    const DevModel &md = *H.model;
+    Hmm h;
+    for (int u = 0; u < 9; u++) h.A[u] = md.A[u];
+    for (int u = 0; u < 3; u++) h.pi[u] = md.pi[u];
+    const double i2v = 0.5 / md.variance;
+    const double cst = (double)b.npix * log(2.0 * M_PI * md.variance) / 2.0;
+    const int ckpt = H.ckpt > 0 ? min(H.ckpt, T) : (int)ceil(sqrt((double)T));
+    const int nseg = (T + ckpt - 1) / ckpt;
+    const size_t NI = (size_t)H.NI;
+    double *ck = H.ck + li;  // slot u at ck[u * NI]
+    auto emr = [&](int t) { return H.em + ((size_t)t * NI + li) * 8; };
+    double a[3] = {0, 0, 0};
+    double L = 0.0;
+    for (int t = 0; t < T; t++) {
+        const double *m8 = emr(t);
+        const double e1 = -m8[0] * i2v - cst;
+        double bon, boff, m;
+        emis(m8[1] * i2v, bon, boff, m);
+        if (t % ckpt == 0) {
+            const int u = 3 * (t / ckpt);
+            ck[u * NI] = a[0];
+            ck[(u + 1) * NI] = a[1];
+            ck[(u + 2) * NI] = a[2];
+        }
+        L += fstep(h, a, bon, boff, t == 0) + e1 + m;
+    }
+    const size_t oi = ((size_t)it.q * b.K + it.k) * b.S + it.s;
+    b.logL[oi] = L;
+    if (!GRAD) return;
+    double g[5] = {0, 0, 0, 0, 0};
+    double beta[3] = {1.0, 1.0, 1.0};
+    const int sa0 = 3 * nseg, sc0 = sa0 + ckpt;  // slots: checkpoints | alpha_t(on) | log c_t
+    double c_next = 0.0, bon_n = 0.0, boff_n = 0.0;
+    for (int sgi = nseg - 1; sgi >= 0; sgi--) {
+        const int t0 = sgi * ckpt, t1 = min(T, t0 + ckpt);
+        double ar[3] = {ck[(3 * sgi) * NI], ck[(3 * sgi + 1) * NI], ck[(3 * sgi + 2) * NI]};
+        for (int t = t0; t < t1; t++) {
+            double bon, boff, m;
+            emis(emr(t)[1] * i2v, bon, boff, m);
+            ck[(sc0 + t - t0) * NI] = fstep(h, ar, bon, boff, t == 0);
+            ck[(sa0 + t - t0) * NI] = ar[0];
+        }
+        for (int t = t1 - 1; t >= t0; t--) {
+            if (t < T - 1) {
+                const double ic = exp(-c_next);
+                const double w0 = bon_n * beta[0], w1 = boff_n * beta[1], w2 = boff_n * beta[2];
+                const double n0 = (h.A[0] * w0 + h.A[1] * w1 + h.A[2] * w2) * ic;
+                const double n1 = (h.A[3] * w0 + h.A[4] * w1 + h.A[5] * w2) * ic;
+                const double n2 = (h.A[6] * w0 + h.A[7] * w1 + h.A[8] * w2) * ic;
+                beta[0] = n0; beta[1] = n1; beta[2] = n2;
+            }
+            const double *m8 = emr(t);
+            const double gam = ck[(sa0 + t - t0) * NI] * beta[0];
+            for (int n = 0; n < 5; n++) g[n] += gam * m8[2 + n];
+            double m;
+            emis(m8[1] * i2v, bon_n, boff_n, m);
+            c_next = ck[(sc0 + t - t0) * NI];
+        }
+    }
+    const double iv = 1.0 / md.variance;
+    for (int n = 0; n < 5; n++) b.grad[oi * 5 + n] = g[n] * iv;
+}
+
+template __global__ void b3_corr<false>(CorrParams);
+template __global__ void b3_corr<true>(CorrParams);
+template __global__ void b3_hmm<false>(HmmParams);
+template __global__ void b3_hmm<true>(HmmParams);
 
 template __global__ void b3_eval<MODE_LOGL>(EvalParams);
 template __global__ void b3_eval<MODE_GRAD>(EvalParams);

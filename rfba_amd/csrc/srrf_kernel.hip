@@ -214,8 +214,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
     // end of sample-table setup
     float *scr = P.scratch + (long long)blockIdx.x * P.scratch_stride;
     const int NU = g.NU, NV = g.NV, NW = g.NW, TXP = g.TXP, noz = g.noz;
-    const int RU = g.RU, RV = g.RV, RW = g.RW;
-    const int NR = RU * RV * RW;
+    const int RU = g.RU, RV = g.RV;
 
     for (;;) {
         __syncthreads();
@@ -239,6 +238,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
             if (tid == 0) atomicOr(P.err, 1);
             continue;
         }
+        // This tile's own extents (a batch plans LDS for its largest tile;
+        // tiles of larger magnification cover fewer low-res rows).  Every tap
+        // of the tile lies inside them; strides stay the planned ones.
+        const int nu = uhi - ulo + 1, nv = vhi - vlo + 1, nw = whi - wlo + 1;
+        const int ru = nu + 2 * g.hx, rv_ = nv + 2 * g.hy, rw = nw + 2 * g.hz;
+        const int nr_raw = ru * rv_ * rw;
 
         // z tables (uniform per tile, frame independent).
         for (int e = tid; e < TZ * noz; e += kThreads) {
@@ -282,19 +287,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
             __syncthreads();
             {
                 const Tin *srct = src + (long long)t * it.st;
-                for (int e = tid; e < NR; e += kThreads) {
-                    const int a = e % RU, bb = (e / RU) % RV, c = e / (RU * RV);
+                for (int e = tid; e < nr_raw; e += kThreads) {
+                    const int a = e % ru, bb = (e / ru) % rv_, c = e / (ru * rv_);
                     const int x = clampi(rx0 + a, 0, W - 1) + it.ox;
                     const int y = clampi(ry0 + bb, 0, H - 1) + it.oy;
                     const int z = clampi(rz0 + c, 0, D - 1) + it.oz;
-                    raw[e] = load_px<Tin>(srct, (long long)x + y * it.sy + z * it.sz);
+                    raw[(c * RV + bb) * RU + a] = load_px<Tin>(srct, (long long)x + y * it.sy + z * it.sz);
                 }
             }
             __syncthreads();
 
             // ---- gradient planes G[c][w][v][u] ------------------------------
-            for (int e = tid; e < NU * NV * NW; e += kThreads) {
-                const int u = e % NU, v = (e / NU) % NV, w = e / (NU * NV);
+            for (int e = tid; e < nu * nv * nw; e += kThreads) {
+                const int u = e % nu, v = (e / nu) % nv, w = e / (nu * nv);
+                const int gi3 = (w * NV + v) * NU + u;
                 const int cu = u + g.hx, cv = v + g.hy, cw = w + g.hz;
                 auto R3 = [&](int du, int dv, int dw) {
                     return raw[((cw + dw) * RV + (cv + dv)) * RU + (cu + du)];
@@ -332,9 +338,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                     gz = s;
                 }
                 const int NP = NU * NV * NW;
-                G[e] = gx;
-                G[NP + e] = gy;
-                G[2 * NP + e] = gz;
+                G[gi3] = gx;
+                G[NP + gi3] = gy;
+                G[2 * NP + gi3] = gz;
             }
 
             for (int z = 0; z < TZ; z++) dsum[z * kThreads + tid] = 0.0f;
@@ -354,8 +360,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                 // ---- x pass: Px[c][v][w][X] = sum_i G[c][w][v][b+i] * wx_i ----
                 {
                     const int NP = NU * NV * NW;
-                    for (int e = tid; e < NV * NW * kTX; e += kThreads) {
-                        const int X = e % kTX, w = (e / kTX) % NW, v = e / (kTX * NW);
+                    for (int e = tid; e < nv * nw * kTX; e += kThreads) {
+                        const int X = e % kTX, w = (e / kTX) % nw, v = e / (kTX * nw);
                         const float *xt = xtab + X * 6;
                         const int ub = __float_as_int(xt[0]);
                         const float w0 = xt[1], w1 = xt[2], w2 = xt[3], w3 = xt[4];
@@ -386,7 +392,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                     axis_weights(y0, M, H, vb, wy0, wy1, wy2, wy3, validy);
                     const int vrel = clampi(vb - vlo, 0, NV - 4);
                     // ---- y pass: Py[w][c][tid] = sum_j Px[c][vrel+j][w][X] * wy_j
-                    for (int w = 0; w < NW; w++) {
+                    for (int w = 0; w < nw; w++) {
 #pragma unroll
                         for (int c = 0; c < 3; c++) {
                             const float *pp = Px + ((c * NV + vrel) * NW + w) * TXP + xl;
@@ -480,8 +486,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                     xt[1] = w0; xt[2] = w1; xt[3] = w2; xt[4] = w3;
                 }
                 __syncthreads();
-                for (int e = tid; e < NV * NW * kTX; e += kThreads) {
-                    const int X = e % kTX, w = (e / kTX) % NW, v = e / (kTX * NW);
+                for (int e = tid; e < nv * nw * kTX; e += kThreads) {
+                    const int X = e % kTX, w = (e / kTX) % nw, v = e / (kTX * nw);
                     const float *xt = xtab + X * 6;
                     const int ub = __float_as_int(xt[0]);
                     const float *rr = raw + ((w + g.hz) * RV + (v + g.hy)) * RU + ub + g.hx;
@@ -498,7 +504,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                 axis_weights(Yc, M, H, vb, wy0, wy1, wy2, wy3, validy);
                 const int vrel = clampi(vb - vlo, 0, NV - 4);
                 const int validx = !((Xc / (float)M) < 1.5f || (Xc / (float)M) > (float)W - 1.5f);
-                for (int w = 0; w < NW; w++) {
+                for (int w = 0; w < nw; w++) {
                     const float *pp = Px + (vrel * NW + w) * TXP + xl;
                     const int vs = NW * TXP;
                     float q = 0.0f;

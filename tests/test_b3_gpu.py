@@ -82,14 +82,14 @@ def test_uneven_blocks_plain_graph_and_per_block_agree(gpu):
         blocks.append(b3.BlockData(data, b3.perturb(params, 2, 200 + i), states[None], 100.0))
     bt = b3.Batch(blocks, m)
     res = {}
-    for mode in ("plain", "graph", "per_block", "graph2"):
-        if mode == "per_block":
-            bt.run_per_block()
+    for mode in ("plain", "graph", "per_block", "per_param_set", "graph2"):
+        if mode.startswith("per_"):
+            bt.run_per_block(per_param_set=(mode == "per_param_set"))
         else:
             bt.run(use_graph=mode.startswith("graph"))
         res[mode] = [bt.results(i) for i in range(len(blocks))]
     bt.close()
-    for mode in ("graph", "per_block", "graph2"):
+    for mode in ("graph", "per_block", "per_param_set", "graph2"):
         for (L, G), (L0, G0) in zip(res[mode], res["plain"]):
             assert np.array_equal(L, L0) and np.array_equal(G, G0), mode
     for b, (L, G) in zip(blocks, res["plain"]):
