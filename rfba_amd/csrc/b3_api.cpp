@@ -21,7 +21,7 @@ __global__ void b3_resid(const DevBlock *__restrict__ blocks, const PlaneTask *_
                          float *__restrict__ R, double *__restrict__ S0, int s0_stride,
                          const DevModel *__restrict__ mdl);
 template <int MODE> __global__ void b3_eval(EvalParams E);
-template <bool GRAD> __global__ void b3_corr(CorrParams C);
+template <bool GRAD> __global__ void b3_corr_mfma(CorrParams C);
 template <bool GRAD> __global__ void b3_hmm(HmmParams H);
 }  // namespace rfba3b
 
@@ -173,14 +173,14 @@ static void batch_free(rfba3b_batch *b) {
 
 typedef void (*CorrFn)(CorrParams);
 typedef void (*HmmFn)(HmmParams);
-static CorrFn corr_fn(bool grad) { return grad ? b3_corr<true> : b3_corr<false>; }
+static CorrFn corr_fn(bool grad) { return grad ? b3_corr_mfma<true> : b3_corr_mfma<false>; }
 static HmmFn hmm_fn(bool grad) { return grad ? b3_hmm<true> : b3_hmm<false>; }
 
 static size_t corr_lds(bool grad, int naxmax) {
-    const int nb = grad ? 8 : 4;  // padded basis stride per pixel and spot
-    return sizeof(double) * kCorrSpots * 6 + sizeof(float) * ((size_t)kCorrFrames * kCorrRow +
-                                                              (size_t)kCorrPix * kCorrSpots * nb +
-                                                              (size_t)kCorrSpots * naxmax * 4);
+    // Q/H | residual chunk [128][36] | basis chunk [32][16-column tiles + 4] | axis tables
+    const size_t bmat = (size_t)kCorrPix * (((grad ? 48 : 8) + 15) / 16 * 16 + 4);
+    return sizeof(double) * kCorrSpots * 6 +
+           sizeof(float) * ((size_t)kCorrFrames * kCorrRow + bmat + (size_t)kCorrSpots * naxmax * 4);
 }
 
 extern "C" int rfba3b_batch_create(const rfba3b_block *blocks, int n, const rfba3b_model *model,

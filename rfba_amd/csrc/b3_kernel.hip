@@ -369,24 +369,31 @@ __global__ __launch_bounds__(256) void b3_eval(EvalParams E) {
 }
 
 // ---------------------------------------------------------------------------
-// Kernel C (b3_corr): the per-frame sums of kernel B for all spots of a
+// Kernel C (b3_corr_mfma): the per-frame sums of kernel B for all spots of a
 // residual plane at once.  With rb = R + [on_k(t)] mu_k,
 //     Soff = S0 + [on] (2 C + Q),   D = 2 C + (on ? Q : -Q),
 //     diff_n = G_n - [off] H_n,
 // where S0(t) = sum R^2 (from kernel A), C(t) = sum mu R, G_n(t) = sum dmu_n R
 // and the frame-independent Q = sum mu^2, H_n = sum dmu_n mu.  C and G are a
 // product of the plane [T x P] with the spots' basis [P x 6 nk]: the
-// workgroup streams 32-pixel chunks of the plane (all frames of the pass)
-// and of the basis through LDS; thread (spot j = tid & 7, frame f = tid >> 3)
-// accumulates frames f, f+32, f+64, f+96 x 6 basis columns in FP32 over a
-// chunk and in FP64 across chunks.  Rows of the chunk are 36 floats apart, so
-// the 8 distinct frame rows a wave reads hit disjoint 4-bank groups.
+// workgroup streams 32-pixel chunks of the plane (128 frames per pass) and
+// of the basis through LDS, so the plane is read once per 8 spots.
+// The product runs on the matrix cores with v_mfma_f32_16x16x4_f32
+// (lane l holds A[l%16][l/16] = R[frame][pixel] and B[l/16][l%16] =
+// basis[pixel][column]; result rows 4(l/16)+r, column l%16).  Passes of 128
+// frames; wave w owns frame tiles 2w, 2w+1 (16 frames each) x all column
+// tiles; FP32 matrix accumulation over two 32-pixel chunks, FP64 beyond.
+// Each lane writes the em fields of its own (frame, column) results.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 template <bool GRAD>
-__global__ __launch_bounds__(256) void b3_corr(CorrParams C) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void b3_corr_mfma(CorrParams C) {
     extern __shared__ double ldsc[];
-    constexpr int NB = GRAD ? 6 : 1;  // basis columns: mu, then dmu_0..4
-    constexpr int NBP = GRAD ? 8 : 4;  // padded LDS stride of a pixel's basis (float4 reads)
-    const int tid = threadIdx.x;
+    constexpr int NB = GRAD ? 6 : 1;
+    constexpr int NCOL = kCorrSpots * NB;   // 48 or 8 basis columns
+    constexpr int NCT = (NCOL + 15) / 16;   // 16-column tiles
+    constexpr int BSTR = NCT * 16 + 4;      // basis chunk row stride (floats)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const CorrTask tk = C.tasks[blockIdx.x];
     const PlaneRef pr = C.planes[tk.plane];
     const DevBlock &b = C.blocks[pr.block];
@@ -397,10 +404,9 @@ __global__ __launch_bounds__(256) void b3_corr(CorrParams C) {
 
     double *qh = ldsc;                                   // [8][6] Q, H_0..4
     float *Rs = (float *)(qh + kCorrSpots * 6);          // [kCorrFrames][kCorrRow]
-    float *Bs = Rs + kCorrFrames * kCorrRow;             // [kCorrPix][8][NBP]
-    float *ax = Bs + kCorrPix * kCorrSpots * NBP;        // [8][nax][4]: exp(s), s, d I/sigma^2
+    float *Bs = Rs + kCorrFrames * kCorrRow;             // [kCorrPix][BSTR]
+    float *ax = Bs + kCorrPix * BSTR;                    // [8][nax][4]
 
-    // per-axis tables of the nk spots (same expressions as kernel B)
     const float zF = (float)md.zfactor;
     const float zdiv = md.z_exact ? zF * zF : 1.0f;
     for (int e = tid; e < nk * nax; e += 256) {
@@ -429,8 +435,12 @@ __global__ __launch_bounds__(256) void b3_corr(CorrParams C) {
         q[1] = sv;
         q[2] = dI;
     }
+    // columns past the spots' basis stay zero
+    for (int e = tid; e < kCorrPix * (BSTR - NCOL); e += 256) {
+        const int c = e / (BSTR - NCOL), o = e - c * (BSTR - NCOL);
+        Bs[c * BSTR + NCOL + o] = 0.0f;
+    }
 
-    // basis builder role: spot jb = tid >> 5, pixel lane cb = tid & 31
     const int jb = tid >> 5, cb = tid & 31;
     float Isb = 0.0f, invb = 0.0f, msigb = 0.0f;
     if (jb < nk) {
@@ -439,127 +449,172 @@ __global__ __launch_bounds__(256) void b3_corr(CorrParams C) {
         invb = (float)(1.0 / (ph[1] * 2.5066282746310002));
         msigb = -Isb / (float)ph[1];
     }
-    double qacc[6] = {0, 0, 0, 0, 0, 0};
+    // Q = sum mu^2 and H_n = sum dmu_n mu over the block (frame independent):
+    // spot jb, pixels cb, cb + 32, ...
+    auto basis = [&](int x, int y, int z, float *v) {
+        const float *qx = ax + 4 * (jb * nax + x);
+        const float *qy = ax + 4 * (jb * nax + X + y);
+        const float *qz = ax + 4 * (jb * nax + X + Y + z);
+        const float E = (qx[0] * qy[0] * invb) * qz[0];
+        v[0] = Isb * E;
+        if (GRAD) {
+            const float sv = (qx[1] + qy[1]) + qz[1];
+            v[NB > 1 ? 1 : 0] = E;
+            v[NB > 2 ? 2 : 0] = E * msigb * (1.0f + 2.0f * sv);
+            v[NB > 3 ? 3 : 0] = E * qx[2];
+            v[NB > 4 ? 4 : 0] = E * qy[2];
+            v[NB > 5 ? 5 : 0] = E * qz[2];
+        }
+    };
+    __syncthreads();  // axis tables
+    {
+        double qacc[NB];
+#pragma unroll
+        for (int n = 0; n < NB; n++) qacc[n] = 0.0;
+        if (jb < nk)
+            for (int p = cb; p < P; p += 32) {
+                const int z = p / XY, rr = p - z * XY;
+                const int y = rr / X, x = rr - y * X;
+                float v[NB];
+                basis(x, y, z, v);
+#pragma unroll
+                for (int n = 0; n < NB; n++) qacc[n] += (double)v[n] * (double)v[0];
+            }
+#pragma unroll
+        for (int n = 0; n < NB; n++) {
+            double v = qacc[n];
+#pragma unroll
+            for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            if (cb == 0) qh[jb * 6 + n] = v;
+        }
+    }
 
-    // compute role: spot j = tid & 7, frame f = tid >> 3
-    const int j = tid & 7, f = tid >> 3;
     const float *Rp = C.R + pr.rslot;
+    const int lr = lane & 15, lk = lane >> 4;
     for (int t0 = 0; t0 < T; t0 += kCorrFrames) {
         const int tn = min(kCorrFrames, T - t0);
-        double acc[4][NB];
+        v4f acc[2][NCT];
+        double a64[2][NCT][4];
 #pragma unroll
-        for (int r = 0; r < 4; r++)
+        for (int f2 = 0; f2 < 2; f2++)
 #pragma unroll
-            for (int n = 0; n < NB; n++) acc[r][n] = 0.0;
-        for (int p0 = 0; p0 < P; p0 += kCorrPix) {
-            __syncthreads();
-            // residual chunk: rows = frames t0 .. t0 + TP - 1, zero-padded
-            for (int e = tid; e < kCorrFrames * kCorrPix; e += 256) {
-                const int r = e >> 5, c = e & 31;
-                const int p = p0 + c;
-                Rs[r * kCorrRow + c] = (r < tn && p < P) ? Rp[(size_t)(t0 + r) * P + p] : 0.0f;
+            for (int ct = 0; ct < NCT; ct++) {
+                acc[f2][ct] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int r = 0; r < 4; r++) a64[f2][ct][r] = 0.0;
             }
-            // basis chunk (and, on the first pass, Q and H)
+        // pixel p = p0 + cb of the basis role, tracked as (x, y, z)
+        int bx = cb % X, by = (cb / X) % Y, bz = cb / XY;
+        // Residual chunk prefetch: thread owns 4 consecutive pixels
+        // (4 (tid & 7)) of rows (tid >> 3) + 32 i; the next chunk's loads are
+        // in flight during this chunk's matrix work.
+        const int pr4 = (tid & 7) * 4, prow = tid >> 3;
+        float4 pre[4];
+        auto fetch = [&](int p0) {
+            const int p = p0 + pr4;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int r = prow + 32 * i;
+                float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (r < tn) {
+                    const float *src = Rp + (size_t)(t0 + r) * P + p;
+                    if (p + 3 < P) {
+                        __builtin_memcpy(&v, src, sizeof v);  // 4-byte aligned source
+                    } else {
+                        if (p < P) v.x = src[0];
+                        if (p + 1 < P) v.y = src[1];
+                        if (p + 2 < P) v.z = src[2];
+                    }
+                }
+                pre[i] = v;
+            }
+        };
+        fetch(0);
+        int nch = 0;
+        for (int p0 = 0; p0 < P; p0 += kCorrPix, nch++) {
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < 4; i++) *(float4 *)(Rs + (prow + 32 * i) * kCorrRow + pr4) = pre[i];
             {
                 const int p = p0 + cb;
                 float v[NB];
 #pragma unroll
                 for (int n = 0; n < NB; n++) v[n] = 0.0f;
-                if (jb < nk && p < P) {
-                    const int z = p / XY, rr = p - z * XY;
-                    const int y = rr / X, x = rr - y * X;
-                    const float *qx = ax + 4 * (jb * nax + x);
-                    const float *qy = ax + 4 * (jb * nax + X + y);
-                    const float *qz = ax + 4 * (jb * nax + X + Y + z);
-                    const float E = (qx[0] * qy[0] * invb) * qz[0];
-                    const float mu = Isb * E;
-                    v[0] = mu;
-                    if (GRAD) {
-                        const float sv = (qx[1] + qy[1]) + qz[1];
-                        v[1] = E;
-                        v[2] = E * msigb * (1.0f + 2.0f * sv);
-                        v[3] = E * qx[2];
-                        v[4] = E * qy[2];
-                        v[5] = E * qz[2];
-                    }
-                    if (t0 == 0) {
+                if (jb < nk && p < P) basis(bx, by, bz, v);
 #pragma unroll
-                        for (int n = 0; n < NB; n++) qacc[n] += (double)v[n] * (double)mu;
+                for (int n = 0; n < NB; n++) Bs[cb * BSTR + jb * NB + n] = v[n];
+                // advance this lane's pixel by kCorrPix
+                bx += kCorrPix;
+                while (bx >= X) {
+                    bx -= X;
+                    if (++by == Y) {
+                        by = 0;
+                        bz++;
                     }
                 }
-                float *bq = Bs + (cb * kCorrSpots + jb) * NBP;
-                *(float4 *)bq = make_float4(v[0], GRAD ? v[1] : 0.0f, GRAD ? v[2] : 0.0f, GRAD ? v[3] : 0.0f);
-                if (GRAD) *(float4 *)(bq + 4) = make_float4(v[NB > 4 ? 4 : 0], v[NB > 5 ? 5 : 0], 0.0f, 0.0f);
             }
             __syncthreads();
-            float a32[4][NB];
+            if (p0 + kCorrPix < P) fetch(p0 + kCorrPix);
 #pragma unroll
-            for (int r = 0; r < 4; r++)
+            for (int k4 = 0; k4 < kCorrPix; k4 += 4) {
+                const int kk = k4 + lk;
+                float av[2], bv[NCT];
 #pragma unroll
-                for (int n = 0; n < NB; n++) a32[r][n] = 0.0f;
-#pragma unroll 2
-            for (int c = 0; c < kCorrPix; c += 4) {
-                float4 rv[4];
+                for (int f2 = 0; f2 < 2; f2++) av[f2] = Rs[((2 * wave + f2) * 16 + lr) * kCorrRow + kk];
 #pragma unroll
-                for (int r = 0; r < 4; r++) rv[r] = *(const float4 *)(Rs + (f + 32 * r) * kCorrRow + c);
+                for (int ct = 0; ct < NCT; ct++) bv[ct] = Bs[kk * BSTR + ct * 16 + lr];
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    float bv[NBP];
-                    const float *bq = Bs + ((c + i) * kCorrSpots + j) * NBP;
-                    const float4 b0 = *(const float4 *)bq;
-                    bv[0] = b0.x; bv[1] = b0.y; bv[2] = b0.z; bv[3] = b0.w;
-                    if (GRAD) {
-                        const float4 b1 = *(const float4 *)(bq + 4);
-                        bv[4 % NBP] = b1.x; bv[5 % NBP] = b1.y;
-                    }
+                for (int f2 = 0; f2 < 2; f2++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const float x = i == 0 ? rv[r].x : (i == 1 ? rv[r].y : (i == 2 ? rv[r].z : rv[r].w));
-#pragma unroll
-                        for (int n = 0; n < NB; n++) a32[r][n] = fmaf(x, bv[n], a32[r][n]);
-                    }
-                }
+                    for (int ct = 0; ct < NCT; ct++)
+                        acc[f2][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[f2], bv[ct], acc[f2][ct], 0, 0, 0);
             }
+            if ((nch & 1) || p0 + kCorrPix >= P) {
 #pragma unroll
-            for (int r = 0; r < 4; r++)
+                for (int f2 = 0; f2 < 2; f2++)
 #pragma unroll
-                for (int n = 0; n < NB; n++) acc[r][n] += (double)a32[r][n];
-        }
-        if (t0 == 0) {
-            // Q, H of spot jb: reduce over the 32 pixel lanes that share it
+                    for (int ct = 0; ct < NCT; ct++) {
 #pragma unroll
-            for (int n = 0; n < NB; n++) {
-                double v = qacc[n];
-#pragma unroll
-                for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o);
-                if (cb == 0 && jb < kCorrSpots) qh[jb * 6 + n] = v;
+                        for (int r = 0; r < 4; r++) a64[f2][ct][r] += (double)acc[f2][ct][r];
+                        acc[f2][ct] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
+                    }
             }
         }
         __syncthreads();
-        if (j < nk) {
-            const int k = tk.k0 + j;
-            const int item = tk.item0 + j;
-            const uint8_t *stk = b.states + ((size_t)pr.s * K + k) * T;
-            const double Q = qh[j * 6];
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int t = t0 + f + 32 * r;
-                if (f + 32 * r < tn) {
+        for (int f2 = 0; f2 < 2; f2++)
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++) {
+                const int col = ct * 16 + lr;
+                const int j = col / NB, n = col - j * NB;
+                if (col >= NCOL || j >= nk) continue;
+                const int k = tk.k0 + j;
+                const int item = tk.item0 + j;
+                const uint8_t *stk = b.states + ((size_t)pr.s * K + k) * T;
+                const double Q = qh[j * 6], Hn = qh[j * 6 + n];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int trel = (2 * wave + f2) * 16 + 4 * lk + r;
+                    if (trel >= tn) continue;
+                    const int t = t0 + trel;
                     const bool on = stk[t] == 0;
-                    const double S0 = C.S0[(size_t)tk.plane * C.Tmax + t];
-                    const double c2 = 2.0 * acc[r][0];
+                    const double v = a64[f2][ct][r];
                     double *m = C.em + ((size_t)t * C.NI + item) * 8;
-                    m[0] = on ? S0 + (c2 + Q) : S0;
-                    m[1] = on ? c2 + Q : c2 - Q;
-                    if (GRAD) {
-#pragma unroll
-                        for (int n = 1; n < NB; n++) m[1 + n] = on ? acc[r][n] : acc[r][n] - qh[j * 6 + n];
+                    if (n == 0) {
+                        const double S0 = C.S0[(size_t)tk.plane * C.Tmax + t];
+                        const double c2 = 2.0 * v;
+                        m[0] = on ? S0 + (c2 + Q) : S0;
+                        m[1] = on ? c2 + Q : c2 - Q;
+                    } else {
+                        m[1 + n] = on ? v : v - Hn;
                     }
                 }
             }
-        }
     }
 }
+
+template __global__ void b3_corr_mfma<false>(CorrParams);
+template __global__ void b3_corr_mfma<true>(CorrParams);
 
 // ---------------------------------------------------------------------------
 // Kernel D (b3_hmm): one lane per item runs the FP64 HMM of kernel B from
@@ -636,8 +691,6 @@ __global__ __launch_bounds__(256) void b3_hmm(HmmParams H) {
     for (int n = 0; n < 5; n++) b.grad[oi * 5 + n] = g[n] * iv;
 }
 
-template __global__ void b3_corr<false>(CorrParams);
-template __global__ void b3_corr<true>(CorrParams);
 template __global__ void b3_hmm<false>(HmmParams);
 template __global__ void b3_hmm<true>(HmmParams);
 
