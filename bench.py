@@ -184,7 +184,9 @@ def bench_srrf_batched(args):
     bt = srrf.Batch(blocks, srrf.RFBA_U16, conf, smp)
     t0 = time.perf_counter()
     tz, tune = bt.autotune()
-    log("c3: autotune %.1f s -> tile depth %d, sample ms %s" % (time.perf_counter() - t0, tz, tune))
+    groups = bt.groups
+    log("c3: autotune %.1f s -> groups (M, tile depth) %s; largest group's sample ms %s"
+        % (time.perf_counter() - t0, groups, tune))
     for i in range(args.warmup):
         log("c3: warmup batched %.1f ms" % bt.run(use_graph=True))
     srrf.lib().rfba_device_synchronize()
@@ -227,7 +229,7 @@ def bench_srrf_batched(args):
                     "T 30-200, seed 4); M per block from {2,3,4} (seed 5); R=0.5, 6 axis directions, TRA, "
                     "smoothing 1, IW 1" % len(split),
         "blocks": len(blocks), "edge_sizes": edges, "sr_voxels": n_sr, "sr_voxel_frames": sr_frames,
-        "tile_depth": tz, "autotune_sample_ms": {str(k): v for k, v in tune.items()},
+        "groups_mag_tile_depth": groups, "autotune_sample_ms_largest_group": {str(k): v for k, v in tune.items()},
         "batched": {"ms": wall_b, "kernel_ms": kb, "sr_voxels_per_s": n_sr / (wall_b * 1e-3)},
         "per_block": {"ms": wall_pb, "kernel_ms": ms_pb, "sr_voxels_per_s": n_sr / (wall_pb * 1e-3)},
         "speedup_batched_vs_per_block": wall_pb / wall_b,

@@ -150,29 +150,38 @@ typedef struct rfba_srrf_block {
 
 typedef struct rfba_srrf_batch rfba_srrf_batch; /* opaque plan */
 
-/* Build a plan (work queue, tables, autotuned tile shape) for n blocks that
- * share dtype, conf (Method, smoothing, ...) and samples.  The plan owns its
- * device tables and is reusable while the block list is unchanged. */
+/* Build a plan for n blocks that share dtype, conf (Method, smoothing, ...)
+ * and samples.  Blocks are grouped by magnification; each group gets its own
+ * tile depth, LDS layout and persistent launch (the launches run back to back
+ * on one stream, one hipGraph).  The plan owns its device tables and is
+ * reusable while the block list is unchanged. */
 int rfba_srrf_batch_create(const rfba_srrf_block *blocks, int n, int dtype,
                            const SRRFConf *conf,
                            const rfba_srrf_samples *samples,
                            rfba_srrf_batch **plan);
-/* Run: one persistent launch over all blocks (captured into a hipGraph on
- * the first call when use_graph != 0, replayed afterwards). */
+/* Run: one persistent launch per magnification group over all its blocks
+ * (captured into a hipGraph on the first call when use_graph != 0, replayed
+ * afterwards). */
 int rfba_srrf_batch_run(rfba_srrf_batch *plan, void *stream, int use_graph,
                         float *kernel_ms);
 /* Reference form for comparison: one launch per block on the same stream. */
 int rfba_srrf_batch_run_per_block(rfba_srrf_batch *plan, void *stream,
                                   float *kernel_ms);
-/* Autotuner (no reference counterpart): times every feasible tile depth
- * (4, 8, 16 SR voxels along z) on a sub-plan of every sample_stride-th block
- * (0: about 64 blocks), keeps the fastest and rebuilds the plan for it.
- * *tz gets the depth, ms_by_tz[3] the sample times (0 = not feasible);
- * either may be NULL.  Synchronous on stream. */
+/* Autotuner (no reference counterpart): per magnification group, times
+ * every feasible tile depth (4, 8, 16 SR voxels along z) on a sub-plan of
+ * every sample_stride-th block of the group (0: about 32 blocks), keeps the
+ * fastest and rebuilds the group for it.  *tz and ms_by_tz[3] (sample times
+ * at depth 4, 8, 16; 0 = not feasible) describe the group with the most
+ * work; either may be NULL.  Writes the sampled blocks' outputs.
+ * Synchronous on stream. */
 int rfba_srrf_batch_autotune(rfba_srrf_batch *plan, void *stream,
                              int sample_stride, int *tz, float *ms_by_tz);
-/* Tile depth the plan currently uses. */
+/* Tile depth of the group with the most work. */
 int rfba_srrf_batch_tile_depth(const rfba_srrf_batch *plan);
+/* Magnification and tile depth of each group (largest work first), up to
+ * max_groups entries; returns the number of groups. */
+int rfba_srrf_batch_groups(const rfba_srrf_batch *plan, int *mags,
+                           int *tile_depths, int max_groups);
 void rfba_srrf_batch_destroy(rfba_srrf_batch *plan);
 
 /* ---- runtime helpers ---------------------------------------------------- */

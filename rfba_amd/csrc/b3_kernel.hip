@@ -71,9 +71,10 @@ __global__ __launch_bounds__(256) void b3_resid(const DevBlock *__restrict__ blo
         __syncthreads();
         const float *dt = b.data + (size_t)tk.t * P;
         float *Rt = R + tk.rslot + (size_t)tk.t * P;
-        // FP32 spot images, FP64 accumulation, one rounding of the residual
-        // (summing b0 + images in FP32 would drop the far tails of every
-        // spot below half an ulp of b0: a one-sided bias).
+        // FP32 spot images summed in FP32 (relative to their own sum), then
+        // data - b0 - sum in FP64 and one rounding of the residual (adding
+        // b0 into the FP32 sum would drop the far tails of every spot below
+        // half an ulp of b0: a one-sided bias).
         // threads over (x,y) column x 8-deep z runs: the spot's xy factor is
         // formed once per run; the product order ((amp gx) gy) gz is kept.
         double ss = 0.0;
@@ -83,21 +84,22 @@ __global__ __launch_bounds__(256) void b3_resid(const DevBlock *__restrict__ blo
             const int zc = e / XY, col = e - zc * XY;
             const int y = col / X, x = col - y * X;
             const int z0 = zc * RZ;
-            double v[RZ];
+            float m[RZ];
 #pragma unroll
-            for (int u = 0; u < RZ; u++) v[u] = z0 + u < Z ? (double)dt[(size_t)(z0 + u) * XY + col] - b.b0 : 0.0;
+            for (int u = 0; u < RZ; u++) m[u] = 0.0f;
             for (int j = 0; j < K; j++) {
                 if (!on[j]) continue;
                 const float *gj = g + j * n_ax;
                 const float a = amp[j] * gj[x] * gj[X + y];
                 const float *gz = gj + X + Y + z0;  // may run past spot j's z entries: padded, never stored
 #pragma unroll
-                for (int u = 0; u < RZ; u++) v[u] -= (double)(a * gz[u]);
+                for (int u = 0; u < RZ; u++) m[u] += a * gz[u];
             }
 #pragma unroll
             for (int u = 0; u < RZ; u++)
                 if (z0 + u < Z) {
-                    const float rf = (float)v[u];
+                    const double v = ((double)dt[(size_t)(z0 + u) * XY + col] - b.b0) - (double)m[u];
+                    const float rf = (float)v;
                     Rt[(size_t)(z0 + u) * XY + col] = rf;
                     ss += (double)rf * (double)rf;
                 }
@@ -382,7 +384,7 @@ __global__ __launch_bounds__(256) void b3_eval(EvalParams E) {
 // (lane l holds A[l%16][l/16] = R[frame][pixel] and B[l/16][l%16] =
 // basis[pixel][column]; result rows 4(l/16)+r, column l%16).  Passes of 128
 // frames; wave w owns frame tiles 2w, 2w+1 (16 frames each) x all column
-// tiles; FP32 matrix accumulation over two 32-pixel chunks, FP64 beyond.
+// tiles; FP32 matrix accumulation over four 32-pixel chunks, FP64 beyond.
 // Each lane writes the em fields of its own (frame, column) results.
 typedef float v4f __attribute__((ext_vector_type(4)));
 
@@ -569,7 +571,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                     for (int ct = 0; ct < NCT; ct++)
                         acc[f2][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[f2], bv[ct], acc[f2][ct], 0, 0, 0);
             }
-            if ((nch & 1) || p0 + kCorrPix >= P) {
+            if ((nch & 3) == 3 || p0 + kCorrPix >= P) {
 #pragma unroll
                 for (int f2 = 0; f2 < 2; f2++)
 #pragma unroll

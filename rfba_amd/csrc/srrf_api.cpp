@@ -727,37 +727,55 @@ extern "C" int SRRFGradientCalc(float *inImage, int *imageDim, float *outImage) 
 // ---------------------------------------------------------------------------
 // batched form
 
+// Blocks are grouped by magnification: each group has its own tile depth,
+// LDS layout, kernel and grid (a mixed batch planned as one would size every
+// tile's LDS for the smallest magnification's halo and lose occupancy).  The
+// groups' persistent launches run back to back on one stream (one graph).
 struct rfba_srrf_batch {
     SRRFConf conf;
     int dtype = 0;
-    Plan pl;
-    KernelFn fn = nullptr;
-    int grid = 0;
-    std::vector<DevItem> items;
+    std::vector<DevItem> items;  // all blocks, caller order
+    DevSamples smp;
     DevSamples *d_smp = nullptr;
     DevItem *d_items = nullptr;
-    DevTile *d_tiles = nullptr;   // queue order
-    DevTile *d_tiles_by_item = nullptr;
-    int *d_ctl = nullptr;
-    float *d_scratch = nullptr;
-    long long scratch_stride = 0;
+    int *d_ctl = nullptr;        // [2 per group]: queue counter, error flag
+    float *d_scratch = nullptr;  // TRAC/TRPPM series, shared by the groups' launches
+    size_t scratch_bytes = 0;
+    struct Group {
+        int M = 0;
+        std::vector<int> ids;         // global item indices
+        std::vector<DevItem> gitems;  // the group's items (planning)
+        Plan pl;
+        KernelFn fn = nullptr;
+        int grid = 0;
+        long long scratch_stride = 0;
+        DevTile *d_tiles = nullptr;          // queue order
+        DevTile *d_tiles_by_item = nullptr;  // per block, for the per-block form
+        double work = 0.0;                   // frames x SR voxels
+        float tune_ms[3] = {0, 0, 0};        // sample time at TZ 4, 8, 16 (0: not timed)
+    };
+    std::vector<Group> groups;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    hipStream_t graph_stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    DevSamples smp;
-    int tuned_tz = 0;
-    float tune_ms[3] = {0, 0, 0};  // sample time at TZ 4, 8, 16 (0: not feasible)
 };
+
+static void batch_drop_graph(rfba_srrf_batch *b) {
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
 
 static void batch_free(rfba_srrf_batch *b) {
     if (!b) return;
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
+    batch_drop_graph(b);
+    for (auto &g : b->groups) {
+        if (g.d_tiles) hipFree(g.d_tiles);
+        if (g.d_tiles_by_item) hipFree(g.d_tiles_by_item);
+    }
     if (b->d_smp) hipFree(b->d_smp);
     if (b->d_items) hipFree(b->d_items);
-    if (b->d_tiles) hipFree(b->d_tiles);
-    if (b->d_tiles_by_item) hipFree(b->d_tiles_by_item);
     if (b->d_ctl) hipFree(b->d_ctl);
     if (b->d_scratch) hipFree(b->d_scratch);
     if (b->ev0) hipEventDestroy(b->ev0);
@@ -765,44 +783,74 @@ static void batch_free(rfba_srrf_batch *b) {
     delete b;
 }
 
-// (Re)build the device side of a plan for the tile depth in pl: tile
-// queue, kernel, grid, TRAC/TRPPM series scratch.  Drops a captured graph.
-static int batch_build(rfba_srrf_batch *b, const DevSamples &s, const Plan &geom) {
+static int realloc_dev(void **p, size_t bytes) {
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    HIP_TRY(hipMalloc(p, std::max<size_t>(bytes, 16)));
+    return RFBA_OK;
+}
+
+// (Re)build one group's device side for the tile depth in geom: tile queue
+// (tiles name global item indices), kernel, grid, series scratch.
+static int group_build(rfba_srrf_batch *b, rfba_srrf_batch::Group &g, const Plan &geom) {
     int rc;
-    if (b->exec) hipGraphExecDestroy(b->exec);
-    if (b->graph) hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
-    b->smp = s;
-    b->pl = geom;
-    plan_tiles(b->items, b->pl.TZ, b->pl);
-    b->fn = pick_kernel(b->dtype, b->pl.TZ, s);
-    if ((rc = occupancy_grid(b->fn, b->pl.lds_bytes, (int)b->pl.tiles.size(), b->grid))) return rc;
-    auto realloc_dev = [&](void **p, size_t bytes) -> int {
-        if (*p) hipFree(*p);
-        *p = nullptr;
-        HIP_TRY(hipMalloc(p, std::max<size_t>(bytes, 16)));
-        return RFBA_OK;
-    };
-    const size_t nt = b->pl.tiles.size();
+    batch_drop_graph(b);
+    g.pl = geom;
+    plan_tiles(g.gitems, g.pl.TZ, g.pl);
+    for (auto &t : g.pl.tiles) t.item = g.ids[t.item];
+    for (auto &t : g.pl.tiles_by_item) t.item = g.ids[t.item];
+    g.fn = pick_kernel(b->dtype, g.pl.TZ, b->smp);
+    if ((rc = occupancy_grid(g.fn, g.pl.lds_bytes, (int)g.pl.tiles.size(), g.grid))) return rc;
+    const size_t nt = g.pl.tiles.size();
+    if ((rc = realloc_dev((void **)&g.d_tiles, nt * sizeof(DevTile))) ||
+        (rc = realloc_dev((void **)&g.d_tiles_by_item, nt * sizeof(DevTile))))
+        return rc;
+    HIP_TRY(hipMemcpy(g.d_tiles, g.pl.tiles.data(), nt * sizeof(DevTile), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(g.d_tiles_by_item, g.pl.tiles_by_item.data(), nt * sizeof(DevTile), hipMemcpyHostToDevice));
+    g.scratch_stride = 0;
+    if (needs_series(b->conf)) {
+        g.scratch_stride = (long long)g.pl.Tmax * g.pl.TZ * kThreads;
+        const size_t need = (size_t)g.scratch_stride * g.grid * sizeof(float);
+        if (need > b->scratch_bytes) {
+            if ((rc = realloc_dev((void **)&b->d_scratch, need))) return rc;
+            b->scratch_bytes = need;
+        }
+    }
+    return RFBA_OK;
+}
+
+// Device tables shared by the groups: samples, items, per-group control words.
+static int batch_tables(rfba_srrf_batch *b) {
+    int rc;
     if ((rc = realloc_dev((void **)&b->d_smp, sizeof(DevSamples))) ||
         (rc = realloc_dev((void **)&b->d_items, b->items.size() * sizeof(DevItem))) ||
-        (rc = realloc_dev((void **)&b->d_tiles, nt * sizeof(DevTile))) ||
-        (rc = realloc_dev((void **)&b->d_tiles_by_item, nt * sizeof(DevTile))) ||
-        (rc = realloc_dev((void **)&b->d_ctl, 2 * sizeof(int))))
+        (rc = realloc_dev((void **)&b->d_ctl, 2 * b->groups.size() * sizeof(int))))
         return rc;
-    if (b->d_scratch) hipFree(b->d_scratch);
-    b->d_scratch = nullptr;
-    b->scratch_stride = 0;
-    if (needs_series(b->conf)) {
-        b->scratch_stride = (long long)b->pl.Tmax * b->pl.TZ * kThreads;
-        if ((rc = realloc_dev((void **)&b->d_scratch, (size_t)b->scratch_stride * b->grid * sizeof(float)))) return rc;
-    }
-    HIP_TRY(hipMemcpy(b->d_smp, &s, sizeof s, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(b->d_smp, &b->smp, sizeof b->smp, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(b->d_items, b->items.data(), b->items.size() * sizeof(DevItem), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(b->d_tiles, b->pl.tiles.data(), nt * sizeof(DevTile), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(b->d_tiles_by_item, b->pl.tiles_by_item.data(), nt * sizeof(DevTile), hipMemcpyHostToDevice));
     return RFBA_OK;
+}
+
+// Group the items by magnification, largest work first.
+static void batch_group(rfba_srrf_batch *b) {
+    b->groups.clear();
+    std::vector<int> mags;
+    for (const auto &it : b->items)
+        if (std::find(mags.begin(), mags.end(), it.M) == mags.end()) mags.push_back(it.M);
+    for (int M : mags) {
+        rfba_srrf_batch::Group g;
+        g.M = M;
+        for (size_t i = 0; i < b->items.size(); i++)
+            if (b->items[i].M == M) {
+                g.ids.push_back((int)i);
+                g.gitems.push_back(b->items[i]);
+                const DevItem &it = b->items[i];
+                g.work += (double)it.T * (double)(it.bx1 - it.bx0) * (it.by1 - it.by0) * (it.bz1 - it.bz0);
+            }
+        b->groups.push_back(std::move(g));
+    }
+    std::stable_sort(b->groups.begin(), b->groups.end(),
+                     [](const rfba_srrf_batch::Group &x, const rfba_srrf_batch::Group &y) { return x.work > y.work; });
 }
 
 extern "C" int rfba_srrf_batch_create(const rfba_srrf_block *blocks, int n, int dtype, const SRRFConf *conf,
@@ -820,6 +868,7 @@ extern "C" int rfba_srrf_batch_create(const rfba_srrf_block *blocks, int n, int 
     rfba_srrf_batch *b = new rfba_srrf_batch();
     b->conf = *conf;
     b->dtype = dtype;
+    b->smp = s;
     for (int i = 0; i < n; i++) {
         const rfba_srrf_block &bl = blocks[i];
         for (int a = 0; a < 3; a++)
@@ -846,7 +895,15 @@ extern "C" int rfba_srrf_batch_create(const rfba_srrf_block *blocks, int n, int 
         it.bx1 = bl.dims[0] * bl.mag; it.by1 = bl.dims[1] * bl.mag; it.bz1 = bl.dims[2] * bl.mag;
         b->items.push_back(it);
     }
-    if ((rc = choose_tz(b->items, s, *conf, b->pl, 0)) || (rc = batch_build(b, s, b->pl))) {
+    batch_group(b);
+    for (auto &g : b->groups) {
+        Plan p;
+        if ((rc = choose_tz(g.gitems, s, *conf, p, 0)) || (rc = group_build(b, g, p))) {
+            batch_free(b);
+            return rc;
+        }
+    }
+    if ((rc = batch_tables(b))) {
         batch_free(b);
         return rc;
     }
@@ -861,32 +918,38 @@ extern "C" int rfba_srrf_batch_create(const rfba_srrf_block *blocks, int n, int 
     return RFBA_OK;
 }
 
-static LaunchParams batch_params(rfba_srrf_batch *b, const DevTile *tiles, int ntiles) {
-    LaunchParams P = make_params(b->conf, b->pl);
+static LaunchParams batch_params(rfba_srrf_batch *b, const rfba_srrf_batch::Group &g, int gi, const DevTile *tiles,
+                                 int ntiles) {
+    LaunchParams P = make_params(b->conf, g.pl);
     P.smp = b->d_smp;
     P.items = b->d_items;
     P.tiles = tiles;
     P.ntiles = ntiles;
-    P.counter = b->d_ctl;
-    P.err = b->d_ctl + 1;
+    P.counter = b->d_ctl + 2 * gi;
+    P.err = b->d_ctl + 2 * gi + 1;
     P.scratch = b->d_scratch;
-    P.scratch_stride = b->scratch_stride;
+    P.scratch_stride = g.scratch_stride;
     return P;
 }
 
 static int batch_enqueue(rfba_srrf_batch *b, hipStream_t st) {
-    LaunchParams P = batch_params(b, b->d_tiles, (int)b->pl.tiles.size());
-    hipLaunchKernelGGL(srrf_reset, dim3(1), dim3(64), 0, st, P.counter, P.err);
-    HIP_TRY(hipGetLastError());
-    hipLaunchKernelGGL(b->fn, dim3(b->grid), dim3(kThreads), b->pl.lds_bytes, st, P);
-    HIP_TRY(hipGetLastError());
+    for (size_t gi = 0; gi < b->groups.size(); gi++) {
+        const auto &g = b->groups[gi];
+        if (g.pl.tiles.empty()) continue;
+        LaunchParams P = batch_params(b, g, (int)gi, g.d_tiles, (int)g.pl.tiles.size());
+        hipLaunchKernelGGL(srrf_reset, dim3(1), dim3(64), 0, st, P.counter, P.err);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(g.fn, dim3(g.grid), dim3(kThreads), g.pl.lds_bytes, st, P);
+        HIP_TRY(hipGetLastError());
+    }
     return RFBA_OK;
 }
 
 static int batch_check(rfba_srrf_batch *b) {
-    int ctl[2] = {0, 0};
-    HIP_TRY(hipMemcpy(ctl, b->d_ctl, sizeof ctl, hipMemcpyDeviceToHost));
-    if (ctl[1]) return fail(RFBA_EDEVICE, "tile geometry exceeded the planned LDS extents");
+    std::vector<int> ctl(2 * b->groups.size());
+    HIP_TRY(hipMemcpy(ctl.data(), b->d_ctl, ctl.size() * sizeof(int), hipMemcpyDeviceToHost));
+    for (size_t gi = 0; gi < b->groups.size(); gi++)
+        if (ctl[2 * gi + 1]) return fail(RFBA_EDEVICE, "tile geometry exceeded the planned LDS extents");
     return RFBA_OK;
 }
 
@@ -930,15 +993,18 @@ extern "C" int rfba_srrf_batch_run_per_block(rfba_srrf_batch *b, void *stream, f
     std::lock_guard<std::mutex> lk(g_ctx.mu);
     hipStream_t st = (hipStream_t)stream;
     HIP_TRY(hipEventRecord(b->ev0, st));
-    for (size_t i = 0; i < b->items.size(); i++) {
-        const auto &r = b->pl.item_tiles[i];
-        if (r.second == 0) continue;
-        LaunchParams P = batch_params(b, b->d_tiles_by_item + r.first, r.second);
-        const int grid = std::min(b->grid, r.second);
-        hipLaunchKernelGGL(srrf_reset, dim3(1), dim3(64), 0, st, P.counter, P.err);
-        HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL(b->fn, dim3(grid), dim3(kThreads), b->pl.lds_bytes, st, P);
-        HIP_TRY(hipGetLastError());
+    for (size_t gi = 0; gi < b->groups.size(); gi++) {
+        const auto &g = b->groups[gi];
+        for (size_t i = 0; i < g.pl.item_tiles.size(); i++) {
+            const auto &r = g.pl.item_tiles[i];
+            if (r.second == 0) continue;
+            LaunchParams P = batch_params(b, g, (int)gi, g.d_tiles_by_item + r.first, r.second);
+            const int grid = std::min(g.grid, r.second);
+            hipLaunchKernelGGL(srrf_reset, dim3(1), dim3(64), 0, st, P.counter, P.err);
+            HIP_TRY(hipGetLastError());
+            hipLaunchKernelGGL(g.fn, dim3(grid), dim3(kThreads), g.pl.lds_bytes, st, P);
+            HIP_TRY(hipGetLastError());
+        }
     }
     HIP_TRY(hipEventRecord(b->ev1, st));
     if (kernel_ms) {
@@ -949,58 +1015,88 @@ extern "C" int rfba_srrf_batch_run_per_block(rfba_srrf_batch *b, void *stream, f
     return RFBA_OK;
 }
 
-// Autotuner: each feasible tile depth (geometry planned over all blocks, so
-// that the LDS layout is the one the full run would use) is timed on a
-// sub-plan of every sample_stride-th block, one warm run then one timed run;
-// the fastest depth is kept and the plan rebuilt for it.
+// Autotuner: for each magnification group, each feasible tile depth
+// (geometry planned over all of the group's blocks, so that the LDS layout
+// is the one the full run uses) is timed on a one-group sub-plan of every
+// sample_stride-th block of the group, one warm run then one timed run; the
+// fastest depth is kept and the group rebuilt for it.  The sampled blocks'
+// outputs are written (with their correct values).
 extern "C" int rfba_srrf_batch_autotune(rfba_srrf_batch *b, void *stream, int sample_stride, int *tz,
                                         float *ms_by_tz) {
     if (!b) return fail(RFBA_EINVAL, "plan is NULL");
     std::lock_guard<std::mutex> lk(g_ctx.mu);
     hipStream_t st = (hipStream_t)stream;
-    const int n = (int)b->items.size();
-    const int stride = sample_stride > 0 ? sample_stride : std::max(1, n / 64);
-    rfba_srrf_batch *sub = new rfba_srrf_batch();
-    sub->conf = b->conf;
-    sub->dtype = b->dtype;
-    for (int i = 0; i < n; i += stride) sub->items.push_back(b->items[i]);
     const int cand[3] = {4, 8, 16};
-    Plan best;
-    float best_ms = 0.0f;
     int rc = RFBA_OK;
-    for (int c = 0; c < 3; c++) {
-        b->tune_ms[c] = 0.0f;
-        Plan p;
-        if (plan_geometry(b->items, b->smp, b->conf, cand[c], p) != RFBA_OK) continue;
-        if ((rc = batch_build(sub, b->smp, p))) break;
-        float ms = 0.0f;
-        for (int rep = 0; rep < 2 && rc == RFBA_OK; rep++) {
-            if ((rc = hip_status(hipEventRecord(b->ev0, st)))) break;
-            if ((rc = batch_enqueue(sub, st))) break;
-            if ((rc = hip_status(hipEventRecord(b->ev1, st)))) break;
-            if ((rc = hip_status(hipEventSynchronize(b->ev1)))) break;
-            if ((rc = hip_status(hipEventElapsedTime(&ms, b->ev0, b->ev1)))) break;
-            rc = batch_check(sub);
+    for (auto &g : b->groups) {
+        const int n = (int)g.ids.size();
+        const int stride = sample_stride > 0 ? sample_stride : std::max(1, n / 32);
+        rfba_srrf_batch *sub = new rfba_srrf_batch();
+        sub->conf = b->conf;
+        sub->dtype = b->dtype;
+        sub->smp = b->smp;
+        rfba_srrf_batch::Group sg;
+        sg.M = g.M;
+        for (int i = 0; i < n; i += stride) {
+            sg.ids.push_back((int)sub->items.size());
+            sg.gitems.push_back(g.gitems[i]);
+            sub->items.push_back(g.gitems[i]);
         }
-        if (rc) break;
-        b->tune_ms[c] = ms;
-        if (best_ms == 0.0f || ms < best_ms) {
-            best_ms = ms;
-            best = p;
+        sub->groups.push_back(std::move(sg));
+        Plan best;
+        float best_ms = 0.0f;
+        if ((rc = batch_tables(sub))) {
+            batch_free(sub);
+            return rc;
         }
+        for (int c = 0; c < 3 && rc == RFBA_OK; c++) {
+            g.tune_ms[c] = 0.0f;
+            Plan p;
+            if (plan_geometry(g.gitems, b->smp, b->conf, cand[c], p) != RFBA_OK) continue;
+            if ((rc = group_build(sub, sub->groups[0], p))) break;
+            float ms = 0.0f;
+            for (int rep = 0; rep < 2 && rc == RFBA_OK; rep++) {
+                if ((rc = hip_status(hipEventRecord(b->ev0, st)))) break;
+                if ((rc = batch_enqueue(sub, st))) break;
+                if ((rc = hip_status(hipEventRecord(b->ev1, st)))) break;
+                if ((rc = hip_status(hipEventSynchronize(b->ev1)))) break;
+                if ((rc = hip_status(hipEventElapsedTime(&ms, b->ev0, b->ev1)))) break;
+                rc = batch_check(sub);
+            }
+            if (rc) break;
+            g.tune_ms[c] = ms;
+            if (best_ms == 0.0f || ms < best_ms) {
+                best_ms = ms;
+                best = p;
+            }
+        }
+        batch_free(sub);
+        if (rc) return rc;
+        if (best_ms == 0.0f) return fail(RFBA_EINVAL, "no feasible tile depth for magnification %d", g.M);
+        if ((rc = group_build(b, g, best))) return rc;
     }
-    batch_free(sub);
-    if (rc) return rc;
-    if (best_ms == 0.0f) return fail(RFBA_EINVAL, "no feasible tile depth");
-    if ((rc = batch_build(b, b->smp, best))) return rc;
-    b->tuned_tz = best.TZ;
-    if (tz) *tz = best.TZ;
-    if (ms_by_tz)
-        for (int c = 0; c < 3; c++) ms_by_tz[c] = b->tune_ms[c];
+    if (!b->groups.empty()) {
+        const auto &g0 = b->groups[0];  // the group with the most work
+        if (tz) *tz = g0.pl.TZ;
+        if (ms_by_tz)
+            for (int c = 0; c < 3; c++) ms_by_tz[c] = g0.tune_ms[c];
+    }
     return RFBA_OK;
 }
 
-extern "C" int rfba_srrf_batch_tile_depth(const rfba_srrf_batch *b) { return b ? b->pl.TZ : 0; }
+extern "C" int rfba_srrf_batch_tile_depth(const rfba_srrf_batch *b) {
+    return (b && !b->groups.empty()) ? b->groups[0].pl.TZ : 0;
+}
+
+extern "C" int rfba_srrf_batch_groups(const rfba_srrf_batch *b, int *mags, int *tile_depths, int max_groups) {
+    if (!b) return fail(RFBA_EINVAL, "plan is NULL");
+    const int n = (int)b->groups.size();
+    for (int i = 0; i < n && i < max_groups; i++) {
+        if (mags) mags[i] = b->groups[i].M;
+        if (tile_depths) tile_depths[i] = b->groups[i].pl.TZ;
+    }
+    return n;
+}
 
 extern "C" void rfba_srrf_batch_destroy(rfba_srrf_batch *b) {
     std::lock_guard<std::mutex> lk(g_ctx.mu);

@@ -115,6 +115,8 @@ def lib() -> ctypes.CDLL:
         L.rfba_srrf_batch_autotune.restype = c_int
         L.rfba_srrf_batch_tile_depth.argtypes = [c_void_p]
         L.rfba_srrf_batch_tile_depth.restype = c_int
+        L.rfba_srrf_batch_groups.argtypes = [c_void_p, POINTER(c_int), POINTER(c_int), c_int]
+        L.rfba_srrf_batch_groups.restype = c_int
         L.rfba_srrf_batch_destroy.argtypes = [c_void_p]
         L.rfba_srrf_batch_destroy.restype = None
         L.rfba_last_error.argtypes = []
@@ -353,8 +355,9 @@ class Batch:
         return ms.value
 
     def autotune(self, sample_stride: int = 0):
-        """Time tile depths 4/8/16 on a sample of blocks, keep the fastest.
-        Returns (depth, {depth: sample ms})."""
+        """Per magnification group, time tile depths 4/8/16 on a sample of its
+        blocks and keep the fastest.  Returns (depth, {depth: sample ms}) of
+        the group with the most work."""
         tz = c_int(0)
         ms = (c_float * 3)()
         _check(lib().rfba_srrf_batch_autotune(self.plan, None, sample_stride, ctypes.byref(tz), ms))
@@ -363,6 +366,13 @@ class Batch:
     @property
     def tile_depth(self) -> int:
         return lib().rfba_srrf_batch_tile_depth(self.plan)
+
+    @property
+    def groups(self):
+        """[(magnification, tile depth)] per group, largest work first."""
+        m, t = (c_int * 64)(), (c_int * 64)()
+        n = lib().rfba_srrf_batch_groups(self.plan, m, t, 64)
+        return [(m[i], t[i]) for i in range(min(n, 64))]
 
     def close(self):
         if self.plan:

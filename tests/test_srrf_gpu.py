@@ -197,9 +197,10 @@ def test_autotuned_batch_matches_single_stacks(gpu):
     b = srrf.Batch(blocks, srrf.RFBA_U16, c, smp)
     b.run(use_graph=True)  # a graph captured before tuning must be dropped
     tz, ms = b.autotune(sample_stride=2)
-    print("autotune: depth %d, sample ms %s" % (tz, ms))
+    print("autotune: depth %d, sample ms %s, groups %s" % (tz, ms, b.groups))
     assert tz in ms and ms[tz] == min(ms.values()) and b.tile_depth == tz
     assert set(ms) <= {4, 8, 16} and 8 in ms
+    assert sorted(m for m, _ in b.groups) == sorted(set(bl.mag for bl in blocks))
     for mode in ("graph", "per_block"):
         for bl in blocks:
             bl.out.zero()
