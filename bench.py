@@ -44,6 +44,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "3D SRRF SR-voxels/s + %HBM roofline; 3B batched logL+grad evals/s vs plain"
+# HBM bytes per launch of srrf_fused on the headline workload (T=200), from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration
+# (profiles/srrf_fused_full_volume_256x256x96_T200.md).  Dominated by the TRAC
+# per-voxel series written and read back (8 B per SR voxel-frame).
+HEADLINE_TRAFFIC_BYTES = 9.089e11
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
@@ -120,7 +125,8 @@ def bench_stack(args, rec):
     alg_flops = float(T) * n_sr * (K * 430 + 128 * conf.IntensityW + 6)
     tflops = alg_flops / (kernel_ms * 1e-3) / 1e12
     roof = hbm_roof(alg_bytes, kernel_ms)
-    roof.update({"traffic": args.traffic, "kernel": "srrf_fused (uint16 input)", "kernel_ms": kernel_ms,
+    traffic = args.traffic if args.traffic is not None else (HEADLINE_TRAFFIC_BYTES if T == 200 else None)
+    roof.update({"traffic": traffic, "kernel": "srrf_fused (uint16 input)", "kernel_ms": kernel_ms,
                  "algorithmic_bytes_per_launch": alg_bytes,
                  "valu_fp32": {"achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                                "frac": tflops / FP32_PEAK_TFLOPS, "algorithmic_flops_per_launch": alg_flops}})
@@ -260,18 +266,32 @@ def b3_model(b3, ckpt):
 
 
 def b3_alg(blocks):
-    """(bytes, flops) per batched launch: data + residual planes read and
-    written once, R read once per spot; 13 FP32 flops per pixel-frame per
-    (q, spot) item and 30 per pixel of the spot tables."""
+    """(bytes, flops) per batched launch sequence, as algorithmic counts:
+    bytes = the block data plus S background planes, 4 T P (1 + S) per block
+    (what a fused evaluation must read); flops = 13 FP32 flops per
+    (pixel, frame) of every (q, spot, sample) item plus 30 per pixel of its
+    spot table."""
     by, fl = 0.0, 0.0
     for b in blocks:
         T, Z, Y, X = b.data.shape
         Q, K, _ = b.params.shape
         S = b.states.shape[0]
         P = X * Y * Z
-        by += 4.0 * T * P * (1 + Q * S * (2 + K))
+        by += 4.0 * T * P * (1 + S)
         fl += float(Q) * K * S * (13.0 * T * P + 30.0 * P)
     return by, fl
+
+
+def b3_roof(by, fl, kernel_ms):
+    """The correlation kernel runs on the FP32 matrix cores: the binding roof
+    is their peak; the HBM figure is reported beside it."""
+    tf = fl / (kernel_ms * 1e-3) / 1e12
+    roof = {"bound": "mfma", "achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": tf / FP32_PEAK_TFLOPS, "kernel": "b3_resid + b3_corr_mfma + b3_hmm (one launch sequence)",
+            "kernel_ms": kernel_ms, "algorithmic_flops_per_launch": fl}
+    roof["hbm"] = hbm_roof(by, kernel_ms)
+    roof["hbm"]["algorithmic_bytes_per_launch"] = by
+    return roof
 
 
 def bench_b3_single(args):
@@ -299,10 +319,7 @@ def bench_b3_single(args):
     log("c4: batched %.2f ms, per-parameter-set %.2f ms" % (wall_b, wall_pb))
     by, fl = b3_alg([blk])
     kb = float(np.mean(kms))
-    roof = hbm_roof(by, kb)
-    roof.update({"kernel": "b3_resid + b3_eval<grad>", "kernel_ms": kb, "algorithmic_bytes_per_launch": by,
-                 "valu_fp32": {"achieved": fl / (kb * 1e-3) / 1e12, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                               "frac": fl / (kb * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, "algorithmic_flops_per_launch": fl}})
+    roof = b3_roof(by, fl, kb)
     res = {
         "workload": "3B logL + adjoint, one 29^3 block, T=100, 40 spots (seed 6), 256 parameter sets (+-0.5 px, "
                     "+-20%% intensity, seed 7), S=1, checkpoints every ceil(sqrt(T)) frames",
@@ -378,10 +395,7 @@ def bench_b3_batched(args):
         % (wall_b, wall_sub, wall_pb, nq, wall_pq))
     by, fl = b3_alg(blocks)
     kb = float(np.mean(kms))
-    roof = hbm_roof(by, kb)
-    roof.update({"kernel": "b3_resid + b3_eval<grad>", "kernel_ms": kb, "algorithmic_bytes_per_launch": by,
-                 "valu_fp32": {"achieved": fl / (kb * 1e-3) / 1e12, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                               "frac": fl / (kb * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, "algorithmic_flops_per_launch": fl}})
+    roof = b3_roof(by, fl, kb)
     n_ev = len(blocks) * Q
     res = {
         "workload": "batched 3B logL + adjoint: 1200 blocks of the 512x512x128 doSplit (29^3, 3-voxel overlap; "

@@ -196,6 +196,18 @@ int rfba_device_synchronize(void);
 /* Plain device-to-device copy kernel of `bytes` (multiple of 16): the
  * reference point used to calibrate HBM byte counters. */
 int rfba_copy_probe(void *d_dst, const void *d_src, size_t bytes, float *ms);
+
+/* ---- caller post-processing and the SRRF.raw file ------------------------
+ * The reference's caller clamps the SR volume (NaN -> 0, negative -> 0) and
+ * keeps its maximum, starting from 1 (multispot5_place_methods.cc:260-269);
+ * then writes it as raw float32, x fastest, to SRRF.raw (:270-274) and reads
+ * it back in split mode (:284-289). */
+int rfba_srrf_postprocess_device(float *d_out, size_t n, void *stream,
+                                 float *fmax);
+int rfba_srrf_write_raw(const char *path, const float *data, size_t n);
+/* Fails (RFBA_EINVAL) when the file holds fewer than n floats; the
+ * reference's fread would leave the tail of the buffer unread. */
+int rfba_srrf_read_raw(const char *path, float *data, size_t n);
 /* Release the library's cached device scratch (kept between calls). */
 void rfba_release_cache(void);
 

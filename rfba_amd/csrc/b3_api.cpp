@@ -177,8 +177,8 @@ static CorrFn corr_fn(bool grad) { return grad ? b3_corr_mfma<true> : b3_corr_mf
 static HmmFn hmm_fn(bool grad) { return grad ? b3_hmm<true> : b3_hmm<false>; }
 
 static size_t corr_lds(bool grad, int naxmax) {
-    // Q/H | residual chunk [128][36] | basis chunk [32][16-column tiles + 4] | axis tables
-    const size_t bmat = (size_t)kCorrPix * (((grad ? 48 : 8) + 15) / 16 * 16 + 4);
+    // Q/H | residual chunk [128][36] | basis chunk [32][16-column tiles] | axis tables
+    const size_t bmat = (size_t)kCorrPix * (((grad ? 48 : 8) + 15) / 16 * 16);
     return sizeof(double) * kCorrSpots * 6 +
            sizeof(float) * ((size_t)kCorrFrames * kCorrRow + bmat + (size_t)kCorrSpots * naxmax * 4);
 }
@@ -358,7 +358,7 @@ static int enqueue_range(rfba3b_batch *b, hipStream_t st, const rfba3b_batch::Ch
         C.S0 = b->d_S0;
         C.model = b->d_model;
         C.em = b->d_em;
-        hipLaunchKernelGGL(corr_fn(b->grad), dim3(ncorr), dim3(256), b->lds_corr, st, C);
+        hipLaunchKernelGGL(corr_fn(b->grad), dim3((ncorr + 7) / 8 * 8), dim3(256), b->lds_corr, st, C);
         HIP_TRY(hipGetLastError());
     }
     if (nitem > 0) {

@@ -84,9 +84,13 @@ __global__ __launch_bounds__(256) void b3_resid(const DevBlock *__restrict__ blo
             const int zc = e / XY, col = e - zc * XY;
             const int y = col / X, x = col - y * X;
             const int z0 = zc * RZ;
-            float m[RZ];
+            // data first: the loads are in flight during the spot sum
+            float dv[RZ], m[RZ];
 #pragma unroll
-            for (int u = 0; u < RZ; u++) m[u] = 0.0f;
+            for (int u = 0; u < RZ; u++) {
+                dv[u] = dt[(size_t)min(z0 + u, Z - 1) * XY + col];
+                m[u] = 0.0f;
+            }
             for (int j = 0; j < K; j++) {
                 if (!on[j]) continue;
                 const float *gj = g + j * n_ax;
@@ -98,7 +102,7 @@ __global__ __launch_bounds__(256) void b3_resid(const DevBlock *__restrict__ blo
 #pragma unroll
             for (int u = 0; u < RZ; u++)
                 if (z0 + u < Z) {
-                    const double v = ((double)dt[(size_t)(z0 + u) * XY + col] - b.b0) - (double)m[u];
+                    const double v = ((double)dv[u] - b.b0) - (double)m[u];
                     const float rf = (float)v;
                     Rt[(size_t)(z0 + u) * XY + col] = rf;
                     ss += (double)rf * (double)rf;
@@ -394,9 +398,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     constexpr int NB = GRAD ? 6 : 1;
     constexpr int NCOL = kCorrSpots * NB;   // 48 or 8 basis columns
     constexpr int NCT = (NCOL + 15) / 16;   // 16-column tiles
-    constexpr int BSTR = NCT * 16 + 4;      // basis chunk row stride (floats)
+    // basis chunk row stride: a multiple of 16 that is 16 or 48 mod 64, so the
+    // four pixel rows a B-operand read touches fall on disjoint bank groups
+    constexpr int BSTR = NCT * 16;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const CorrTask tk = C.tasks[blockIdx.x];
+    // Workgroup b runs on XCD b % 8: give each XCD a contiguous range of
+    // tasks so that the spot groups of one plane share that XCD's L2 (the
+    // grid is padded to a multiple of 8).
+    const int per_xcd = gridDim.x >> 3;
+    const int task = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    if (task >= C.ntasks) return;
+    const CorrTask tk = C.tasks[task];
     const PlaneRef pr = C.planes[tk.plane];
     const DevBlock &b = C.blocks[pr.block];
     const int X = b.X, Y = b.Y, Z = b.Z, T = b.T, K = b.K;
@@ -438,19 +450,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
         q[2] = dI;
     }
     // columns past the spots' basis stay zero
-    for (int e = tid; e < kCorrPix * (BSTR - NCOL); e += 256) {
+    for (int e = tid; BSTR > NCOL && e < kCorrPix * (BSTR - NCOL); e += 256) {
         const int c = e / (BSTR - NCOL), o = e - c * (BSTR - NCOL);
         Bs[c * BSTR + NCOL + o] = 0.0f;
     }
 
-    const int jb = tid >> 5, cb = tid & 31;
+    // basis roles: spot jb, pixel lane cb.  The Q/H pre-pass reduces over
+    // the 32 lanes of a half-wave (jb = tid >> 5); the chunk loop writes the
+    // 8 spots of a pixel from neighbouring lanes (jb = tid & 7), which keeps
+    // its LDS stores nearly conflict-free.
+    int jb = tid >> 5, cb = tid & 31;
     float Isb = 0.0f, invb = 0.0f, msigb = 0.0f;
-    if (jb < nk) {
-        const double *ph = b.params + ((size_t)pr.q * K + tk.k0 + jb) * 5;
-        Isb = (float)ph[0];
-        invb = (float)(1.0 / (ph[1] * 2.5066282746310002));
-        msigb = -Isb / (float)ph[1];
-    }
+    auto spot_consts = [&]() {
+        Isb = invb = msigb = 0.0f;
+        if (jb < nk) {
+            const double *ph = b.params + ((size_t)pr.q * K + tk.k0 + jb) * 5;
+            Isb = (float)ph[0];
+            invb = (float)(1.0 / (ph[1] * 2.5066282746310002));
+            msigb = -Isb / (float)ph[1];
+        }
+    };
+    spot_consts();
     // Q = sum mu^2 and H_n = sum dmu_n mu over the block (frame independent):
     // spot jb, pixels cb, cb + 32, ...
     auto basis = [&](int x, int y, int z, float *v) {
@@ -491,6 +511,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
         }
     }
 
+    jb = tid & 7;
+    cb = tid >> 3;
+    spot_consts();
     const float *Rp = C.R + pr.rslot;
     const int lr = lane & 15, lk = lane >> 4;
     for (int t0 = 0; t0 < T; t0 += kCorrFrames) {

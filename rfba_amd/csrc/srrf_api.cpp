@@ -20,6 +20,8 @@ namespace rfba {
 template <typename Tin, int TZ, bool RP2> __global__ void srrf_fused(LaunchParams P);
 __global__ void srrf_reset(int *counter, int *err);
 __global__ void copy_probe(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n);
+__global__ void srrf_clamp_init(unsigned *maxbits);
+__global__ void srrf_clamp(float *__restrict__ d, size_t n, unsigned *__restrict__ maxbits);
 }  // namespace rfba
 
 using namespace rfba;
@@ -1148,6 +1150,54 @@ extern "C" void rfba_release_cache(void) {
     rel((void *&)g_ctx.d_out, g_ctx.out_bytes);
     rel((void *&)g_ctx.d_items, g_ctx.items_cap);
     rel((void *&)g_ctx.d_tiles, g_ctx.tiles_cap);
+}
+
+// ---------------------------------------------------------------------------
+// caller post-processing and the SRRF.raw file (multispot5_place_methods.cc:
+// 260-274 write, 284-289 read)
+
+extern "C" int rfba_srrf_postprocess_device(float *d_out, size_t n, void *stream, float *fmax) {
+    if (!d_out && n) return fail(RFBA_EINVAL, "NULL buffer");
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    int rc;
+    if ((rc = ctx_init())) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    unsigned *d_max = nullptr;
+    HIP_TRY(hipMalloc(&d_max, sizeof(unsigned)));
+    hipLaunchKernelGGL(srrf_clamp_init, dim3(1), dim3(64), 0, st, d_max);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && n) {
+        const size_t blocks = std::min<size_t>((n + 255) / 256, (size_t)g_ctx.ncu * 16);
+        hipLaunchKernelGGL(srrf_clamp, dim3((unsigned)blocks), dim3(256), 0, st, d_out, n, d_max);
+        e = hipGetLastError();
+    }
+    unsigned bits = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&bits, d_max, sizeof bits, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(d_max);
+    if (e != hipSuccess) return fail(RFBA_EDEVICE, "post-processing: %s", hipGetErrorString(e));
+    if (fmax) memcpy(fmax, &bits, sizeof bits);
+    return RFBA_OK;
+}
+
+extern "C" int rfba_srrf_write_raw(const char *path, const float *data, size_t n) {
+    if (!path || (!data && n)) return fail(RFBA_EINVAL, "NULL argument");
+    FILE *fp = fopen(path, "wb");
+    if (!fp) return fail(RFBA_EINVAL, "cannot open %s for writing", path);
+    const size_t w = n ? fwrite(data, sizeof(float), n, fp) : 0;
+    const int c = fclose(fp);
+    if (w != n || c != 0) return fail(RFBA_EINVAL, "short write to %s", path);
+    return RFBA_OK;
+}
+
+extern "C" int rfba_srrf_read_raw(const char *path, float *data, size_t n) {
+    if (!path || (!data && n)) return fail(RFBA_EINVAL, "NULL argument");
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return fail(RFBA_EINVAL, "cannot open %s", path);
+    const size_t r = n ? fread(data, sizeof(float), n, fp) : 0;
+    fclose(fp);
+    if (r != n) return fail(RFBA_EINVAL, "%s holds %zu of %zu floats", path, r, n);
+    return RFBA_OK;
 }
 
 extern "C" int rfba_copy_probe(void *d_dst, const void *d_src, size_t bytes, float *ms) {

@@ -149,6 +149,37 @@ __global__ void srrf_reset(int *counter, int *err) {
     }
 }
 
+// Caller post-processing of the SR volume (multispot5_place_methods.cc:260-269):
+// NaN -> 0, negative -> 0, in place, and the running maximum starting at 1.
+// Non-negative floats order like their bit patterns, so the maximum is an
+// unsigned atomic max of the bits (one per workgroup).
+__global__ void srrf_clamp_init(unsigned *maxbits) {
+    if (threadIdx.x == 0) *maxbits = __float_as_uint(1.0f);
+}
+
+__global__ void srrf_clamp(float *__restrict__ d, size_t n, unsigned *__restrict__ maxbits) {
+    __shared__ float red[4];
+    float mx = 1.0f;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float v = d[i];
+        if (v != v) v = 0.0f;
+        if (v < 0.0f) v = 0.0f;
+        d[i] = v;
+        mx = v > mx ? v : mx;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const float u = __shfl_xor(mx, o);
+        mx = u > mx ? u : mx;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = red[w] > m ? red[w] : m;
+        atomicMax(maxbits, __float_as_uint(m));
+    }
+}
+
 // Plain streaming copy of n 16-byte vectors: the reference point for HBM
 // byte counters (each byte read once and written once).
 __global__ void copy_probe(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {

@@ -137,6 +137,12 @@ def lib() -> ctypes.CDLL:
         L.rfba_device_synchronize.restype = c_int
         L.rfba_copy_probe.argtypes = [c_void_p, c_void_p, c_size_t, POINTER(c_float)]
         L.rfba_copy_probe.restype = c_int
+        L.rfba_srrf_postprocess_device.argtypes = [c_void_p, c_size_t, c_void_p, POINTER(c_float)]
+        L.rfba_srrf_postprocess_device.restype = c_int
+        L.rfba_srrf_write_raw.argtypes = [c_char_p, POINTER(c_float), c_size_t]
+        L.rfba_srrf_write_raw.restype = c_int
+        L.rfba_srrf_read_raw.argtypes = [c_char_p, POINTER(c_float), c_size_t]
+        L.rfba_srrf_read_raw.restype = c_int
         L.rfba_release_cache.argtypes = []
         L.rfba_release_cache.restype = None
         _sig_done = True
@@ -384,3 +390,25 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- caller post-processing
+def postprocess_device(d_out: DeviceBuffer, n: int) -> float:
+    """In place on the device: NaN -> 0, negative -> 0; returns max(1, max)
+    (multispot5_place_methods.cc:260-269)."""
+    fm = c_float(0)
+    _check(lib().rfba_srrf_postprocess_device(d_out.ptr, c_size_t(n), None, ctypes.byref(fm)))
+    return fm.value
+
+
+def write_raw(path: str, volume: np.ndarray):
+    """SRRF.raw: raw float32, x fastest (multispot5_place_methods.cc:270-274)."""
+    a = np.ascontiguousarray(volume, np.float32)
+    _check(lib().rfba_srrf_write_raw(path.encode(), a.ctypes.data_as(POINTER(c_float)), c_size_t(a.size)))
+
+
+def read_raw(path: str, shape_zyx) -> np.ndarray:
+    """Read SRRF.raw back (multispot5_place_methods.cc:284-289)."""
+    out = np.empty(shape_zyx, np.float32)
+    _check(lib().rfba_srrf_read_raw(path.encode(), out.ctypes.data_as(POINTER(c_float)), c_size_t(out.size)))
+    return out
