@@ -284,6 +284,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
             int b, valid;
             float w0, w1, w2, w3;
             axis_weights(z0, M, D, b, w0, w1, w2, w3, valid);
+            // An out-of-range sample (interpolateG returns 0, :146-147) gets
+            // zero weights on the axis that fails: its interpolated gradient
+            // is then +-0, which takes the same path as an exact 0 (g2 = 0:
+            // the general term, NaN).  Likewise for x and y below.
+            if (!valid) w0 = w1 = w2 = w3 = 0.0f;
             float *zt = ztab + e * 8;
             zt[0] = __int_as_float(clampi(b - wlo, 0, NW - 4));
             zt[1] = w0; zt[2] = w1; zt[3] = w2; zt[4] = w3;
@@ -383,6 +388,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                     int b, valid;
                     float w0, w1, w2, w3;
                     axis_weights(((float)(X0 + tid) + 0.5f) + oxv, M, W, b, w0, w1, w2, w3, valid);
+                    if (!valid) w0 = w1 = w2 = w3 = 0.0f;
                     float *xt = xtab + tid * 6;
                     xt[0] = __int_as_float(clampi(b - ulo, 0, NU - 4));
                     xt[1] = w0; xt[2] = w1; xt[3] = w2; xt[4] = w3;
@@ -412,7 +418,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
 
                 const float x0 = Xc + oxv;
                 const float dxs = x0 - Xc;
-                const int validx = !((x0 / (float)M) < 1.5f || (x0 / (float)M) > (float)W - 1.5f);
                 const int r0 = st_grp[3 * gi + 1], nr = st_grp[3 * gi + 2];
                 for (int r = r0; r < r0 + nr; r++) {
                     const int iy = st_run[5 * r + 2];
@@ -421,6 +426,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                     int vb, validy;
                     float wy0, wy1, wy2, wy3;
                     axis_weights(y0, M, H, vb, wy0, wy1, wy2, wy3, validy);
+                    if (!validy) wy0 = wy1 = wy2 = wy3 = 0.0f;
                     const int vrel = clampi(vb - vlo, 0, NV - 4);
                     // ---- y pass: Py[w][c][tid] = sum_j Px[c][vrel+j][w][X] * wy_j
                     for (int w = 0; w < nw; w++) {
@@ -435,7 +441,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                             Py[(w * 3 + c) * kThreads + tid] = q;
                         }
                     }
-                    const int validxy = validx & validy;
                     const int s0 = st_run[5 * r], ns = st_run[5 * r + 1];
                     const float rxs = __int_as_float(st_run[5 * r + 3]), rys = __int_as_float(st_run[5 * r + 4]);
                     // Voxel pairs (Z, Z+1): two independent chains per sample.
@@ -454,8 +459,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                             const float *zt0 = ztab + (Z * noz + iz) * 8;
                             const float *zt1 = ztab + ((Z + 1) * noz + iz) * 8;
                             const int wr0 = __float_as_int(zt0[0]), wr1 = __float_as_int(zt1[0]);
-                            const int v0 = validxy & __float_as_int(zt0[5]);
-                            const int v1 = validxy & __float_as_int(zt1[5]);
                             a0.z0 = zt0[6]; a0.dz = zt0[7];
                             a1.z0 = zt1[6]; a1.dz = zt1[7];
                             // ---- z pass (interpolateG :154-167, innermost sum last)
@@ -472,8 +475,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                                 s1v = s1v + q1[6 * kThreads] * zt1[3];
                                 s0v = s0v + q0[9 * kThreads] * zt0[4];
                                 s1v = s1v + q1[9 * kThreads] * zt1[4];
-                                G0[c] = v0 ? s0v : 0.0f;
-                                G1[c] = v1 ? s1v : 0.0f;
+                                G0[c] = s0v;
+                                G1[c] = s1v;
                             }
                             bool b0, b1;
                             float t0 = term_fast<RP2>(G0[0], G0[1], G0[2], a0, sR, sInvR, b0);
