@@ -73,6 +73,30 @@ int rfba3b_forward_messages(const int *xyz, int P, int T, const float *data, con
  * params[q][k] over the background b0 + sum_{j != k} [state_s,j(t) == on]
  * mu(params[q][j]) (the other spots' sampled intensities, as remove_spot in
  * multispot5.cc:1602-1603), giving logL[q][k][s] and grad[q][k][s][5]. */
+/* Forward filtering / backward sampling of one spot's blink states: the
+ * forward messages of rfba3b_forward_messages, then backward sampling
+ * (forward_algorithm.h:541-566) on the device.  uniforms[T] are the
+ * generator's draws in the reference's order (uniforms[0] picks the last
+ * frame's state).  states[T] gets 0 on / 1 off / 2 bleached; delta [T][3]
+ * and logL may be NULL.  Host buffers. */
+int rfba3b_ffbs(const int *xyz, int P, int T, const float *data, const float *bg,
+                const double *phi, const rfba3b_model *model,
+                const double *uniforms, uint8_t *states, double *delta,
+                double *logL);
+
+/* Backward sampling of n chains at once on the device: d_delta [n][T][3]
+ * forward messages, d_uniforms [n][T] draws, d_states [n][T] output.
+ * Asynchronous on stream. */
+int rfba3b_backward_sample_device(const double *d_delta, int n, int T,
+                                  const rfba3b_model *model,
+                                  const double *d_uniforms, uint8_t *d_states,
+                                  void *stream);
+
+/* The reference's random stream (MT19937 as seeded by mt19937.h:48-51 and
+ * mersenne.cpp RandomInit; draws as mt19937.h:61-63): n draws after skipping
+ * `skip` draws, for callers that reproduce a reference run. */
+int rfba3b_mt19937_uniforms(int seed, unsigned long long skip, size_t n, double *out);
+
 typedef struct rfba3b_block {
     int dims[3];              /* X, Y, Z (pixel (x,y,z) has coordinates x, y, z) */
     int T, K, Q, S;

@@ -122,6 +122,43 @@ double forward_delta(const double *A, const double *pi, const double *on, const 
     return Z + ln(sum);
 }
 
+// select_random_element (forward_algorithm.h:502-513): the first i whose
+// running sum reaches choice = u * scale, else the last element.
+int select_random_element(const double *v, int n, double scale, double u) {
+    double total = 0, choice = u * scale;
+    for (int i = 0; i < n; i++) {
+        total += v[i];
+        if (choice <= total) return i;
+    }
+    return n - 1;
+}
+
+// sample_unscaled_log (forward_algorithm.h:520-532)
+int sample_unscaled_log(const double *a_in, double u) {
+    double a[3] = {a_in[0], a_in[1], a_in[2]};
+    const double hi = std::max(a[0], std::max(a[1], a[2]));
+    double sum = 0;
+    for (int i = 0; i < 3; i++) {
+        a[i] = std::exp(a[i] - hi);
+        sum += a[i];
+    }
+    return select_random_element(a, 3, sum, u);
+}
+
+// backward_sampling (forward_algorithm.h:541-566): the k-th draw of the
+// generator is u[k]; the last frame is drawn first.
+void backward_sampling(const double *A, const double *delta, int T, const double *u, int *states) {
+    double lA[9];
+    for (int i = 0; i < 9; i++) lA[i] = ln(A[i]);
+    int k = 0;
+    states[T - 1] = sample_unscaled_log(delta + 3 * (T - 1), u[k++]);
+    for (int t = T - 2; t >= 0; t--) {
+        double rp[3];
+        for (int j = 0; j < 3; j++) rp[j] = delta[3 * t + j] + lA[3 * j + states[t + 1]];
+        states[t] = sample_unscaled_log(rp, u[k++]);
+    }
+}
+
 // diff_log_log_normal, storm.h:434-437
 inline double diff_log_log_normal(double x, double mu, double sigma) { return -(1 + (ln(x) - mu) / sq(sigma)) / x; }
 
@@ -169,6 +206,11 @@ double oracle_b3_forward_delta(const double *A, const double *pi, const double *
 }
 
 double oracle_b3_diff_log_log_normal(double x, double mu, double sigma) { return diff_log_log_normal(x, mu, sigma); }
+
+int oracle_b3_backward_sampling(const double *A, const double *delta, int T, const double *u, int *states) {
+    backward_sampling(A, delta, T, u, states);
+    return 0;
+}
 
 // logL and d logL/dphi of one spot over one background (data, bg [T][P]).
 int oracle_b3_logl_grad(const int *xyz, int P, int T, const double *data, const double *bg, const double *phi,

@@ -24,6 +24,8 @@ def lib():
         L.oracle_b3_forward_deriv.argtypes = [dp, dp, dp, dp, dp, c_int, dp, dp]
         L.oracle_b3_forward_delta.argtypes = [dp, dp, dp, dp, c_int, dp]
         L.oracle_b3_forward_delta.restype = c_double
+        L.oracle_b3_backward_sampling.argtypes = [dp, dp, c_int, dp, ip]
+        L.oracle_b3_backward_sampling.restype = c_int
         L.oracle_b3_diff_log_log_normal.argtypes = [c_double, c_double, c_double]
         L.oracle_b3_diff_log_log_normal.restype = c_double
         L.oracle_b3_logl_grad.argtypes = [ip, c_int, c_int, dp, dp, dp, mp, dp, dp]
@@ -94,6 +96,16 @@ def neg_grad(xyz, data, bg, phi, model):
     Ls = np.empty(S)
     lib().oracle_b3_neg_grad(_p(xyz, c_int), P, T, _p(data), _p(bg), S, _p(phi), ctypes.byref(model), _p(g), _p(Ls))
     return g, Ls
+
+
+def backward_sampling(A, delta, u):
+    """States [T] sampled from forward messages delta [T][3] with draws u [T]
+    (u[0] for the last frame)."""
+    A, delta, u = _d(A), _d(delta), _d(u)
+    T = delta.shape[0]
+    st = np.empty(T, np.int32)
+    lib().oracle_b3_backward_sampling(_p(A), _p(delta), T, _p(u), _p(st, c_int))
+    return st
 
 
 def diff_log_log_normal(x, mu, sigma):

@@ -86,6 +86,13 @@ def lib():
         L.rfba3b_neg_grad_sampled.restype = c_int
         L.rfba3b_forward_messages.argtypes = [ip, c_int, c_int, fp, fp, dp, POINTER(Model), dp, dp]
         L.rfba3b_forward_messages.restype = c_int
+        L.rfba3b_ffbs.argtypes = [ip, c_int, c_int, fp, fp, dp, POINTER(Model), dp, POINTER(c_uint8), dp, dp]
+        L.rfba3b_ffbs.restype = c_int
+        L.rfba3b_backward_sample_device.argtypes = [c_void_p, c_int, c_int, POINTER(Model), c_void_p, c_void_p,
+                                                    c_void_p]
+        L.rfba3b_backward_sample_device.restype = c_int
+        L.rfba3b_mt19937_uniforms.argtypes = [c_int, ctypes.c_ulonglong, ctypes.c_size_t, dp]
+        L.rfba3b_mt19937_uniforms.restype = c_int
         L.rfba3b_batch_create.argtypes = [POINTER(_Block), c_int, POINTER(Model), POINTER(c_void_p)]
         L.rfba3b_batch_create.restype = c_int
         L.rfba3b_batch_run.argtypes = [c_void_p, c_void_p, c_int, POINTER(c_float)]
@@ -138,6 +145,48 @@ def forward_messages(xyz, data, bg, phi, model: Model):
     _check(lib().rfba3b_forward_messages(_p(xyz, c_int), P, T, _p(data, c_float), _p(bg, c_float),
                                          _p(phi, c_double), ctypes.byref(model), _p(delta, c_double), ctypes.byref(L)))
     return delta, L.value
+
+
+def ffbs(xyz, data, bg, phi, model: Model, uniforms):
+    """Forward filtering / backward sampling of one spot's states (0 on, 1
+    off, 2 bleached).  uniforms [T]: the generator's draws, uniforms[0] for
+    the last frame.  Returns (states [T] uint8, delta [T,3], logL)."""
+    xyz = np.ascontiguousarray(xyz, np.int32)
+    data = np.ascontiguousarray(data, np.float32)
+    bg = np.ascontiguousarray(bg, np.float32)
+    phi = np.ascontiguousarray(phi, np.float64).ravel()
+    u = np.ascontiguousarray(uniforms, np.float64)
+    T, P = data.shape
+    states = np.empty(T, np.uint8)
+    delta = np.empty((T, 3), np.float64)
+    L = c_double(0)
+    _check(lib().rfba3b_ffbs(_p(xyz, c_int), P, T, _p(data, c_float), _p(bg, c_float), _p(phi, c_double),
+                             ctypes.byref(model), _p(u, c_double), _p(states, c_uint8), _p(delta, c_double),
+                             ctypes.byref(L)))
+    return states, delta, L.value
+
+
+def backward_sample(delta, uniforms, model: Model):
+    """Backward sampling of n chains on the device: delta [n,T,3], uniforms
+    [n,T] -> states [n,T] uint8."""
+    delta = np.ascontiguousarray(delta, np.float64)
+    u = np.ascontiguousarray(uniforms, np.float64)
+    n, T, _ = delta.shape
+    dd, du, ds = DeviceBuffer.from_array(delta), DeviceBuffer.from_array(u), DeviceBuffer(n * T)
+    try:
+        _check(lib().rfba3b_backward_sample_device(dd.ptr, n, T, ctypes.byref(model), du.ptr, ds.ptr, None))
+        return ds.to_array((n, T), np.uint8)
+    finally:
+        for b in (dd, du, ds):
+            b.close()
+
+
+def mt19937_uniforms(seed: int, n: int, skip: int = 0) -> np.ndarray:
+    """The reference's rng() stream for a seed (MT19937, 37 outputs discarded
+    at seeding, draws = output * 2^-32)."""
+    out = np.empty(n, np.float64)
+    _check(lib().rfba3b_mt19937_uniforms(int(seed), skip, n, _p(out, c_double)))
+    return out
 
 
 @dataclass

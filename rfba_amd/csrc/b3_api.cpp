@@ -11,6 +11,7 @@
 #include <cstring>
 #include <limits>
 #include <mutex>
+#include <random>
 #include <vector>
 
 #include "b3_internal.h"
@@ -23,6 +24,11 @@ __global__ void b3_resid(const DevBlock *__restrict__ blocks, const PlaneTask *_
 template <int MODE> __global__ void b3_eval(EvalParams E);
 template <bool GRAD> __global__ void b3_corr_mfma(CorrParams C);
 template <bool GRAD> __global__ void b3_hmm(HmmParams H);
+struct LogA {
+    double v[9];
+};
+__global__ void b3_backward_sample(const double *__restrict__ delta, int n, int T, LogA lA,
+                                   const double *__restrict__ u, uint8_t *__restrict__ states);
 }  // namespace rfba3b
 
 using namespace rfba3b;
@@ -615,5 +621,73 @@ extern "C" int rfba3b_forward_messages(const int *xyz, int P, int T, const float
     if (rc) return rc;
     memcpy(delta, D.data(), 3 * (size_t)T * sizeof(double));
     if (logL) *logL = L[0];
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// forward filtering / backward sampling
+
+static LogA log_a(const rfba3b_model *m) {
+    LogA l;
+    for (int i = 0; i < 9; i++) l.v[i] = ln0(m->A[i]);
+    return l;
+}
+
+extern "C" int rfba3b_backward_sample_device(const double *d_delta, int n, int T, const rfba3b_model *model,
+                                             const double *d_uniforms, uint8_t *d_states, void *stream) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if ((rc = check_model(model)) || (rc = dev_init())) return rc;
+    if (n < 0 || T <= 0 || (n > 0 && (!d_delta || !d_uniforms || !d_states))) return fail(-1, "invalid arguments");
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(b3_backward_sample, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, d_delta, n, T,
+                       log_a(model), d_uniforms, d_states);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int rfba3b_ffbs(const int *xyz, int P, int T, const float *data, const float *bg, const double *phi,
+                           const rfba3b_model *model, const double *uniforms, uint8_t *states, double *delta,
+                           double *logL) {
+    if (!uniforms || !states || !phi) return fail(-1, "invalid arguments");
+    if (!(phi[0] > 0 && phi[1] > 0)) return fail(-1, "invalid spot parameters");
+    std::vector<double> D;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        std::vector<double> L, G;
+        int rc = eval_explicit(xyz, P, T, data, bg, 1, phi, 1, model, MODE_DELTA, L, G, &D);
+        if (rc) return rc;
+        if (logL) *logL = L[0];
+    }
+    if (delta) memcpy(delta, D.data(), 3 * (size_t)T * sizeof(double));
+    double *dd = nullptr, *du = nullptr;
+    uint8_t *ds = nullptr;
+    hipError_t e = hipMalloc(&dd, D.size() * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&du, (size_t)T * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&ds, (size_t)T);
+    if (e == hipSuccess) e = hipMemcpy(dd, D.data(), D.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(du, uniforms, (size_t)T * sizeof(double), hipMemcpyHostToDevice);
+    int rc = 0;
+    if (e == hipSuccess) {
+        rc = rfba3b_backward_sample_device(dd, 1, T, model, du, ds, nullptr);
+        if (!rc) e = hipMemcpy(states, ds, (size_t)T, hipMemcpyDeviceToHost);
+    }
+    hipFree(dd);
+    hipFree(du);
+    hipFree(ds);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(-3, "backward sampling: %s", hipGetErrorString(e));
+    return 0;
+}
+
+// The reference's MT19937 stream (mt19937.h:48-63 over mersenne.cpp): seeded
+// by the standard MT19937 initialisation, 37 outputs discarded (RandomInit),
+// then each draw is the 32-bit output times 2^-32.  std::mt19937 implements
+// the same generator and initialisation.
+extern "C" int rfba3b_mt19937_uniforms(int seed, unsigned long long skip, size_t n, double *out) {
+    if (!out && n) return fail(-1, "NULL output");
+    std::mt19937 g((uint32_t)seed);
+    g.discard(37 + skip);
+    for (size_t i = 0; i < n; i++) out[i] = (double)g() * (1.0 / (65536.0 * 65536.0));
     return 0;
 }

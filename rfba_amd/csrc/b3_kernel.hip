@@ -719,6 +719,48 @@ __global__ __launch_bounds__(256) void b3_hmm(HmmParams H) {
 template __global__ void b3_hmm<false>(HmmParams);
 template __global__ void b3_hmm<true>(HmmParams);
 
+// ---------------------------------------------------------------------------
+// Backward sampling of forward filtering / backward sampling, one lane per
+// chain (forward_algorithm.h:502-566): the last frame's state is drawn from
+// its forward message, then each earlier frame's from its message plus
+// ln A[j][next]; a draw picks the first state whose running sum of
+// exp(a - max) reaches u * sum.  Draw k of a chain is u[k] (u[0]: last frame).
+struct LogA {
+    double v[9];  // ln A[i][j], ln 0 = -1e100
+};
+
+__device__ __forceinline__ int sample_unscaled_log3(double a0, double a1, double a2, double u) {
+    const double hi = fmax(a0, fmax(a1, a2));
+    const double e0 = exp(a0 - hi), e1 = exp(a1 - hi), e2 = exp(a2 - hi);
+    double sum = 0.0;
+    sum += e0;
+    sum += e1;
+    sum += e2;
+    const double choice = u * sum;
+    double total = e0;
+    if (choice <= total) return 0;
+    total += e1;
+    if (choice <= total) return 1;
+    return 2;  // either the third running sum reaches choice or the last element is returned
+}
+
+__global__ void b3_backward_sample(const double *__restrict__ delta, int n, int T, LogA lA,
+                                   const double *__restrict__ u, uint8_t *__restrict__ states) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const double *d = delta + (size_t)c * T * 3;
+    const double *uc = u + (size_t)c * T;
+    uint8_t *sc = states + (size_t)c * T;
+    int k = 0;
+    int next = sample_unscaled_log3(d[3 * (T - 1)], d[3 * (T - 1) + 1], d[3 * (T - 1) + 2], uc[k++]);
+    sc[T - 1] = (uint8_t)next;
+    for (int t = T - 2; t >= 0; t--) {
+        next = sample_unscaled_log3(d[3 * t] + lA.v[next], d[3 * t + 1] + lA.v[3 + next], d[3 * t + 2] + lA.v[6 + next],
+                                    uc[k++]);
+        sc[t] = (uint8_t)next;
+    }
+}
+
 template __global__ void b3_eval<MODE_LOGL>(EvalParams);
 template __global__ void b3_eval<MODE_GRAD>(EvalParams);
 template __global__ void b3_eval<MODE_DELTA>(EvalParams);

@@ -167,3 +167,38 @@ def test_config4_shape_subset(gpu):
     (L, G), = run_batch([b3.BlockData(data, P, states[None], 100.0)], m, graph=True)
     Lo, Go = bo.model_eval(data, 100.0, m, P, states[None])
     compare(L, G, Lo, Go, "29^3 T=100 K=40")
+
+
+def test_ffbs_matches_oracle_sampler(gpu):
+    # forward messages on the GPU, then backward sampling on the GPU; the
+    # oracle's sampler on the same messages and draws gives the same states
+    m = model()
+    data, params, states = b3.synth_block((9, 8, 7), 40, 3, 51)
+    xyz = grid(9, 8, 7)
+    d2 = data.reshape(40, -1)
+    k = int(np.argmax((states == 0).sum(1)))  # the spot that is on most often
+    assert (states[k] == 0).sum() >= 5
+    bg = np.full(d2.shape, 100.0)
+    for j in range(3):
+        if j != k:
+            bg += (states[j] == 0)[:, None] * bo.spot(xyz, params[j], 2.0)[0][None]
+    u = b3.mt19937_uniforms(7, 40)
+    st, delta, L = b3.ffbs(xyz, d2.astype(np.float32), bg.astype(np.float32), params[k], m, u)
+    so = bo.backward_sampling(b3.blink_matrix(), delta, u)
+    assert np.array_equal(st, so.astype(np.uint8))
+    # the sampled path follows the data: frames where the spot is on
+    agree = np.mean((st == 0) == (states[k] == 0))
+    print("ffbs: %d of %d on-frames sampled, %.2f agreement with the true states"
+          % ((st == 0).sum(), (states[k] == 0).sum(), agree))
+    assert agree > 0.9
+
+
+def test_backward_sample_device_many_chains(gpu):
+    rng = np.random.default_rng(12)
+    A, pi = b3.blink_matrix(), np.array([0.5, 0.5, 0.0])
+    n, T = 300, 25
+    deltas = np.stack([bo.forward_delta(A, pi, rng.normal(0, 2, T), rng.normal(0, 2, T))[0] for _ in range(n)])
+    U = rng.uniform(size=(n, T))
+    got = b3.backward_sample(deltas, U, model())
+    ref = np.stack([bo.backward_sampling(A, deltas[i], U[i]) for i in range(n)]).astype(np.uint8)
+    assert np.array_equal(got, ref)

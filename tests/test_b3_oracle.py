@@ -224,3 +224,69 @@ def test_model_eval_matches_composition():
                 Lr, Gr = bo.logl_grad(xyz, d2, bg, P[q, k], m)
                 assert L[q, k, s] == pytest.approx(Lr, rel=1e-13)
                 np.testing.assert_allclose(G[q, k, s], Gr, rtol=1e-10, atol=1e-12)
+
+
+# ------------------------------------------------------------------ backward sampling
+def path_posterior(A, pi, on, off):
+    """Exact P(path | data) for every path (brute force), as a dict."""
+    T = len(on)
+    A = np.asarray(A)
+    logs = {}
+    for path in itertools.product(range(3), repeat=T):
+        if pi[path[0]] == 0:
+            continue
+        lp = np.log(pi[path[0]])
+        ok = True
+        for t in range(1, T):
+            a = A[path[t - 1], path[t]]
+            if a == 0:
+                ok = False
+                break
+            lp += np.log(a)
+        if ok:
+            logs[path] = lp + sum(on[t] if s == 0 else off[t] for t, s in enumerate(path))
+    m = max(logs.values())
+    z = sum(np.exp(v - m) for v in logs.values())
+    return {k: np.exp(v - m) / z for k, v in logs.items()}
+
+
+def test_backward_sampling_matches_exact_posterior():
+    # FFBS draws paths from P(path | data); compare frequencies of 40000 draws
+    # with brute-force enumeration (binomial sd <= 0.0025; bar 0.012)
+    rng = np.random.default_rng(9)
+    T = 4
+    A, pi = b3.blink_matrix(), np.array([0.5, 0.5, 0.0])
+    on, off = rng.normal(0, 1.0, T), rng.normal(0, 1.0, T)
+    delta, _ = bo.forward_delta(A, pi, on, off)
+    post = path_posterior(A, pi, on, off)
+    N = 40000
+    U = rng.uniform(size=(N, T))
+    counts = {}
+    for i in range(N):
+        p = tuple(int(v) for v in bo.backward_sampling(A, delta, U[i]))
+        counts[p] = counts.get(p, 0) + 1
+    assert set(counts) <= set(post)
+    err = max(abs(counts.get(k, 0) / N - v) for k, v in post.items())
+    assert err < 0.012, err
+
+
+def test_backward_sampling_selection_rule():
+    # select_random_element: first state whose running sum reaches u * sum
+    A = b3.blink_matrix()
+    delta = np.log(np.array([[0.2, 0.3, 0.5]]))  # T = 1: weights 0.2, 0.3, 0.5
+    for u, s in [(0.0, 0), (0.199999, 0), (0.2000001, 1), (0.499999, 1), (0.5000001, 2), (0.999999, 2)]:
+        assert bo.backward_sampling(A, delta, [u])[0] == s, u
+    # bleached cannot precede on (A[2][0] = 0, ln 0 = -1e100): drawing the
+    # predecessor of an 'on' frame with u near 1 stops at 'off'
+    d = np.zeros((2, 3))
+    st = bo.backward_sampling(A, d, [0.1, 0.999999])  # last frame: equal weights, u = 0.1 -> on
+    assert list(st) == [1, 0]
+
+
+def test_mt19937_stream_known_answer():
+    # MT19937 with the standard seeding: the 10000th output for seed 5489 is
+    # 4123659995 (a published known answer of the generator)
+    u = b3.mt19937_uniforms(5489, 1, skip=10000 - 37 - 1)
+    assert u[0] * 2.0 ** 32 == 4123659995
+    v = b3.mt19937_uniforms(5489, 5)
+    assert np.all((v >= 0) & (v < 1)) and np.all(v * 2.0 ** 32 == np.floor(v * 2.0 ** 32))
