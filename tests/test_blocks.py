@@ -1,7 +1,9 @@
 """Block planner (rfba_amd.blocks.do_split) against hand-derived cases of
 Matlab Scripts/doSplit.m: [start, length] pairs, 1-based, neighbours
 overlapping by 2*border, the last block taking the remainder."""
-from rfba_amd.blocks import do_split, volume_blocks
+import numpy as np
+
+from rfba_amd.blocks import do_split, interior_pixels, volume_blocks
 
 
 def test_single_block_when_it_fits():
@@ -40,3 +42,17 @@ def test_coverage_and_overlap():
         assert covered == set(range(1, xall + 1))
         for (s0, n0), (s1, _) in zip(parts, parts[1:]):
             assert s0 + n0 - s1 == 2 * b
+
+
+def test_interior_filter_hand_case():
+    # loadResult.m: block (2nd x block starting at 24, length 29), border 3:
+    # local x = X - 24 + 1 must satisfy 3 < x < 26, i.e. X in 27..48
+    xl = do_split(512, 29, 3)
+    yl = zl = [(1, 29)]
+    X = np.array([26, 27, 48, 49, 30, 30], np.float64)
+    Y = np.array([10, 10, 10, 10, 3, 4], np.float64)
+    Z = np.full(6, 10.0)
+    pix = np.stack([np.ones(6), np.ones(6), X, Y, Z])
+    kept = interior_pixels(pix, 3, xl, yl, zl, (1, 0, 0))
+    assert kept[2].tolist() == [27, 48, 30]
+    assert kept[3].tolist() == [10, 10, 4]
