@@ -216,6 +216,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
     int *st_smp = (int *)(st_ox + g.st_smp);  // iz, rz
     const float sR = S->R, sInvR = S->invR, sDiv = S->divisor;
     const int ngroups = S->ngroups;
+    const int n_ox = S->nox, n_oy = S->noy;
     const bool fast_ok = S->fast_ok != 0;
     const float oxmin = S->oxmin, oxmax = S->oxmax, oymin = S->oymin, oymax = S->oymax;
     const float ozmin = S->ozmin, ozmax = S->ozmax;
@@ -311,6 +312,30 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
         const int Xs = X0 + xl, Ys = Y0 + yl;
         const float Xc = (float)Xs + 0.5f, Yc = (float)Ys + 0.5f;
 
+        // Voxels whose value needs no sample loop: a sample out of range on
+        // any axis makes every frame's radiality NaN (its gradient is zero:
+        // the general term, NaN, and a sum with NaN), and voxels past the
+        // block are not stored.  zmask: tile slices to skip; colbad: this
+        // thread's column.  The range test is axis_weights' own.
+        __syncthreads();  // z tables
+        unsigned zmask = 0;
+        for (int Z = 0; Z < TZ; Z++) {
+            bool bad = (Z0 + Z) < it.bz0 || (Z0 + Z) >= it.bz1;
+            for (int iz = 0; iz < noz; iz++) bad = bad || !__float_as_int(ztab[(Z * noz + iz) * 8 + 5]);
+            if (bad) zmask |= 1u << Z;
+        }
+        bool colbad = Xs < it.bx0 || Xs >= it.bx1 || Ys < it.by0 || Ys >= it.by1;
+        for (int i = 0; i < n_ox; i++) {
+            const float xx = (Xc + st_ox[i]) / (float)M;
+            colbad = colbad || xx < 1.5f || xx > (float)W - 1.5f;
+        }
+        for (int i = 0; i < n_oy; i++) {
+            const float yy = (Yc + st_oy[i]) / (float)M;
+            colbad = colbad || yy < 1.5f || yy > (float)H - 1.5f;
+        }
+        const unsigned zall = (TZ >= 32) ? ~0u : ((1u << TZ) - 1u);
+        const bool tile_skip = __syncthreads_and(colbad || zmask == zall) != 0;
+
         float acc0[TZ], acc1[TZ];
 #pragma unroll
         for (int z = 0; z < TZ; z++) { acc0[z] = 0.0f; acc1[z] = 0.0f; }
@@ -333,8 +358,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
             }
             __syncthreads();
 
-            // ---- gradient planes G[c][w][v][u] ------------------------------
-            for (int e = tid; e < nu * nv * nw; e += kThreads) {
+            // ---- gradient planes G[c][w][v][u] (only the sample passes read them)
+            for (int e = tid; e < (tile_skip ? 0 : nu * nv * nw); e += kThreads) {
                 const int u = e % nu, v = (e / nu) % nv, w = e / (nu * nv);
                 const int gi3 = (w * NV + v) * NU + u;
                 const int cu = u + g.hx, cv = v + g.hy, cw = w + g.hz;
@@ -381,7 +406,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
 
             for (int z = 0; z < TZ; z++) dsum[z * kThreads + tid] = 0.0f;
 
-            for (int gi = 0; gi < ngroups; gi++) {
+            for (int gi = 0; gi < ngroups && !tile_skip; gi++) {
                 const int ix = st_grp[3 * gi];
                 const float oxv = st_ox[ix];
                 if (tid < kTX) {
@@ -419,7 +444,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                 const float x0 = Xc + oxv;
                 const float dxs = x0 - Xc;
                 const int r0 = st_grp[3 * gi + 1], nr = st_grp[3 * gi + 2];
-                for (int r = r0; r < r0 + nr; r++) {
+                // a wave whose voxels all skip does no y pass or samples
+                const bool wave_skip = __all(colbad || zmask == zall);
+                for (int r = r0; r < r0 + nr && !wave_skip; r++) {
                     const int iy = st_run[5 * r + 2];
                     const float y0 = Yc + st_oy[iy];
                     const float dys = y0 - Yc;
@@ -445,6 +472,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                     const float rxs = __int_as_float(st_run[5 * r + 3]), rys = __int_as_float(st_run[5 * r + 4]);
                     // Voxel pairs (Z, Z+1): two independent chains per sample.
                     for (int Z = 0; Z < TZ; Z += 2) {
+                        if (__all(colbad || ((zmask >> Z) & 3u) == 3u)) continue;
                         float dd0 = dsum[Z * kThreads + tid], dd1 = dsum[(Z + 1) * kThreads + tid];
                         TermIn a0, a1;
                         a0.rx = a1.rx = rxs; a0.ry = a1.ry = rys;
@@ -501,7 +529,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
             float rv[TZ];
 #pragma unroll
             for (int Z = 0; Z < TZ; Z++) {
-                float dd = dsum[Z * kThreads + tid] / sDiv;
+                float dd = (colbad || ((zmask >> Z) & 1u)) ? __builtin_nanf("") : dsum[Z * kThreads + tid] / sDiv;
                 if (P.renorm == 1) dd = 0.5f + (dd / 2.0f);
                 rv[Z] = (P.remove_con == 1) ? fmaxf(dd, 0.0f) : dd;
             }
