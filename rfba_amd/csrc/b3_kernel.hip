@@ -46,22 +46,33 @@ __global__ __launch_bounds__(256) void b3_resid(const DevBlock *__restrict__ blo
         const DevBlock &b = blocks[tk.block];
         const int X = b.X, Y = b.Y, Z = b.Z, K = b.K, T = b.T;
         const int P = X * Y * Z;
-        const int n_ax = X + Y + Z;
-        // tables: g[K][n_ax] (x | y | z entries), amp[K], on[K]
-        float *g = lds;
-        float *amp = g + K * n_ax;
+        // tables: gxy[K][X+Y] (x | y factors, padded to 4 floats), gz[K][Zp]
+        // (z factors, Zp = Z rounded up to 8, zero past Z: 16-byte aligned
+        // rows read as two float4 per 8-deep run), amp[K], on[K]
+        const int nxy = X + Y, nxyp = (K * nxy + 3) & ~3, Zp = (Z + 7) & ~7;
+        float *gxy = lds;
+        float *gzt = gxy + nxyp;
+        float *amp = gzt + K * Zp;
         int *on = (int *)(amp + K);
         __syncthreads();
         const double zF = mdl->zfactor;
-        for (int e = tid; e < K * n_ax; e += 256) {
-            const int j = e / n_ax, a = e % n_ax;
+        for (int e = tid; e < K * nxy; e += 256) {
+            const int j = e / nxy, a = e - j * nxy;
             const double *ph = b.params + ((size_t)tk.q * K + j) * 5;
             const float sig = (float)ph[1];
-            float d;
-            if (a < X) d = (float)(a + b.ox) - (float)ph[2];
-            else if (a < X + Y) d = (float)(a - X + b.oy) - (float)ph[3];
-            else d = ((float)(a - X - Y + b.oz) - (float)ph[4]) / (float)zF;
-            g[e] = expf(-(d * d) / (2.0f * sig * sig));
+            const float d = a < X ? (float)(a + b.ox) - (float)ph[2] : (float)(a - X + b.oy) - (float)ph[3];
+            gxy[e] = expf(-(d * d) / (2.0f * sig * sig));
+        }
+        for (int e = tid; e < K * Zp; e += 256) {
+            const int j = e / Zp, z = e - j * Zp;
+            float v = 0.0f;
+            if (z < Z) {
+                const double *ph = b.params + ((size_t)tk.q * K + j) * 5;
+                const float sig = (float)ph[1];
+                const float d = ((float)(z + b.oz) - (float)ph[4]) / (float)zF;
+                v = expf(-(d * d) / (2.0f * sig * sig));
+            }
+            gzt[e] = v;
         }
         for (int j = tid; j < K; j += 256) {
             const double *ph = b.params + ((size_t)tk.q * K + j) * 5;
@@ -93,11 +104,12 @@ __global__ __launch_bounds__(256) void b3_resid(const DevBlock *__restrict__ blo
             }
             for (int j = 0; j < K; j++) {
                 if (!on[j]) continue;
-                const float *gj = g + j * n_ax;
+                const float *gj = gxy + j * nxy;
                 const float a = amp[j] * gj[x] * gj[X + y];
-                const float *gz = gj + X + Y + z0;  // may run past spot j's z entries: padded, never stored
-#pragma unroll
-                for (int u = 0; u < RZ; u++) m[u] += a * gz[u];
+                const float4 *gz = (const float4 *)(gzt + j * Zp + z0);
+                const float4 g0 = gz[0], g1 = gz[1];
+                m[0] += a * g0.x; m[1] += a * g0.y; m[2] += a * g0.z; m[3] += a * g0.w;
+                m[4] += a * g1.x; m[5] += a * g1.y; m[6] += a * g1.z; m[7] += a * g1.w;
             }
 #pragma unroll
             for (int u = 0; u < RZ; u++)
