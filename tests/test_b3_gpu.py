@@ -202,3 +202,15 @@ def test_backward_sample_device_many_chains(gpu):
     got = b3.backward_sample(deltas, U, model())
     ref = np.stack([bo.backward_sampling(A, deltas[i], U[i]) for i in range(n)]).astype(np.uint8)
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("T,K", [(1, 1), (2, 3), (3, 9)])
+def test_short_series_and_single_spot(gpu, T, K):
+    # T = 1 (no transitions, checkpoint stride 1), one spot, more spots than
+    # a correlation group's 8
+    m = model(checkpoint=0)
+    data, params, states = b3.synth_block((7, 6, 5), T, K, 70 + T)
+    P = b3.perturb(params, 2, 71)
+    (L, G), = run_batch([b3.BlockData(data, P, states[None], 100.0)], m)
+    Lo, Go = bo.model_eval(data, 100.0, m, P, states[None])
+    compare(L, G, Lo, Go, "T=%d K=%d" % (T, K))

@@ -245,3 +245,19 @@ def test_radius_variants(gpu, radius):
     g = srrf.srrf_stack(s, c, smp)
     o = oracle.srrf_stack(s.astype(np.float32), c, smp.xyz, smp.divisor)
     parity(g, o, "R=%g" % radius)
+
+
+@pytest.mark.parametrize("method,poscon", [(1, 0), (0, 0), (3, 0), (2, 1)])
+def test_float_input_with_nan_and_inf(gpu, method, poscon):
+    # non-finite input values propagate exactly as in the reference arithmetic
+    # (gradient sums, interpolation, the general radiality term)
+    s = series((9, 10, 14, 13), 6, 61).astype(np.float32)
+    s[2, 5, 7, 6] = np.nan
+    s[4, 3, 3, 9] = np.inf
+    s[6, 7, 10, 2] = -np.inf
+    smp = srrf.directions(26, 0.5)
+    c = conf(Method=method, RemoveCon=poscon, IntensityW=1, Magnification=2, ITC=0, TRACOrder=2)
+    g = srrf.srrf_stack(s, c, smp)
+    o = oracle.srrf_stack(s, c, smp.xyz, smp.divisor)
+    assert np.array_equal(g, o, equal_nan=True)
+    print("method %d: %d of %d outputs NaN" % (method, np.isnan(o).sum(), o.size))
