@@ -208,6 +208,21 @@ int rfba_srrf_write_raw(const char *path, const float *data, size_t n);
 /* Fails (RFBA_EINVAL) when the file holds fewer than n floats; the
  * reference's fread would leave the tail of the buffer unread. */
 int rfba_srrf_read_raw(const char *path, float *data, size_t n);
+
+/* SRRF-guided spot placement (multispot5_place_methods.cc:293-341): every
+ * low-res pixel (x, y, z) of the region (pixels [npix][3]) expands to its
+ * mag^3 SR pixels (x offset outermost, z innermost); their values are read
+ * from the device SR volume d_sr [Z*mag][Y*mag][X*mag] (dims = X, Y, Z);
+ * spots are then placed by rejection sampling against the region's maximum
+ * with the reference's MT19937 stream for `seed` (get_defaults, :69-71):
+ * spots[k] = (intensity_mode, blur_mode, x, y, z), positions in low-res
+ * pixels.  Returns the number of spots placed (0 when the maximum is
+ * negative, as the reference returns none) or a negative status;
+ * *draws_used (may be NULL) = generator draws consumed. */
+int rfba_srrf_place_spots(const float *d_sr, const int *dims, int mag,
+                          const int *pixels, int npix, int num_spots,
+                          double intensity_mode, double blur_mode, int seed,
+                          double *spots, unsigned long long *draws_used);
 /* Release the library's cached device scratch (kept between calls). */
 void rfba_release_cache(void);
 

@@ -153,3 +153,20 @@ def srrf_tiled(series, conf, fill=np.nan, nthreads=None):
     if rc:
         raise ValueError("oracle rejected the input")
     return out
+
+
+def place_spots(sr, mag, pixels, num_spots, intensity_mode, blur_mode, seed):
+    """Spots [n, 5] placed by the restated place_spots_intensity_sampled."""
+    L = lib()
+    L.oracle_place_spots.argtypes = [ctypes.POINTER(c_float), ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int, c_double, c_double,
+                                     ctypes.c_int, ctypes.POINTER(c_double)]
+    L.oracle_place_spots.restype = ctypes.c_int
+    sr = np.ascontiguousarray(sr, np.float32)
+    Zs, Ys, Xs = sr.shape
+    dims = (ctypes.c_int * 3)(Xs // mag, Ys // mag, Zs // mag)
+    px = np.ascontiguousarray(pixels, np.int32)
+    out = np.zeros((num_spots, 5))
+    n = L.oracle_place_spots(_p(sr, c_float), dims, mag, _p(px, ctypes.c_int), len(px), num_spots, intensity_mode,
+                             blur_mode, seed, _p(out, c_double))
+    return out[:n]

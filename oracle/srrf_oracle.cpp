@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <random>
 #include <thread>
 #include <vector>
 
@@ -570,5 +571,49 @@ int oracle_srrf_tiled_f32(const float *in, const int *imageDim, const SRRFConf *
 }
 
 int oracle_hardware_threads(void) { return (int)std::thread::hardware_concurrency(); }
+
+// place_spots_intensity_sampled (multispot5_place_methods.cc:293-341) on a
+// host SR volume, with MT19937 seeded as get_defaults (:69-71): the
+// generator is std::mt19937 (the same recurrence and seeding) after the 37
+// warm-up outputs of mersenne.cpp's RandomInit; draws are output * 2^-32.
+int oracle_place_spots(const float *sr, const int *dims, int mag, const int *pixels, int npix, int num_spots,
+                       double intensity_mode, double blur_mode, int seed, double *spots) {
+    std::vector<int> hx, hy, hz;
+    for (int i = 0; i < npix; i++)
+        for (int j1 = 0; j1 < mag; j1++)
+            for (int j2 = 0; j2 < mag; j2++)
+                for (int j3 = 0; j3 < mag; j3++) {
+                    hx.push_back(pixels[3 * i] * mag + j1);
+                    hy.push_back(pixels[3 * i + 1] * mag + j2);
+                    hz.push_back(pixels[3 * i + 2] * mag + j3);
+                }
+    std::vector<float> inten(hx.size());
+    for (size_t i = 0; i < hx.size(); i++) {
+        const size_t idx = (size_t)hz[i] * dims[0] * dims[1] * mag * mag + (size_t)hy[i] * dims[0] * mag + hx[i];
+        inten[i] = sr[idx];
+    }
+    const double max_intensity = *std::max_element(inten.begin(), inten.end());
+    if (max_intensity < 0) return 0;
+    std::mt19937 g((uint32_t)seed);
+    g.discard(37);
+    auto rng = [&]() { return (double)g() * (1. / (65536. * 65536.)); };
+    int placed = 0;
+    while (placed < num_spots) {
+        int element = (int)std::floor(rng() * hx.size());
+        double yv = rng() * max_intensity;
+        if (yv <= inten[element]) {
+            float x = hx[element] + rng() - .5;
+            float y = hy[element] + rng() - .5;
+            float z = hz[element] + rng() - .5;
+            x = x / mag;
+            y = y / mag;
+            z = z / mag;
+            double *sp = spots + 5 * (size_t)placed;
+            sp[0] = intensity_mode; sp[1] = blur_mode; sp[2] = x; sp[3] = y; sp[4] = z;
+            placed++;
+        }
+    }
+    return placed;
+}
 
 } // extern "C"

@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,8 @@ __global__ void srrf_reset(int *counter, int *err);
 __global__ void copy_probe(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n);
 __global__ void srrf_clamp_init(unsigned *maxbits);
 __global__ void srrf_clamp(float *__restrict__ d, size_t n, unsigned *__restrict__ maxbits);
+__global__ void srrf_gather_region(const float *__restrict__ sr, int X, int Y, int M, const int *__restrict__ px,
+                                   long long n, float *__restrict__ out);
 }  // namespace rfba
 
 using namespace rfba;
@@ -1198,6 +1201,86 @@ extern "C" int rfba_srrf_read_raw(const char *path, float *data, size_t n) {
     fclose(fp);
     if (r != n) return fail(RFBA_EINVAL, "%s holds %zu of %zu floats", path, r, n);
     return RFBA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// SRRF-guided spot placement (place_spots_intensity_sampled,
+// multispot5_place_methods.cc:293-341): the region's SR values are gathered
+// on the device; the rejection sampling runs on the host with the
+// reference's generator (MT19937 seeded as get_defaults :69-71 does, 37
+// outputs discarded by RandomInit, draws = output * 2^-32).
+
+extern "C" int rfba_srrf_place_spots(const float *d_sr, const int *dims, int mag, const int *pixels, int npix,
+                                     int num_spots, double intensity_mode, double blur_mode, int seed, double *spots,
+                                     unsigned long long *draws_used) {
+    if (!d_sr || !dims || !pixels || npix <= 0 || mag <= 0 || num_spots < 0 || (num_spots && !spots))
+        return fail(RFBA_EINVAL, "invalid arguments");
+    const int X = dims[0], Y = dims[1], Z = dims[2];
+    for (int i = 0; i < npix; i++)
+        if (pixels[3 * i] < 0 || pixels[3 * i] >= X || pixels[3 * i + 1] < 0 || pixels[3 * i + 1] >= Y ||
+            pixels[3 * i + 2] < 0 || pixels[3 * i + 2] >= Z)
+            return fail(RFBA_EINVAL, "pixel %d outside the volume", i);
+    const long long n = (long long)npix * mag * mag * mag;
+    std::vector<float> val((size_t)n);
+    {
+        std::lock_guard<std::mutex> lk(g_ctx.mu);
+        int rc;
+        if ((rc = ctx_init())) return rc;
+        int *d_px = nullptr;
+        float *d_val = nullptr;
+        hipError_t e = hipMalloc(&d_px, (size_t)npix * 3 * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc(&d_val, (size_t)n * sizeof(float));
+        if (e == hipSuccess) e = hipMemcpy(d_px, pixels, (size_t)npix * 3 * sizeof(int), hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            const long long blocks = std::min<long long>((n + 255) / 256, (long long)g_ctx.ncu * 16);
+            hipLaunchKernelGGL(srrf_gather_region, dim3((unsigned)blocks), dim3(256), 0, nullptr, d_sr, X, Y, mag, d_px,
+                               n, d_val);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpy(val.data(), d_val, (size_t)n * sizeof(float), hipMemcpyDeviceToHost);
+        hipFree(d_px);
+        hipFree(d_val);
+        if (e != hipSuccess) return fail(RFBA_EDEVICE, "region gather: %s", hipGetErrorString(e));
+    }
+    double mx = val[0];
+    for (long long i = 1; i < n; i++) mx = std::max(mx, (double)val[(size_t)i]);
+    std::mt19937 g((uint32_t)seed);
+    g.discard(37);
+    unsigned long long used = 0;
+    auto rng = [&]() {
+        used++;
+        return (double)g() * (1.0 / (65536.0 * 65536.0));
+    };
+    int placed = 0;
+    if (!(mx < 0)) {
+        const int M3 = mag * mag * mag;
+        while (placed < num_spots) {
+            const long long element = (long long)std::floor(rng() * (double)n);
+            const double yv = rng() * mx;
+            if (yv <= (double)val[(size_t)element]) {
+                const long long p = element / M3;
+                const int j = (int)(element - p * M3);
+                const int xs = pixels[3 * p] * mag + j / (mag * mag);
+                const int ys = pixels[3 * p + 1] * mag + (j / mag) % mag;
+                const int zs = pixels[3 * p + 2] * mag + j % mag;
+                float x = (float)(xs + rng() - .5);
+                float y = (float)(ys + rng() - .5);
+                float z = (float)(zs + rng() - .5);
+                x = x / mag;
+                y = y / mag;
+                z = z / mag;
+                double *sp = spots + 5 * (size_t)placed;
+                sp[0] = intensity_mode;
+                sp[1] = blur_mode;
+                sp[2] = x;
+                sp[3] = y;
+                sp[4] = z;
+                placed++;
+            }
+        }
+    }
+    if (draws_used) *draws_used = used;
+    return placed;
 }
 
 extern "C" int rfba_copy_probe(void *d_dst, const void *d_src, size_t bytes, float *ms) {

@@ -180,6 +180,22 @@ __global__ void srrf_clamp(float *__restrict__ d, size_t n, unsigned *__restrict
     }
 }
 
+// SR values of a region's SR pixels (multispot5_place_methods.cc:293-317):
+// low-res pixel i expands to its M^3 SR pixels, x offset outermost and z
+// offset innermost; value = SR volume at (x M + j1, y M + j2, z M + j3).
+__global__ void srrf_gather_region(const float *__restrict__ sr, int X, int Y, int M, const int *__restrict__ px,
+                                   long long n, float *__restrict__ out) {
+    const int M3 = M * M * M;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const long long p = i / M3;
+        const int j = (int)(i - p * M3);
+        const int j1 = j / (M * M), j2 = (j / M) % M, j3 = j % M;
+        const long long xs = (long long)px[3 * p] * M + j1, ys = (long long)px[3 * p + 1] * M + j2,
+                        zs = (long long)px[3 * p + 2] * M + j3;
+        out[i] = sr[zs * X * Y * M * M + ys * X * M + xs];
+    }
+}
+
 // Plain streaming copy of n 16-byte vectors: the reference point for HBM
 // byte counters (each byte read once and written once).
 __global__ void copy_probe(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {

@@ -143,6 +143,10 @@ def lib() -> ctypes.CDLL:
         L.rfba_srrf_write_raw.restype = c_int
         L.rfba_srrf_read_raw.argtypes = [c_char_p, POINTER(c_float), c_size_t]
         L.rfba_srrf_read_raw.restype = c_int
+        L.rfba_srrf_place_spots.argtypes = [c_void_p, POINTER(c_int), c_int, POINTER(c_int), c_int, c_int, ctypes.c_double,
+                                            ctypes.c_double, c_int, POINTER(ctypes.c_double),
+                                            POINTER(ctypes.c_ulonglong)]
+        L.rfba_srrf_place_spots.restype = c_int
         L.rfba_release_cache.argtypes = []
         L.rfba_release_cache.restype = None
         _sig_done = True
@@ -412,3 +416,20 @@ def read_raw(path: str, shape_zyx) -> np.ndarray:
     out = np.empty(shape_zyx, np.float32)
     _check(lib().rfba_srrf_read_raw(path.encode(), out.ctypes.data_as(POINTER(c_float)), c_size_t(out.size)))
     return out
+
+
+def place_spots(d_sr: DeviceBuffer, dims_xyz, mag: int, pixels, num_spots: int, intensity_mode: float,
+                blur_mode: float, seed: int):
+    """SRRF-guided spot placement (multispot5_place_methods.cc:293-341) from a
+    device SR volume.  pixels [n, 3] low-res (x, y, z).  Returns (spots [k, 5]
+    as (I, sigma, x, y, z), generator draws used)."""
+    px = np.ascontiguousarray(pixels, np.int32)
+    dims = (c_int * 3)(*dims_xyz)
+    out = np.zeros((max(num_spots, 1), 5), np.float64)
+    used = ctypes.c_ulonglong(0)
+    rc = lib().rfba_srrf_place_spots(d_sr.ptr, dims, mag, px.ctypes.data_as(POINTER(c_int)), len(px), num_spots,
+                                     intensity_mode, blur_mode, seed, out.ctypes.data_as(POINTER(ctypes.c_double)),
+                                     ctypes.byref(used))
+    if rc < 0:
+        _check(rc)
+    return out[:rc], used.value
