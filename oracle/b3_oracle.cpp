@@ -5,6 +5,12 @@
 // No product code path links or calls it.  Each function follows the cited
 // reference lines in operation order (the gradient in forward mode, as the
 // reference computes it; the GPU computes it in reverse mode).
+//
+// Pinning: every function is checked in tests/test_b3_oracle.py against
+// finite differences, closed forms, brute-force enumeration of all state
+// paths or the exact path posterior (see the test docstrings).  The
+// reference's own 3B headers cannot be compiled here without stand-ins for
+// MSVC-only names, and the reference ships no vectors for this path.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
