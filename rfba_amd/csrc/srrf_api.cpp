@@ -386,10 +386,12 @@ size_t lds_layout(LaunchGeom &g, int TZ, const DevSamples &s) {
         if ((g.NW * (kTX + p)) % 32 == 16) { g.TXP = kTX + p; break; }
     g.noz = noz;
     int off = 0;
-    g.off_raw = off; off += g.RU * g.RV * g.RW;
+    // The raw halo is read only before the sample passes (gradient and the
+    // voxel-centre intensity); the y-interpolated columns reuse its space.
     g.off_g = off; off += 3 * g.NU * g.NV * g.NW;
     g.off_px = off; off += 3 * g.NV * g.NW * g.TXP;
-    g.off_py = off; off += 3 * g.NW * kThreads;
+    g.off_py = off;
+    g.off_raw = off; off += std::max(3 * g.NW * kThreads, g.RU * g.RV * g.RW);
     g.off_xtab = off; off += kTX * 6;
     g.off_ztab = off; off += TZ * noz * 8;
     g.off_zc = off; off += TZ;

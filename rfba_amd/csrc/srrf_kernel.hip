@@ -420,6 +420,65 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                 G[2 * NP + gi3] = gz;
             }
 
+            // ---- intensity at the voxel centre (interpolateIntensity :257-287),
+            // before the sample passes: the raw halo is dead afterwards and
+            // its LDS is reused by the y-interpolated columns.
+            float Iv[TZ];
+#pragma unroll
+            for (int Z = 0; Z < TZ; Z++) Iv[Z] = 0.0f;
+            if (P.iw == 1) {
+                if (tid < kTX) {
+                    int b, valid;
+                    float w0, w1, w2, w3;
+                    axis_weights((float)(X0 + tid) + 0.5f, M, W, b, w0, w1, w2, w3, valid);
+                    float *xt = xtab + tid * 6;
+                    xt[0] = __int_as_float(clampi(b - ulo, 0, NU - 4));
+                    xt[1] = w0; xt[2] = w1; xt[3] = w2; xt[4] = w3;
+                }
+                __syncthreads();
+                for (int e = tid; e < nv * nw * kTX; e += kThreads) {
+                    const int X = e % kTX, w = (e / kTX) % nw, v = e / (kTX * nw);
+                    const float *xt = xtab + X * 6;
+                    const int ub = __float_as_int(xt[0]);
+                    const float *rr = raw + ((w + g.hz) * RV + (v + g.hy)) * RU + ub + g.hx;
+                    float p = 0.0f;
+                    p = p + rr[0] * xt[1];
+                    p = p + rr[1] * xt[2];
+                    p = p + rr[2] * xt[3];
+                    p = p + rr[3] * xt[4];
+                    Px[(v * NW + w) * TXP + X] = p;
+                }
+                __syncthreads();
+                int vb, validy;
+                float wy0, wy1, wy2, wy3;
+                axis_weights(Yc, M, H, vb, wy0, wy1, wy2, wy3, validy);
+                const int vrel = clampi(vb - vlo, 0, NV - 4);
+                const int validx = !((Xc / (float)M) < 1.5f || (Xc / (float)M) > (float)W - 1.5f);
+                for (int w = 0; w < nw; w++) {
+                    const float *pp = Px + (vrel * NW + w) * TXP + xl;
+                    const int vs = NW * TXP;
+                    float q = 0.0f;
+                    q = q + pp[0] * wy0;
+                    q = q + pp[vs] * wy1;
+                    q = q + pp[2 * vs] * wy2;
+                    q = q + pp[3 * vs] * wy3;
+                    Py[w * kThreads + tid] = q;
+                }
+#pragma unroll
+                for (int Z = 0; Z < TZ; Z++) {
+                    const float *ct = ctab + Z * 6;
+                    const int wrel = __float_as_int(ct[0]);
+                    const float *qq = Py + wrel * kThreads + tid;
+                    float sv = 0.0f;
+                    sv = sv + qq[0] * ct[1];
+                    sv = sv + qq[kThreads] * ct[2];
+                    sv = sv + qq[2 * kThreads] * ct[3];
+                    sv = sv + qq[3 * kThreads] * ct[4];
+                    const int valid = validx & validy & __float_as_int(ct[5]);
+                    Iv[Z] = valid ? fmaxf(sv, 0.0f) : 0.0f;
+                }
+            }
+
             for (int z = 0; z < TZ; z++) dsum[z * kThreads + tid] = 0.0f;
 
             for (int gi = 0; gi < ngroups && !tile_skip; gi++) {
@@ -548,63 +607,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                 float dd = (colbad || ((zmask >> Z) & 1u)) ? __builtin_nanf("") : dsum[Z * kThreads + tid] / sDiv;
                 if (P.renorm == 1) dd = 0.5f + (dd / 2.0f);
                 rv[Z] = (P.remove_con == 1) ? fmaxf(dd, 0.0f) : dd;
-            }
-
-            // ---- intensity at the voxel centre (interpolateIntensity :257-287)
-            float Iv[TZ];
-#pragma unroll
-            for (int Z = 0; Z < TZ; Z++) Iv[Z] = 0.0f;
-            if (P.iw == 1) {
-                if (tid < kTX) {
-                    int b, valid;
-                    float w0, w1, w2, w3;
-                    axis_weights((float)(X0 + tid) + 0.5f, M, W, b, w0, w1, w2, w3, valid);
-                    float *xt = xtab + tid * 6;
-                    xt[0] = __int_as_float(clampi(b - ulo, 0, NU - 4));
-                    xt[1] = w0; xt[2] = w1; xt[3] = w2; xt[4] = w3;
-                }
-                __syncthreads();
-                for (int e = tid; e < nv * nw * kTX; e += kThreads) {
-                    const int X = e % kTX, w = (e / kTX) % nw, v = e / (kTX * nw);
-                    const float *xt = xtab + X * 6;
-                    const int ub = __float_as_int(xt[0]);
-                    const float *rr = raw + ((w + g.hz) * RV + (v + g.hy)) * RU + ub + g.hx;
-                    float p = 0.0f;
-                    p = p + rr[0] * xt[1];
-                    p = p + rr[1] * xt[2];
-                    p = p + rr[2] * xt[3];
-                    p = p + rr[3] * xt[4];
-                    Px[(v * NW + w) * TXP + X] = p;
-                }
-                __syncthreads();
-                int vb, validy;
-                float wy0, wy1, wy2, wy3;
-                axis_weights(Yc, M, H, vb, wy0, wy1, wy2, wy3, validy);
-                const int vrel = clampi(vb - vlo, 0, NV - 4);
-                const int validx = !((Xc / (float)M) < 1.5f || (Xc / (float)M) > (float)W - 1.5f);
-                for (int w = 0; w < nw; w++) {
-                    const float *pp = Px + (vrel * NW + w) * TXP + xl;
-                    const int vs = NW * TXP;
-                    float q = 0.0f;
-                    q = q + pp[0] * wy0;
-                    q = q + pp[vs] * wy1;
-                    q = q + pp[2 * vs] * wy2;
-                    q = q + pp[3 * vs] * wy3;
-                    Py[w * kThreads + tid] = q;
-                }
-#pragma unroll
-                for (int Z = 0; Z < TZ; Z++) {
-                    const float *ct = ctab + Z * 6;
-                    const int wrel = __float_as_int(ct[0]);
-                    const float *qq = Py + wrel * kThreads + tid;
-                    float sv = 0.0f;
-                    sv = sv + qq[0] * ct[1];
-                    sv = sv + qq[kThreads] * ct[2];
-                    sv = sv + qq[2 * kThreads] * ct[3];
-                    sv = sv + qq[3 * kThreads] * ct[4];
-                    const int valid = validx & validy & __float_as_int(ct[5]);
-                    Iv[Z] = valid ? fmaxf(sv, 0.0f) : 0.0f;
-                }
             }
 
             // ---- temporal accumulation (same order as the reference)
