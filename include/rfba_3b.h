@@ -120,6 +120,70 @@ int rfba3b_batch_run(rfba3b_batch *plan, void *stream, int use_graph, float *ker
 int rfba3b_batch_run_per_block(rfba3b_batch *plan, void *stream, int per_param_set, float *kernel_ms);
 void rfba3b_batch_destroy(rfba3b_batch *plan);
 
+/* ---- Gibbs sampling of the blink states --------------------------------
+ * The reference's GibbsSampler2 (sampled_multispot.h:362-730), whose next()
+ * (:681-709) visits every spot in turn: remove the spot from the current
+ * sample intensities over its mask (remove_spot, :115-122), compute the
+ * masked on/off log-probabilities of every frame (SpotWithBackgroundMasked,
+ * spot_with_background.hh:106-151), run the forward recursion
+ * (forward_algorithm_delta2, forward_algorithm.h:394-433), draw the spot's
+ * state path by backward sampling (:542-566) and add the spot back
+ * (add_spot, sampled_multispot.h:124-131).
+ *
+ * A chain is one such sampler over a list of P pixels with K spots.  The
+ * current sample intensities are held as the residual
+ *   d_resid[t][p] = data[t][p] - (base + sum over spots on in frame t of spot[k][p])
+ * in FP64 (base = 0 reproduces the reference's empty sample).  Each chain
+ * owns an MT19937 generator seeded like the reference's (mt19937.h:48-51,
+ * 37 draws discarded), advanced by `skip` draws; it persists across runs.
+ * One spot update consumes exactly T draws (backward_sampling). */
+typedef struct rfba3b_gibbs_chain {
+    int P, T, K;
+    const float *d_data;       /* device [T][P] pixel intensities */
+    const double *d_spot;      /* device [K][P] spot intensities (compute_spot_intensity) */
+    const int *d_mask_off;     /* device [K + 1]: spot k's mask is d_mask[off[k] .. off[k+1]) */
+    const int *d_mask;         /* device pixel indices in [0, P) (spot_pixels) */
+    double base;               /* intensity of the empty sample; 0 for the reference */
+    uint8_t *d_sample;         /* device [K][T] current sample (in/out) */
+    double *d_resid;           /* device [T][P] residual of the current sample (in/out) */
+    uint8_t *d_samples;        /* device [S][K][T]: the sample after each of the S steps
+                                  of rfba3b_gibbs_run (the layout of rfba3b_block.d_states),
+                                  or NULL */
+    unsigned int seed;         /* MT19937 seed */
+    unsigned long long skip;   /* draws skipped after seeding */
+} rfba3b_gibbs_chain;
+
+typedef struct rfba3b_gibbs rfba3b_gibbs;  /* opaque plan */
+
+/* Validates the chains, seeds their generators; chains stay caller-owned. */
+int rfba3b_gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_model *model, rfba3b_gibbs **plan);
+/* GibbsSampler2::reset (sampled_multispot.h:430-438): every spot state 2
+ * (off, bleached) in every frame, residual = data - base.  Asynchronous. */
+int rfba3b_gibbs_reset(rfba3b_gibbs *plan, void *stream);
+/* `samples` consecutive next() calls of every chain (one launch, one
+ * workgroup per chain), each of `sample_iterations` sweeps over the spots in
+ * index order; after step s the sample is copied to d_samples[s].
+ * variance: host [samples] noise variance of each step (set_variance,
+ * :423-426), or NULL for the model's.  Synchronous; kernel_ms may be NULL. */
+int rfba3b_gibbs_run(rfba3b_gibbs *plan, int samples, int sample_iterations, const double *variance, void *stream,
+                     float *kernel_ms);
+/* Draws consumed by chain i since its seeding, skip included. */
+int rfba3b_gibbs_draws(const rfba3b_gibbs *plan, int chain, unsigned long long *draws);
+void rfba3b_gibbs_destroy(rfba3b_gibbs *plan);
+
+/* get_spot_pixels (multispot5.cc:1001-1020) for K spots: the pixels of a
+ * ball of radius phi[1] * 6 + 1 (getBall, :1256-1269) around the spot's
+ * rounded position (ir_rounded3D, :989-995), found in the pixel list xyz
+ * [P][3] (sorted by z, then y, then x, as the reference's region).  Host
+ * buffers.  offsets [K + 1] is always filled; idx (capacity cap) gets the
+ * pixel indices when non-NULL. */
+int rfba3b_spot_pixels(const int *xyz, int P, const double *phi, int K, int *offsets, int *idx, size_t cap);
+/* compute_spot_intensity (sampled_multispot.h:183-191, storm.h:51-57,
+ * 229-235) of K spots at P pixels on the device: d_xyz [P][3], d_phi [K][5],
+ * d_spot [K][P] (FP64).  Asynchronous. */
+int rfba3b_spot_intensities_device(const int *d_xyz, int P, const double *d_phi, int K, double zfactor,
+                                   double *d_spot, void *stream);
+
 const char *rfba3b_last_error(void);
 
 #ifdef __cplusplus

@@ -12,6 +12,7 @@
 // reference's own 3B headers cannot be compiled here without stand-ins for
 // MSVC-only names, and the reference ships no vectors for this path.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdint>
 #include <limits>
@@ -286,6 +287,125 @@ int oracle_b3_model_eval(const int *dims, int T, const double *data, double b0, 
                 for (int n = 0; n < 5; n++) grad[item * 5 + n] = gr[n];
         }
     });
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Gibbs sampling of the blink states.
+
+// getBall (multispot5.cc:1256-1269): the offsets of a ball, z outermost,
+// x innermost; the test compares Ref3D::mag_squared (unsigned,
+// image3d.h:78-82) converted to float with radius * radius in float.
+static std::vector<std::array<int, 3>> get_ball(float radius) {
+    std::vector<std::array<int, 3>> pts;
+    const int r = (int)std::ceil(radius + 1);
+    for (int z = -r; z <= r; z++)
+        for (int y = -r; y <= r; y++)
+            for (int x = -r; x <= r; x++) {
+                const unsigned int m2 = unsigned(x * x) + unsigned(y * y) + unsigned(z * z);
+                if (m2 <= radius * radius) pts.push_back({x, y, z});
+            }
+    return pts;
+}
+
+// get_spot_pixels (multispot5.cc:1001-1020) with ir_rounded3D (:989-995):
+// the ball of radius spot[1] * 6 + 1 around the rounded position, looked up
+// in the pixel list (sorted by Ref3D::operator<, image3d.h:50-53) with
+// lower_bound.
+static void get_spot_pixels(const int *xyz, int P, const double *phi, std::vector<int> &out) {
+    auto less = [](const std::array<int, 3> &a, const std::array<int, 3> &b) {
+        return a[2] < b[2] || (a[2] == b[2] && a[1] < b[1]) || (a[2] == b[2] && a[1] == b[1] && a[0] < b[0]);
+    };
+    std::vector<std::array<int, 3>> pixels(P);
+    for (int p = 0; p < P; p++) pixels[p] = {xyz[3 * p], xyz[3 * p + 1], xyz[3 * p + 2]};
+    const std::vector<std::array<int, 3>> pix = get_ball((float)(phi[1] * 6 + 1));
+    std::array<int, 3> offset;
+    for (int j = 0; j < 3; j++) {
+        const double v = phi[2 + j];
+        offset[j] = static_cast<int>(v > 0.0 ? v + 0.5 : v - 0.5);
+    }
+    out.clear();
+    for (const auto &b : pix) {
+        const std::array<int, 3> q = {b[0] + offset[0], b[1] + offset[1], b[2] + offset[2]};
+        const int pos = (int)(std::lower_bound(pixels.begin(), pixels.end(), q, less) - pixels.begin());
+        if (pos != P && pixels[pos] == q) out.push_back(pos);
+    }
+}
+
+// remove_spot / add_spot over a mask (sampled_multispot.h:113-131).
+static void masked_spot(std::vector<double> &intens, int P, int T, const double *spot, const uint8_t *smp,
+                        const int *mask, int nm, double sign) {
+    for (int t = 0; t < T; t++)
+        if (smp[t] == 0)
+            for (int p = 0; p < nm; p++) {
+                double &v = intens[(size_t)t * P + mask[p]];
+                v = sign < 0 ? v - spot[mask[p]] : v + spot[mask[p]];
+            }
+}
+
+// SpotWithBackgroundMasked (spot_with_background.hh:106-151 over the mask):
+// per frame, log P(frame | on) and log P(frame | off).
+static void masked_emissions(const double *px, const std::vector<double> &bg, int P, int T, const double *spot,
+                             const int *mask, int nm, double var, double *on, double *off) {
+    for (int t = 0; t < T; t++) {
+        double lo = 0, lf = 0;
+        for (int p = 0; p < nm; p++) {
+            const size_t i = (size_t)t * P + mask[p];
+            const double e = px[i] - (bg[i] + spot[mask[p]]);
+            lf -= sq(px[i] - bg[i]);
+            lo -= sq(e);
+        }
+        on[t] = lo / (2 * var) - nm * std::log(2 * M_PI * var) / 2;
+        off[t] = lf / (2 * var) - nm * std::log(2 * M_PI * var) / 2;
+    }
+}
+
+extern "C" {
+
+int oracle_b3_spot_pixels(const int *xyz, int P, const double *phi, int K, int *offsets, int *idx, size_t cap) {
+    std::vector<int> out;
+    size_t tot = 0;
+    offsets[0] = 0;
+    for (int k = 0; k < K; k++) {
+        get_spot_pixels(xyz, P, phi + 5 * (size_t)k, out);
+        for (int v : out) {
+            if (idx && tot < cap) idx[tot] = v;
+            tot++;
+        }
+        offsets[k + 1] = (int)tot;
+    }
+    return (idx && tot > cap) ? -1 : 0;
+}
+
+// GibbsSampler2::next (sampled_multispot.h:681-709): `iters` sweeps over the
+// spots in index order.  data, intens [T][P]; spot [K][P]; masks as offsets
+// [K+1] + indices; sample [K][T] (0 on, 1 off, 2 bleached).  Draw k of the
+// generator is u[k]; *used returns the number of draws consumed.
+int oracle_b3_gibbs_next(int P, int T, int K, const double *data, const double *spot, const int *moff,
+                         const int *midx, const double *A, const double *pi, double var, int iters, uint8_t *sample,
+                         double *intens, const double *u, size_t *used) {
+    std::vector<double> cur(intens, intens + (size_t)T * P);
+    std::vector<double> on(T), off(T), delta(3 * (size_t)T);
+    std::vector<int> st(T);
+    size_t k_u = 0;
+    for (int j = 0; j < iters; j++)
+        for (int k = 0; k < K; k++) {
+            const double *sp = spot + (size_t)k * P;
+            const int *mask = midx + moff[k];
+            const int nm = moff[k + 1] - moff[k];
+            uint8_t *smp = sample + (size_t)k * T;
+            masked_spot(cur, P, T, sp, smp, mask, nm, -1.0);
+            masked_emissions(data, cur, P, T, sp, mask, nm, var, on.data(), off.data());
+            forward_delta(A, pi, on.data(), off.data(), T, delta.data());
+            backward_sampling(A, delta.data(), T, u + k_u, st.data());
+            k_u += T;
+            for (int t = 0; t < T; t++) smp[t] = (uint8_t)st[t];
+            masked_spot(cur, P, T, sp, smp, mask, nm, +1.0);
+        }
+    std::copy(cur.begin(), cur.end(), intens);
+    if (used) *used = k_u;
     return 0;
 }
 

@@ -32,6 +32,11 @@ def lib():
         L.oracle_b3_neg_grad.argtypes = [ip, c_int, c_int, dp, dp, c_int, dp, mp, dp, dp]
         L.oracle_b3_model_eval.argtypes = [ip, c_int, dp, c_double, mp, c_int, dp, c_int, POINTER(c_uint8), c_int,
                                            dp, dp, c_int]
+        L.oracle_b3_spot_pixels.argtypes = [ip, c_int, dp, c_int, ip, ip, ctypes.c_size_t]
+        L.oracle_b3_spot_pixels.restype = c_int
+        L.oracle_b3_gibbs_next.argtypes = [c_int, c_int, c_int, dp, dp, ip, ip, dp, dp, c_double, c_int,
+                                           POINTER(c_uint8), dp, dp, POINTER(ctypes.c_size_t)]
+        L.oracle_b3_gibbs_next.restype = c_int
         _sig = True
     return L
 
@@ -126,3 +131,36 @@ def model_eval(data, b0, model, params, states, threads=None):
     lib().oracle_b3_model_eval(dims, T, _p(data), b0, ctypes.byref(model), K, _p(params), Q, _p(states, c_uint8), S,
                                _p(L), _p(G), threads or _o.threads())
     return L, G
+
+
+def spot_pixels(xyz, phi):
+    """get_spot_pixels for spots phi [K,5] over the pixel list xyz [P,3]:
+    (offsets [K+1], indices)."""
+    xyz = np.ascontiguousarray(xyz, np.int32)
+    phi = _d(np.atleast_2d(phi))
+    K, P = phi.shape[0], len(xyz)
+    off = np.empty(K + 1, np.int32)
+    lib().oracle_b3_spot_pixels(_p(xyz, c_int), P, _p(phi), K, _p(off, c_int), None, 0)
+    idx = np.empty(max(int(off[-1]), 1), np.int32)
+    rc = lib().oracle_b3_spot_pixels(_p(xyz, c_int), P, _p(phi), K, _p(off, c_int), _p(idx, c_int), idx.size)
+    assert rc == 0
+    return off, idx[:int(off[-1])]
+
+
+def gibbs_next(data, spot, offsets, idx, A, pi, var, iters, sample, intens, u):
+    """GibbsSampler2::next: data [T,P], spot [K,P], masks (offsets, idx),
+    sample [K,T] uint8 and intens [T,P] updated in place; u the draws.
+    Returns the number of draws used."""
+    data, spot = _d(data), _d(spot)
+    T, P = data.shape
+    K = spot.shape[0]
+    offsets = np.ascontiguousarray(offsets, np.int32)
+    idx = np.ascontiguousarray(idx if len(idx) else np.zeros(1), np.int32)
+    assert sample.dtype == np.uint8 and sample.flags.c_contiguous and sample.shape == (K, T)
+    assert intens.dtype == np.float64 and intens.flags.c_contiguous and intens.shape == (T, P)
+    u = _d(u)
+    used = ctypes.c_size_t(0)
+    assert u.size >= iters * K * T
+    lib().oracle_b3_gibbs_next(P, T, K, _p(data), _p(spot), _p(offsets, c_int), _p(idx, c_int), _p(_d(A)),
+                               _p(_d(pi)), var, iters, _p(sample, c_uint8), _p(intens), _p(u), ctypes.byref(used))
+    return used.value

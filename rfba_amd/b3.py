@@ -77,6 +77,13 @@ class _Block(ctypes.Structure):
 _sig = False
 
 
+class _GibbsChain(ctypes.Structure):
+    _fields_ = [("P", c_int), ("T", c_int), ("K", c_int), ("d_data", c_void_p), ("d_spot", c_void_p),
+                ("d_mask_off", c_void_p), ("d_mask", c_void_p), ("base", c_double), ("d_sample", c_void_p),
+                ("d_resid", c_void_p), ("d_samples", c_void_p), ("seed", ctypes.c_uint),
+                ("skip", ctypes.c_ulonglong)]
+
+
 def lib():
     global _sig
     L = _lib.load("rfba_3b")
@@ -101,6 +108,21 @@ def lib():
         L.rfba3b_batch_run_per_block.restype = c_int
         L.rfba3b_batch_destroy.argtypes = [c_void_p]
         L.rfba3b_batch_destroy.restype = None
+        L.rfba3b_gibbs_create.argtypes = [POINTER(_GibbsChain), c_int, POINTER(Model), POINTER(c_void_p)]
+        L.rfba3b_gibbs_create.restype = c_int
+        L.rfba3b_gibbs_reset.argtypes = [c_void_p, c_void_p]
+        L.rfba3b_gibbs_reset.restype = c_int
+        L.rfba3b_gibbs_run.argtypes = [c_void_p, c_int, c_int, POINTER(c_double), c_void_p, POINTER(c_float)]
+        L.rfba3b_gibbs_run.restype = c_int
+        L.rfba3b_gibbs_draws.argtypes = [c_void_p, c_int, POINTER(ctypes.c_ulonglong)]
+        L.rfba3b_gibbs_draws.restype = c_int
+        L.rfba3b_gibbs_destroy.argtypes = [c_void_p]
+        L.rfba3b_gibbs_destroy.restype = None
+        L.rfba3b_spot_pixels.argtypes = [POINTER(c_int), c_int, POINTER(c_double), c_int, POINTER(c_int),
+                                         POINTER(c_int), ctypes.c_size_t]
+        L.rfba3b_spot_pixels.restype = c_int
+        L.rfba3b_spot_intensities_device.argtypes = [c_void_p, c_int, c_void_p, c_int, c_double, c_void_p, c_void_p]
+        L.rfba3b_spot_intensities_device.restype = c_int
         L.rfba3b_last_error.argtypes = []
         L.rfba3b_last_error.restype = c_char_p
         _sig = True
@@ -250,6 +272,137 @@ class Batch:
             self.plan = c_void_p()
         for b in self._bufs:
             b.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------ Gibbs sampling
+def box_pixels(dims_xyz, origin_xyz=(0, 0, 0)):
+    """Pixel list [X*Y*Z, 3] of a dense box in the region order (z, then y,
+    then x), coordinates offset by origin."""
+    X, Y, Z = dims_xyz
+    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    xyz = np.stack([x.ravel() + origin_xyz[0], y.ravel() + origin_xyz[1], z.ravel() + origin_xyz[2]], 1)
+    return np.ascontiguousarray(xyz, np.int32)
+
+
+def spot_pixels(xyz, phi):
+    """get_spot_pixels (multispot5.cc:1001-1020) of spots phi [K,5] in the
+    pixel list xyz [P,3] (sorted by z, y, x): (offsets [K+1], indices)."""
+    xyz = np.ascontiguousarray(xyz, np.int32)
+    phi = np.ascontiguousarray(np.atleast_2d(phi), np.float64)
+    K, P = phi.shape[0], len(xyz)
+    off = np.empty(K + 1, np.int32)
+    _check(lib().rfba3b_spot_pixels(_p(xyz, c_int), P, _p(phi, c_double), K, _p(off, c_int), None, 0))
+    idx = np.empty(max(int(off[-1]), 1), np.int32)
+    _check(lib().rfba3b_spot_pixels(_p(xyz, c_int), P, _p(phi, c_double), K, _p(off, c_int), _p(idx, c_int),
+                                    idx.size))
+    return off, idx[:int(off[-1])]
+
+
+@dataclass
+class ChainData:
+    data: np.ndarray         # float32 [T, P] (or [T, Z, Y, X] over box_pixels order)
+    xyz: np.ndarray          # int32 [P, 3], sorted by (z, y, x)
+    spots: np.ndarray        # float64 [K, 5]
+    base: float = 0.0        # intensity of the empty sample
+    seed: int = 0
+    skip: int = 0
+    slots: int = 0           # S: sample copies kept by Gibbs.run (d_samples [S, K, T])
+
+
+class Gibbs:
+    """Device Gibbs sampler over chains (GibbsSampler2, one workgroup per
+    chain).  Spot tables are computed on the device, masks on the host."""
+
+    def __init__(self, chains: list[ChainData], model: Model):
+        self.model = model
+        self._bufs = []
+        self.meta = []
+        arr = (_GibbsChain * len(chains))()
+        for i, ch in enumerate(chains):
+            T = ch.data.shape[0]
+            data = np.ascontiguousarray(ch.data, np.float32).reshape(T, -1)
+            P = data.shape[1]
+            phi = np.ascontiguousarray(np.atleast_2d(ch.spots), np.float64).reshape(-1, 5)
+            K = phi.shape[0]
+            assert ch.xyz.shape == (P, 3)
+            off, idx = spot_pixels(ch.xyz, phi)
+            dd = DeviceBuffer.from_array(data)
+            dx = DeviceBuffer.from_array(np.ascontiguousarray(ch.xyz, np.int32))
+            dphi = DeviceBuffer.from_array(phi) if K else None
+            dspot = DeviceBuffer(max(K * P, 1) * 8)
+            doff = DeviceBuffer.from_array(off)
+            didx = DeviceBuffer.from_array(idx if idx.size else np.zeros(1, np.int32))
+            dsmp = DeviceBuffer(max(K * T, 1))
+            dres = DeviceBuffer(T * P * 8)
+            dout = DeviceBuffer(max(ch.slots * K * T, 1)) if ch.slots else None
+            if K:
+                _check(lib().rfba3b_spot_intensities_device(dx.ptr, P, dphi.ptr, K, model.zfactor, dspot.ptr, None))
+            self._bufs += [b for b in (dd, dx, dphi, dspot, doff, didx, dsmp, dres, dout) if b is not None]
+            self.meta.append(dict(P=P, T=T, K=K, slots=ch.slots, spot=dspot, sample=dsmp, resid=dres, out=dout,
+                                  offsets=off, idx=idx))
+            a = arr[i]
+            a.P, a.T, a.K = P, T, K
+            a.d_data, a.d_spot, a.d_mask_off, a.d_mask = dd.ptr, dspot.ptr, doff.ptr, didx.ptr
+            a.base = float(ch.base)
+            a.d_sample, a.d_resid = dsmp.ptr, dres.ptr
+            a.d_samples = dout.ptr if dout else None
+            a.seed = int(ch.seed) & 0xFFFFFFFF
+            a.skip = int(ch.skip)
+        self.plan = c_void_p()
+        _check(lib().rfba3b_gibbs_create(arr, len(chains), ctypes.byref(model), ctypes.byref(self.plan)))
+        self.reset()
+
+    def reset(self):
+        _check(lib().rfba3b_gibbs_reset(self.plan, None))
+
+    def run(self, samples=1, sample_iterations=1, variance=None) -> float:
+        """`samples` next() steps of every chain; returns the kernel time (ms)."""
+        for m in self.meta:
+            if m["out"] is not None and samples > m["slots"]:
+                raise B3Error("chain keeps %d sample slots, %d requested" % (m["slots"], samples))
+        var = None
+        if variance is not None:
+            var = np.ascontiguousarray(np.broadcast_to(np.asarray(variance, np.float64), (samples,)))
+        ms = c_float(0)
+        _check(lib().rfba3b_gibbs_run(self.plan, samples, sample_iterations,
+                                      _p(var, c_double) if var is not None else None, None, ctypes.byref(ms)))
+        return ms.value
+
+    def sample(self, i):
+        m = self.meta[i]
+        return m["sample"].to_array((m["K"], m["T"]), np.uint8)
+
+    def samples(self, i, n=None):
+        m = self.meta[i]
+        n = m["slots"] if n is None else n
+        return m["out"].to_array((n, m["K"], m["T"]), np.uint8)
+
+    def resid(self, i):
+        m = self.meta[i]
+        return m["resid"].to_array((m["T"], m["P"]), np.float64)
+
+    def spot_table(self, i):
+        m = self.meta[i]
+        return m["spot"].to_array((m["K"], m["P"]), np.float64)
+
+    def draws(self, i):
+        v = ctypes.c_ulonglong(0)
+        _check(lib().rfba3b_gibbs_draws(self.plan, i, ctypes.byref(v)))
+        return v.value
+
+    def close(self):
+        if self.plan:
+            lib().rfba3b_gibbs_destroy(self.plan)
+            self.plan = c_void_p()
+        for b in self._bufs:
+            b.close()
+        self._bufs = []
 
     def __del__(self):
         try:

@@ -52,7 +52,8 @@ TARGETS = {
     # 3B: FP32 pixel terms checked against an FP64 oracle with a tolerance;
     # contraction to FMA is allowed there.
     "b3": lambda: _hip(os.path.join(PKG, "librfba_3b.so"),
-                       [os.path.join(CSRC, "b3_kernel.hip"), os.path.join(CSRC, "b3_api.cpp")], contract="fast"),
+                       [os.path.join(CSRC, "b3_kernel.hip"), os.path.join(CSRC, "b3_gibbs.hip"),
+                        os.path.join(CSRC, "b3_api.cpp")], contract="fast"),
     "synth": lambda: ["g++", *GXX_FLAGS, "-O3", os.path.join(CSRC, "synth.cpp"),
                       "-o", os.path.join(PKG, "librfba_synth.so")],
     "oracle": lambda: ["g++", *GXX_FLAGS,

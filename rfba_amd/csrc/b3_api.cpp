@@ -120,6 +120,23 @@ int eval_grid(EvalFn fn, size_t lds, int nitems, int &grid) {
 
 }  // namespace
 
+namespace rfba3b {
+int host_fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+int host_dev_init(int *ncu) {
+    const int rc = dev_init();
+    if (ncu) *ncu = g_ncu;
+    return rc;
+}
+int host_check_model(const rfba3b_model *m) { return check_model(m); }
+std::mutex &host_mutex() { return g_mu; }
+}  // namespace rfba3b
+
 // ---------------------------------------------------------------------------
 // batched plan
 

@@ -2,9 +2,19 @@
 #pragma once
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/rfba_3b.h"
 
 namespace rfba3b {
+
+// Host helpers shared by the library's translation units (b3_api.cpp):
+// error message of rfba3b_last_error, device check (gfx950, CU count),
+// model validation and the library-wide call lock.
+int host_fail(int code, const char *fmt, ...);
+int host_dev_init(int *ncu);
+int host_check_model(const rfba3b_model *m);
+std::mutex &host_mutex();
 
 enum { MODE_LOGL = 0, MODE_GRAD = 1, MODE_DELTA = 2 };
 
