@@ -20,9 +20,12 @@ Beside it, in "workloads" of the same JSON line:
   b3_batched    1200 blocks of the same split (seed 8), K ~ Poisson(40)
                 clipped to 5-120, T=120, 32 parameter sets per block,
                 checkpoint stride 11: batched vs one launch per (block,
-                parameter set), the latter on the first 100 blocks.
+                parameter set), the latter on the first 100 blocks;
+  b3_gibbs      Gibbs sampling of the blink states of the same 1200 blocks
+                (true spot parameters): 10 samples of one sweep per run,
+                all blocks in one launch vs one launch per block (first 20).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only c2,c3,c4,c5]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only c2,c3,c4,c5,gibbs]
 
 Prints one JSON line.  value = SR voxels / s of the headline workload;
 roofline = algorithmic HBM bytes per launch / measured kernel time against
@@ -349,21 +352,36 @@ def bench_b3_single(args):
     return res
 
 
+_C5 = {}
+
+
+def c5_blocks():
+    """The 1200 blocks of the 512x512x128 doSplit (seed 8): K ~ Poisson(40)
+    clipped to 5-120, T=120; (data [T,Z,Y,X], true params [K,5], states
+    [K,T]) per block, generated once per process."""
+    if not _C5:
+        from rfba_amd import b3
+        from rfba_amd.blocks import volume_blocks
+        split = volume_blocks((512, 512, 128), 29, 3)
+        rng = np.random.default_rng(8)
+        idx = np.sort(rng.choice(len(split), 1200, replace=False))
+        K = np.clip(rng.poisson(40, 1200), 5, 120)
+        T = 120
+        dims = [split[i][1] for i in idx]
+        t0 = time.perf_counter()
+        _C5["gen"] = b3.synth_blocks(dims, T, K, 8000)
+        _C5["K"], _C5["T"] = K, T
+        log("c5: generated %d blocks in %.1f s" % (len(dims), time.perf_counter() - t0))
+    return _C5["gen"], _C5["K"], _C5["T"]
+
+
 def bench_b3_batched(args):
     from rfba_amd import b3
-    from rfba_amd.blocks import volume_blocks
     from rfba_amd.synth import default_threads
     m = b3_model(b3, 11)
-    split = volume_blocks((512, 512, 128), 29, 3)
-    rng = np.random.default_rng(8)
-    idx = np.sort(rng.choice(len(split), 1200, replace=False))
-    K = np.clip(rng.poisson(40, 1200), 5, 120)
-    T, Q = 120, 32
-    dims = [split[i][1] for i in idx]
-    t0 = time.perf_counter()
-    gen = b3.synth_blocks(dims, T, K, 8000)
+    gen, K, T = c5_blocks()
+    Q = 32
     blocks = [b3.BlockData(d, b3.perturb(p, Q, 80000 + i), s[None], 100.0) for i, (d, p, s) in enumerate(gen)]
-    log("c5: generated %d blocks in %.1f s" % (len(blocks), time.perf_counter() - t0))
     bt = b3.Batch(blocks, m)
     log("c5: warmup batched %.1f ms" % bt.run(use_graph=True))
     kms = []
@@ -426,6 +444,118 @@ def bench_b3_batched(args):
     return res
 
 
+def gibbs_changes(prev, samples):
+    """Frames whose on/off status changes at each spot update of a run:
+    prev [K,T] the sample before the run, samples [S,K,T] after each step
+    (one sweep per step)."""
+    on = np.concatenate([prev[None], samples]) == 0
+    return (on[1:] != on[:-1]).sum(axis=(0, 2))  # per spot, summed over steps
+
+
+def bench_b3_gibbs(args):
+    """Gibbs sampling of the blink states over the 1200 blocks: one run =
+    main.gibbs.samples = 10 next() steps of every block (multispot5.cfg:28),
+    each one sweep (gibbs.mixing_iterations = 1, :20)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from rfba_amd import b3
+    from rfba_amd.synth import default_threads
+    m = b3_model(b3, 0)
+    gen, K, T = c5_blocks()
+    S = 10
+    t0 = time.perf_counter()
+    chains = []
+    for i, (d, p, _) in enumerate(gen):
+        Tz, Z, Y, X = d.shape
+        chains.append(b3.ChainData(d.reshape(Tz, -1), b3.box_pixels((X, Y, Z)), p, base=100.0, seed=1000 + i,
+                                   slots=S))
+    g = b3.Gibbs(chains, m)
+    log("gibbs: %d chains set up in %.1f s" % (len(chains), time.perf_counter() - t0))
+    masks = [int(mm["offsets"][-1]) for mm in g.meta]
+    log("gibbs: warmup %.1f ms" % g.run(S, 1))
+    first = [g.samples(i, 1)[0].copy() for i in range(min(16, len(chains)))] if not args.no_cpu else []
+    kms, walls = [], []
+    for i in range(max(1, args.steps)):
+        prev = [g.sample(c) for c in range(len(chains))]
+        t1 = time.perf_counter()
+        kms.append(g.run(S, 1))
+        walls.append((time.perf_counter() - t1) * 1e3)
+        log("gibbs: step %d %.1f ms" % (i, kms[-1]))
+    # algorithmic bytes of the last run: per spot update, the FP64 residual
+    # over the mask in every frame (8 T m), the mask's indices and spot values
+    # (12 m), and a read-modify-write of the residual over the mask in every
+    # frame whose on/off status changed (16 m per change)
+    by = 0.0
+    ups = 0
+    for c, mm in enumerate(g.meta):
+        off = mm["offsets"].astype(np.int64)
+        msz = off[1:] - off[:-1]
+        ch = gibbs_changes(prev[c], g.samples(c, S))
+        by += float(S * (8.0 * T * msz + 12.0 * msz).sum() + 16.0 * (msz * ch).sum())
+        ups += S * mm["K"]
+    kb = float(np.mean(kms))
+    wall = float(np.mean(walls))
+    roof = hbm_roof(by, kb)
+    roof.update({"kernel": "b3_gibbs (one workgroup per chain)", "kernel_ms": kb,
+                 "algorithmic_bytes_per_launch": by})
+    # plain form: one launch per block (a plan per chain) on the first blocks
+    nsub = min(20, len(chains))
+    subs = [b3.Gibbs([chains[i]], m) for i in range(nsub)]
+    for sg in subs:
+        sg.run(S, 1)
+    t1 = time.perf_counter()
+    ms_pb = sum(sg.run(S, 1) for sg in subs)
+    wall_pb = (time.perf_counter() - t1) * 1e3
+    ups_sub = S * int(sum(chains[i].spots.shape[0] for i in range(nsub)))
+    for sg in subs:
+        sg.close()
+    g.close()
+    res = {
+        "workload": "Gibbs sampling of blink states (GibbsSampler2::next): the 1200 blocks of the 512x512x128 "
+                    "doSplit (seed 8), K ~ Poisson(40) clipped 5-120, T=120, masks of radius 6 sigma + 1 "
+                    "(sigma 1.3), 10 samples of one sweep each per run, MT19937 per block",
+        "chains": len(chains), "spot_updates": ups, "mask_pixels_mean": float(np.sum(masks) / max(1, K.sum())),
+        "batched": {"ms": wall, "kernel_ms": kb, "spot_updates_per_s": ups / (wall * 1e-3),
+                    "block_samples_per_s": len(chains) * S / (wall * 1e-3)},
+        "plain_per_block": {"blocks": nsub, "ms": wall_pb, "kernel_ms": ms_pb,
+                            "spot_updates_per_s": ups_sub / (wall_pb * 1e-3)},
+        "speedup_batched_vs_per_block": (ups / wall) / (ups_sub / wall_pb),
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        from oracle import b3oracle as bo
+        th = default_threads()
+        n = len(first)
+
+        def one(i):
+            d, p, _ = gen[i]
+            Tz = d.shape[0]
+            Z, Y, X = d.shape[1:]
+            xyz = b3.box_pixels((X, Y, Z))
+            data = d.reshape(Tz, -1).astype(np.float64) - 100.0
+            spot = np.stack([bo.spot(xyz, q, zfactor=m.zfactor)[0] for q in p])
+            off, idx = bo.spot_pixels(xyz, p)
+            u = b3.mt19937_uniforms(1000 + i, len(p) * Tz)
+            smp = np.full((len(p), Tz), 2, np.uint8)
+            intens = np.zeros(data.shape)
+            t = time.perf_counter()
+            bo.gibbs_next(data, spot, off, idx, np.array(m.A[:]).reshape(3, 3), np.array(m.pi[:]), m.variance, 1,
+                          smp, intens, u)
+            return time.perf_counter() - t, smp, len(p)
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(th) as ex:
+            out = list(ex.map(one, range(n)))
+        dt = time.perf_counter() - t0
+        res["first_sample_equals_oracle"] = {"blocks": n,
+                                             "equal": bool(all(np.array_equal(o[1], first[i])
+                                                               for i, o in enumerate(out)))}
+        res["cpu_baseline"] = {"value": sum(o[2] for o in out) / dt, "unit": "spot-updates/s", "cores": th,
+                               "kind": "port",
+                               "sample": "oracle GibbsSampler2::next, first sweep of blocks 0-%d, %d threads, "
+                                         "%.1f s" % (n - 1, th, dt)}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -434,8 +564,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-crop", type=str, default="12,12,8")
     ap.add_argument("--frames", type=int, default=200)
-    ap.add_argument("--only", type=str, default="c2,c3,c4,c5",
-                    help="workloads to run: c2 (headline stack), c3 (batched SRRF), c4, c5 (3B)")
+    ap.add_argument("--only", type=str, default="c2,c3,c4,c5,gibbs",
+                    help="workloads to run: c2 (headline stack), c3 (batched SRRF), c4, c5 (3B), "
+                         "gibbs (3B blink-state sampling)")
     ap.add_argument("--c5-plain-blocks", type=int, default=100)
     ap.add_argument("--traffic", type=float, default=None,
                     help="measured HBM bytes per launch of the headline kernel (rocprofv3 FETCH_SIZE+WRITE_SIZE)")
@@ -468,6 +599,8 @@ def main():
         rec["workloads"]["b3_single"] = bench_b3_single(args)
     if "c5" in only:
         rec["workloads"]["b3_batched"] = bench_b3_batched(args)
+    if "gibbs" in only:
+        rec["workloads"]["b3_gibbs"] = bench_b3_gibbs(args)
     print(json.dumps(rec), flush=True)
 
 

@@ -79,8 +79,8 @@ _sig = False
 
 class _GibbsChain(ctypes.Structure):
     _fields_ = [("P", c_int), ("T", c_int), ("K", c_int), ("d_data", c_void_p), ("d_spot", c_void_p),
-                ("d_mask_off", c_void_p), ("d_mask", c_void_p), ("base", c_double), ("d_sample", c_void_p),
-                ("d_resid", c_void_p), ("d_samples", c_void_p), ("seed", ctypes.c_uint),
+                ("d_mask_off", c_void_p), ("d_mask", c_void_p), ("base", c_double), ("xyz", c_void_p),
+                ("d_sample", c_void_p), ("d_samples", c_void_p), ("seed", ctypes.c_uint),
                 ("skip", ctypes.c_ulonglong)]
 
 
@@ -114,6 +114,8 @@ def lib():
         L.rfba3b_gibbs_reset.restype = c_int
         L.rfba3b_gibbs_run.argtypes = [c_void_p, c_int, c_int, POINTER(c_double), c_void_p, POINTER(c_float)]
         L.rfba3b_gibbs_run.restype = c_int
+        L.rfba3b_gibbs_sample_intensities.argtypes = [c_void_p, c_int, c_void_p, c_void_p]
+        L.rfba3b_gibbs_sample_intensities.restype = c_int
         L.rfba3b_gibbs_draws.argtypes = [c_void_p, c_int, POINTER(ctypes.c_ulonglong)]
         L.rfba3b_gibbs_draws.restype = c_int
         L.rfba3b_gibbs_destroy.argtypes = [c_void_p]
@@ -317,46 +319,50 @@ class ChainData:
 
 class Gibbs:
     """Device Gibbs sampler over chains (GibbsSampler2, one workgroup per
-    chain).  Spot tables are computed on the device, masks on the host."""
+    chain).  Spot tables are computed on the device, masks on the host.
+    bricks=False keeps each residual in the pixel list's order."""
 
-    def __init__(self, chains: list[ChainData], model: Model):
+    def __init__(self, chains: list[ChainData], model: Model, bricks=True):
         self.model = model
         self._bufs = []
         self.meta = []
         arr = (_GibbsChain * len(chains))()
+        keep = []
         for i, ch in enumerate(chains):
             T = ch.data.shape[0]
             data = np.ascontiguousarray(ch.data, np.float32).reshape(T, -1)
             P = data.shape[1]
             phi = np.ascontiguousarray(np.atleast_2d(ch.spots), np.float64).reshape(-1, 5)
             K = phi.shape[0]
-            assert ch.xyz.shape == (P, 3)
-            off, idx = spot_pixels(ch.xyz, phi)
+            xyz = np.ascontiguousarray(ch.xyz, np.int32)
+            assert xyz.shape == (P, 3)
+            keep.append(xyz)
+            off, idx = spot_pixels(xyz, phi)
             dd = DeviceBuffer.from_array(data)
-            dx = DeviceBuffer.from_array(np.ascontiguousarray(ch.xyz, np.int32))
+            dx = DeviceBuffer.from_array(xyz)
             dphi = DeviceBuffer.from_array(phi) if K else None
             dspot = DeviceBuffer(max(K * P, 1) * 8)
             doff = DeviceBuffer.from_array(off)
             didx = DeviceBuffer.from_array(idx if idx.size else np.zeros(1, np.int32))
             dsmp = DeviceBuffer(max(K * T, 1))
-            dres = DeviceBuffer(T * P * 8)
             dout = DeviceBuffer(max(ch.slots * K * T, 1)) if ch.slots else None
             if K:
                 _check(lib().rfba3b_spot_intensities_device(dx.ptr, P, dphi.ptr, K, model.zfactor, dspot.ptr, None))
-            self._bufs += [b for b in (dd, dx, dphi, dspot, doff, didx, dsmp, dres, dout) if b is not None]
-            self.meta.append(dict(P=P, T=T, K=K, slots=ch.slots, spot=dspot, sample=dsmp, resid=dres, out=dout,
+            self._bufs += [b for b in (dd, dx, dphi, dspot, doff, didx, dsmp, dout) if b is not None]
+            self.meta.append(dict(P=P, T=T, K=K, slots=ch.slots, spot=dspot, sample=dsmp, out=dout,
                                   offsets=off, idx=idx))
             a = arr[i]
             a.P, a.T, a.K = P, T, K
             a.d_data, a.d_spot, a.d_mask_off, a.d_mask = dd.ptr, dspot.ptr, doff.ptr, didx.ptr
             a.base = float(ch.base)
-            a.d_sample, a.d_resid = dsmp.ptr, dres.ptr
+            a.xyz = xyz.ctypes.data if bricks else None
+            a.d_sample = dsmp.ptr
             a.d_samples = dout.ptr if dout else None
             a.seed = int(ch.seed) & 0xFFFFFFFF
             a.skip = int(ch.skip)
         self.plan = c_void_p()
         _check(lib().rfba3b_gibbs_create(arr, len(chains), ctypes.byref(model), ctypes.byref(self.plan)))
-        self.reset()
+        del keep
 
     def reset(self):
         _check(lib().rfba3b_gibbs_reset(self.plan, None))
@@ -383,9 +389,15 @@ class Gibbs:
         n = m["slots"] if n is None else n
         return m["out"].to_array((n, m["K"], m["T"]), np.uint8)
 
-    def resid(self, i):
+    def sample_intensities(self, i):
+        """Current sample intensities + base of chain i, [T, P]."""
         m = self.meta[i]
-        return m["resid"].to_array((m["T"], m["P"]), np.float64)
+        out = DeviceBuffer(m["T"] * m["P"] * 8)
+        try:
+            _check(lib().rfba3b_gibbs_sample_intensities(self.plan, i, out.ptr, None))
+            return out.to_array((m["T"], m["P"]), np.float64)
+        finally:
+            out.close()
 
     def spot_table(self, i):
         m = self.meta[i]

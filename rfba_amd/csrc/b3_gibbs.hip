@@ -32,16 +32,20 @@
 namespace rfba3b {
 
 constexpr int kGibbsThreads = 256;
-constexpr int kMaskChunk = 1024;  // mask pixels staged in LDS at a time
+constexpr int kMaskChunk = 2048;  // mask pixels staged in LDS at a time
 constexpr int kMtN = 624;         // MT19937 state words (mersenne.cpp MERS_N)
 
+typedef __attribute__((address_space(1))) double gdouble;
+
 struct GibbsDev {
-    int P, T, K, pad;
+    int P, T, K, S;       // S: residual slots per frame (>= P)
     const double *spot;   // [K][P]
     const int *moff;      // [K + 1]
-    const int *midx;      // mask pixel indices
+    const int *mpix;      // mask entries: pixel index (sorted by slot within a spot)
+    const int *mslot;     // mask entries: residual slot
+    const int *slot;      // [P] residual slot of each pixel
     uint8_t *sample;      // [K][T]
-    double *resid;        // [T][P]
+    double *resid;        // [T][S]
     uint8_t *samples;     // [S][K][T] or null
     const float *data;    // [T][P]
     double base;
@@ -116,19 +120,21 @@ __device__ __forceinline__ int select3(double w0, double w1, double w2, double u
 
 // Forward recursion over normalised messages and backward sampling, on one
 // lane.  ex[3t..3t+2] = (x_t, b_on, b_off) with x_t = B_on - B_off,
-// b_on = e^min(x,0), b_off = e^min(-x,0).  alpha_t is the normalised forward
-// message (forward_algorithm_delta2's delta_t minus a per-frame constant);
-// backward sampling (forward_algorithm.h:542-566) draws the last frame from
-// alpha_{T-1} and frame t from alpha_t(i) A[i][s_{t+1}], which is
-// sample_unscaled_log's distribution (:520-532).
-__device__ void ffbs_lane(const GibbsParams &G, int T, const double *ex, double *alpha, const uint32_t *ring,
-                          int head, int RC, uint8_t *sn) {
-    const double *A = G.A;
+// b_on = e^min(x,0), b_off = e^min(-x,0).  alpha_t is the forward message
+// up to a positive per-frame factor (forward_algorithm_delta2's delta_t
+// minus a per-frame constant); backward sampling (forward_algorithm.h:
+// 542-566) draws the last frame from alpha_{T-1} and frame t from
+// alpha_t(i) A[i][s_{t+1}], which is sample_unscaled_log's distribution
+// (:520-532) -- a per-frame factor changes neither.  The frames whose on/off
+// status changes are listed in chg as (t << 1) | was_on; returns their count.
+__device__ int ffbs_lane(const double *Api, int T, const double *ex, double *alpha, const uint32_t *ring,
+                         int head, int RC, const uint8_t *so, uint8_t *sn, int *chg) {
+    const double *A = Api, *pi = Api + 9;
     double a0 = 0, a1 = 0, a2 = 0;
     for (int t = 0; t < T; t++) {
         double p0, p1, p2;
         if (t == 0) {
-            p0 = G.pi[0]; p1 = G.pi[1]; p2 = G.pi[2];
+            p0 = pi[0]; p1 = pi[1]; p2 = pi[2];
         } else {
             p0 = a0 * A[0] + a1 * A[3] + a2 * A[6];
             p1 = a0 * A[1] + a1 * A[4] + a2 * A[7];
@@ -138,7 +144,8 @@ __device__ void ffbs_lane(const GibbsParams &G, int T, const double *ex, double 
         const double n0 = p0 * bon, n1 = p1 * boff, n2 = p2 * boff;
         const double c = n0 + n1 + n2;
         if (c > 1e-250) {
-            const double ic = 1.0 / c;
+            // any positive scale will do: the hardware reciprocal
+            const double ic = __builtin_amdgcn_rcp(c);
             a0 = n0 * ic; a1 = n1 * ic; a2 = n2 * ic;
         } else {
             // the same step in log space (the emission ratio underflowed)
@@ -171,13 +178,39 @@ __device__ void ffbs_lane(const GibbsParams &G, int T, const double *ex, double 
         next = select3(alpha[3 * t] * A[next], alpha[3 * t + 1] * A[3 + next], alpha[3 * t + 2] * A[6 + next], u);
         sn[t] = (uint8_t)next;
     }
+    int n = 0;
+    for (int t = 0; t < T; t++) {
+        const bool was = so[t] == 0, now = sn[t] == 0;
+        if (was != now) chg[n++] = (t << 1) | (was ? 1 : 0);
+    }
+    return n;
+}
+
+// Sums of kFB per-lane values over the wave in 10 exchanges: halving the
+// set of values at each of the first three steps; lane l ends with the total
+// of value (l >> 3) & 7.
+constexpr int kFB = 8;  // frames per wave batch
+__device__ __forceinline__ double wave_sum8(const double (&a)[kFB], int lane) {
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+    double v4[4], v2[2];
+#pragma unroll
+    for (int j = 0; j < 4; j++) v4[j] = (b5 ? a[j + 4] : a[j]) + __shfl_xor(b5 ? a[j] : a[j + 4], 32);
+#pragma unroll
+    for (int j = 0; j < 2; j++) v2[j] = (b4 ? v4[j + 2] : v4[j]) + __shfl_xor(b4 ? v4[j] : v4[j + 2], 16);
+    double v = (b3 ? v2[1] : v2[0]) + __shfl_xor(b3 ? v2[0] : v2[1], 8);
+    v += __shfl_xor(v, 4);
+    v += __shfl_xor(v, 2);
+    v += __shfl_xor(v, 1);
+    return v;
 }
 
 __global__ __launch_bounds__(kGibbsThreads) void b3_gibbs(GibbsParams G) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    constexpr int kWaves = kGibbsThreads / 64;
     const GibbsDev c = G.chains[G.order[blockIdx.x]];
-    const int T = c.T, P = c.P, K = c.K, Tc = G.Tcap;
+    const int T = c.T, P = c.S, K = c.K, Tc = G.Tcap;  // P: the residual's row length
     const int RC = Tc + kMtN;  // draw ring capacity
 
     double *alpha = (double *)lds;          // [Tc][3]
@@ -185,52 +218,76 @@ __global__ __launch_bounds__(kGibbsThreads) void b3_gibbs(GibbsParams G) {
     double *dsum = ex + 3 * Tc;             // [Tc]
     double *muL = dsum + Tc;                // [kMaskChunk]
     int *idxL = (int *)(muL + kMaskChunk);  // [kMaskChunk]
-    uint32_t *mt = (uint32_t *)(idxL + kMaskChunk);  // [kMtN]
+    int *chg = idxL + kMaskChunk;           // [Tc] frames whose on/off status changed
+    int *nchg = chg + Tc;                   // [4]
+    uint32_t *mt = (uint32_t *)(nchg + 4);  // [kMtN]
     uint32_t *ring = mt + kMtN;             // [RC]
     uint8_t *so = (uint8_t *)(ring + RC);   // [Tc] state before the update
     uint8_t *sn = so + Tc;                  // [Tc] state after the update
+    __shared__ double Api[12];              // A[9], pi[3]
+    // the residual through global-address-space pointers: rows are uniform
+    // per wave, pixel offsets per lane
+    gdouble *R = (gdouble *)c.resid;
 
+    if (tid < 9) Api[tid] = G.A[tid];
+    else if (tid < 12) Api[tid] = G.pi[tid - 9];
     for (int i = tid; i < kMtN; i += kGibbsThreads) mt[i] = c.mt[i];
     const int pos0 = (int)c.mt[kMtN];
     __syncthreads();
     // Every thread tracks the ring's head and count; they change uniformly.
     int head = 0, cnt = kMtN - pos0;
     for (int i = tid; i < cnt; i += kGibbsThreads) ring[i] = mt_temper(mt[pos0 + i]);
+    const int nblk = (T + kFB - 1) / kFB;
 
     for (int s = 0; s < G.samples; s++) {
         const double i2v = 0.5 / G.var[s];
         for (int it = 0; it < G.iters; it++) {
             for (int k = 0; k < K; k++) {
                 const int m0 = c.moff[k], m = c.moff[k + 1] - m0;
-                const double *spk = c.spot + (size_t)k * P;
+                const double *spk = c.spot + (size_t)k * c.P;
                 uint8_t *smp = c.sample + (size_t)k * T;
                 for (int t = tid; t < T; t += kGibbsThreads) {
                     so[t] = smp[t];
                     dsum[t] = 0.0;
                 }
                 __syncthreads();
-                // 1. d_t over the mask, the spot taken out where it was on
+                // 1. d_t over the mask, the spot taken out where it was on;
+                //    each wave takes kFB consecutive frames at a time
                 for (int c0 = 0; c0 < m; c0 += kMaskChunk) {
                     const int nc = min(kMaskChunk, m - c0);
                     for (int i = tid; i < nc; i += kGibbsThreads) {
-                        const int p = c.midx[m0 + c0 + i];
-                        idxL[i] = p;
-                        muL[i] = spk[p];
+                        idxL[i] = c.mslot[m0 + c0 + i];
+                        muL[i] = spk[c.mpix[m0 + c0 + i]];
                     }
                     __syncthreads();
-                    for (int t = wave; t < T; t += kGibbsThreads / 64) {
-                        const bool on = so[t] == 0;
-                        const double *rt = c.resid + (size_t)t * P;
-                        double acc = 0.0;
-#pragma unroll 4
-                        for (int i = lane; i < nc; i += 64) {
-                            const double mu = muL[i];
-                            double r = rt[idxL[i]];
-                            if (on) r += mu;
-                            acc += mu * (mu - 2.0 * r);
+                    for (int b = wave; b < nblk; b += kWaves) {
+                        const int t0 = b * kFB;
+                        const gdouble *rt[kFB];
+                        bool on[kFB];
+                        double acc[kFB];
+#pragma unroll
+                        for (int f = 0; f < kFB; f++) {
+                            const int t = min(t0 + f, T - 1);  // frames past T repeat the last, discarded
+                            rt[f] = R + (size_t)t * P;
+                            on[f] = so[t] == 0;
+                            acc[f] = 0.0;
                         }
-                        acc = wave_sum(acc);
-                        if (lane == 0) dsum[t] += acc;
+#pragma unroll 2
+                        for (int i = lane; i < nc; i += 64) {
+                            const int p = idxL[i];
+                            const double mu = muL[i];
+                            double r[kFB];
+#pragma unroll
+                            for (int f = 0; f < kFB; f++) r[f] = rt[f][p];
+#pragma unroll
+                            for (int f = 0; f < kFB; f++) {
+                                const double r0 = on[f] ? r[f] + mu : r[f];
+                                acc[f] += mu * (mu - 2.0 * r0);
+                            }
+                        }
+                        const double v = wave_sum8(acc, lane);
+                        const int t = t0 + ((lane >> 3) & 7);
+                        if ((lane & 7) == 0 && t < T) dsum[t] += v;
                     }
                     __syncthreads();
                 }
@@ -251,34 +308,50 @@ __global__ __launch_bounds__(kGibbsThreads) void b3_gibbs(GibbsParams G) {
                     cnt += kMtN;
                 }
                 __syncthreads();
-                if (tid == 0) ffbs_lane(G, T, ex, alpha, ring, head, RC, sn);
+                if (tid == 0) nchg[0] = ffbs_lane(Api, T, ex, alpha, ring, head, RC, so, sn, chg);
                 head += T;
                 head = head >= RC ? head - RC : head;
                 cnt -= T;
                 __syncthreads();
-                // 3. the new sample, and the residual where on/off changed
+                // 3. the new sample, and the residual in the frames whose
+                //    on/off status changed (kFB of them per wave at a time)
                 for (int t = tid; t < T; t += kGibbsThreads) smp[t] = sn[t];
+                const int nch = nchg[0];
                 const bool restage = m > kMaskChunk;
-                for (int c0 = 0; c0 < m; c0 += kMaskChunk) {
-                    const int nc = min(kMaskChunk, m - c0);
-                    if (restage) {
-                        for (int i = tid; i < nc; i += kGibbsThreads) {
-                            const int p = c.midx[m0 + c0 + i];
-                            idxL[i] = p;
-                            muL[i] = spk[p];
+                if (nch > 0) {
+                    for (int c0 = 0; c0 < m; c0 += kMaskChunk) {
+                        const int nc = min(kMaskChunk, m - c0);
+                        if (restage) {
+                            for (int i = tid; i < nc; i += kGibbsThreads) {
+                                idxL[i] = c.mslot[m0 + c0 + i];
+                                muL[i] = spk[c.mpix[m0 + c0 + i]];
+                            }
+                            __syncthreads();
                         }
-                        __syncthreads();
-                    }
-                    for (int t = wave; t < T; t += kGibbsThreads / 64) {
-                        const bool was = so[t] == 0, now = sn[t] == 0;
-                        if (was == now) continue;
-                        double *rt = c.resid + (size_t)t * P;
-                        for (int i = lane; i < nc; i += 64) {
-                            const int p = idxL[i];
-                            rt[p] = was ? rt[p] + muL[i] : rt[p] - muL[i];
+                        for (int j0 = wave * kFB; j0 < nch; j0 += kWaves * kFB) {
+                            gdouble *rt[kFB];
+                            double sg[kFB];
+#pragma unroll
+                            for (int f = 0; f < kFB; f++) {
+                                const int e = __builtin_amdgcn_readfirstlane(chg[min(j0 + f, nch - 1)]);
+                                rt[f] = R + (size_t)(e >> 1) * P;
+                                sg[f] = j0 + f < nch ? ((e & 1) ? 1.0 : -1.0) : 0.0;  // +mu: spot removed
+                            }
+#pragma unroll 1
+                            for (int i = lane; i < nc; i += 64) {
+                                const int p = idxL[i];
+                                const double mu = muL[i];
+                                double r[kFB];
+#pragma unroll
+                                for (int f = 0; f < kFB; f++)
+                                    if (sg[f] != 0.0) r[f] = rt[f][p];
+#pragma unroll
+                                for (int f = 0; f < kFB; f++)
+                                    if (sg[f] != 0.0) rt[f][p] = r[f] + sg[f] * mu;
+                            }
                         }
+                        if (restage) __syncthreads();
                     }
-                    if (restage) __syncthreads();
                 }
                 __syncthreads();
             }
@@ -300,38 +373,21 @@ __global__ void b3_gibbs_reset(const GibbsDev *chains) {
     const GibbsDev c = chains[blockIdx.y];
     const size_t nt = (size_t)c.T * c.P, ns = (size_t)c.K * c.T;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nt; e += stride)
-        c.resid[e] = (double)c.data[e] - c.base;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nt; e += stride) {
+        const size_t t = e / c.P, p = e - t * c.P;
+        c.resid[t * c.S + c.slot[p]] = (double)c.data[e] - c.base;
+    }
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < ns; e += stride) c.sample[e] = 2;
 }
 
-// Mask checks of one chain per workgroup: offsets nondecreasing from 0,
-// indices in [0, P) and strictly increasing within each spot.  out[2c] =
-// number of violations, out[2c + 1] = mask entries.
-__global__ void b3_gibbs_check(const GibbsDev *chains, int *out) {
-    const GibbsDev c = chains[blockIdx.x];
-    __shared__ int bad;
-    if (threadIdx.x == 0) bad = 0;
-    __syncthreads();
-    int nb = 0;
-    if (threadIdx.x == 0 && c.K > 0 && c.moff[0] != 0) nb++;
-    for (int k = 0; k < c.K; k++) {
-        const int a = c.moff[k], b = c.moff[k + 1];
-        if (b < a) {
-            nb += threadIdx.x == 0;
-            continue;
-        }
-        for (int i = a + threadIdx.x; i < b; i += blockDim.x) {
-            const int p = c.midx[i];
-            if (p < 0 || p >= c.P) nb++;
-            else if (i > a && c.midx[i - 1] >= p) nb++;
-        }
-    }
-    if (nb) atomicAdd(&bad, nb);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        out[2 * blockIdx.x] = bad;
-        out[2 * blockIdx.x + 1] = c.K > 0 ? c.moff[c.K] : 0;
+// sample intensities (+ base) in the pixel list's order: data - residual
+__global__ void b3_gibbs_export(const GibbsDev *chains, int i, double *out) {
+    const GibbsDev c = chains[i];
+    const size_t nt = (size_t)c.T * c.P;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nt; e += stride) {
+        const size_t t = e / c.P, p = e - t * c.P;
+        out[e] = (double)c.data[e] - c.resid[t * c.S + c.slot[p]];
     }
 }
 
@@ -402,19 +458,51 @@ struct HostMt {
 };
 
 size_t gibbs_lds(int Tc) {
-    return 8 * (7 * (size_t)Tc + kMaskChunk) + 4 * (size_t)kMaskChunk + 4 * ((size_t)kMtN + Tc + kMtN) +
-           2 * (size_t)Tc;
+    return 8 * (7 * (size_t)Tc + kMaskChunk) + 4 * ((size_t)kMaskChunk + Tc + 4) +
+           4 * ((size_t)kMtN + Tc + kMtN) + 2 * (size_t)Tc;
 }
 
 }  // namespace
 
+// Residual layout of a chain: with coordinates, voxel (x, y, z) of the
+// pixels' bounding box goes to brick ((z/2 * NBY + y/2) * NBX + x/4), slot
+// (z&1) * 8 + (y&1) * 4 + (x&3) within the brick's 16 (one 128-byte line of
+// FP64); a region whose bricks would hold more than twice its pixels keeps
+// the list order.  Returns the slots per frame.
+static int brick_layout(const int *xyz, int P, std::vector<int> &slot) {
+    slot.resize(P);
+    if (xyz && P > 0) {
+        int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+        for (int p = 0; p < P; p++)
+            for (int j = 0; j < 3; j++) {
+                lo[j] = std::min(lo[j], xyz[3 * p + j]);
+                hi[j] = std::max(hi[j], xyz[3 * p + j]);
+            }
+        const long long nbx = ((long long)hi[0] - lo[0] + 4) / 4, nby = ((long long)hi[1] - lo[1] + 2) / 2,
+                        nbz = ((long long)hi[2] - lo[2] + 2) / 2;
+        const long long S = nbx * nby * nbz * 16;
+        if (S <= 2LL * P + 4096) {
+            for (int p = 0; p < P; p++) {
+                const long long x = xyz[3 * p] - lo[0], y = xyz[3 * p + 1] - lo[1], z = xyz[3 * p + 2] - lo[2];
+                slot[p] = (int)((((z >> 1) * nby + (y >> 1)) * nbx + (x >> 2)) * 16 + (z & 1) * 8 + (y & 1) * 4 + (x & 3));
+            }
+            return (int)S;
+        }
+    }
+    for (int p = 0; p < P; p++) slot[p] = p;
+    return P;
+}
+
 struct rfba3b_gibbs {
     rfba3b_model model;
     std::vector<rfba3b_gibbs_chain> chains;
+    std::vector<GibbsDev> dev;
     std::vector<unsigned long long> draws;
     GibbsDev *d_chains = nullptr;
     int *d_order = nullptr;
     uint32_t *d_mt = nullptr;
+    int *d_ints = nullptr;       // offsets, mask pixels, mask slots, pixel slots of every chain
+    double *d_resid = nullptr;   // residuals of every chain
     double *d_var = nullptr;
     int var_cap = 0;
     int Tmax = 0;
@@ -423,6 +511,8 @@ struct rfba3b_gibbs {
         if (d_chains) (void)hipFree(d_chains);
         if (d_order) (void)hipFree(d_order);
         if (d_mt) (void)hipFree(d_mt);
+        if (d_ints) (void)hipFree(d_ints);
+        if (d_resid) (void)hipFree(d_resid);
         if (d_var) (void)hipFree(d_var);
     }
 };
@@ -438,8 +528,7 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
     for (int i = 0; i < n; i++) {
         const rfba3b_gibbs_chain &c = chains[i];
         if (c.P <= 0 || c.T <= 0 || c.K < 0) return host_fail(-1, "chain %d: P and T must be positive, K >= 0", i);
-        if ((size_t)c.P * c.T > (size_t)LLONG_MAX / 8) return host_fail(-1, "chain %d: T x P too large", i);
-        if (!c.d_data || !c.d_resid || !c.d_sample) return host_fail(-1, "chain %d: data, residual or sample is NULL", i);
+        if (!c.d_data || !c.d_sample) return host_fail(-1, "chain %d: data or sample is NULL", i);
         if (c.K > 0 && (!c.d_spot || !c.d_mask_off || !c.d_mask))
             return host_fail(-1, "chain %d: spot table or masks are NULL", i);
         if (!std::isfinite(c.base)) return host_fail(-1, "chain %d: base is not finite", i);
@@ -456,8 +545,59 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
     g->Tmax = Tmax;
     g->lds = lds;
 
+    // masks to the host: checked, mapped to residual slots, sorted by slot
+    std::vector<int> ints;                 // all chains' integer tables, concatenated
+    std::vector<size_t> o_off(n), o_pix(n), o_slotm(n), o_slot(n), o_res(n);
+    std::vector<int> S(n);
+    std::vector<long long> work(n);
+    size_t nres = 0;
+    for (int i = 0; i < n; i++) {
+        const rfba3b_gibbs_chain &c = chains[i];
+        std::vector<int> off(c.K + 1, 0), mask, slot;
+        if (c.K > 0) {
+            GHIP(hipMemcpy(off.data(), c.d_mask_off, (c.K + 1) * sizeof(int), hipMemcpyDeviceToHost));
+            if (off[0] != 0) return host_fail(-1, "chain %d: mask offsets must start at 0", i);
+            for (int k = 0; k < c.K; k++)
+                if (off[k + 1] < off[k]) return host_fail(-1, "chain %d: mask offsets decrease at spot %d", i, k);
+            mask.resize(off[c.K]);
+            if (!mask.empty())
+                GHIP(hipMemcpy(mask.data(), c.d_mask, mask.size() * sizeof(int), hipMemcpyDeviceToHost));
+            for (int k = 0; k < c.K; k++)
+                for (int j = off[k]; j < off[k + 1]; j++)
+                    if (mask[j] < 0 || mask[j] >= c.P || (j > off[k] && mask[j - 1] >= mask[j]))
+                        return host_fail(-1, "chain %d: mask of spot %d is not increasing within [0, P) at %d", i,
+                                         k, j - off[k]);
+        }
+        S[i] = brick_layout(c.xyz, c.P, slot);
+        if ((size_t)S[i] * c.T > (size_t)LLONG_MAX / 8) return host_fail(-1, "chain %d: T x P too large", i);
+        o_off[i] = ints.size();
+        ints.insert(ints.end(), off.begin(), off.end());
+        o_pix[i] = ints.size();
+        o_slotm[i] = o_pix[i] + mask.size();
+        ints.resize(ints.size() + 2 * mask.size());
+        for (int k = 0; k < c.K; k++) {
+            std::vector<std::pair<int, int>> e;  // (slot, pixel)
+            for (int j = off[k]; j < off[k + 1]; j++) e.emplace_back(slot[mask[j]], mask[j]);
+            std::sort(e.begin(), e.end());
+            for (size_t j = 0; j < e.size(); j++) {
+                ints[o_pix[i] + off[k] + j] = e[j].second;
+                ints[o_slotm[i] + off[k] + j] = e[j].first;
+            }
+        }
+        o_slot[i] = ints.size();
+        ints.insert(ints.end(), slot.begin(), slot.end());
+        o_res[i] = nres;
+        nres += ((size_t)S[i] * c.T + 31) & ~(size_t)31;  // 256-byte aligned chains
+        work[i] = (long long)c.T * std::max(off[c.K], 1) + (long long)c.K * c.T;
+    }
+    GHIP(hipMalloc(&g->d_ints, std::max<size_t>(ints.size(), 1) * sizeof(int)));
+    GHIP(hipMemcpy(g->d_ints, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice));
+    GHIP(hipMalloc(&g->d_resid, nres * sizeof(double)));
+    GHIP(hipMemset(g->d_resid, 0, nres * sizeof(double)));  // brick slots outside the region stay 0
+
     std::vector<uint32_t> mts((size_t)n * (kMtN + 1));
-    std::vector<GibbsDev> dev(n);
+    std::vector<GibbsDev> &dev = g->dev;
+    dev.resize(n);
     GHIP(hipMalloc(&g->d_mt, mts.size() * sizeof(uint32_t)));
     for (int i = 0; i < n; i++) {
         const rfba3b_gibbs_chain &c = chains[i];
@@ -467,9 +607,13 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
         mts[(size_t)i * (kMtN + 1) + kMtN] = (uint32_t)h.pos;
         g->draws[i] = c.skip;
         GibbsDev &d = dev[i];
-        d.P = c.P; d.T = c.T; d.K = c.K; d.pad = 0;
-        d.spot = c.d_spot; d.moff = c.d_mask_off; d.midx = c.d_mask;
-        d.sample = c.d_sample; d.resid = c.d_resid; d.samples = c.d_samples;
+        d.P = c.P; d.T = c.T; d.K = c.K; d.S = S[i];
+        d.spot = c.d_spot;
+        d.moff = g->d_ints + o_off[i];
+        d.mpix = g->d_ints + o_pix[i];
+        d.mslot = g->d_ints + o_slotm[i];
+        d.slot = g->d_ints + o_slot[i];
+        d.sample = c.d_sample; d.resid = g->d_resid + o_res[i]; d.samples = c.d_samples;
         d.data = c.d_data; d.base = c.base;
         d.mt = g->d_mt + (size_t)i * (kMtN + 1);
     }
@@ -477,22 +621,7 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
     GHIP(hipMalloc(&g->d_chains, n * sizeof(GibbsDev)));
     GHIP(hipMemcpy(g->d_chains, dev.data(), n * sizeof(GibbsDev), hipMemcpyHostToDevice));
 
-    // mask checks on the device, then the launch order by work (T x mask entries)
-    int *d_chk = nullptr;
-    GHIP(hipMalloc(&d_chk, 2 * (size_t)n * sizeof(int)));
-    hipLaunchKernelGGL(b3_gibbs_check, dim3(n), dim3(256), 0, nullptr, (const GibbsDev *)g->d_chains, d_chk);
-    std::vector<int> chk(2 * (size_t)n);
-    hipError_t e1 = hipGetLastError();
-    hipError_t e2 = hipMemcpy(chk.data(), d_chk, chk.size() * sizeof(int), hipMemcpyDeviceToHost);
-    (void)hipFree(d_chk);
-    GHIP(e1);
-    GHIP(e2);
-    std::vector<long long> work(n);
-    for (int i = 0; i < n; i++) {
-        if (chk[2 * i]) return host_fail(-1, "chain %d: %d invalid mask entries (offsets must start at 0 and not "
-                                             "decrease; indices in [0, P), increasing within a spot)", i, chk[2 * i]);
-        work[i] = (long long)chains[i].T * std::max(chk[2 * i + 1], 1) + (long long)chains[i].K * chains[i].T;
-    }
+    // launch order: largest work (T x mask entries) first
     std::vector<int> order(n);
     for (int i = 0; i < n; i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
@@ -504,6 +633,9 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
     int per_cu = 0;
     GHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)b3_gibbs, kGibbsThreads, lds));
     if (per_cu <= 0) return host_fail(-1, "Gibbs kernel does not fit on a CU (LDS %zu bytes)", lds);
+    hipLaunchKernelGGL(b3_gibbs_reset, dim3(64, n), dim3(256), 0, nullptr, (const GibbsDev *)g->d_chains);
+    GHIP(hipGetLastError());
+    GHIP(hipDeviceSynchronize());
     *out = own.release();
     return 0;
 }
@@ -574,6 +706,16 @@ extern "C" int rfba3b_gibbs_run(rfba3b_gibbs *g, int samples, int sample_iterati
     for (size_t i = 0; i < g->chains.size(); i++)
         g->draws[i] += (unsigned long long)samples * sample_iterations * g->chains[i].K * g->chains[i].T;
     if (kernel_ms) *kernel_ms = ms;
+    return 0;
+}
+
+extern "C" int rfba3b_gibbs_sample_intensities(const rfba3b_gibbs *g, int chain, double *d_out, void *stream) {
+    std::lock_guard<std::mutex> lk(host_mutex());
+    if (!g || !d_out) return host_fail(-1, "plan or output is NULL");
+    if (chain < 0 || chain >= (int)g->chains.size()) return host_fail(-1, "chain %d out of range", chain);
+    hipLaunchKernelGGL(b3_gibbs_export, dim3(256), dim3(256), 0, (hipStream_t)stream, (const GibbsDev *)g->d_chains,
+                       chain, d_out);
+    GHIP(hipGetLastError());
     return 0;
 }
 
