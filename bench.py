@@ -472,7 +472,8 @@ def bench_b3_gibbs(args):
     log("gibbs: %d chains set up in %.1f s" % (len(chains), time.perf_counter() - t0))
     masks = [int(mm["offsets"][-1]) for mm in g.meta]
     log("gibbs: warmup %.1f ms" % g.run(S, 1))
-    first = [g.samples(i, 1)[0].copy() for i in range(min(16, len(chains)))] if not args.no_cpu else []
+    # first samples of every block, for the oracle's check (a bounded CPU sample: one sweep per block)
+    first = [g.samples(i, 1)[0].copy() for i in range(len(chains))] if not args.no_cpu else []
     kms, walls = [], []
     for i in range(max(1, args.steps)):
         prev = [g.sample(c) for c in range(len(chains))]
@@ -546,13 +547,12 @@ def bench_b3_gibbs(args):
         with ThreadPoolExecutor(th) as ex:
             out = list(ex.map(one, range(n)))
         dt = time.perf_counter() - t0
-        res["first_sample_equals_oracle"] = {"blocks": n,
-                                             "equal": bool(all(np.array_equal(o[1], first[i])
-                                                               for i, o in enumerate(out)))}
+        neq = sum(bool(np.array_equal(o[1], first[i])) for i, o in enumerate(out))
+        res["first_sample_equals_oracle"] = {"blocks": n, "equal_blocks": neq, "equal": neq == n}
         res["cpu_baseline"] = {"value": sum(o[2] for o in out) / dt, "unit": "spot-updates/s", "cores": th,
                                "kind": "port",
-                               "sample": "oracle GibbsSampler2::next, first sweep of blocks 0-%d, %d threads, "
-                                         "%.1f s" % (n - 1, th, dt)}
+                               "sample": "oracle GibbsSampler2::next, first sweep of every block (%d blocks), %d "
+                                         "threads, %.1f s" % (n, th, dt)}
     return res
 
 
