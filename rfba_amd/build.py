@@ -6,6 +6,7 @@
 Targets (all outputs next to their sources, git-ignored):
   rfba_amd/librfba_srrf.so   HIP kernels + C ABI of include/rfba_srrf.h (gfx950)
   rfba_amd/librfba_3b.so     HIP kernels + C ABI of include/rfba_3b.h (gfx950)
+  rfba_amd/librfba_prep.so   TIFF loading + normalisation, C ABI of include/rfba_prep.h (gfx950)
   rfba_amd/librfba_synth.so  host-only synthetic data generator
   oracle/liboracle.so        the CPU oracle (test infrastructure)
 """
@@ -54,10 +55,12 @@ TARGETS = {
     "b3": lambda: _hip(os.path.join(PKG, "librfba_3b.so"),
                        [os.path.join(CSRC, "b3_kernel.hip"), os.path.join(CSRC, "b3_gibbs.hip"),
                         os.path.join(CSRC, "b3_api.cpp")], contract="fast"),
+    # input loading and normalisation: FP32 order of libcvd's FIR, no contraction
+    "prep": lambda: _hip(os.path.join(PKG, "librfba_prep.so"), [os.path.join(CSRC, "prep.hip")]),
     "synth": lambda: ["g++", *GXX_FLAGS, "-O3", os.path.join(CSRC, "synth.cpp"),
                       "-o", os.path.join(PKG, "librfba_synth.so")],
     "oracle": lambda: ["g++", *GXX_FLAGS,
-                       *[os.path.join(ORACLE, f) for f in ("srrf_oracle.cpp", "b3_oracle.cpp")
+                       *[os.path.join(ORACLE, f) for f in ("srrf_oracle.cpp", "b3_oracle.cpp", "prep_oracle.cpp")
                          if os.path.exists(os.path.join(ORACLE, f))],
                        "-o", os.path.join(ORACLE, "liboracle.so")],
 }
@@ -71,7 +74,7 @@ def _available(name):
 
 def build(targets=None, verbose=True):
     names = [n for n in (targets or list(TARGETS)) if _available(n)]
-    if any(n in ("srrf", "b3") for n in names) and not shutil.which(HIPCC) and not os.path.exists(HIPCC):
+    if any(n in ("srrf", "b3", "prep") for n in names) and not shutil.which(HIPCC) and not os.path.exists(HIPCC):
         raise RuntimeError("hipcc not found at %s" % HIPCC)
     with ThreadPoolExecutor(max_workers=4) as ex:
         futs = {n: ex.submit(_run, TARGETS[n]()) for n in names}

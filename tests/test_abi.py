@@ -26,7 +26,8 @@ def exported(path):
     return set(l.split()[-1] for l in out.splitlines() if l.strip())
 
 
-@pytest.mark.parametrize("header,lib", [("rfba_srrf.h", "rfba_srrf"), ("rfba_3b.h", "rfba_3b")])
+@pytest.mark.parametrize("header,lib", [("rfba_srrf.h", "rfba_srrf"), ("rfba_3b.h", "rfba_3b"),
+                                        ("rfba_prep.h", "rfba_prep")])
 def test_all_declared_symbols_exported(header, lib):
     if not os.path.exists(os.path.join(ROOT, "include", header)):
         pytest.skip("header not present")
