@@ -23,7 +23,10 @@ Beside it, in "workloads" of the same JSON line:
                 parameter set), the latter on the first 100 blocks;
   b3_gibbs      Gibbs sampling of the blink states of the same 1200 blocks
                 (true spot parameters): 10 samples of one sweep per run,
-                all blocks in one launch vs one launch per block (first 20).
+                all blocks in one launch vs one launch per block (first 20);
+  prep          image normalisation of the 512x512x128 T=120 series (per
+                z-slice high-pass, mean/variance, scaling), and the TIFF path
+                end to end on 8 of its frames.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only c2,c3,c4,c5,gibbs]
 
@@ -155,13 +158,25 @@ class _View:
         self.ptr = ctypes.c_void_p(base.ptr.value + offset)
 
 
+_C3 = {}
+
+
+def c3_series():
+    """The 512x512x128 T=120 uint16 series (8000 emitters, seed 3), generated
+    once per process."""
+    if not _C3:
+        from rfba_amd import synth
+        t0 = time.perf_counter()
+        _C3["series"] = synth.srrf_series((512, 512, 128, 120), 8000, 3)
+        log("c3: generated %s series in %.1f s" % (_C3["series"].shape, time.perf_counter() - t0))
+    return _C3["series"]
+
+
 def bench_srrf_batched(args):
     from rfba_amd import srrf, synth
     from rfba_amd.blocks import volume_blocks
     vol, T, n_em = (512, 512, 128), 120, 8000
-    t0 = time.perf_counter()
-    series = synth.srrf_series(vol + (T,), n_em, 3)
-    log("c3: generated %s series in %.1f s" % (series.shape, time.perf_counter() - t0))
+    series = c3_series()
     split = volume_blocks(vol, 29, 3)
     rng4 = np.random.default_rng(4)
     extra_dims = rng4.integers(8, 30, (64, 3))
@@ -556,6 +571,86 @@ def bench_b3_gibbs(args):
     return res
 
 
+def bench_prep(args):
+    """load_and_normalize_images on the 512x512x128 T=120 series: per z-slice
+    high-pass (sigma 5, 31 taps per axis), mean/variance, scaling; plus the
+    TIFF path end to end on 8 of its frames."""
+    import tempfile
+    from rfba_amd import prep
+    from rfba_amd.synth import default_threads
+    series = c3_series()
+    T, Z, Y, X = series.shape
+    c = prep.conf(lpf=5.0)
+    d_raw = prep.Device.from_array(series)
+    d_out = prep.Device(series.size * 4)
+    stats, ms = prep.normalize_device(d_raw, series.dtype, series.shape, c, d_out)
+    log("prep: warmup %.1f ms" % ms)
+    kms, walls = [], []
+    for i in range(max(3, args.steps)):
+        t0 = time.perf_counter()
+        stats, ms = prep.normalize_device(d_raw, series.dtype, series.shape, c, d_out)
+        walls.append((time.perf_counter() - t0) * 1e3)
+        kms.append(ms)
+        log("prep: step %d %.1f ms" % (i, ms))
+    d_raw.close()
+    d_out.close()
+    n = float(series.size)
+    kb = float(np.mean(kms))
+    # algorithmic bytes per voxel-frame: the uint16 input read and the float
+    # high-pass written (2 + 4), then read and written scaled (4 + 4)
+    by = 14.0 * n
+    roof = hbm_roof(by, kb)
+    k, _ = prep.gaussian_kernel(5.0)
+    ks = len(k)
+    roof.update({"kernel": "prep_highpass + prep_scale + 2 x prep_reduce", "kernel_ms": kb,
+                 "algorithmic_bytes_per_launch": by,
+                 "fp32_ops_per_voxel": 2 * (3 * ks + 1) + 1})
+    res = {
+        "workload": "load_and_normalize_images on the 512x512x128 T=120 uint16 series (8000 emitters, seed 3): "
+                    "per z-slice high-pass (sigma 5, half-width 15), mean/variance, division by the std",
+        "voxel_frames": int(n),
+        "normalize": {"ms": float(np.mean(walls)), "kernel_ms": kb, "voxel_frames_per_s": n / (kb * 1e-3),
+                      "stats": [float(v) for v in stats]},
+        "roofline": roof,
+    }
+    # the TIFF path end to end: 8 frames written as stacks, read back
+    tmp = tempfile.mkdtemp(prefix="rfba_prep_", dir=os.environ.get("TMPDIR", "/tmp"))
+    try:
+        paths = []
+        for t in range(8):
+            p = os.path.join(tmp, "frame%03d.tif" % t)
+            prep.tiff_write(p, series[t], big_endian=bool(t & 1))
+            paths.append(p)
+        prep.load_and_normalize(paths, c)
+        t0 = time.perf_counter()
+        out8, st8 = prep.load_and_normalize(paths, c)
+        dt = time.perf_counter() - t0
+        mb = 8 * series[0].nbytes / 1e6
+        res["tiff_load_and_normalize"] = {"frames": 8, "MB": mb, "s": dt, "MB_per_s": mb / dt,
+                                          "note": "files just written: read from the page cache"}
+    finally:
+        for p in os.listdir(tmp):
+            os.remove(os.path.join(tmp, p))
+        os.rmdir(tmp)
+    if not args.no_cpu:
+        from oracle import preporacle as po
+        th = default_threads()
+        nf = min(16, T)
+        sub = series[:nf]
+        t0 = time.perf_counter()
+        oo, ost = po.normalize(sub.astype(np.float32), lpf=5.0, threads=th)
+        dt = time.perf_counter() - t0
+        go, gst = prep.normalize(sub, c)
+        res["first_frames_equal_oracle"] = {"frames": nf, "equal": bool(np.array_equal(go, oo) and
+                                                                         np.array_equal(gst, ost))}
+        del oo, go
+        res["cpu_baseline"] = {"value": sub.size / dt, "unit": "voxel-frames/s", "cores": min(th, nf), "kind": "port",
+                               "sample": "oracle on frames 0-%d of the same series (%d voxel-frames), %.1f s; "
+                                         "high-pass one thread per frame, statistics and scaling on one thread"
+                                         % (nf - 1, sub.size, dt)}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -564,9 +659,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-crop", type=str, default="12,12,8")
     ap.add_argument("--frames", type=int, default=200)
-    ap.add_argument("--only", type=str, default="c2,c3,c4,c5,gibbs",
+    ap.add_argument("--only", type=str, default="c2,c3,c4,c5,gibbs,prep",
                     help="workloads to run: c2 (headline stack), c3 (batched SRRF), c4, c5 (3B), "
-                         "gibbs (3B blink-state sampling)")
+                         "gibbs (3B blink-state sampling), prep (image normalisation)")
     ap.add_argument("--c5-plain-blocks", type=int, default=100)
     ap.add_argument("--traffic", type=float, default=None,
                     help="measured HBM bytes per launch of the headline kernel (rocprofv3 FETCH_SIZE+WRITE_SIZE)")
@@ -601,6 +696,8 @@ def main():
         rec["workloads"]["b3_batched"] = bench_b3_batched(args)
     if "gibbs" in only:
         rec["workloads"]["b3_gibbs"] = bench_b3_gibbs(args)
+    if "prep" in only:
+        rec["workloads"]["prep"] = bench_prep(args)
     print(json.dumps(rec), flush=True)
 
 

@@ -196,15 +196,23 @@ __device__ __forceinline__ float to_f(uint8_t v) { return (float)v; }
 __device__ __forceinline__ float to_f(uint16_t v) { return (float)v; }
 __device__ __forceinline__ float to_f(float v) { return v; }
 
-template <typename Tin>
+// KS > 0: the half-width fixed at compile time, each thread computing 8
+// consecutive outputs of a row (then of a column) from a window of
+// 8 + 2 KS values held in registers; KS = 0: any half-width, one output at a
+// time from LDS.  Both evaluate the same terms in the same order.
+constexpr int kRun = 8;
+constexpr int kHS = kTX + 1;  // row stride of the blurred rows in LDS
+
+template <typename Tin, int KS>
 __global__ __launch_bounds__(kThreads) void prep_highpass(const Tin *__restrict__ raw, int X, int Y, int ntx,
                                                            int nty, Fir F, int skip, float *__restrict__ out,
                                                            double2 *__restrict__ partial) {
     extern __shared__ float lds[];
-    const int ks = F.ksize;
+    const int ks = KS > 0 ? KS : F.ksize;
     const int WX = kTX + 2 * ks, WY = kTY + 2 * ks;
-    float *win = lds;             // [WY][WX] input window
-    float *hrow = lds + WY * WX;  // [WY][kTX] horizontally blurred rows
+    const int WS = WX | 1;        // odd row strides: lanes on consecutive rows hit distinct banks
+    float *win = lds;             // [WY][WS] input window
+    float *hrow = lds + WY * WS;  // [WY][kHS] horizontally blurred rows
     const int tid = threadIdx.x;
     const long long tile = blockIdx.x;
     const int tx = (int)(tile % ntx), ty = (int)((tile / ntx) % nty);
@@ -215,43 +223,84 @@ __global__ __launch_bounds__(kThreads) void prep_highpass(const Tin *__restrict_
     for (int e = tid; e < WX * WY; e += kThreads) {
         const int r = e / WX, c = e - r * WX;
         const int gy = min(max(y0 - ks + r, 0), Y - 1), gx = min(max(x0 - ks + c, 0), X - 1);
-        win[e] = to_f(src[(size_t)gy * X + gx]);
+        win[r * WS + c] = to_f(src[(size_t)gy * X + gx]);
     }
     __syncthreads();
     if (!skip) {
-        for (int e = tid; e < WY * kTX; e += kThreads) {
-            const int r = e / kTX, c = e - r * kTX;
-            const int j = x0 + c;
-            const float *in = win + r * WX + c + ks;
-            float h;
-            if (j < ks || j >= X - ks) h = (float)((double)in[0] * F.factor_d);
-            else h = in[0] * F.factor_f;
-            for (int k = 0; k < ks; k++) h += (in[-k - 1] + in[k + 1]) * F.kern[k];
-            hrow[e] = h;
+        if (KS > 0) {
+            // rows: (row r, columns c0 .. c0 + 7) per item, lanes on consecutive rows
+            for (int e = tid; e < WY * (kTX / kRun); e += kThreads) {
+                const int c0 = (e / WY) * kRun, r = e - (e / WY) * WY;
+                float w[kRun + 2 * (KS > 0 ? KS : 1)];
+                const float *in = win + r * WS + c0;
+#pragma unroll
+                for (int i = 0; i < kRun + 2 * KS; i++) w[i] = in[i];
+#pragma unroll
+                for (int i = 0; i < kRun; i++) {
+                    const int j = x0 + c0 + i;
+                    float h;
+                    if (j < KS || j >= X - KS) h = (float)((double)w[KS + i] * F.factor_d);
+                    else h = w[KS + i] * F.factor_f;
+#pragma unroll
+                    for (int k = 0; k < KS; k++) h += (w[KS + i - k - 1] + w[KS + i + k + 1]) * F.kern[k];
+                    hrow[r * kHS + c0 + i] = h;
+                }
+            }
+        } else {
+            for (int e = tid; e < WY * kTX; e += kThreads) {
+                const int r = e / kTX, c = e - r * kTX;
+                const int j = x0 + c;
+                const float *in = win + r * WS + c + ks;
+                float h;
+                if (j < ks || j >= X - ks) h = (float)((double)in[0] * F.factor_d);
+                else h = in[0] * F.factor_f;
+                for (int k = 0; k < ks; k++) h += (in[-k - 1] + in[k + 1]) * F.kern[k];
+                hrow[r * kHS + c] = h;
+            }
         }
         __syncthreads();
     }
     double s1 = 0.0, s2 = 0.0;
     float *dst = out + slice * (long long)X * Y;
-    for (int e = tid; e < kTY * kTX; e += kThreads) {
-        const int r = e / kTX, c = e - r * kTX;
+    auto emit = [&](int r, int c, float v) {
         const int gy = y0 + r, gx = x0 + c;
-        if (gy >= Y || gx >= X) continue;
-        const float x = win[(r + ks) * WX + c + ks];
+        if (gy >= Y || gx >= X) return;
+        const float x = win[(r + ks) * WS + c + ks];
         float f;
-        if (skip) {
-            f = x;
-        } else if (gy >= 1 && gy <= Y - 2 && gx >= 1 && gx <= X - 2) {
-            const float *hc = hrow + (r + ks) * kTX + c;
-            float v = (float)((double)hc[0] * F.factor_d);
-            for (int k = 0; k < ks; k++) v += (hc[-(k + 1) * kTX] + hc[(k + 1) * kTX]) * F.kern[k];
-            f = x - v;
-        } else {
-            f = 0.0f;
-        }
+        if (skip) f = x;
+        else if (gy >= 1 && gy <= Y - 2 && gx >= 1 && gx <= X - 2) f = x - v;
+        else f = 0.0f;
         dst[(size_t)gy * X + gx] = f;
         s1 += (double)f;
         s2 += (double)(f * f);
+    };
+    if (KS > 0 && !skip) {
+        // columns: (column c, rows r0 .. r0 + 7) per item
+        for (int e = tid; e < kTX * (kTY / kRun); e += kThreads) {
+            const int c = e % kTX, r0 = (e / kTX) * kRun;
+            float w[kRun + 2 * (KS > 0 ? KS : 1)];
+            const float *hc = hrow + r0 * kHS + c;
+#pragma unroll
+            for (int i = 0; i < kRun + 2 * KS; i++) w[i] = hc[i * kHS];
+#pragma unroll
+            for (int i = 0; i < kRun; i++) {
+                float v = (float)((double)w[KS + i] * F.factor_d);
+#pragma unroll
+                for (int k = 0; k < KS; k++) v += (w[KS + i - k - 1] + w[KS + i + k + 1]) * F.kern[k];
+                emit(r0 + i, c, v);
+            }
+        }
+    } else {
+        for (int e = tid; e < kTY * kTX; e += kThreads) {
+            const int r = e / kTX, c = e - r * kTX;
+            float v = 0.0f;
+            if (!skip) {
+                const float *hc = hrow + (r + ks) * kHS + c;
+                v = (float)((double)hc[0] * F.factor_d);
+                for (int k = 0; k < ks; k++) v += (hc[-(k + 1) * kHS] + hc[(k + 1) * kHS]) * F.kern[k];
+            }
+            emit(r, c, v);
+        }
     }
     __shared__ double red[2][kThreads / 64];
     for (int o = 32; o > 0; o >>= 1) {
@@ -275,12 +324,25 @@ __global__ __launch_bounds__(kThreads) void prep_highpass(const Tin *__restrict_
 
 // (x - sub) / div (fixed scaling) or x / div (div = sqrt of the variance),
 // in double, rounded to float; per-block sums of the results and their
-// float squares.  Block b covers elements [b * chunk, (b + 1) * chunk).
+// float squares.  Block b covers elements [b * chunk, (b + 1) * chunk), chunk
+// a multiple of 4 (16-byte accesses; the tail past a multiple of 4 singly).
 __global__ __launch_bounds__(kThreads) void prep_scale(float *__restrict__ v, long long n, long long chunk,
                                                        double sub, double div, double2 *__restrict__ partial) {
-    const long long b0 = (long long)blockIdx.x * chunk, b1 = min(n, b0 + chunk);
+    const long long b0 = (long long)blockIdx.x * chunk, b1 = max(b0, min(n, b0 + chunk));  // empty past n
     double s1 = 0.0, s2 = 0.0;
-    for (long long i = b0 + threadIdx.x; i < b1; i += kThreads) {
+    const long long n4 = b0 + ((b1 - b0) & ~3LL);
+    for (long long i = b0 + 4 * (long long)threadIdx.x; i < n4; i += 4 * kThreads) {
+        float4 q = *(const float4 *)(v + i);
+        float r[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            r[j] = (float)(((double)r[j] - sub) / div);
+            s1 += (double)r[j];
+            s2 += (double)(r[j] * r[j]);
+        }
+        *(float4 *)(v + i) = make_float4(r[0], r[1], r[2], r[3]);
+    }
+    for (long long i = n4 + threadIdx.x; i < b1; i += kThreads) {
         const float f = (float)(((double)v[i] - sub) / div);
         v[i] = f;
         s1 += (double)f;
@@ -306,11 +368,14 @@ __global__ __launch_bounds__(kThreads) void prep_scale(float *__restrict__ v, lo
     }
 }
 
-// Sum of n partials by one workgroup of 1024 threads in a fixed order.
-__global__ __launch_bounds__(1024) void prep_reduce(const double2 *__restrict__ partial, long long n,
-                                                    double *__restrict__ out) {
+// Sums of n partials in a fixed order: block b of gridDim.x sums the
+// contiguous range [b * chunk, (b + 1) * chunk) into out[b] (a double2 per
+// block), so that a second launch with one block finishes the total.
+__global__ __launch_bounds__(1024) void prep_reduce(const double2 *__restrict__ partial, long long n, long long chunk,
+                                                    double2 *__restrict__ out) {
+    const long long b0 = (long long)blockIdx.x * chunk, b1 = min(n, b0 + chunk);
     double s1 = 0.0, s2 = 0.0;
-    for (long long i = threadIdx.x; i < n; i += 1024) {
+    for (long long i = b0 + threadIdx.x; i < b1; i += 1024) {
         s1 += partial[i].x;
         s2 += partial[i].y;
     }
@@ -330,8 +395,7 @@ __global__ __launch_bounds__(1024) void prep_reduce(const double2 *__restrict__ 
             a += red[0][w];
             b += red[1][w];
         }
-        out[0] = a;
-        out[1] = b;
+        out[blockIdx.x] = make_double2(a, b);
     }
 }
 
@@ -358,14 +422,24 @@ void stats_of(const double *s, double area, double *out) {
     out[1] = (float)(q - m * m);
 }
 
+constexpr int kRed = 256;  // blocks of the first reduction level
+
 struct Scratch {
     double2 *partial = nullptr;
     size_t cap = 0;
-    double *sums = nullptr;
+    double2 *sums = nullptr;  // [kRed] first level, then [kRed] the total
 } g_scr;
 
+// Two launches: kRed blocks over n partials, then one block over kRed.
+void reduce2(long long n, hipStream_t st) {
+    const long long chunk = (n + kRed - 1) / kRed;
+    hipLaunchKernelGGL(prep_reduce, dim3(kRed), dim3(1024), 0, st, (const double2 *)g_scr.partial, n, chunk, g_scr.sums);
+    hipLaunchKernelGGL(prep_reduce, dim3(1), dim3(1024), 0, st, (const double2 *)g_scr.sums, (long long)kRed,
+                       (long long)kRed, g_scr.sums + kRed);
+}
+
 int scratch(size_t n) {
-    if (!g_scr.sums) PHIP(hipMalloc(&g_scr.sums, 2 * sizeof(double)));
+    if (!g_scr.sums) PHIP(hipMalloc(&g_scr.sums, 2 * kRed * sizeof(double2)));
     if (n > g_scr.cap) {
         if (g_scr.partial) PHIP(hipFree(g_scr.partial));
         g_scr.partial = nullptr;
@@ -400,32 +474,30 @@ int normalize(const void *d_raw, int type, const int *dims, const rfba_prep_conf
     if (ntiles > 0x7fffffffLL) return fail(-1, "series too large for one launch");
     const long long n = (long long)X * Y * Z * T;
     const int sblocks = 4096;
-    const long long chunk = (n + sblocks - 1) / sblocks;
+    const long long chunk = ((n + sblocks - 1) / sblocks + 3) & ~3LL;  // a multiple of 4 elements
     if (int rc = scratch(std::max<long long>(ntiles, sblocks))) return rc;
-    const size_t lds = 4 * ((size_t)(kTX + 2 * F.ksize) * (kTY + 2 * F.ksize) + (size_t)(kTY + 2 * F.ksize) * kTX);
-    const void *fn = type == RFBA_PREP_U8 ? (const void *)prep_highpass<uint8_t>
-                     : type == RFBA_PREP_U16 ? (const void *)prep_highpass<uint16_t> : (const void *)prep_highpass<float>;
-    if (lds > 64 * 1024) PHIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const size_t lds = 4 * (size_t)(kTY + 2 * F.ksize) * (((size_t)(kTX + 2 * F.ksize) | 1) + kHS);
+    // the default blur (sigma 5: half-width 15) runs the register-window form
+    const bool k15 = !conf->skip && F.ksize == 15;
+    typedef void (*HpFn)(const void *, int, int, int, int, Fir, int, float *, double2 *);
+    HpFn fn;
+    if (type == RFBA_PREP_U8) fn = k15 ? (HpFn)prep_highpass<uint8_t, 15> : (HpFn)prep_highpass<uint8_t, 0>;
+    else if (type == RFBA_PREP_U16) fn = k15 ? (HpFn)prep_highpass<uint16_t, 15> : (HpFn)prep_highpass<uint16_t, 0>;
+    else fn = k15 ? (HpFn)prep_highpass<float, 15> : (HpFn)prep_highpass<float, 0>;
+    if (lds > 64 * 1024) PHIP(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
 
     hipEvent_t e0, e1;
     PHIP(hipEventCreate(&e0));
     PHIP(hipEventCreate(&e1));
     PHIP(hipEventRecord(e0, st));
     const int skip = conf->skip ? 1 : 0;
-    if (type == RFBA_PREP_U8)
-        hipLaunchKernelGGL(prep_highpass<uint8_t>, dim3((unsigned)ntiles), dim3(kThreads), lds, st,
-                           (const uint8_t *)d_raw, X, Y, (int)ntx, (int)nty, F, skip, d_out, g_scr.partial);
-    else if (type == RFBA_PREP_U16)
-        hipLaunchKernelGGL(prep_highpass<uint16_t>, dim3((unsigned)ntiles), dim3(kThreads), lds, st,
-                           (const uint16_t *)d_raw, X, Y, (int)ntx, (int)nty, F, skip, d_out, g_scr.partial);
-    else
-        hipLaunchKernelGGL(prep_highpass<float>, dim3((unsigned)ntiles), dim3(kThreads), lds, st,
-                           (const float *)d_raw, X, Y, (int)ntx, (int)nty, F, skip, d_out, g_scr.partial);
+    hipLaunchKernelGGL(fn, dim3((unsigned)ntiles), dim3(kThreads), lds, st, d_raw, X, Y, (int)ntx, (int)nty, F, skip,
+                       d_out, g_scr.partial);
     PHIP(hipGetLastError());
-    hipLaunchKernelGGL(prep_reduce, dim3(1), dim3(1024), 0, st, (const double2 *)g_scr.partial, ntiles, g_scr.sums);
+    reduce2(ntiles, st);
     PHIP(hipGetLastError());
     double sums[2], before[2], after[2];
-    PHIP(hipMemcpyAsync(sums, g_scr.sums, sizeof sums, hipMemcpyDeviceToHost, st));
+    PHIP(hipMemcpyAsync(sums, g_scr.sums + kRed, sizeof sums, hipMemcpyDeviceToHost, st));
     PHIP(hipStreamSynchronize(st));
     const double area = (double)n;
     stats_of(sums, area, before);
@@ -434,10 +506,9 @@ int normalize(const void *d_raw, int type, const int *dims, const rfba_prep_conf
     const double div = conf->fixed_scaling ? conf->divide : std::sqrt(before[1]);
     hipLaunchKernelGGL(prep_scale, dim3(sblocks), dim3(kThreads), 0, st, d_out, n, chunk, sub, div, g_scr.partial);
     PHIP(hipGetLastError());
-    hipLaunchKernelGGL(prep_reduce, dim3(1), dim3(1024), 0, st, (const double2 *)g_scr.partial, (long long)sblocks,
-                       g_scr.sums);
+    reduce2(sblocks, st);
     PHIP(hipGetLastError());
-    PHIP(hipMemcpyAsync(sums, g_scr.sums, sizeof sums, hipMemcpyDeviceToHost, st));
+    PHIP(hipMemcpyAsync(sums, g_scr.sums + kRed, sizeof sums, hipMemcpyDeviceToHost, st));
     PHIP(hipEventRecord(e1, st));
     PHIP(hipEventSynchronize(e1));
     float ms = 0;
