@@ -219,7 +219,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
     float *dsum = lds + g.off_dsum;  // TZ x 256: per-voxel signed sums (thread-private)
 
     const int tid = threadIdx.x;
-    const int xl = tid % kTX, yl = tid / kTX;
+    // Each wave covers an 8 x 8 patch of the tile's columns, so that the
+    // padding columns past a block's x edge fill whole waves (which skip).
+    static_assert(kTX == 16 && kTY == 16 && kThreads == 256, "8 x 8 wave patches of a 16 x 16 tile");
+    const int xl = ((tid >> 6) & 1) * 8 + (tid & 7), yl = (tid >> 7) * 8 + ((tid >> 3) & 7);
     const DevSamples *__restrict__ S = P.smp;
     // Sample tables are read in the innermost loops with uniform indices:
     // keep them in LDS (global reads there would wait on memory every
