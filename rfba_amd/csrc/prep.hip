@@ -205,8 +205,8 @@ constexpr int kHS = kTX + 1;  // row stride of the blurred rows in LDS
 
 template <typename Tin, int KS>
 __global__ __launch_bounds__(kThreads) void prep_highpass(const Tin *__restrict__ raw, int X, int Y, int ntx,
-                                                           int nty, Fir F, int skip, float *__restrict__ out,
-                                                           double2 *__restrict__ partial) {
+                                                           int nty, long long ntiles, Fir F, int skip,
+                                                           float *__restrict__ out, double2 *__restrict__ partial) {
     extern __shared__ float lds[];
     const int ks = KS > 0 ? KS : F.ksize;
     const int WX = kTX + 2 * ks, WY = kTY + 2 * ks;
@@ -214,7 +214,11 @@ __global__ __launch_bounds__(kThreads) void prep_highpass(const Tin *__restrict_
     float *win = lds;             // [WY][WS] input window
     float *hrow = lds + WY * WS;  // [WY][kHS] horizontally blurred rows
     const int tid = threadIdx.x;
-    const long long tile = blockIdx.x;
+    // Workgroup b runs on XCD b % 8: each XCD takes a contiguous range of
+    // tiles, so that neighbouring tiles' overlapping windows meet in its L2
+    // (the grid is padded to a multiple of 8).
+    const long long tile = (long long)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    if (tile >= ntiles) return;
     const int tx = (int)(tile % ntx), ty = (int)((tile / ntx) % nty);
     const long long slice = tile / ((long long)ntx * nty);
     const int x0 = tx * kTX, y0 = ty * kTY;
@@ -471,7 +475,7 @@ int normalize(const void *d_raw, int type, const int *dims, const rfba_prep_conf
     if (int rc = dev_init()) return rc;
     const long long ntx = (X + kTX - 1) / kTX, nty = (Y + kTY - 1) / kTY;
     const long long ntiles = ntx * nty * (long long)Z * T;
-    if (ntiles > 0x7fffffffLL) return fail(-1, "series too large for one launch");
+    if (ntiles > 0x7ffffff0LL) return fail(-1, "series too large for one launch");
     const long long n = (long long)X * Y * Z * T;
     const int sblocks = 4096;
     const long long chunk = ((n + sblocks - 1) / sblocks + 3) & ~3LL;  // a multiple of 4 elements
@@ -479,7 +483,7 @@ int normalize(const void *d_raw, int type, const int *dims, const rfba_prep_conf
     const size_t lds = 4 * (size_t)(kTY + 2 * F.ksize) * (((size_t)(kTX + 2 * F.ksize) | 1) + kHS);
     // the default blur (sigma 5: half-width 15) runs the register-window form
     const bool k15 = !conf->skip && F.ksize == 15;
-    typedef void (*HpFn)(const void *, int, int, int, int, Fir, int, float *, double2 *);
+    typedef void (*HpFn)(const void *, int, int, int, int, long long, Fir, int, float *, double2 *);
     HpFn fn;
     if (type == RFBA_PREP_U8) fn = k15 ? (HpFn)prep_highpass<uint8_t, 15> : (HpFn)prep_highpass<uint8_t, 0>;
     else if (type == RFBA_PREP_U16) fn = k15 ? (HpFn)prep_highpass<uint16_t, 15> : (HpFn)prep_highpass<uint16_t, 0>;
@@ -491,8 +495,8 @@ int normalize(const void *d_raw, int type, const int *dims, const rfba_prep_conf
     PHIP(hipEventCreate(&e1));
     PHIP(hipEventRecord(e0, st));
     const int skip = conf->skip ? 1 : 0;
-    hipLaunchKernelGGL(fn, dim3((unsigned)ntiles), dim3(kThreads), lds, st, d_raw, X, Y, (int)ntx, (int)nty, F, skip,
-                       d_out, g_scr.partial);
+    hipLaunchKernelGGL(fn, dim3((unsigned)((ntiles + 7) / 8 * 8)), dim3(kThreads), lds, st, d_raw, X, Y, (int)ntx,
+                       (int)nty, ntiles, F, skip, d_out, g_scr.partial);
     PHIP(hipGetLastError());
     reduce2(ntiles, st);
     PHIP(hipGetLastError());
