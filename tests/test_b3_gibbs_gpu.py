@@ -120,6 +120,22 @@ def test_gibbs_runs_continue_the_stream(gpu):
         g2.close()
 
 
+def test_gibbs_long_series_uses_large_lds(gpu):
+    # T = 1500: per-chain tables of ~129 KB of LDS (more than 64 KB)
+    m = model()
+    T = 1500
+    data, params, _ = b3.synth_block((6, 6, 4), T, 3, 51, A=A_REF, pi=PI_REF)
+    ch = b3.ChainData(data.reshape(T, -1), b3.box_pixels((6, 6, 4)), params, base=100.0, seed=21, skip=3, slots=2)
+    g = b3.Gibbs([ch], m)
+    try:
+        g.run(2, 1)
+        smp, intens, _ = oracle_chain(ch, m, 2, 1, [100.0, 100.0])
+        assert np.array_equal(g.samples(0, 2), smp)
+        assert np.allclose(g.sample_intensities(0), intens, rtol=0, atol=1e-9)
+    finally:
+        g.close()
+
+
 def test_gibbs_samples_feed_the_batched_likelihood(gpu):
     # the device samples are the [S][K][T] state input of the batched logL
     m = model()

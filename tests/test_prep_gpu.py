@@ -30,6 +30,7 @@ CASES = [
     ((2, 2, 70, 130), np.uint8, dict(lpf=5.0)),               # several tiles per slice
     ((2, 2, 40, 33), np.float32, dict(lpf=2.5)),              # float input, 8-tap half-width
     ((1, 4, 17, 9), np.uint16, dict(lpf=1.0)),                # 3-tap half-width
+    ((1, 2, 80, 75), np.uint16, dict(lpf=12.0)),              # half-width 36: an 84 KB LDS window
     ((2, 2, 5, 7), np.uint16, dict(skip=1)),                  # no high-pass
     ((2, 2, 9, 11), np.uint16, dict(skip=1, fixed_scaling=1, subtract=100.0, divide=7.5)),
 ]
