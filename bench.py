@@ -484,7 +484,9 @@ def bench_b3_gibbs(args):
         chains.append(b3.ChainData(d.reshape(Tz, -1), b3.box_pixels((X, Y, Z)), p, base=100.0, seed=1000 + i,
                                    slots=S))
     g = b3.Gibbs(chains, m)
-    log("gibbs: %d chains set up in %.1f s" % (len(chains), time.perf_counter() - t0))
+    setup_ms = g.setup_ms
+    log("gibbs: %d chains set up in %.1f s (statistics kernels %.1f ms)"
+        % (len(chains), time.perf_counter() - t0, setup_ms))
     masks = [int(mm["offsets"][-1]) for mm in g.meta]
     log("gibbs: warmup %.1f ms" % g.run(S, 1))
     # first samples of every block, for the oracle's check (a bounded CPU sample: one sweep per block)
@@ -496,23 +498,30 @@ def bench_b3_gibbs(args):
         kms.append(g.run(S, 1))
         walls.append((time.perf_counter() - t1) * 1e3)
         log("gibbs: step %d %.1f ms" % (i, kms[-1]))
-    # algorithmic bytes of the last run: per spot update, the FP64 residual
-    # over the mask in every frame (8 T m), the mask's indices and spot values
-    # (12 m), and a read-modify-write of the residual over the mask in every
-    # frame whose on/off status changed (16 m per change)
+    # algorithmic bytes of the last run: per spot update, the spot's C and A
+    # rows (16 T), its sample row read and written (2 T), and for every frame
+    # whose on/off status changed, O and A of the other spots (24 K); the
+    # pixels are read only by the plan's statistics (setup below)
     by = 0.0
     ups = 0
+    setup_by = 0.0
     for c, mm in enumerate(g.meta):
         off = mm["offsets"].astype(np.int64)
         msz = off[1:] - off[:-1]
+        Kc = mm["K"]
         ch = gibbs_changes(prev[c], g.samples(c, S))
-        by += float(S * (8.0 * T * msz + 12.0 * msz).sum() + 16.0 * (msz * ch).sum())
-        ups += S * mm["K"]
+        by += float(S * Kc * 18.0 * T + 24.0 * Kc * ch.sum())
+        ups += S * Kc
+        # statistics: data over each mask in every frame (4 T m), the mask and
+        # spot values (12 m), the overlaps (12 m per other spot)
+        setup_by += float((4.0 * T * msz + 12.0 * msz + 12.0 * msz * Kc).sum())
     kb = float(np.mean(kms))
     wall = float(np.mean(walls))
     roof = hbm_roof(by, kb)
-    roof.update({"kernel": "b3_gibbs (one workgroup per chain)", "kernel_ms": kb,
-                 "algorithmic_bytes_per_launch": by})
+    roof.update({"kernel": "b3_gibbs (one 64-thread workgroup per chain)", "kernel_ms": kb,
+                 "algorithmic_bytes_per_launch": by,
+                 "note": "bound by the serial forward recursion and backward sampling of each spot update "
+                         "(T dependent FP64 steps on one lane), not by memory"})
     # plain form: one launch per block (a plan per chain) on the first blocks
     nsub = min(20, len(chains))
     subs = [b3.Gibbs([chains[i]], m) for i in range(nsub)]
@@ -535,6 +544,10 @@ def bench_b3_gibbs(args):
         "plain_per_block": {"blocks": nsub, "ms": wall_pb, "kernel_ms": ms_pb,
                             "spot_updates_per_s": ups_sub / (wall_pb * 1e-3)},
         "speedup_batched_vs_per_block": (ups / wall) / (ups_sub / wall_pb),
+        "setup": {"kernel_ms": setup_ms, "algorithmic_bytes": setup_by,
+                  "GB_per_s": setup_by / (setup_ms * 1e-3) / 1e9 if setup_ms > 0 else None,
+                  "what": "per plan: S2, C, mask bitmaps and overlaps of every spot (b3_gibbs_stats, "
+                          "b3_gibbs_member, b3_gibbs_overlap)"},
         "roofline": roof,
     }
     if not args.no_cpu:

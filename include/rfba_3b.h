@@ -130,17 +130,21 @@ void rfba3b_batch_destroy(rfba3b_batch *plan);
  * state path by backward sampling (:542-566) and add the spot back
  * (add_spot, sampled_multispot.h:124-131).
  *
- * A chain is one such sampler over a list of P pixels with K spots.  The
- * plan keeps each chain's current sample intensities on the device as the
- * residual data[t][p] - (base + sum of the spots on in frame t), in FP64
- * (base = 0 reproduces the reference's empty sample).  Given the pixels'
- * coordinates, the residual is stored in 4 x 2 x 2 voxel bricks of one
- * 128-byte line each, so that the ball-shaped masks read few partial lines;
- * rfba3b_gibbs_sample_intensities returns it in the pixel list's order.
- * Each chain owns an MT19937 generator seeded like the reference's
- * (mt19937.h:48-51, 37 draws discarded), advanced by `skip` draws; it
- * persists across runs.  One spot update consumes exactly T draws
- * (backward_sampling). */
+ * A chain is one such sampler over a list of P pixels with K spots.  Spot
+ * k's masked on/off log-probabilities differ, in frame t, by
+ * -d_t(k) / (2 variance) with
+ *   d_t(k) = S2_k - 2 (C_t(k) - A_t(k)),
+ *   S2_k = sum over M_k of spot_k^2,  C_t(k) = sum over M_k of spot_k (data_t - base),
+ *   A_t(k) = sum over the other spots j on in frame t of O_kj,
+ *   O_kj = sum over M_k and M_j of spot_k spot_j,
+ * because the sample intensities hold each on spot over its own mask M_j
+ * (add_spot); everything else is common to the three states of a frame.
+ * The plan computes S2, C and O once, on the device, and keeps A in FP64
+ * as states change, so a sweep reads no pixels.  base = 0 reproduces the
+ * reference's empty sample.  Each chain owns an MT19937 generator seeded
+ * like the reference's (mt19937.h:48-51, 37 draws discarded), advanced by
+ * `skip` draws; it persists across runs.  One spot update consumes exactly
+ * T draws (backward_sampling). */
 typedef struct rfba3b_gibbs_chain {
     int P, T, K;
     const float *d_data;       /* device [T][P] pixel intensities */
@@ -148,7 +152,6 @@ typedef struct rfba3b_gibbs_chain {
     const int *d_mask_off;     /* device [K + 1]: spot k's mask is d_mask[off[k] .. off[k+1]) */
     const int *d_mask;         /* device pixel indices in [0, P) (spot_pixels) */
     double base;               /* intensity of the empty sample; 0 for the reference */
-    const int *xyz;            /* host [P][3] pixel coordinates, or NULL (residual in list order) */
     uint8_t *d_sample;         /* device [K][T] current sample (in/out) */
     uint8_t *d_samples;        /* device [S][K][T]: the sample after each of the S steps
                                   of rfba3b_gibbs_run (the layout of rfba3b_block.d_states),
@@ -159,22 +162,25 @@ typedef struct rfba3b_gibbs_chain {
 
 typedef struct rfba3b_gibbs rfba3b_gibbs;  /* opaque plan */
 
-/* Validates the chains and their masks, lays out the residuals, seeds the
+/* Validates the chains and their masks, computes S2, C and O, seeds the
  * generators; the caller's buffers stay caller-owned and must outlive the
- * plan.  Starts from reset. */
-int rfba3b_gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_model *model, rfba3b_gibbs **plan);
+ * plan.  Starts from reset.  setup_ms (may be NULL) gets the statistics'
+ * kernel time. */
+int rfba3b_gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_model *model, rfba3b_gibbs **plan,
+                        float *setup_ms);
 /* GibbsSampler2::reset (sampled_multispot.h:430-438): every spot state 2
  * (off, bleached) in every frame, sample intensities = base.  Asynchronous. */
 int rfba3b_gibbs_reset(rfba3b_gibbs *plan, void *stream);
 /* `samples` consecutive next() calls of every chain (one launch, one
- * workgroup per chain), each of `sample_iterations` sweeps over the spots in
- * index order; after step s the sample is copied to d_samples[s].
+ * 64-thread workgroup per chain), each of `sample_iterations` sweeps over
+ * the spots in index order; after step s the sample is copied to
+ * d_samples[s].
  * variance: host [samples] noise variance of each step (set_variance,
  * :423-426), or NULL for the model's.  Synchronous; kernel_ms may be NULL. */
 int rfba3b_gibbs_run(rfba3b_gibbs *plan, int samples, int sample_iterations, const double *variance, void *stream,
                      float *kernel_ms);
 /* GibbsSampler2::sample_intensities (sampled_multispot.h:725-728) of chain
- * i, plus base: d_out [T][P] in the pixel list's order.  Asynchronous. */
+ * i, plus base, from its current sample: d_out [T][P].  Asynchronous. */
 int rfba3b_gibbs_sample_intensities(const rfba3b_gibbs *plan, int chain, double *d_out, void *stream);
 /* Draws consumed by chain i since its seeding, skip included. */
 int rfba3b_gibbs_draws(const rfba3b_gibbs *plan, int chain, unsigned long long *draws);

@@ -79,9 +79,8 @@ _sig = False
 
 class _GibbsChain(ctypes.Structure):
     _fields_ = [("P", c_int), ("T", c_int), ("K", c_int), ("d_data", c_void_p), ("d_spot", c_void_p),
-                ("d_mask_off", c_void_p), ("d_mask", c_void_p), ("base", c_double), ("xyz", c_void_p),
-                ("d_sample", c_void_p), ("d_samples", c_void_p), ("seed", ctypes.c_uint),
-                ("skip", ctypes.c_ulonglong)]
+                ("d_mask_off", c_void_p), ("d_mask", c_void_p), ("base", c_double), ("d_sample", c_void_p),
+                ("d_samples", c_void_p), ("seed", ctypes.c_uint), ("skip", ctypes.c_ulonglong)]
 
 
 def lib():
@@ -108,7 +107,8 @@ def lib():
         L.rfba3b_batch_run_per_block.restype = c_int
         L.rfba3b_batch_destroy.argtypes = [c_void_p]
         L.rfba3b_batch_destroy.restype = None
-        L.rfba3b_gibbs_create.argtypes = [POINTER(_GibbsChain), c_int, POINTER(Model), POINTER(c_void_p)]
+        L.rfba3b_gibbs_create.argtypes = [POINTER(_GibbsChain), c_int, POINTER(Model), POINTER(c_void_p),
+                                          POINTER(c_float)]
         L.rfba3b_gibbs_create.restype = c_int
         L.rfba3b_gibbs_reset.argtypes = [c_void_p, c_void_p]
         L.rfba3b_gibbs_reset.restype = c_int
@@ -319,15 +319,14 @@ class ChainData:
 
 class Gibbs:
     """Device Gibbs sampler over chains (GibbsSampler2, one workgroup per
-    chain).  Spot tables are computed on the device, masks on the host.
-    bricks=False keeps each residual in the pixel list's order."""
+    chain).  Spot tables are computed on the device, masks on the host;
+    setup_ms is the device time of the plan's statistics."""
 
-    def __init__(self, chains: list[ChainData], model: Model, bricks=True):
+    def __init__(self, chains: list[ChainData], model: Model):
         self.model = model
         self._bufs = []
         self.meta = []
         arr = (_GibbsChain * len(chains))()
-        keep = []
         for i, ch in enumerate(chains):
             T = ch.data.shape[0]
             data = np.ascontiguousarray(ch.data, np.float32).reshape(T, -1)
@@ -336,7 +335,6 @@ class Gibbs:
             K = phi.shape[0]
             xyz = np.ascontiguousarray(ch.xyz, np.int32)
             assert xyz.shape == (P, 3)
-            keep.append(xyz)
             off, idx = spot_pixels(xyz, phi)
             dd = DeviceBuffer.from_array(data)
             dx = DeviceBuffer.from_array(xyz)
@@ -355,14 +353,15 @@ class Gibbs:
             a.P, a.T, a.K = P, T, K
             a.d_data, a.d_spot, a.d_mask_off, a.d_mask = dd.ptr, dspot.ptr, doff.ptr, didx.ptr
             a.base = float(ch.base)
-            a.xyz = xyz.ctypes.data if bricks else None
             a.d_sample = dsmp.ptr
             a.d_samples = dout.ptr if dout else None
             a.seed = int(ch.seed) & 0xFFFFFFFF
             a.skip = int(ch.skip)
         self.plan = c_void_p()
-        _check(lib().rfba3b_gibbs_create(arr, len(chains), ctypes.byref(model), ctypes.byref(self.plan)))
-        del keep
+        ms = c_float(0)
+        _check(lib().rfba3b_gibbs_create(arr, len(chains), ctypes.byref(model), ctypes.byref(self.plan),
+                                         ctypes.byref(ms)))
+        self.setup_ms = ms.value
 
     def reset(self):
         _check(lib().rfba3b_gibbs_reset(self.plan, None))

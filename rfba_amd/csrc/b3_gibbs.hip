@@ -4,20 +4,20 @@
 // get_spot_pixels (multispot5.cc:1001-1020) and compute_spot_intensity
 // (sampled_multispot.h:183-191).
 //
-// One workgroup runs one chain.  A spot update is
-//   1. one pass over the spot's mask in every frame (pixels across lanes,
-//      frames across waves) giving d_t = sum (r - mu)^2 - r^2, where r is
-//      the residual with the spot removed; the reference's masked on/off
-//      log-probabilities (spot_with_background.hh:120-147) differ by
-//      B_on - B_off = -d_t / (2 var), and everything common to the three
-//      states of a frame cancels in both the forward recursion and the
-//      backward sampling;
-//   2. the forward recursion on one lane over normalised messages (FP64),
-//      then backward sampling with the chain's MT19937 draws;
-//   3. a second pass that touches only the frames whose on/off status
-//      changed, adding or removing the spot in the residual.
-// The residual (data - sample intensities) lives in HBM in FP64; the mask's
-// pixel indices and spot values are staged through LDS in chunks.
+// The masked on/off log-probabilities of spot k in frame t
+// (spot_with_background.hh:120-147 over the mask M_k) differ by
+// B_on - B_off = -d_t(k) / (2 var), with r0 the data minus the base minus
+// the other spots that are on, each over its own mask (add_spot,
+// sampled_multispot.h:124-131):
+//   d_t(k) = sum_{M_k} (r0 - mu_k)^2 - r0^2 = S2_k - 2 (C_t(k) - A_t(k)),
+//   S2_k = sum_{M_k} mu_k^2,   C_t(k) = sum_{M_k} mu_k (data_t - base),
+//   A_t(k) = sum_{j != k on in frame t} O_kj,   O_kj = sum_{M_k and M_j} mu_k mu_j.
+// Everything else in B is common to the three states of a frame and cancels
+// in both the forward recursion and the backward sampling.  S2, C and O are
+// computed once per plan on the device (FP64 sums over the masks); A is kept
+// in FP64 and changed by +-O_kj in the frames where spot j turns on or off.
+// A sweep therefore reads no pixels: per spot update, the frames' d_t, the
+// forward recursion and backward sampling on one lane, and the A updates.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -31,23 +31,24 @@
 
 namespace rfba3b {
 
-constexpr int kGibbsThreads = 256;
-constexpr int kMaskChunk = 2048;  // mask pixels staged in LDS at a time
-constexpr int kMtN = 624;         // MT19937 state words (mersenne.cpp MERS_N)
-
-typedef __attribute__((address_space(1))) double gdouble;
+constexpr int kSweepThreads = 64;   // one wave per chain
+constexpr int kStatThreads = 256;   // statistics kernels
+constexpr int kMaskChunk = 2048;    // mask entries staged in LDS at a time
+constexpr int kMtN = 624;           // MT19937 state words (mersenne.cpp MERS_N)
 
 struct GibbsDev {
-    int P, T, K, S;       // S: residual slots per frame (>= P)
+    int P, T, K, W;       // W: 32-bit words per spot in the mask bitmap
+    const float *data;    // [T][P]
     const double *spot;   // [K][P]
     const int *moff;      // [K + 1]
-    const int *mpix;      // mask entries: pixel index (sorted by slot within a spot)
-    const int *mslot;     // mask entries: residual slot
-    const int *slot;      // [P] residual slot of each pixel
+    const int *midx;      // mask pixel indices
     uint8_t *sample;      // [K][T]
-    double *resid;        // [T][S]
     uint8_t *samples;     // [S][K][T] or null
-    const float *data;    // [T][P]
+    double *C;            // [K][T]
+    double *A;            // [K][T]
+    double *O;            // [K][K]
+    double *S2;           // [K]
+    uint32_t *bits;       // [K][W] mask membership
     double base;
     uint32_t *mt;         // [kMtN + 1]: state words, then the position of the next draw
 };
@@ -58,6 +59,10 @@ struct GibbsParams {
     int samples, iters, Tcap, pad;
     const double *var;    // [samples]
     double A[9], pi[3];
+};
+
+struct SpotTask {
+    int chain, k;
 };
 
 // ---------------------------------------------------------------------------
@@ -77,23 +82,48 @@ __device__ __forceinline__ uint32_t mt_word(uint32_t cur, uint32_t nxt, uint32_t
 }
 
 // The sequential update new[i] = f(old[i], old[i+1], (i < 227 ? old : new)[i+397 mod 624])
-// in four dependent ranges: [0,227) reads only old words, [227,454) and
-// [454,623) read the new words 227 below them, and 623 reads new[0], new[396].
-__device__ void mt_twist(uint32_t *mt, int tid) {
-    uint32_t v = 0;
-    if (tid < 227) v = mt_word(mt[tid], mt[tid + 1], mt[tid + 397]);
+// by one wave in four dependent ranges: [0,227) reads only old words,
+// [227,454) and [454,623) read the new words 227 below them, and 623 reads
+// new[0] and new[396].
+__device__ void mt_twist64(uint32_t *mt, int lane) {
+    uint32_t v[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int i = lane + 64 * r;
+        if (i < 227) v[r] = mt_word(mt[i], mt[i + 1], mt[i + 397]);
+    }
     __syncthreads();
-    if (tid < 227) mt[tid] = v;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int i = lane + 64 * r;
+        if (i < 227) mt[i] = v[r];
+    }
     __syncthreads();
-    if (tid < 227) v = mt_word(mt[tid + 227], mt[tid + 228], mt[tid]);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int i = 227 + lane + 64 * r;
+        if (i < 454) v[r] = mt_word(mt[i], mt[i + 1], mt[i - 227]);
+    }
     __syncthreads();
-    if (tid < 227) mt[tid + 227] = v;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int i = 227 + lane + 64 * r;
+        if (i < 454) mt[i] = v[r];
+    }
     __syncthreads();
-    if (tid < 169) v = mt_word(mt[tid + 454], mt[tid + 455], mt[tid + 227]);
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const int i = 454 + lane + 64 * r;
+        if (i < 623) v[r] = mt_word(mt[i], mt[i + 1], mt[i - 227]);
+    }
     __syncthreads();
-    if (tid < 169) mt[tid + 454] = v;
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const int i = 454 + lane + 64 * r;
+        if (i < 623) mt[i] = v[r];
+    }
     __syncthreads();
-    if (tid == 0) mt[623] = mt_word(mt[623], mt[0], mt[396]);
+    if (lane == 0) mt[623] = mt_word(mt[623], mt[0], mt[396]);
     __syncthreads();
 }
 
@@ -204,103 +234,166 @@ __device__ __forceinline__ double wave_sum8(const double (&a)[kFB], int lane) {
     return v;
 }
 
-__global__ __launch_bounds__(kGibbsThreads) void b3_gibbs(GibbsParams G) {
+// ---------------------------------------------------------------------------
+// statistics of the plan: S2, C (b3_gibbs_stats), mask bitmaps
+// (b3_gibbs_member) and the overlaps O (b3_gibbs_overlap), one workgroup
+// per (chain, spot)
+
+// S2_k and C_t(k) for every frame: the mask's pixel indices and spot values
+// staged in LDS in chunks; each wave takes kFB consecutive frames at a time
+// (pixels across lanes), the chunks' sums added in order.
+__global__ __launch_bounds__(kStatThreads) void b3_gibbs_stats(const GibbsDev *chains, const SpotTask *tasks) {
     extern __shared__ __align__(16) unsigned char lds[];
+    const SpotTask tk = tasks[blockIdx.x];
+    const GibbsDev c = chains[tk.chain];
+    const int T = c.T, P = c.P, k = tk.k;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    constexpr int kWaves = kGibbsThreads / 64;
+    double *dsum = (double *)lds;               // [T]
+    double *muL = dsum + T;                     // [kMaskChunk]
+    int *idxL = (int *)(muL + kMaskChunk);      // [kMaskChunk]
+    __shared__ double s2red[kStatThreads / 64];
+    const int m0 = c.moff[k], m = c.moff[k + 1] - m0;
+    const double *spk = c.spot + (size_t)k * P;
+    for (int t = tid; t < T; t += kStatThreads) dsum[t] = 0.0;
+    double s2 = 0.0;
+    const int nblk = (T + kFB - 1) / kFB;
+    for (int c0 = 0; c0 < m; c0 += kMaskChunk) {
+        const int nc = min(kMaskChunk, m - c0);
+        __syncthreads();
+        for (int i = tid; i < nc; i += kStatThreads) {
+            const int p = c.midx[m0 + c0 + i];
+            idxL[i] = p;
+            muL[i] = spk[p];
+            s2 += spk[p] * spk[p];
+        }
+        __syncthreads();
+        for (int b = wave; b < nblk; b += kStatThreads / 64) {
+            const int t0 = b * kFB;
+            const float *rt[kFB];
+            double acc[kFB];
+#pragma unroll
+            for (int f = 0; f < kFB; f++) {
+                rt[f] = c.data + (size_t)min(t0 + f, T - 1) * P;  // frames past T repeat the last, discarded
+                acc[f] = 0.0;
+            }
+            for (int i = lane; i < nc; i += 64) {
+                const int p = idxL[i];
+                const double mu = muL[i];
+#pragma unroll
+                for (int f = 0; f < kFB; f++) acc[f] += mu * ((double)rt[f][p] - c.base);
+            }
+            const double v = wave_sum8(acc, lane);
+            const int t = t0 + ((lane >> 3) & 7);
+            if ((lane & 7) == 0 && t < T) dsum[t] += v;
+        }
+    }
+    s2 = wave_sum(s2);
+    if (lane == 0) s2red[wave] = s2;
+    __syncthreads();
+    for (int t = tid; t < T; t += kStatThreads) c.C[(size_t)k * T + t] = dsum[t];
+    if (tid == 0) {
+        double a = 0.0;
+        for (int w = 0; w < kStatThreads / 64; w++) a += s2red[w];
+        c.S2[k] = a;
+    }
+}
+
+// bits[k][p / 32] has bit p % 32 set for every pixel p of mask k.
+__global__ __launch_bounds__(kStatThreads) void b3_gibbs_member(const GibbsDev *chains, const SpotTask *tasks) {
+    const SpotTask tk = tasks[blockIdx.x];
+    const GibbsDev c = chains[tk.chain];
+    const int m0 = c.moff[tk.k], m1 = c.moff[tk.k + 1];
+    uint32_t *bk = c.bits + (size_t)tk.k * c.W;
+    for (int i = m0 + threadIdx.x; i < m1; i += kStatThreads) {
+        const int p = c.midx[i];
+        atomicOr(bk + (p >> 5), 1u << (p & 31));
+    }
+}
+
+// O_kj = sum over the pixels of mask k that are in mask j of mu_k mu_j, for
+// every j (O_kk = 0).  Each thread sums its pixels in order, then the
+// workgroup in a fixed tree.
+__global__ __launch_bounds__(kStatThreads) void b3_gibbs_overlap(const GibbsDev *chains, const SpotTask *tasks) {
+    const SpotTask tk = tasks[blockIdx.x];
+    const GibbsDev c = chains[tk.chain];
+    const int k = tk.k, K = c.K, P = c.P;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ double red[kStatThreads / 64];
+    const int m0 = c.moff[k], m = c.moff[k + 1] - m0;
+    const double *spk = c.spot + (size_t)k * P;
+    for (int j = 0; j < K; j++) {
+        double a = 0.0;
+        if (j != k) {
+            const double *spj = c.spot + (size_t)j * P;
+            const uint32_t *bj = c.bits + (size_t)j * c.W;
+            for (int i = tid; i < m; i += kStatThreads) {
+                const int p = c.midx[m0 + i];
+                if ((bj[p >> 5] >> (p & 31)) & 1u) a += spk[p] * spj[p];
+            }
+        }
+        a = wave_sum(a);
+        __syncthreads();  // red free from the previous j
+        if (lane == 0) red[wave] = a;
+        __syncthreads();
+        if (tid == 0) {
+            double o = 0.0;
+            for (int w = 0; w < kStatThreads / 64; w++) o += red[w];
+            c.O[(size_t)k * K + j] = o;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the sweeps: one 64-thread workgroup per chain
+
+__global__ __launch_bounds__(kSweepThreads) void b3_gibbs(GibbsParams G) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = threadIdx.x;
     const GibbsDev c = G.chains[G.order[blockIdx.x]];
-    const int T = c.T, P = c.S, K = c.K, Tc = G.Tcap;  // P: the residual's row length
+    const int T = c.T, K = c.K, Tc = G.Tcap;
     const int RC = Tc + kMtN;  // draw ring capacity
 
     double *alpha = (double *)lds;          // [Tc][3]
     double *ex = alpha + 3 * Tc;            // [Tc][3]
-    double *dsum = ex + 3 * Tc;             // [Tc]
-    double *muL = dsum + Tc;                // [kMaskChunk]
-    int *idxL = (int *)(muL + kMaskChunk);  // [kMaskChunk]
-    int *chg = idxL + kMaskChunk;           // [Tc] frames whose on/off status changed
-    int *nchg = chg + Tc;                   // [4]
-    uint32_t *mt = (uint32_t *)(nchg + 4);  // [kMtN]
+    int *chg = (int *)(ex + 3 * Tc);        // [Tc] frames whose on/off status changed
+    uint32_t *mt = (uint32_t *)(chg + Tc);  // [kMtN]
     uint32_t *ring = mt + kMtN;             // [RC]
     uint8_t *so = (uint8_t *)(ring + RC);   // [Tc] state before the update
     uint8_t *sn = so + Tc;                  // [Tc] state after the update
     __shared__ double Api[12];              // A[9], pi[3]
-    // the residual through global-address-space pointers: rows are uniform
-    // per wave, pixel offsets per lane
-    gdouble *R = (gdouble *)c.resid;
+    __shared__ int nchg;
 
-    if (tid < 9) Api[tid] = G.A[tid];
-    else if (tid < 12) Api[tid] = G.pi[tid - 9];
-    for (int i = tid; i < kMtN; i += kGibbsThreads) mt[i] = c.mt[i];
+    if (lane < 9) Api[lane] = G.A[lane];
+    else if (lane < 12) Api[lane] = G.pi[lane - 9];
+    for (int i = lane; i < kMtN; i += kSweepThreads) mt[i] = c.mt[i];
     const int pos0 = (int)c.mt[kMtN];
     __syncthreads();
     // Every thread tracks the ring's head and count; they change uniformly.
     int head = 0, cnt = kMtN - pos0;
-    for (int i = tid; i < cnt; i += kGibbsThreads) ring[i] = mt_temper(mt[pos0 + i]);
-    const int nblk = (T + kFB - 1) / kFB;
+    for (int i = lane; i < cnt; i += kSweepThreads) ring[i] = mt_temper(mt[pos0 + i]);
 
     for (int s = 0; s < G.samples; s++) {
         const double i2v = 0.5 / G.var[s];
         for (int it = 0; it < G.iters; it++) {
             for (int k = 0; k < K; k++) {
-                const int m0 = c.moff[k], m = c.moff[k + 1] - m0;
-                const double *spk = c.spot + (size_t)k * c.P;
+                const double *Ck = c.C + (size_t)k * T;
+                const double *Ak = c.A + (size_t)k * T;
+                const double S2 = c.S2[k];
                 uint8_t *smp = c.sample + (size_t)k * T;
-                for (int t = tid; t < T; t += kGibbsThreads) {
+                // 1. B_on - B_off of every frame
+                for (int t = lane; t < T; t += kSweepThreads) {
                     so[t] = smp[t];
-                    dsum[t] = 0.0;
-                }
-                __syncthreads();
-                // 1. d_t over the mask, the spot taken out where it was on;
-                //    each wave takes kFB consecutive frames at a time
-                for (int c0 = 0; c0 < m; c0 += kMaskChunk) {
-                    const int nc = min(kMaskChunk, m - c0);
-                    for (int i = tid; i < nc; i += kGibbsThreads) {
-                        idxL[i] = c.mslot[m0 + c0 + i];
-                        muL[i] = spk[c.mpix[m0 + c0 + i]];
-                    }
-                    __syncthreads();
-                    for (int b = wave; b < nblk; b += kWaves) {
-                        const int t0 = b * kFB;
-                        const gdouble *rt[kFB];
-                        bool on[kFB];
-                        double acc[kFB];
-#pragma unroll
-                        for (int f = 0; f < kFB; f++) {
-                            const int t = min(t0 + f, T - 1);  // frames past T repeat the last, discarded
-                            rt[f] = R + (size_t)t * P;
-                            on[f] = so[t] == 0;
-                            acc[f] = 0.0;
-                        }
-#pragma unroll 2
-                        for (int i = lane; i < nc; i += 64) {
-                            const int p = idxL[i];
-                            const double mu = muL[i];
-                            double r[kFB];
-#pragma unroll
-                            for (int f = 0; f < kFB; f++) r[f] = rt[f][p];
-#pragma unroll
-                            for (int f = 0; f < kFB; f++) {
-                                const double r0 = on[f] ? r[f] + mu : r[f];
-                                acc[f] += mu * (mu - 2.0 * r0);
-                            }
-                        }
-                        const double v = wave_sum8(acc, lane);
-                        const int t = t0 + ((lane >> 3) & 7);
-                        if ((lane & 7) == 0 && t < T) dsum[t] += v;
-                    }
-                    __syncthreads();
-                }
-                for (int t = tid; t < T; t += kGibbsThreads) {
-                    const double x = -dsum[t] * i2v;
+                    const double d = S2 - 2.0 * (Ck[t] - Ak[t]);
+                    const double x = -d * i2v;
                     ex[3 * t] = x;
                     ex[3 * t + 1] = x >= 0.0 ? 1.0 : exp(x);
                     ex[3 * t + 2] = x >= 0.0 ? exp(-x) : 1.0;
                 }
                 // 2. T draws, then the forward recursion and backward sampling
                 while (cnt < T) {
-                    mt_twist(mt, tid);
-                    for (int i = tid; i < kMtN; i += kGibbsThreads) {
+                    mt_twist64(mt, lane);
+                    for (int i = lane; i < kMtN; i += kSweepThreads) {
                         int w = head + cnt + i;
                         w = w >= RC ? w - RC : w;
                         ring[w] = mt_temper(mt[i]);
@@ -308,86 +401,68 @@ __global__ __launch_bounds__(kGibbsThreads) void b3_gibbs(GibbsParams G) {
                     cnt += kMtN;
                 }
                 __syncthreads();
-                if (tid == 0) nchg[0] = ffbs_lane(Api, T, ex, alpha, ring, head, RC, so, sn, chg);
+                if (lane == 0) nchg = ffbs_lane(Api, T, ex, alpha, ring, head, RC, so, sn, chg);
                 head += T;
                 head = head >= RC ? head - RC : head;
                 cnt -= T;
                 __syncthreads();
-                // 3. the new sample, and the residual in the frames whose
-                //    on/off status changed (kFB of them per wave at a time)
-                for (int t = tid; t < T; t += kGibbsThreads) smp[t] = sn[t];
-                const int nch = nchg[0];
-                const bool restage = m > kMaskChunk;
-                if (nch > 0) {
-                    for (int c0 = 0; c0 < m; c0 += kMaskChunk) {
-                        const int nc = min(kMaskChunk, m - c0);
-                        if (restage) {
-                            for (int i = tid; i < nc; i += kGibbsThreads) {
-                                idxL[i] = c.mslot[m0 + c0 + i];
-                                muL[i] = spk[c.mpix[m0 + c0 + i]];
-                            }
-                            __syncthreads();
-                        }
-                        for (int j0 = wave * kFB; j0 < nch; j0 += kWaves * kFB) {
-                            gdouble *rt[kFB];
-                            double sg[kFB];
-#pragma unroll
-                            for (int f = 0; f < kFB; f++) {
-                                const int e = __builtin_amdgcn_readfirstlane(chg[min(j0 + f, nch - 1)]);
-                                rt[f] = R + (size_t)(e >> 1) * P;
-                                sg[f] = j0 + f < nch ? ((e & 1) ? 1.0 : -1.0) : 0.0;  // +mu: spot removed
-                            }
-#pragma unroll 1
-                            for (int i = lane; i < nc; i += 64) {
-                                const int p = idxL[i];
-                                const double mu = muL[i];
-                                double r[kFB];
-#pragma unroll
-                                for (int f = 0; f < kFB; f++)
-                                    if (sg[f] != 0.0) r[f] = rt[f][p];
-#pragma unroll
-                                for (int f = 0; f < kFB; f++)
-                                    if (sg[f] != 0.0) rt[f][p] = r[f] + sg[f] * mu;
-                            }
-                        }
-                        if (restage) __syncthreads();
-                    }
+                // 3. the new sample, and A of the other spots in the frames
+                //    where this spot turned on (+O) or off (-O)
+                for (int t = lane; t < T; t += kSweepThreads) smp[t] = sn[t];
+                const int nch = nchg;
+                for (int e = lane; e < nch * K; e += kSweepThreads) {
+                    const int f = e / K, j = e - f * K;
+                    if (j == k) continue;
+                    const int code = chg[f];
+                    const double o = c.O[(size_t)j * K + k];
+                    double *a = c.A + (size_t)j * T + (code >> 1);
+                    *a = (code & 1) ? *a - o : *a + o;
                 }
                 __syncthreads();
             }
         }
         if (c.samples) {
             uint8_t *dst = c.samples + (size_t)s * K * T;
-            for (int i = tid; i < K * T; i += kGibbsThreads) dst[i] = c.sample[i];
+            for (int i = lane; i < K * T; i += kSweepThreads) dst[i] = c.sample[i];
             __syncthreads();
         }
     }
     // the generator's state and the position of its next draw
-    for (int i = tid; i < kMtN; i += kGibbsThreads) c.mt[i] = mt[i];
-    if (tid == 0) c.mt[kMtN] = (uint32_t)(kMtN - cnt);
+    for (int i = lane; i < kMtN; i += kSweepThreads) c.mt[i] = mt[i];
+    if (lane == 0) c.mt[kMtN] = (uint32_t)(kMtN - cnt);
 }
 
 // GibbsSampler2::reset (sampled_multispot.h:430-438): all spots in state 2,
-// sample intensities = base, i.e. residual = data - base.
+// no spot in the sample intensities (A = 0).
 __global__ void b3_gibbs_reset(const GibbsDev *chains) {
     const GibbsDev c = chains[blockIdx.y];
-    const size_t nt = (size_t)c.T * c.P, ns = (size_t)c.K * c.T;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nt; e += stride) {
-        const size_t t = e / c.P, p = e - t * c.P;
-        c.resid[t * c.S + c.slot[p]] = (double)c.data[e] - c.base;
+    const size_t n = (size_t)c.K * c.T;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        c.A[e] = 0.0;
+        c.sample[e] = 2;
     }
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < ns; e += stride) c.sample[e] = 2;
 }
 
-// sample intensities (+ base) in the pixel list's order: data - residual
-__global__ void b3_gibbs_export(const GibbsDev *chains, int i, double *out) {
+// Sample intensities of chain i from its sample: base, plus each on spot
+// over its mask, spots in index order (a barrier between spots: masks
+// overlap).
+__global__ __launch_bounds__(kStatThreads) void b3_gibbs_export(const GibbsDev *chains, int i, double *out) {
     const GibbsDev c = chains[i];
-    const size_t nt = (size_t)c.T * c.P;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nt; e += stride) {
-        const size_t t = e / c.P, p = e - t * c.P;
-        out[e] = (double)c.data[e] - c.resid[t * c.S + c.slot[p]];
+    const int T = c.T, P = c.P;
+    for (size_t e = threadIdx.x; e < (size_t)T * P; e += kStatThreads) out[e] = c.base;
+    __syncthreads();
+    for (int j = 0; j < c.K; j++) {
+        const int m0 = c.moff[j], m1 = c.moff[j + 1];
+        const double *spj = c.spot + (size_t)j * P;
+        for (int t = 0; t < T; t++) {
+            if (c.sample[(size_t)j * T + t] != 0) continue;
+            double *o = out + (size_t)t * P;
+            for (int q = m0 + threadIdx.x; q < m1; q += kStatThreads) {
+                const int p = c.midx[q];
+                o[p] += spj[p];
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -457,52 +532,25 @@ struct HostMt {
     }
 };
 
+// LDS of the sweep kernel for frames up to Tc
 size_t gibbs_lds(int Tc) {
-    return 8 * (7 * (size_t)Tc + kMaskChunk) + 4 * ((size_t)kMaskChunk + Tc + 4) +
-           4 * ((size_t)kMtN + Tc + kMtN) + 2 * (size_t)Tc;
+    return 8 * (6 * (size_t)Tc) + 4 * (size_t)Tc + 4 * ((size_t)kMtN + Tc + kMtN) + 2 * (size_t)Tc;
 }
+
+size_t stats_lds(int Tc) { return 8 * ((size_t)Tc + kMaskChunk) + 4 * (size_t)kMaskChunk; }
 
 }  // namespace
-
-// Residual layout of a chain: with coordinates, voxel (x, y, z) of the
-// pixels' bounding box goes to brick ((z/2 * NBY + y/2) * NBX + x/4), slot
-// (z&1) * 8 + (y&1) * 4 + (x&3) within the brick's 16 (one 128-byte line of
-// FP64); a region whose bricks would hold more than twice its pixels keeps
-// the list order.  Returns the slots per frame.
-static int brick_layout(const int *xyz, int P, std::vector<int> &slot) {
-    slot.resize(P);
-    if (xyz && P > 0) {
-        int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-        for (int p = 0; p < P; p++)
-            for (int j = 0; j < 3; j++) {
-                lo[j] = std::min(lo[j], xyz[3 * p + j]);
-                hi[j] = std::max(hi[j], xyz[3 * p + j]);
-            }
-        const long long nbx = ((long long)hi[0] - lo[0] + 4) / 4, nby = ((long long)hi[1] - lo[1] + 2) / 2,
-                        nbz = ((long long)hi[2] - lo[2] + 2) / 2;
-        const long long S = nbx * nby * nbz * 16;
-        if (S <= 2LL * P + 4096) {
-            for (int p = 0; p < P; p++) {
-                const long long x = xyz[3 * p] - lo[0], y = xyz[3 * p + 1] - lo[1], z = xyz[3 * p + 2] - lo[2];
-                slot[p] = (int)((((z >> 1) * nby + (y >> 1)) * nbx + (x >> 2)) * 16 + (z & 1) * 8 + (y & 1) * 4 + (x & 3));
-            }
-            return (int)S;
-        }
-    }
-    for (int p = 0; p < P; p++) slot[p] = p;
-    return P;
-}
 
 struct rfba3b_gibbs {
     rfba3b_model model;
     std::vector<rfba3b_gibbs_chain> chains;
-    std::vector<GibbsDev> dev;
     std::vector<unsigned long long> draws;
     GibbsDev *d_chains = nullptr;
     int *d_order = nullptr;
     uint32_t *d_mt = nullptr;
-    int *d_ints = nullptr;       // offsets, mask pixels, mask slots, pixel slots of every chain
-    double *d_resid = nullptr;   // residuals of every chain
+    int *d_ints = nullptr;       // every chain's mask offsets and indices
+    double *d_dbl = nullptr;     // every chain's C, A, O, S2
+    uint32_t *d_bits = nullptr;  // every chain's mask bitmaps
     double *d_var = nullptr;
     int var_cap = 0;
     int Tmax = 0;
@@ -512,15 +560,19 @@ struct rfba3b_gibbs {
         if (d_order) (void)hipFree(d_order);
         if (d_mt) (void)hipFree(d_mt);
         if (d_ints) (void)hipFree(d_ints);
-        if (d_resid) (void)hipFree(d_resid);
+        if (d_dbl) (void)hipFree(d_dbl);
+        if (d_bits) (void)hipFree(d_bits);
         if (d_var) (void)hipFree(d_var);
     }
 };
 
-static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_model *model, rfba3b_gibbs **out) {
+static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_model *model, rfba3b_gibbs **out,
+                        float *setup_ms) {
     if (!out) return host_fail(-1, "plan pointer is NULL");
     *out = nullptr;
+    if (setup_ms) *setup_ms = 0.0f;
     if (!chains || n <= 0) return host_fail(-1, "no chains");
+    if (n > 65535) return host_fail(-1, "%d chains: at most 65535 per plan", n);
     if (int rc = host_check_model(model)) return rc;
     int ncu = 0;
     if (int rc = host_dev_init(&ncu)) return rc;
@@ -528,14 +580,16 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
     for (int i = 0; i < n; i++) {
         const rfba3b_gibbs_chain &c = chains[i];
         if (c.P <= 0 || c.T <= 0 || c.K < 0) return host_fail(-1, "chain %d: P and T must be positive, K >= 0", i);
+        if ((size_t)c.P * c.T > (size_t)INT_MAX * 64) return host_fail(-1, "chain %d: T x P too large", i);
         if (!c.d_data || !c.d_sample) return host_fail(-1, "chain %d: data or sample is NULL", i);
         if (c.K > 0 && (!c.d_spot || !c.d_mask_off || !c.d_mask))
             return host_fail(-1, "chain %d: spot table or masks are NULL", i);
         if (!std::isfinite(c.base)) return host_fail(-1, "chain %d: base is not finite", i);
         Tmax = std::max(Tmax, c.T);
     }
-    const size_t lds = gibbs_lds(Tmax);
-    if (lds > 160 * 1024) return host_fail(-1, "T = %d needs %zu bytes of LDS per chain (at most 160 KiB)", Tmax, lds);
+    const size_t lds = gibbs_lds(Tmax), slds = stats_lds(Tmax);
+    if (lds > 160 * 1024 || slds > 160 * 1024)
+        return host_fail(-1, "T = %d needs %zu bytes of LDS per chain (at most 160 KiB)", Tmax, std::max(lds, slds));
 
     auto *g = new rfba3b_gibbs();
     std::unique_ptr<rfba3b_gibbs> own(g);
@@ -545,15 +599,15 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
     g->Tmax = Tmax;
     g->lds = lds;
 
-    // masks to the host: checked, mapped to residual slots, sorted by slot
-    std::vector<int> ints;                 // all chains' integer tables, concatenated
-    std::vector<size_t> o_off(n), o_pix(n), o_slotm(n), o_slot(n), o_res(n);
-    std::vector<int> S(n);
+    // masks to the host for checking; offsets and indices kept by the plan
+    std::vector<int> ints;
+    std::vector<size_t> o_off(n), o_idx(n), o_dbl(n), o_bits(n);
     std::vector<long long> work(n);
-    size_t nres = 0;
+    std::vector<SpotTask> tasks;
+    size_t ndbl = 0, nbits = 0;
     for (int i = 0; i < n; i++) {
         const rfba3b_gibbs_chain &c = chains[i];
-        std::vector<int> off(c.K + 1, 0), mask, slot;
+        std::vector<int> off(c.K + 1, 0), mask;
         if (c.K > 0) {
             GHIP(hipMemcpy(off.data(), c.d_mask_off, (c.K + 1) * sizeof(int), hipMemcpyDeviceToHost));
             if (off[0] != 0) return host_fail(-1, "chain %d: mask offsets must start at 0", i);
@@ -568,36 +622,25 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
                         return host_fail(-1, "chain %d: mask of spot %d is not increasing within [0, P) at %d", i,
                                          k, j - off[k]);
         }
-        S[i] = brick_layout(c.xyz, c.P, slot);
-        if ((size_t)S[i] * c.T > (size_t)LLONG_MAX / 8) return host_fail(-1, "chain %d: T x P too large", i);
         o_off[i] = ints.size();
         ints.insert(ints.end(), off.begin(), off.end());
-        o_pix[i] = ints.size();
-        o_slotm[i] = o_pix[i] + mask.size();
-        ints.resize(ints.size() + 2 * mask.size());
-        for (int k = 0; k < c.K; k++) {
-            std::vector<std::pair<int, int>> e;  // (slot, pixel)
-            for (int j = off[k]; j < off[k + 1]; j++) e.emplace_back(slot[mask[j]], mask[j]);
-            std::sort(e.begin(), e.end());
-            for (size_t j = 0; j < e.size(); j++) {
-                ints[o_pix[i] + off[k] + j] = e[j].second;
-                ints[o_slotm[i] + off[k] + j] = e[j].first;
-            }
-        }
-        o_slot[i] = ints.size();
-        ints.insert(ints.end(), slot.begin(), slot.end());
-        o_res[i] = nres;
-        nres += ((size_t)S[i] * c.T + 31) & ~(size_t)31;  // 256-byte aligned chains
-        work[i] = (long long)c.T * std::max(off[c.K], 1) + (long long)c.K * c.T;
+        o_idx[i] = ints.size();
+        ints.insert(ints.end(), mask.begin(), mask.end());
+        o_dbl[i] = ndbl;  // C [K][T], A [K][T], O [K][K], S2 [K], 256-byte aligned
+        ndbl += ((2 * (size_t)c.K * c.T + (size_t)c.K * c.K + c.K) + 31) & ~(size_t)31;
+        o_bits[i] = nbits;
+        nbits += (((size_t)c.K * ((c.P + 31) / 32)) + 63) & ~(size_t)63;
+        for (int k = 0; k < c.K; k++) tasks.push_back({i, k});
+        work[i] = (long long)c.K * c.T;
     }
     GHIP(hipMalloc(&g->d_ints, std::max<size_t>(ints.size(), 1) * sizeof(int)));
-    GHIP(hipMemcpy(g->d_ints, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice));
-    GHIP(hipMalloc(&g->d_resid, nres * sizeof(double)));
-    GHIP(hipMemset(g->d_resid, 0, nres * sizeof(double)));  // brick slots outside the region stay 0
+    if (!ints.empty()) GHIP(hipMemcpy(g->d_ints, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice));
+    GHIP(hipMalloc(&g->d_dbl, std::max<size_t>(ndbl, 1) * sizeof(double)));
+    GHIP(hipMalloc(&g->d_bits, std::max<size_t>(nbits, 1) * sizeof(uint32_t)));
+    GHIP(hipMemset(g->d_bits, 0, std::max<size_t>(nbits, 1) * sizeof(uint32_t)));
 
     std::vector<uint32_t> mts((size_t)n * (kMtN + 1));
-    std::vector<GibbsDev> &dev = g->dev;
-    dev.resize(n);
+    std::vector<GibbsDev> dev(n);
     GHIP(hipMalloc(&g->d_mt, mts.size() * sizeof(uint32_t)));
     for (int i = 0; i < n; i++) {
         const rfba3b_gibbs_chain &c = chains[i];
@@ -607,21 +650,57 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
         mts[(size_t)i * (kMtN + 1) + kMtN] = (uint32_t)h.pos;
         g->draws[i] = c.skip;
         GibbsDev &d = dev[i];
-        d.P = c.P; d.T = c.T; d.K = c.K; d.S = S[i];
-        d.spot = c.d_spot;
+        d.P = c.P; d.T = c.T; d.K = c.K; d.W = (c.P + 31) / 32;
+        d.data = c.d_data; d.spot = c.d_spot;
         d.moff = g->d_ints + o_off[i];
-        d.mpix = g->d_ints + o_pix[i];
-        d.mslot = g->d_ints + o_slotm[i];
-        d.slot = g->d_ints + o_slot[i];
-        d.sample = c.d_sample; d.resid = g->d_resid + o_res[i]; d.samples = c.d_samples;
-        d.data = c.d_data; d.base = c.base;
+        d.midx = g->d_ints + o_idx[i];
+        d.sample = c.d_sample; d.samples = c.d_samples;
+        double *b = g->d_dbl + o_dbl[i];
+        d.C = b;
+        d.A = b + (size_t)c.K * c.T;
+        d.O = b + 2 * (size_t)c.K * c.T;
+        d.S2 = d.O + (size_t)c.K * c.K;
+        d.bits = g->d_bits + o_bits[i];
+        d.base = c.base;
         d.mt = g->d_mt + (size_t)i * (kMtN + 1);
     }
     GHIP(hipMemcpy(g->d_mt, mts.data(), mts.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     GHIP(hipMalloc(&g->d_chains, n * sizeof(GibbsDev)));
     GHIP(hipMemcpy(g->d_chains, dev.data(), n * sizeof(GibbsDev), hipMemcpyHostToDevice));
 
-    // launch order: largest work (T x mask entries) first
+    // statistics: bitmaps, then S2 and C, then O (the overlaps read the bitmaps)
+    if (!tasks.empty()) {
+        SpotTask *d_tasks = nullptr;
+        GHIP(hipMalloc(&d_tasks, tasks.size() * sizeof(SpotTask)));
+        hipError_t e = hipMemcpy(d_tasks, tasks.data(), tasks.size() * sizeof(SpotTask), hipMemcpyHostToDevice);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (e == hipSuccess) e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        if (e == hipSuccess && slds > 64 * 1024)
+            e = hipFuncSetAttribute((const void *)b3_gibbs_stats, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds);
+        if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+        if (e == hipSuccess) {
+            const dim3 grid((unsigned)tasks.size());
+            hipLaunchKernelGGL(b3_gibbs_member, grid, dim3(kStatThreads), 0, nullptr, (const GibbsDev *)g->d_chains,
+                               (const SpotTask *)d_tasks);
+            hipLaunchKernelGGL(b3_gibbs_stats, grid, dim3(kStatThreads), slds, nullptr, (const GibbsDev *)g->d_chains,
+                               (const SpotTask *)d_tasks);
+            hipLaunchKernelGGL(b3_gibbs_overlap, grid, dim3(kStatThreads), 0, nullptr,
+                               (const GibbsDev *)g->d_chains, (const SpotTask *)d_tasks);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.0f;
+        if (e == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipFree(d_tasks);
+        GHIP(e);
+        if (setup_ms) *setup_ms = ms;
+    }
+
+    // launch order: largest work (spots x frames) first
     std::vector<int> order(n);
     for (int i = 0; i < n; i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
@@ -631,9 +710,9 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
     if (lds > 64 * 1024)
         GHIP(hipFuncSetAttribute((const void *)b3_gibbs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
-    GHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)b3_gibbs, kGibbsThreads, lds));
+    GHIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)b3_gibbs, kSweepThreads, lds));
     if (per_cu <= 0) return host_fail(-1, "Gibbs kernel does not fit on a CU (LDS %zu bytes)", lds);
-    hipLaunchKernelGGL(b3_gibbs_reset, dim3(64, n), dim3(256), 0, nullptr, (const GibbsDev *)g->d_chains);
+    hipLaunchKernelGGL(b3_gibbs_reset, dim3(16, n), dim3(256), 0, nullptr, (const GibbsDev *)g->d_chains);
     GHIP(hipGetLastError());
     GHIP(hipDeviceSynchronize());
     *out = own.release();
@@ -641,16 +720,16 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
 }
 
 extern "C" int rfba3b_gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_model *model,
-                                   rfba3b_gibbs **plan) {
+                                   rfba3b_gibbs **plan, float *setup_ms) {
     std::lock_guard<std::mutex> lk(host_mutex());
-    return gibbs_create(chains, n, model, plan);
+    return gibbs_create(chains, n, model, plan, setup_ms);
 }
 
 extern "C" int rfba3b_gibbs_reset(rfba3b_gibbs *g, void *stream) {
     std::lock_guard<std::mutex> lk(host_mutex());
     if (!g) return host_fail(-1, "plan is NULL");
     const int n = (int)g->chains.size();
-    hipLaunchKernelGGL(b3_gibbs_reset, dim3(64, n), dim3(256), 0, (hipStream_t)stream,
+    hipLaunchKernelGGL(b3_gibbs_reset, dim3(16, n), dim3(256), 0, (hipStream_t)stream,
                        (const GibbsDev *)g->d_chains);
     GHIP(hipGetLastError());
     return 0;
@@ -693,7 +772,7 @@ extern "C" int rfba3b_gibbs_run(rfba3b_gibbs *g, int samples, int sample_iterati
     GHIP(hipEventCreate(&e0));
     GHIP(hipEventCreate(&e1));
     GHIP(hipEventRecord(e0, st));
-    hipLaunchKernelGGL(b3_gibbs, dim3((unsigned)g->chains.size()), dim3(kGibbsThreads), g->lds, st, P);
+    hipLaunchKernelGGL(b3_gibbs, dim3((unsigned)g->chains.size()), dim3(kSweepThreads), g->lds, st, P);
     const hipError_t le = hipGetLastError();
     GHIP(hipEventRecord(e1, st));
     const hipError_t se = hipEventSynchronize(e1);
@@ -713,8 +792,8 @@ extern "C" int rfba3b_gibbs_sample_intensities(const rfba3b_gibbs *g, int chain,
     std::lock_guard<std::mutex> lk(host_mutex());
     if (!g || !d_out) return host_fail(-1, "plan or output is NULL");
     if (chain < 0 || chain >= (int)g->chains.size()) return host_fail(-1, "chain %d out of range", chain);
-    hipLaunchKernelGGL(b3_gibbs_export, dim3(256), dim3(256), 0, (hipStream_t)stream, (const GibbsDev *)g->d_chains,
-                       chain, d_out);
+    hipLaunchKernelGGL(b3_gibbs_export, dim3(1), dim3(kStatThreads), 0, (hipStream_t)stream,
+                       (const GibbsDev *)g->d_chains, chain, d_out);
     GHIP(hipGetLastError());
     return 0;
 }
