@@ -148,72 +148,95 @@ __device__ __forceinline__ int select3(double w0, double w1, double w2, double u
     return 2;
 }
 
-// Forward recursion over normalised messages and backward sampling, on one
-// lane.  ex[3t..3t+2] = (x_t, b_on, b_off) with x_t = B_on - B_off,
-// b_on = e^min(x,0), b_off = e^min(-x,0).  alpha_t is the forward message
-// up to a positive per-frame factor (forward_algorithm_delta2's delta_t
-// minus a per-frame constant); backward sampling (forward_algorithm.h:
-// 542-566) draws the last frame from alpha_{T-1} and frame t from
-// alpha_t(i) A[i][s_{t+1}], which is sample_unscaled_log's distribution
-// (:520-532) -- a per-frame factor changes neither.  The frames whose on/off
-// status changes are listed in chg as (t << 1) | was_on; returns their count.
-__device__ int ffbs_lane(const double *Api, int T, const double *ex, double *alpha, const uint32_t *ring,
-                         int head, int RC, const uint8_t *so, uint8_t *sn, int *chg) {
-    const double *A = Api, *pi = Api + 9;
-    double a0 = 0, a1 = 0, a2 = 0;
-    for (int t = 0; t < T; t++) {
-        double p0, p1, p2;
-        if (t == 0) {
-            p0 = pi[0]; p1 = pi[1]; p2 = pi[2];
+// The value of lane l for the whole wave (v_readlane); l must be uniform.
+__device__ __forceinline__ double read_lane(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// One step of the forward recursion, the same on every lane: the message of
+// a frame from the previous one's (a0, a1, a2) and the frame's x = B_on -
+// B_off, b_on = e^min(x,0), b_off = e^min(-x,0), normalised to sum 1 (any
+// positive per-frame factor leaves sample_unscaled_log's distribution
+// unchanged), with the step in log space when the emission ratio
+// underflowed.  alpha_t is forward_algorithm_delta2's delta_t (forward_
+// algorithm.h:344-382) less a per-frame constant.
+__device__ __forceinline__ void forward_step(const double *A, const double *pi, bool first, double x, double bon,
+                                             double boff, double &a0, double &a1, double &a2) {
+    double p0, p1, p2;
+    if (first) {
+        p0 = pi[0]; p1 = pi[1]; p2 = pi[2];
+    } else {
+        p0 = a0 * A[0] + a1 * A[3] + a2 * A[6];
+        p1 = a0 * A[1] + a1 * A[4] + a2 * A[7];
+        p2 = a0 * A[2] + a1 * A[5] + a2 * A[8];
+    }
+    const double n0 = p0 * bon, n1 = p1 * boff, n2 = p2 * boff;
+    const double c = n0 + n1 + n2;
+    if (c > 1e-250) {
+        // any positive scale will do: the hardware reciprocal
+        const double ic = __builtin_amdgcn_rcp(c);
+        a0 = n0 * ic; a1 = n1 * ic; a2 = n2 * ic;
+    } else {
+        const double ninf = -__builtin_inf();
+        const double l0 = (p0 > 0 ? log(p0) : ninf) + fmin(x, 0.0);
+        const double l1 = (p1 > 0 ? log(p1) : ninf) + fmin(-x, 0.0);
+        const double l2 = (p2 > 0 ? log(p2) : ninf) + fmin(-x, 0.0);
+        const double mx = fmax(l0, fmax(l1, l2));
+        if (mx == ninf) {
+            a0 = a1 = a2 = 0.0;
         } else {
-            p0 = a0 * A[0] + a1 * A[3] + a2 * A[6];
-            p1 = a0 * A[1] + a1 * A[4] + a2 * A[7];
-            p2 = a0 * A[2] + a1 * A[5] + a2 * A[8];
+            const double e0 = exp(l0 - mx), e1 = exp(l1 - mx), e2 = exp(l2 - mx);
+            const double is = 1.0 / (e0 + e1 + e2);
+            a0 = e0 * is; a1 = e1 * is; a2 = e2 * is;
         }
-        const double x = ex[3 * t], bon = ex[3 * t + 1], boff = ex[3 * t + 2];
-        const double n0 = p0 * bon, n1 = p1 * boff, n2 = p2 * boff;
-        const double c = n0 + n1 + n2;
-        if (c > 1e-250) {
-            // any positive scale will do: the hardware reciprocal
-            const double ic = __builtin_amdgcn_rcp(c);
-            a0 = n0 * ic; a1 = n1 * ic; a2 = n2 * ic;
-        } else {
-            // the same step in log space (the emission ratio underflowed)
-            const double ninf = -__builtin_inf();
-            const double l0 = (p0 > 0 ? log(p0) : ninf) + fmin(x, 0.0);
-            const double l1 = (p1 > 0 ? log(p1) : ninf) + fmin(-x, 0.0);
-            const double l2 = (p2 > 0 ? log(p2) : ninf) + fmin(-x, 0.0);
-            const double mx = fmax(l0, fmax(l1, l2));
-            if (mx == ninf) {
-                a0 = a1 = a2 = 0.0;
-            } else {
-                const double e0 = exp(l0 - mx), e1 = exp(l1 - mx), e2 = exp(l2 - mx);
-                const double is = 1.0 / (e0 + e1 + e2);
-                a0 = e0 * is; a1 = e1 * is; a2 = e2 * is;
-            }
+    }
+}
+
+// Prefix products of the frames' 3 x 3 transfer matrices across the wave:
+// on entry S is lane l's matrix M_l (row: state in its frame, column: state
+// in the previous frame); on return S = M_l M_{l-1} ... M_0, every partial
+// product scaled to sum 1 (any positive factor per frame will do).  Returns
+// false where a partial product summed to 1e-250 or less, i.e. where
+// entries may have underflowed.
+__device__ __forceinline__ bool prefix_products(double (&S)[9], int lane) {
+    bool ok = true;
+#pragma unroll
+    for (int d = 1; d < kSweepThreads; d <<= 1) {
+        double Q[9];
+#pragma unroll
+        for (int e = 0; e < 9; e++) Q[e] = __shfl_up(S[e], d);
+        if (lane >= d) {
+            double R[9], sum = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+#pragma unroll
+                for (int i = 0; i < 3; i++) R[3 * j + i] = S[3 * j] * Q[i] + S[3 * j + 1] * Q[3 + i] + S[3 * j + 2] * Q[6 + i];
+#pragma unroll
+            for (int e = 0; e < 9; e++) sum += R[e];
+            const bool big = sum > 1e-250;
+            ok = ok && big;
+            const double is = big ? __builtin_amdgcn_rcp(sum) : 1.0;
+#pragma unroll
+            for (int e = 0; e < 9; e++) S[e] = R[e] * is;
         }
-        alpha[3 * t] = a0;
-        alpha[3 * t + 1] = a1;
-        alpha[3 * t + 2] = a2;
     }
-    const double scale = 1.0 / (65536.0 * 65536.0);
-    int h = head;
-    double u = (double)ring[h] * scale;
-    h = (h + 1 == RC) ? 0 : h + 1;
-    int next = select3(alpha[3 * (T - 1)], alpha[3 * (T - 1) + 1], alpha[3 * (T - 1) + 2], u);
-    sn[T - 1] = (uint8_t)next;
-    for (int t = T - 2; t >= 0; t--) {
-        u = (double)ring[h] * scale;
-        h = (h + 1 == RC) ? 0 : h + 1;
-        next = select3(alpha[3 * t] * A[next], alpha[3 * t + 1] * A[3 + next], alpha[3 * t + 2] * A[6 + next], u);
-        sn[t] = (uint8_t)next;
+    return ok;
+}
+
+// Maps of {0, 1, 2} into itself, 2 bits per argument (n -> bits 2n..2n+1);
+// compose3(G, H) is n -> G(H(n)).
+constexpr int kIdentity3 = 0 | (1 << 2) | (2 << 4);
+__device__ __forceinline__ int compose3(int G, int H) {
+    int r = 0;
+#pragma unroll
+    for (int n = 0; n < 3; n++) {
+        const int h = (H >> (2 * n)) & 3;
+        r |= ((G >> (2 * h)) & 3) << (2 * n);
     }
-    int n = 0;
-    for (int t = 0; t < T; t++) {
-        const bool was = so[t] == 0, now = sn[t] == 0;
-        if (was != now) chg[n++] = (t << 1) | (was ? 1 : 0);
-    }
-    return n;
+    return r;
 }
 
 // Sums of kFB per-lane values over the wave in 10 exchanges: halving the
@@ -346,6 +369,22 @@ __global__ __launch_bounds__(kStatThreads) void b3_gibbs_overlap(const GibbsDev 
 
 // ---------------------------------------------------------------------------
 // the sweeps: one 64-thread workgroup per chain
+//
+// Per spot update, frames are taken 64 at a time, one per lane:
+// 1. each lane forms its frame's x, b_on, b_off and transfer matrix
+//    M_t[j][i] = b_j A[i][j] (b_j pi_j in frame 0); a 6-step scan across the
+//    wave forms the prefix products, which applied to the message of the
+//    frame before the group give every frame's forward message.  Where a
+//    partial product underflowed, the group is redone one frame at a time
+//    with forward_step (which has the log-space step);
+// 2. backward sampling (forward_algorithm.h:542-566) draws the last frame
+//    from alpha_{T-1} and frame t from alpha_t(i) A[i][s_{t+1}] with draw
+//    u_t; each lane evaluates select_random_element for its frame under
+//    each of the three possible s_{t+1} (the arithmetic of the sequential
+//    rule), and a 6-step scan composes these maps, so that every frame's
+//    state follows from the state after the group;
+// 3. the frames whose on/off status changed are compacted with a ballot,
+//    and A of the other spots is updated there.
 
 __global__ __launch_bounds__(kSweepThreads) void b3_gibbs(GibbsParams G) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -355,42 +394,31 @@ __global__ __launch_bounds__(kSweepThreads) void b3_gibbs(GibbsParams G) {
     const int RC = Tc + kMtN;  // draw ring capacity
 
     double *alpha = (double *)lds;          // [Tc][3]
-    double *ex = alpha + 3 * Tc;            // [Tc][3]
-    int *chg = (int *)(ex + 3 * Tc);        // [Tc] frames whose on/off status changed
+    int *chg = (int *)(alpha + 3 * Tc);     // [Tc] frames whose on/off status changed
     uint32_t *mt = (uint32_t *)(chg + Tc);  // [kMtN]
     uint32_t *ring = mt + kMtN;             // [RC]
-    uint8_t *so = (uint8_t *)(ring + RC);   // [Tc] state before the update
-    uint8_t *sn = so + Tc;                  // [Tc] state after the update
-    __shared__ double Api[12];              // A[9], pi[3]
-    __shared__ int nchg;
+    const double *A = G.A, *pi = G.pi;
+    double *__restrict__ Aall = c.A;
+    const double *__restrict__ Oall = c.O;
 
-    if (lane < 9) Api[lane] = G.A[lane];
-    else if (lane < 12) Api[lane] = G.pi[lane - 9];
     for (int i = lane; i < kMtN; i += kSweepThreads) mt[i] = c.mt[i];
     const int pos0 = (int)c.mt[kMtN];
     __syncthreads();
     // Every thread tracks the ring's head and count; they change uniformly.
     int head = 0, cnt = kMtN - pos0;
     for (int i = lane; i < cnt; i += kSweepThreads) ring[i] = mt_temper(mt[pos0 + i]);
+    const double scale = 1.0 / (65536.0 * 65536.0);
+    const int tlast = ((T - 1) / kSweepThreads) * kSweepThreads;
 
     for (int s = 0; s < G.samples; s++) {
         const double i2v = 0.5 / G.var[s];
         for (int it = 0; it < G.iters; it++) {
             for (int k = 0; k < K; k++) {
                 const double *Ck = c.C + (size_t)k * T;
-                const double *Ak = c.A + (size_t)k * T;
+                const double *Ak = Aall + (size_t)k * T;
                 const double S2 = c.S2[k];
                 uint8_t *smp = c.sample + (size_t)k * T;
-                // 1. B_on - B_off of every frame
-                for (int t = lane; t < T; t += kSweepThreads) {
-                    so[t] = smp[t];
-                    const double d = S2 - 2.0 * (Ck[t] - Ak[t]);
-                    const double x = -d * i2v;
-                    ex[3 * t] = x;
-                    ex[3 * t + 1] = x >= 0.0 ? 1.0 : exp(x);
-                    ex[3 * t + 2] = x >= 0.0 ? exp(-x) : 1.0;
-                }
-                // 2. T draws, then the forward recursion and backward sampling
+                // T draws in the ring
                 while (cnt < T) {
                     mt_twist64(mt, lane);
                     for (int i = lane; i < kMtN; i += kSweepThreads) {
@@ -400,22 +428,110 @@ __global__ __launch_bounds__(kSweepThreads) void b3_gibbs(GibbsParams G) {
                     }
                     cnt += kMtN;
                 }
+                // 1. forward messages: per group of 64 frames, prefix products
+                //    of the transfer matrices applied to the message of the
+                //    frame before the group (the carry; any for the first
+                //    group, whose first matrix ignores it); the group is
+                //    redone one frame at a time where a product underflowed
+                double a0 = 1.0, a1 = 0.0, a2 = 0.0;
+                for (int t0 = 0; t0 < T; t0 += kSweepThreads) {
+                    const int t = t0 + lane;
+                    double x = 0.0, bon = 1.0, boff = 1.0;
+                    if (t < T) {
+                        const double d = S2 - 2.0 * (Ck[t] - Ak[t]);
+                        x = -d * i2v;
+                        bon = x >= 0.0 ? 1.0 : exp(x);
+                        boff = x >= 0.0 ? exp(-x) : 1.0;
+                    }
+                    double M[9];
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        const double bj = j == 0 ? bon : boff;
+#pragma unroll
+                        for (int i = 0; i < 3; i++)
+                            M[3 * j + i] = t >= T ? (i == j ? 1.0 : 0.0) : bj * (t == 0 ? pi[j] : A[3 * i + j]);
+                    }
+                    bool ok = prefix_products(M, lane);
+                    double w0 = M[0] * a0 + M[1] * a1 + M[2] * a2;
+                    double w1 = M[3] * a0 + M[4] * a1 + M[5] * a2;
+                    double w2 = M[6] * a0 + M[7] * a1 + M[8] * a2;
+                    const double sw = w0 + w1 + w2;
+                    ok = ok && sw > 1e-250;
+                    const int nt = min(kSweepThreads, T - t0);
+                    if (__all(ok || t >= T)) {
+                        const double is = __builtin_amdgcn_rcp(sw);
+                        w0 *= is; w1 *= is; w2 *= is;
+                    } else {
+                        for (int j = 0; j < nt; j++) {
+                            forward_step(A, pi, t0 + j == 0, read_lane(x, j), read_lane(bon, j), read_lane(boff, j),
+                                         a0, a1, a2);
+                            if (lane == j) {
+                                w0 = a0; w1 = a1; w2 = a2;
+                            }
+                        }
+                    }
+                    a0 = read_lane(w0, nt - 1);
+                    a1 = read_lane(w1, nt - 1);
+                    a2 = read_lane(w2, nt - 1);
+                    if (t < T) {
+                        alpha[3 * t] = w0;
+                        alpha[3 * t + 1] = w1;
+                        alpha[3 * t + 2] = w2;
+                    }
+                }
                 __syncthreads();
-                if (lane == 0) nchg = ffbs_lane(Api, T, ex, alpha, ring, head, RC, so, sn, chg);
+                // 2. backward sampling, last frame first; frame t takes draw
+                //    head + (T - 1 - t)
+                int sn = 0, nchg = 0;
+                for (int t0 = tlast; t0 >= 0; t0 -= kSweepThreads) {
+                    const int t = t0 + lane;
+                    int f = 0;  // the state drawn as a map of s_{t+1}
+                    if (t < T) {
+                        int r = head + (T - 1 - t);
+                        r = r >= RC ? r - RC : r;
+                        const double u = (double)ring[r] * scale;
+                        const double v0 = alpha[3 * t], v1 = alpha[3 * t + 1], v2 = alpha[3 * t + 2];
+                        if (t == T - 1) {
+                            f = select3(v0, v1, v2, u) * 21;
+                        } else {
+#pragma unroll
+                            for (int n = 0; n < 3; n++) f |= select3(v0 * A[n], v1 * A[3 + n], v2 * A[6 + n], u) << (2 * n);
+                        }
+                    }
+                    // suffix compositions f_t o f_{t+1} o ... over the group,
+                    // applied to the state of the frame after the group
+                    int G = t < T ? f : kIdentity3;
+#pragma unroll
+                    for (int d = 1; d < kSweepThreads; d <<= 1) {
+                        const int H = __shfl_down(G, d);
+                        if (lane + d < kSweepThreads) G = compose3(G, H);
+                    }
+                    const int sv = (G >> (2 * sn)) & 3;
+                    sn = __builtin_amdgcn_readlane(sv, 0);
+                    bool ch = false;
+                    int code = 0;
+                    if (t < T) {
+                        const int old = smp[t];
+                        smp[t] = (uint8_t)sv;
+                        ch = (old == 0) != (sv == 0);
+                        code = (t << 1) | (old == 0 ? 1 : 0);
+                    }
+                    const unsigned long long bal = __ballot(ch);
+                    if (ch) chg[nchg + __popcll(bal & ((1ull << lane) - 1))] = code;
+                    nchg += __popcll(bal);
+                }
                 head += T;
                 head = head >= RC ? head - RC : head;
                 cnt -= T;
                 __syncthreads();
-                // 3. the new sample, and A of the other spots in the frames
-                //    where this spot turned on (+O) or off (-O)
-                for (int t = lane; t < T; t += kSweepThreads) smp[t] = sn[t];
-                const int nch = nchg;
-                for (int e = lane; e < nch * K; e += kSweepThreads) {
+                // 3. A of the other spots in the frames where this spot
+                //    turned on (+O) or off (-O)
+                for (int e = lane; e < nchg * K; e += kSweepThreads) {
                     const int f = e / K, j = e - f * K;
                     if (j == k) continue;
                     const int code = chg[f];
-                    const double o = c.O[(size_t)j * K + k];
-                    double *a = c.A + (size_t)j * T + (code >> 1);
+                    const double o = Oall[(size_t)j * K + k];
+                    double *a = Aall + (size_t)j * T + (code >> 1);
                     *a = (code & 1) ? *a - o : *a + o;
                 }
                 __syncthreads();
@@ -533,9 +649,7 @@ struct HostMt {
 };
 
 // LDS of the sweep kernel for frames up to Tc
-size_t gibbs_lds(int Tc) {
-    return 8 * (6 * (size_t)Tc) + 4 * (size_t)Tc + 4 * ((size_t)kMtN + Tc + kMtN) + 2 * (size_t)Tc;
-}
+size_t gibbs_lds(int Tc) { return 8 * (3 * (size_t)Tc) + 4 * (size_t)Tc + 4 * ((size_t)kMtN + Tc + kMtN); }
 
 size_t stats_lds(int Tc) { return 8 * ((size_t)Tc + kMaskChunk) + 4 * (size_t)kMaskChunk; }
 

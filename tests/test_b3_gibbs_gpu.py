@@ -122,9 +122,10 @@ def test_gibbs_runs_continue_the_stream(gpu):
 
 
 def test_gibbs_long_series_uses_large_lds(gpu):
-    # T = 1500: per-chain tables of ~96 KB of LDS (more than 64 KB)
+    # T = 2500: 85 KB of LDS per chain (more than 64 KB), 40 frame groups of
+    # 64 with a partial last one
     m = model()
-    T = 1500
+    T = 2500
     data, params, _ = b3.synth_block((6, 6, 4), T, 3, 51, A=A_REF, pi=PI_REF)
     ch = b3.ChainData(data.reshape(T, -1), b3.box_pixels((6, 6, 4)), params, base=100.0, seed=21, skip=3, slots=2)
     g = b3.Gibbs([ch], m)
@@ -188,10 +189,11 @@ def test_gibbs_rejects_bad_masks_and_sizes(gpu):
             plan = ctypes.c_void_p()
             rc = b3.lib().rfba3b_gibbs_create(arr, 1, ctypes.byref(m), ctypes.byref(plan), None)
             assert rc == -1 and b"mask" in b3.lib().rfba3b_last_error(), bad
-        # T beyond the LDS budget
+        # T beyond the LDS budget (32 T + 4992 bytes > 160 KiB), rejected
+        # before anything is read
         arr = (b3._GibbsChain * 1)()
         a = arr[0]
-        a.P, a.T, a.K = P, 4000, 0
+        a.P, a.T, a.K = P, 6000, 0
         a.d_data, a.d_sample = dd.ptr, dsm.ptr
         plan = ctypes.c_void_p()
         assert b3.lib().rfba3b_gibbs_create(arr, 1, ctypes.byref(m), ctypes.byref(plan), None) == -1
