@@ -512,9 +512,9 @@ def bench_b3_gibbs(args):
         ch = gibbs_changes(prev[c], g.samples(c, S))
         by += float(S * Kc * 18.0 * T + 24.0 * Kc * ch.sum())
         ups += S * Kc
-        # statistics: data over each mask in every frame (4 T m), the mask and
-        # spot values (12 m), the overlaps (12 m per other spot)
-        setup_by += float((4.0 * T * msz + 12.0 * msz + 12.0 * msz * Kc).sum())
+        # statistics: the frames once (4 T P), the mask indices and spot values
+        # at them (12 m), C, O and S2 written (8 K T + 8 K^2 + 8 K)
+        setup_by += float(4.0 * T * mm["P"] + 12.0 * msz.sum() + 8.0 * (Kc * T + Kc * Kc + Kc))
     kb = float(np.mean(kms))
     wall = float(np.mean(walls))
     roof = hbm_roof(by, kb)
@@ -546,8 +546,9 @@ def bench_b3_gibbs(args):
         "speedup_batched_vs_per_block": (ups / wall) / (ups_sub / wall_pb),
         "setup": {"kernel_ms": setup_ms, "algorithmic_bytes": setup_by,
                   "GB_per_s": setup_by / (setup_ms * 1e-3) / 1e9 if setup_ms > 0 else None,
-                  "what": "per plan: S2, C, mask bitmaps and overlaps of every spot (b3_gibbs_stats, "
-                          "b3_gibbs_member, b3_gibbs_overlap)"},
+                  "what": "per plan: spot values at the mask pixels and S2 (b3_gibbs_mask_values), C per "
+                          "(pixel range, 8 frames) (b3_gibbs_cpart), mask overlaps per (spot, range) "
+                          "(b3_gibbs_opart), partial sums added per spot (b3_gibbs_reduce)"},
         "roofline": roof,
     }
     if not args.no_cpu:

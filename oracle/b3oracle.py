@@ -16,28 +16,30 @@ def lib():
     global _sig
     L = _o.lib()
     if not _sig:
-        from rfba_amd.b3 import Model
-        ip, dp = POINTER(c_int), POINTER(c_double)
-        mp = POINTER(Model)
-        L.oracle_b3_spot.argtypes = [ip, c_int, dp, c_double, c_int, dp, dp]
-        L.oracle_b3_emissions.argtypes = [dp, dp, c_int, c_int, dp, dp, c_double, dp, dp, dp]
-        L.oracle_b3_forward_deriv.argtypes = [dp, dp, dp, dp, dp, c_int, dp, dp]
-        L.oracle_b3_forward_delta.argtypes = [dp, dp, dp, dp, c_int, dp]
-        L.oracle_b3_forward_delta.restype = c_double
-        L.oracle_b3_backward_sampling.argtypes = [dp, dp, c_int, dp, ip]
-        L.oracle_b3_backward_sampling.restype = c_int
-        L.oracle_b3_diff_log_log_normal.argtypes = [c_double, c_double, c_double]
-        L.oracle_b3_diff_log_log_normal.restype = c_double
-        L.oracle_b3_logl_grad.argtypes = [ip, c_int, c_int, dp, dp, dp, mp, dp, dp]
-        L.oracle_b3_neg_grad.argtypes = [ip, c_int, c_int, dp, dp, c_int, dp, mp, dp, dp]
-        L.oracle_b3_model_eval.argtypes = [ip, c_int, dp, c_double, mp, c_int, dp, c_int, POINTER(c_uint8), c_int,
-                                           dp, dp, c_int]
-        L.oracle_b3_spot_pixels.argtypes = [ip, c_int, dp, c_int, ip, ip, ctypes.c_size_t]
-        L.oracle_b3_spot_pixels.restype = c_int
-        L.oracle_b3_gibbs_next.argtypes = [c_int, c_int, c_int, dp, dp, ip, ip, dp, dp, c_double, c_int,
-                                           POINTER(c_uint8), dp, dp, POINTER(ctypes.c_size_t)]
-        L.oracle_b3_gibbs_next.restype = c_int
-        _sig = True
+        with _o.LOCK:
+            if not _sig:
+                from rfba_amd.b3 import Model
+                ip, dp = POINTER(c_int), POINTER(c_double)
+                mp = POINTER(Model)
+                L.oracle_b3_spot.argtypes = [ip, c_int, dp, c_double, c_int, dp, dp]
+                L.oracle_b3_emissions.argtypes = [dp, dp, c_int, c_int, dp, dp, c_double, dp, dp, dp]
+                L.oracle_b3_forward_deriv.argtypes = [dp, dp, dp, dp, dp, c_int, dp, dp]
+                L.oracle_b3_forward_delta.argtypes = [dp, dp, dp, dp, c_int, dp]
+                L.oracle_b3_forward_delta.restype = c_double
+                L.oracle_b3_backward_sampling.argtypes = [dp, dp, c_int, dp, ip]
+                L.oracle_b3_backward_sampling.restype = c_int
+                L.oracle_b3_diff_log_log_normal.argtypes = [c_double, c_double, c_double]
+                L.oracle_b3_diff_log_log_normal.restype = c_double
+                L.oracle_b3_logl_grad.argtypes = [ip, c_int, c_int, dp, dp, dp, mp, dp, dp]
+                L.oracle_b3_neg_grad.argtypes = [ip, c_int, c_int, dp, dp, c_int, dp, mp, dp, dp]
+                L.oracle_b3_model_eval.argtypes = [ip, c_int, dp, c_double, mp, c_int, dp, c_int, POINTER(c_uint8), c_int,
+                                                   dp, dp, c_int]
+                L.oracle_b3_spot_pixels.argtypes = [ip, c_int, dp, c_int, ip, ip, ctypes.c_size_t]
+                L.oracle_b3_spot_pixels.restype = c_int
+                L.oracle_b3_gibbs_next.argtypes = [c_int, c_int, c_int, dp, dp, ip, ip, dp, dp, c_double, c_int,
+                                                   POINTER(c_uint8), dp, dp, POINTER(ctypes.c_size_t)]
+                L.oracle_b3_gibbs_next.restype = c_int
+                _sig = True
     return L
 
 

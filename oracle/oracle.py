@@ -8,41 +8,47 @@ from __future__ import annotations
 
 import ctypes
 import os
+import threading
 from ctypes import POINTER, c_double, c_float, c_int
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 _L = None
+# one CDLL per process, whose signatures are set once, also when several
+# threads ask for it first
+LOCK = threading.RLock()
 
 
 def lib():
     global _L
     if _L is None:
-        path = os.path.join(HERE, "liboracle.so")
-        if not os.path.exists(path):
-            raise ImportError("oracle not built: run `python -m rfba_amd.build oracle`")
-        L = ctypes.CDLL(path)
-        fp, dp, ip = POINTER(c_float), POINTER(c_double), POINTER(c_int)
-        from rfba_amd.srrf import SRRFConf
-        cp = POINTER(SRRFConf)
-        L.oracle_build_ring_f32.argtypes = [c_int, c_float, fp]
-        L.oracle_build_ring_f64.argtypes = [c_int, c_double, dp]
-        L.oracle_cubic_f32.argtypes = [c_float]
-        L.oracle_cubic_f32.restype = c_float
-        L.oracle_cubic_f64.argtypes = [c_double]
-        L.oracle_cubic_f64.restype = c_double
-        L.oracle_gradient_f32.argtypes = [fp, ip, cp, fp, fp, fp]
-        L.oracle_interp_f32.argtypes = [fp, ip, c_int, c_int, fp, c_int, fp]
-        L.oracle_interp_f64.argtypes = [dp, ip, c_int, c_int, dp, c_int, dp]
-        L.oracle_srrf_stack_f32.argtypes = [fp, ip, cp, fp, c_int, c_float, fp, fp, c_int]
-        L.oracle_srrf_stack_f64.argtypes = [dp, ip, cp, dp, c_int, c_double, dp, dp, c_int]
-        L.oracle_srrf_stack_box_f32.argtypes = [fp, ip, cp, fp, c_int, c_float, ip, fp, c_int]
-        L.oracle_temporal_f32.argtypes = [fp, fp, ip, cp, fp]
-        L.oracle_temporal_f64.argtypes = [dp, dp, ip, cp, dp]
-        L.oracle_srrf_tiled_f32.argtypes = [fp, ip, cp, fp, c_int]
-        L.oracle_hardware_threads.restype = c_int
-        _L = L
+        with LOCK:
+            if _L is None:
+                path = os.path.join(HERE, "liboracle.so")
+                if not os.path.exists(path):
+                    raise ImportError("oracle not built: run `python -m rfba_amd.build oracle`")
+                L = ctypes.CDLL(path)
+                fp, dp, ip = POINTER(c_float), POINTER(c_double), POINTER(c_int)
+                from rfba_amd.srrf import SRRFConf
+                cp = POINTER(SRRFConf)
+                L.oracle_build_ring_f32.argtypes = [c_int, c_float, fp]
+                L.oracle_build_ring_f64.argtypes = [c_int, c_double, dp]
+                L.oracle_cubic_f32.argtypes = [c_float]
+                L.oracle_cubic_f32.restype = c_float
+                L.oracle_cubic_f64.argtypes = [c_double]
+                L.oracle_cubic_f64.restype = c_double
+                L.oracle_gradient_f32.argtypes = [fp, ip, cp, fp, fp, fp]
+                L.oracle_interp_f32.argtypes = [fp, ip, c_int, c_int, fp, c_int, fp]
+                L.oracle_interp_f64.argtypes = [dp, ip, c_int, c_int, dp, c_int, dp]
+                L.oracle_srrf_stack_f32.argtypes = [fp, ip, cp, fp, c_int, c_float, fp, fp, c_int]
+                L.oracle_srrf_stack_f64.argtypes = [dp, ip, cp, dp, c_int, c_double, dp, dp, c_int]
+                L.oracle_srrf_stack_box_f32.argtypes = [fp, ip, cp, fp, c_int, c_float, ip, fp, c_int]
+                L.oracle_temporal_f32.argtypes = [fp, fp, ip, cp, fp]
+                L.oracle_temporal_f64.argtypes = [dp, dp, ip, cp, dp]
+                L.oracle_srrf_tiled_f32.argtypes = [fp, ip, cp, fp, c_int]
+                L.oracle_hardware_threads.restype = c_int
+                _L = L
     return _L
 
 

@@ -20,12 +20,14 @@ def lib():
     global _sig
     L = _o.lib()
     if not _sig:
-        fp, dp, ip = POINTER(c_float), POINTER(c_double), POINTER(c_int)
-        L.oracle_prep_kernel.argtypes = [c_double, c_double, fp, c_int, ip, dp]
-        L.oracle_prep_blur.argtypes = [fp, c_int, c_int, c_double, c_double, fp]
-        L.oracle_prep_normalize.argtypes = [fp, c_int, c_int, c_int, c_int, c_int, c_double, c_int, c_double,
-                                            c_double, fp, dp, c_int]
-        _sig = True
+        with _o.LOCK:
+            if not _sig:
+                fp, dp, ip = POINTER(c_float), POINTER(c_double), POINTER(c_int)
+                L.oracle_prep_kernel.argtypes = [c_double, c_double, fp, c_int, ip, dp]
+                L.oracle_prep_blur.argtypes = [fp, c_int, c_int, c_double, c_double, fp]
+                L.oracle_prep_normalize.argtypes = [fp, c_int, c_int, c_int, c_int, c_int, c_double, c_int, c_double,
+                                                    c_double, fp, dp, c_int]
+                _sig = True
     return L
 
 

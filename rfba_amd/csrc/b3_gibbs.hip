@@ -33,11 +33,11 @@ namespace rfba3b {
 
 constexpr int kSweepThreads = 64;   // one wave per chain
 constexpr int kStatThreads = 256;   // statistics kernels
-constexpr int kMaskChunk = 2048;    // mask entries staged in LDS at a time
+constexpr int kRange = 1024;        // pixels per range of the statistics kernels
 constexpr int kMtN = 624;           // MT19937 state words (mersenne.cpp MERS_N)
 
 struct GibbsDev {
-    int P, T, K, W;       // W: 32-bit words per spot in the mask bitmap
+    int P, T, K;
     const float *data;    // [T][P]
     const double *spot;   // [K][P]
     const int *moff;      // [K + 1]
@@ -48,9 +48,30 @@ struct GibbsDev {
     double *A;            // [K][T]
     double *O;            // [K][K]
     double *S2;           // [K]
-    uint32_t *bits;       // [K][W] mask membership
     double base;
     uint32_t *mt;         // [kMtN + 1]: state words, then the position of the next draw
+};
+
+// A chain as the statistics kernels see it (plan creation only).
+struct StatDev {
+    int P, T, K, ne;      // ne: list entries
+    const float *data;    // [T][P]
+    const double *spot;   // [K][P]
+    const int *moff;      // [K + 1]
+    const int *midx;      // mask pixel indices
+    double base;
+    const int *roff;      // [R + 1]: the list entries of range r
+    const int *lst;       // [ne][4]: (spot, range, first mask entry, end)
+    const int *koff;      // [K + 1]: the list entries of spot k in kent
+    const int *kent;      // [ne]: list entries of each spot, in range order
+    double *mu;           // [mask entries]: the spot's value at each mask pixel
+    double *Cp;           // [ne][T]
+    double *Op;           // [ne][K]
+    double *C, *O, *S2;
+};
+
+struct StatTask {
+    int chain, r, g;      // range, frame group
 };
 
 struct GibbsParams {
@@ -258,112 +279,196 @@ __device__ __forceinline__ double wave_sum8(const double (&a)[kFB], int lane) {
 }
 
 // ---------------------------------------------------------------------------
-// statistics of the plan: S2, C (b3_gibbs_stats), mask bitmaps
-// (b3_gibbs_member) and the overlaps O (b3_gibbs_overlap), one workgroup
-// per (chain, spot)
+// statistics of the plan.  The pixels of a chain are cut into ranges of
+// kRange; a list entry is a (spot, range) whose mask has pixels in the
+// range, with the segment of the (increasing) mask that falls there.
+// b3_gibbs_mask_values: the spot values at the mask pixels and S2;
+// b3_gibbs_cpart: per (range, group of kFB frames), each entry's sums;
+// b3_gibbs_opart: per entry, its dot products with the other entries of
+// the range; b3_gibbs_reduce: C and O per spot, the ranges added in order.
 
-// S2_k and C_t(k) for every frame: the mask's pixel indices and spot values
-// staged in LDS in chunks; each wave takes kFB consecutive frames at a time
-// (pixels across lanes), the chunks' sums added in order.
-__global__ __launch_bounds__(kStatThreads) void b3_gibbs_stats(const GibbsDev *chains, const SpotTask *tasks) {
-    extern __shared__ __align__(16) unsigned char lds[];
-    const SpotTask tk = tasks[blockIdx.x];
-    const GibbsDev c = chains[tk.chain];
-    const int T = c.T, P = c.P, k = tk.k;
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    double *dsum = (double *)lds;               // [T]
-    double *muL = dsum + T;                     // [kMaskChunk]
-    int *idxL = (int *)(muL + kMaskChunk);      // [kMaskChunk]
-    __shared__ double s2red[kStatThreads / 64];
-    const int m0 = c.moff[k], m = c.moff[k + 1] - m0;
-    const double *spk = c.spot + (size_t)k * P;
-    for (int t = tid; t < T; t += kStatThreads) dsum[t] = 0.0;
+// Workgroups are dispatched to the 8 XCDs round-robin (b mod 8).  Giving
+// each XCD a contiguous range of tasks keeps the tasks of one chain, which
+// read the same frames and masks, on one L2.  The grid is a multiple of 8;
+// returns the task index, or -1 for the padding.
+__device__ __forceinline__ int xcd_task(int ntasks) {
+    const int per = gridDim.x >> 3, b = blockIdx.x;
+    const int t = (b & 7) * per + (b >> 3);
+    return t < ntasks ? t : -1;
+}
+
+// mu[e] = spot[k][midx[e]] for the entries of mask k (task.k = k), and
+// S2_k = sum of their squares.
+__global__ __launch_bounds__(kStatThreads) void b3_gibbs_mask_values(const StatDev *chains, const SpotTask *tasks,
+                                                                    int ntasks) {
+    const int ti = xcd_task(ntasks);
+    if (ti < 0) return;
+    const SpotTask tk = tasks[ti];
+    const StatDev c = chains[tk.chain];
+    const int k = tk.k, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    __shared__ double red[kStatThreads / 64];
+    const double *spk = c.spot + (size_t)k * c.P;
     double s2 = 0.0;
-    const int nblk = (T + kFB - 1) / kFB;
-    for (int c0 = 0; c0 < m; c0 += kMaskChunk) {
-        const int nc = min(kMaskChunk, m - c0);
-        __syncthreads();
-        for (int i = tid; i < nc; i += kStatThreads) {
-            const int p = c.midx[m0 + c0 + i];
-            idxL[i] = p;
-            muL[i] = spk[p];
-            s2 += spk[p] * spk[p];
-        }
-        __syncthreads();
-        for (int b = wave; b < nblk; b += kStatThreads / 64) {
-            const int t0 = b * kFB;
-            const float *rt[kFB];
-            double acc[kFB];
-#pragma unroll
-            for (int f = 0; f < kFB; f++) {
-                rt[f] = c.data + (size_t)min(t0 + f, T - 1) * P;  // frames past T repeat the last, discarded
-                acc[f] = 0.0;
-            }
-            for (int i = lane; i < nc; i += 64) {
-                const int p = idxL[i];
-                const double mu = muL[i];
-#pragma unroll
-                for (int f = 0; f < kFB; f++) acc[f] += mu * ((double)rt[f][p] - c.base);
-            }
-            const double v = wave_sum8(acc, lane);
-            const int t = t0 + ((lane >> 3) & 7);
-            if ((lane & 7) == 0 && t < T) dsum[t] += v;
-        }
+    for (int e = c.moff[k] + tid; e < c.moff[k + 1]; e += kStatThreads) {
+        const double v = spk[c.midx[e]];
+        c.mu[e] = v;
+        s2 += v * v;
     }
     s2 = wave_sum(s2);
-    if (lane == 0) s2red[wave] = s2;
+    if (lane == 0) red[wave] = s2;
     __syncthreads();
-    for (int t = tid; t < T; t += kStatThreads) c.C[(size_t)k * T + t] = dsum[t];
     if (tid == 0) {
         double a = 0.0;
-        for (int w = 0; w < kStatThreads / 64; w++) a += s2red[w];
+        for (int w = 0; w < kStatThreads / 64; w++) a += red[w];
         c.S2[k] = a;
     }
 }
 
-// bits[k][p / 32] has bit p % 32 set for every pixel p of mask k.
-__global__ __launch_bounds__(kStatThreads) void b3_gibbs_member(const GibbsDev *chains, const SpotTask *tasks) {
-    const SpotTask tk = tasks[blockIdx.x];
-    const GibbsDev c = chains[tk.chain];
-    const int m0 = c.moff[tk.k], m1 = c.moff[tk.k + 1];
-    uint32_t *bk = c.bits + (size_t)tk.k * c.W;
-    for (int i = m0 + threadIdx.x; i < m1; i += kStatThreads) {
-        const int p = c.midx[i];
-        atomicOr(bk + (p >> 5), 1u << (p & 31));
+// Per (range r, frames t0..t0+kFB-1): the frames over the range are staged
+// in LDS with coalesced reads (each pixel value is read from HBM once);
+// each wave takes list entries of the range and sums mu (data - base) over
+// the entry's segment for every frame, four mask entries per lane in
+// flight, into Cp[entry][t].
+__global__ __launch_bounds__(kStatThreads) void b3_gibbs_cpart(const StatDev *chains, const StatTask *tasks,
+                                                              int ntasks) {
+    __shared__ float fr[kFB * kRange];
+    const int ti = xcd_task(ntasks);
+    if (ti < 0) return;
+    const StatTask tk = tasks[ti];
+    const StatDev c = chains[tk.chain];
+    const int T = c.T, P = c.P, r = tk.r;
+    const int t0 = tk.g * kFB, nf = min(kFB, T - t0);
+    const int p0 = r * kRange, np = min(kRange, P - p0);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    {
+        // every load of the group in flight at once; rows past T repeat the
+        // last frame and are discarded
+        constexpr int kU = kRange / kStatThreads;
+        float v[kU][kFB];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int i = tid + u * kStatThreads;
+#pragma unroll
+            for (int f = 0; f < kFB; f++) v[u][f] = i < np ? c.data[(size_t)(t0 + min(f, nf - 1)) * P + p0 + i] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++)
+#pragma unroll
+            for (int f = 0; f < kFB; f++) fr[f * kRange + tid + u * kStatThreads] = v[u][f];
+    }
+    __syncthreads();
+    for (int q = c.roff[r] + wave; q < c.roff[r + 1]; q += kStatThreads / 64) {
+        const int e0 = c.lst[4 * q + 2], e1 = c.lst[4 * q + 3];
+        double acc[kFB];
+#pragma unroll
+        for (int f = 0; f < kFB; f++) acc[f] = 0.0;
+        for (int e = e0 + lane; e < e1; e += 4 * 64) {
+            int x[4];
+            double mu[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int eu = e + 64 * u;
+                const bool in = eu < e1;
+                x[u] = in ? c.midx[eu] - p0 : 0;
+                mu[u] = in ? c.mu[eu] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const bool in = e + 64 * u < e1;
+#pragma unroll
+                for (int f = 0; f < kFB; f++) {
+                    const double term = mu[u] * ((double)fr[f * kRange + x[u]] - c.base);
+                    acc[f] += in ? term : 0.0;
+                }
+            }
+        }
+        const double v = wave_sum8(acc, lane);
+        const int f = (lane >> 3) & 7;
+        if ((lane & 7) == 0 && f < nf) c.Cp[(size_t)q * T + t0 + f] = v;
     }
 }
 
-// O_kj = sum over the pixels of mask k that are in mask j of mu_k mu_j, for
-// every j (O_kk = 0).  Each thread sums its pixels in order, then the
-// workgroup in a fixed tree.
-__global__ __launch_bounds__(kStatThreads) void b3_gibbs_overlap(const GibbsDev *chains, const SpotTask *tasks) {
-    const SpotTask tk = tasks[blockIdx.x];
-    const GibbsDev c = chains[tk.chain];
-    const int k = tk.k, K = c.K, P = c.P;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    __shared__ double red[kStatThreads / 64];
-    const int m0 = c.moff[k], m = c.moff[k + 1] - m0;
-    const double *spk = c.spot + (size_t)k * P;
-    for (int j = 0; j < K; j++) {
-        double a = 0.0;
-        if (j != k) {
-            const double *spj = c.spot + (size_t)j * P;
-            const uint32_t *bj = c.bits + (size_t)j * c.W;
-            for (int i = tid; i < m; i += kStatThreads) {
-                const int p = c.midx[m0 + i];
-                if ((bj[p >> 5] >> (p & 31)) & 1u) a += spk[p] * spj[p];
+// Per list entry q = (k, r): mu_k over its segment scattered into a dense
+// LDS row (0 elsewhere in the range); each wave takes the other entries
+// (j, r) of the range and sums mu_j times the row over their segments,
+// four mask entries per lane in flight.  Op[q][j] for every j (0 for the
+// spots that do not meet the range).
+__global__ __launch_bounds__(kStatThreads) void b3_gibbs_opart(const StatDev *chains, const SpotTask *tasks,
+                                                              int ntasks) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int ti = xcd_task(ntasks);
+    if (ti < 0) return;
+    const SpotTask tk = tasks[ti];
+    const StatDev c = chains[tk.chain];
+    const int q = tk.k, K = c.K;
+    const int k = c.lst[4 * q], r = c.lst[4 * q + 1], ke0 = c.lst[4 * q + 2], ke1 = c.lst[4 * q + 3];
+    const int p0 = r * kRange;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    double *row = (double *)lds;  // [kRange]
+    double *Ol = row + kRange;    // [K]
+    for (int i = tid; i < kRange; i += kStatThreads) row[i] = 0.0;
+    for (int j = tid; j < K; j += kStatThreads) Ol[j] = 0.0;
+    __syncthreads();
+    for (int e = ke0 + tid; e < ke1; e += kStatThreads) row[c.midx[e] - p0] = c.mu[e];
+    __syncthreads();
+    const int q0 = c.roff[r], nq = c.roff[r + 1] - q0;
+    for (int b = 0; b < nq; b += 64) {
+        // up to 64 entries of the range, one per lane, then taken in turn
+        int lj = 0, l0 = 0, l1 = 0;
+        if (b + lane < nq) {
+            const int4 ent = *(const int4 *)(c.lst + 4 * (q0 + b + lane));
+            lj = ent.x; l0 = ent.z; l1 = ent.w;
+        }
+        for (int qi = wave; qi < min(64, nq - b); qi += kStatThreads / 64) {
+            const int j = __builtin_amdgcn_readlane(lj, qi), e0 = __builtin_amdgcn_readlane(l0, qi),
+                      e1 = __builtin_amdgcn_readlane(l1, qi);
+            if (j == k) continue;
+            double a = 0.0;
+            for (int e = e0 + lane; e < e1; e += 4 * 64) {
+                int x[4];
+                double mu[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int eu = e + 64 * u;
+                    const bool in = eu < e1;
+                    x[u] = in ? c.midx[eu] - p0 : 0;
+                    mu[u] = in ? c.mu[eu] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const double term = mu[u] * row[x[u]];
+                    a += e + 64 * u < e1 ? term : 0.0;
+                }
             }
+            a = wave_sum(a);
+            if (lane == 0) Ol[j] = a;
         }
-        a = wave_sum(a);
-        __syncthreads();  // red free from the previous j
-        if (lane == 0) red[wave] = a;
-        __syncthreads();
-        if (tid == 0) {
-            double o = 0.0;
-            for (int w = 0; w < kStatThreads / 64; w++) o += red[w];
-            c.O[(size_t)k * K + j] = o;
-        }
+    }
+    __syncthreads();
+    for (int j = tid; j < K; j += kStatThreads) c.Op[(size_t)q * K + j] = Ol[j];
+}
+
+// C[k][t] and O[k][j] (task.k = k): the partial sums of spot k's list
+// entries added in range order.
+__global__ __launch_bounds__(kStatThreads) void b3_gibbs_reduce(const StatDev *chains, const SpotTask *tasks,
+                                                               int ntasks) {
+    const int ti = xcd_task(ntasks);
+    if (ti < 0) return;
+    const SpotTask tk = tasks[ti];
+    const StatDev c = chains[tk.chain];
+    const int k = tk.k, T = c.T, K = c.K;
+    const int q0 = c.koff[k], q1 = c.koff[k + 1];
+    for (int t = threadIdx.x; t < T; t += kStatThreads) {
+        double a = 0.0;
+        for (int q = q0; q < q1; q++) a += c.Cp[(size_t)c.kent[q] * T + t];
+        c.C[(size_t)k * T + t] = a;
+    }
+    for (int j = threadIdx.x; j < K; j += kStatThreads) {
+        double a = 0.0;
+        for (int q = q0; q < q1; q++) a += c.Op[(size_t)c.kent[q] * K + j];
+        c.O[(size_t)k * K + j] = a;
     }
 }
 
@@ -651,7 +756,8 @@ struct HostMt {
 // LDS of the sweep kernel for frames up to Tc
 size_t gibbs_lds(int Tc) { return 8 * (3 * (size_t)Tc) + 4 * (size_t)Tc + 4 * ((size_t)kMtN + Tc + kMtN); }
 
-size_t stats_lds(int Tc) { return 8 * ((size_t)Tc + kMaskChunk) + 4 * (size_t)kMaskChunk; }
+// LDS of the overlap kernel for up to K spots per chain
+size_t overlap_lds(int K) { return 8 * ((size_t)kRange + K); }
 
 }  // namespace
 
@@ -664,7 +770,6 @@ struct rfba3b_gibbs {
     uint32_t *d_mt = nullptr;
     int *d_ints = nullptr;       // every chain's mask offsets and indices
     double *d_dbl = nullptr;     // every chain's C, A, O, S2
-    uint32_t *d_bits = nullptr;  // every chain's mask bitmaps
     double *d_var = nullptr;
     int var_cap = 0;
     int Tmax = 0;
@@ -675,7 +780,6 @@ struct rfba3b_gibbs {
         if (d_mt) (void)hipFree(d_mt);
         if (d_ints) (void)hipFree(d_ints);
         if (d_dbl) (void)hipFree(d_dbl);
-        if (d_bits) (void)hipFree(d_bits);
         if (d_var) (void)hipFree(d_var);
     }
 };
@@ -690,7 +794,7 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
     if (int rc = host_check_model(model)) return rc;
     int ncu = 0;
     if (int rc = host_dev_init(&ncu)) return rc;
-    int Tmax = 1;
+    int Tmax = 1, Kmax = 0;
     for (int i = 0; i < n; i++) {
         const rfba3b_gibbs_chain &c = chains[i];
         if (c.P <= 0 || c.T <= 0 || c.K < 0) return host_fail(-1, "chain %d: P and T must be positive, K >= 0", i);
@@ -700,10 +804,13 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
             return host_fail(-1, "chain %d: spot table or masks are NULL", i);
         if (!std::isfinite(c.base)) return host_fail(-1, "chain %d: base is not finite", i);
         Tmax = std::max(Tmax, c.T);
+        Kmax = std::max(Kmax, c.K);
     }
-    const size_t lds = gibbs_lds(Tmax), slds = stats_lds(Tmax);
-    if (lds > 160 * 1024 || slds > 160 * 1024)
-        return host_fail(-1, "T = %d needs %zu bytes of LDS per chain (at most 160 KiB)", Tmax, std::max(lds, slds));
+    const size_t lds = gibbs_lds(Tmax), olds = overlap_lds(Kmax);
+    if (lds > 160 * 1024)
+        return host_fail(-1, "T = %d needs %zu bytes of LDS per chain (at most 160 KiB)", Tmax, lds);
+    if (olds > 160 * 1024)
+        return host_fail(-1, "K = %d needs %zu bytes of LDS in the overlaps (at most 160 KiB)", Kmax, olds);
 
     auto *g = new rfba3b_gibbs();
     std::unique_ptr<rfba3b_gibbs> own(g);
@@ -713,12 +820,15 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
     g->Tmax = Tmax;
     g->lds = lds;
 
-    // masks to the host for checking; offsets and indices kept by the plan
+    // masks to the host for checking; offsets and indices kept by the plan,
+    // with the statistics' range lists
     std::vector<int> ints;
-    std::vector<size_t> o_off(n), o_idx(n), o_dbl(n), o_bits(n);
+    std::vector<size_t> o_off(n), o_idx(n), o_roff(n), o_lst(n), o_koff(n), o_kent(n), o_dbl(n), o_scr(n);
+    std::vector<int> nent(n), nmask(n);
     std::vector<long long> work(n);
-    std::vector<SpotTask> tasks;
-    size_t ndbl = 0, nbits = 0;
+    std::vector<SpotTask> stasks, etasks;  // (chain, spot), (chain, list entry)
+    std::vector<StatTask> ctasks;          // (chain, range, frame group)
+    size_t ndbl = 0, nscr = 0;
     for (int i = 0; i < n; i++) {
         const rfba3b_gibbs_chain &c = chains[i];
         std::vector<int> off(c.K + 1, 0), mask;
@@ -740,18 +850,55 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
         ints.insert(ints.end(), off.begin(), off.end());
         o_idx[i] = ints.size();
         ints.insert(ints.end(), mask.begin(), mask.end());
+        // list entries (spot, range, segment) by range, then spot
+        const int R = (c.P + kRange - 1) / kRange;
+        std::vector<int> roff(R + 1, 0), lst, koff(c.K + 1, 0), kent;
+        std::vector<std::vector<int>> per_spot(c.K);
+        std::vector<int> cur(off.begin(), off.end() - 1);  // next mask entry of each spot
+        for (int r = 0; r < R; r++) {
+            roff[r] = (int)(lst.size() / 4);
+            const int pend = std::min((r + 1) * kRange, c.P);
+            for (int k = 0; k < c.K; k++) {
+                int e = cur[k];
+                while (e < off[k + 1] && mask[e] < pend) e++;
+                if (e > cur[k]) {
+                    per_spot[k].push_back((int)(lst.size() / 4));
+                    lst.insert(lst.end(), {k, r, cur[k], e});
+                }
+                cur[k] = e;
+            }
+        }
+        roff[R] = (int)(lst.size() / 4);
+        for (int k = 0; k < c.K; k++) {
+            koff[k] = (int)kent.size();
+            kent.insert(kent.end(), per_spot[k].begin(), per_spot[k].end());
+        }
+        koff[c.K] = (int)kent.size();
+        nent[i] = roff[R];
+        nmask[i] = off[c.K];
+        o_roff[i] = ints.size();
+        ints.insert(ints.end(), roff.begin(), roff.end());
+        while (ints.size() % 4) ints.push_back(0);  // entries read as int4
+        o_lst[i] = ints.size();
+        ints.insert(ints.end(), lst.begin(), lst.end());
+        o_koff[i] = ints.size();
+        ints.insert(ints.end(), koff.begin(), koff.end());
+        o_kent[i] = ints.size();
+        ints.insert(ints.end(), kent.begin(), kent.end());
         o_dbl[i] = ndbl;  // C [K][T], A [K][T], O [K][K], S2 [K], 256-byte aligned
         ndbl += ((2 * (size_t)c.K * c.T + (size_t)c.K * c.K + c.K) + 31) & ~(size_t)31;
-        o_bits[i] = nbits;
-        nbits += (((size_t)c.K * ((c.P + 31) / 32)) + 63) & ~(size_t)63;
-        for (int k = 0; k < c.K; k++) tasks.push_back({i, k});
+        o_scr[i] = nscr;  // mu [mask entries], Cp [ne][T], Op [ne][K]
+        nscr += ((size_t)off[c.K] + (size_t)nent[i] * (c.T + c.K) + 31) & ~(size_t)31;
+        for (int k = 0; k < c.K; k++) stasks.push_back({i, k});
+        for (int q = 0; q < nent[i]; q++) etasks.push_back({i, q});
+        for (int r = 0; r < R; r++)
+            if (roff[r + 1] > roff[r])
+                for (int b = 0; b < (c.T + kFB - 1) / kFB; b++) ctasks.push_back({i, r, b});
         work[i] = (long long)c.K * c.T;
     }
     GHIP(hipMalloc(&g->d_ints, std::max<size_t>(ints.size(), 1) * sizeof(int)));
     if (!ints.empty()) GHIP(hipMemcpy(g->d_ints, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice));
     GHIP(hipMalloc(&g->d_dbl, std::max<size_t>(ndbl, 1) * sizeof(double)));
-    GHIP(hipMalloc(&g->d_bits, std::max<size_t>(nbits, 1) * sizeof(uint32_t)));
-    GHIP(hipMemset(g->d_bits, 0, std::max<size_t>(nbits, 1) * sizeof(uint32_t)));
 
     std::vector<uint32_t> mts((size_t)n * (kMtN + 1));
     std::vector<GibbsDev> dev(n);
@@ -764,7 +911,7 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
         mts[(size_t)i * (kMtN + 1) + kMtN] = (uint32_t)h.pos;
         g->draws[i] = c.skip;
         GibbsDev &d = dev[i];
-        d.P = c.P; d.T = c.T; d.K = c.K; d.W = (c.P + 31) / 32;
+        d.P = c.P; d.T = c.T; d.K = c.K;
         d.data = c.d_data; d.spot = c.d_spot;
         d.moff = g->d_ints + o_off[i];
         d.midx = g->d_ints + o_idx[i];
@@ -774,7 +921,6 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
         d.A = b + (size_t)c.K * c.T;
         d.O = b + 2 * (size_t)c.K * c.T;
         d.S2 = d.O + (size_t)c.K * c.K;
-        d.bits = g->d_bits + o_bits[i];
         d.base = c.base;
         d.mt = g->d_mt + (size_t)i * (kMtN + 1);
     }
@@ -782,25 +928,67 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
     GHIP(hipMalloc(&g->d_chains, n * sizeof(GibbsDev)));
     GHIP(hipMemcpy(g->d_chains, dev.data(), n * sizeof(GibbsDev), hipMemcpyHostToDevice));
 
-    // statistics: bitmaps, then S2 and C, then O (the overlaps read the bitmaps)
-    if (!tasks.empty()) {
-        SpotTask *d_tasks = nullptr;
-        GHIP(hipMalloc(&d_tasks, tasks.size() * sizeof(SpotTask)));
-        hipError_t e = hipMemcpy(d_tasks, tasks.data(), tasks.size() * sizeof(SpotTask), hipMemcpyHostToDevice);
+    // statistics, in buffers that live until they are done
+    if (!stasks.empty()) {
+        struct DevFree {
+            void operator()(void *p) const { (void)hipFree(p); }
+        };
+        std::vector<std::unique_ptr<void, DevFree>> tmp;
+        auto dev_alloc = [&](size_t bytes) -> void * {
+            void *p = nullptr;
+            if (hipMalloc(&p, std::max<size_t>(bytes, 8)) != hipSuccess) return nullptr;
+            tmp.emplace_back(p);
+            return p;
+        };
+        double *scr = (double *)dev_alloc(nscr * sizeof(double));
+        StatDev *d_sdev = (StatDev *)dev_alloc(n * sizeof(StatDev));
+        SpotTask *d_st = (SpotTask *)dev_alloc(stasks.size() * sizeof(SpotTask));
+        SpotTask *d_et = (SpotTask *)dev_alloc(etasks.size() * sizeof(SpotTask));
+        StatTask *d_ct = (StatTask *)dev_alloc(ctasks.size() * sizeof(StatTask));
+        if (!scr || !d_sdev || !d_st || !d_et || !d_ct) return host_fail(-2, "out of device memory (statistics)");
+        std::vector<StatDev> sdev(n);
+        for (int i = 0; i < n; i++) {
+            const rfba3b_gibbs_chain &c = chains[i];
+            StatDev &s = sdev[i];
+            s.P = c.P; s.T = c.T; s.K = c.K; s.ne = nent[i];
+            s.data = c.d_data; s.spot = c.d_spot;
+            s.moff = dev[i].moff; s.midx = dev[i].midx;
+            s.base = c.base;
+            s.roff = g->d_ints + o_roff[i];
+            s.lst = g->d_ints + o_lst[i];
+            s.koff = g->d_ints + o_koff[i];
+            s.kent = g->d_ints + o_kent[i];
+            s.mu = scr + o_scr[i];
+            s.Cp = s.mu + nmask[i];
+            s.Op = s.Cp + (size_t)s.ne * c.T;
+            s.C = dev[i].C; s.O = dev[i].O; s.S2 = dev[i].S2;
+        }
+        GHIP(hipMemcpy(d_sdev, sdev.data(), n * sizeof(StatDev), hipMemcpyHostToDevice));
+        GHIP(hipMemcpy(d_st, stasks.data(), stasks.size() * sizeof(SpotTask), hipMemcpyHostToDevice));
+        if (!etasks.empty())
+            GHIP(hipMemcpy(d_et, etasks.data(), etasks.size() * sizeof(SpotTask), hipMemcpyHostToDevice));
+        if (!ctasks.empty())
+            GHIP(hipMemcpy(d_ct, ctasks.data(), ctasks.size() * sizeof(StatTask), hipMemcpyHostToDevice));
+        if (olds > 64 * 1024)
+            GHIP(hipFuncSetAttribute((const void *)b3_gibbs_opart, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)olds));
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (e == hipSuccess) e = hipEventCreate(&e0);
+        hipError_t e = hipEventCreate(&e0);
         if (e == hipSuccess) e = hipEventCreate(&e1);
-        if (e == hipSuccess && slds > 64 * 1024)
-            e = hipFuncSetAttribute((const void *)b3_gibbs_stats, hipFuncAttributeMaxDynamicSharedMemorySize, (int)slds);
         if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
         if (e == hipSuccess) {
-            const dim3 grid((unsigned)tasks.size());
-            hipLaunchKernelGGL(b3_gibbs_member, grid, dim3(kStatThreads), 0, nullptr, (const GibbsDev *)g->d_chains,
-                               (const SpotTask *)d_tasks);
-            hipLaunchKernelGGL(b3_gibbs_stats, grid, dim3(kStatThreads), slds, nullptr, (const GibbsDev *)g->d_chains,
-                               (const SpotTask *)d_tasks);
-            hipLaunchKernelGGL(b3_gibbs_overlap, grid, dim3(kStatThreads), 0, nullptr,
-                               (const GibbsDev *)g->d_chains, (const SpotTask *)d_tasks);
+            auto grid = [](size_t nt) { return dim3((unsigned)((nt + 7) / 8 * 8)); };
+            const int ns = (int)stasks.size(), ne = (int)etasks.size(), nc = (int)ctasks.size();
+            hipLaunchKernelGGL(b3_gibbs_mask_values, grid(ns), dim3(kStatThreads), 0, nullptr,
+                               (const StatDev *)d_sdev, (const SpotTask *)d_st, ns);
+            if (nc > 0)
+                hipLaunchKernelGGL(b3_gibbs_cpart, grid(nc), dim3(kStatThreads), 0, nullptr, (const StatDev *)d_sdev,
+                                   (const StatTask *)d_ct, nc);
+            if (ne > 0)
+                hipLaunchKernelGGL(b3_gibbs_opart, grid(ne), dim3(kStatThreads), olds, nullptr,
+                                   (const StatDev *)d_sdev, (const SpotTask *)d_et, ne);
+            hipLaunchKernelGGL(b3_gibbs_reduce, grid(ns), dim3(kStatThreads), 0, nullptr, (const StatDev *)d_sdev,
+                               (const SpotTask *)d_st, ns);
             e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
@@ -809,7 +997,6 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
         if (e == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
-        (void)hipFree(d_tasks);
         GHIP(e);
         if (setup_ms) *setup_ms = ms;
     }
