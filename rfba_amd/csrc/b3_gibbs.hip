@@ -177,74 +177,95 @@ __device__ __forceinline__ double read_lane(double v, int l) {
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
-// One step of the forward recursion, the same on every lane: the message of
-// a frame from the previous one's (a0, a1, a2) and the frame's x = B_on -
-// B_off, b_on = e^min(x,0), b_off = e^min(-x,0), normalised to sum 1 (any
-// positive per-frame factor leaves sample_unscaled_log's distribution
-// unchanged), with the step in log space when the emission ratio
-// underflowed.  alpha_t is forward_algorithm_delta2's delta_t (forward_
-// algorithm.h:344-382) less a per-frame constant.
-__device__ __forceinline__ void forward_step(const double *A, const double *pi, bool first, double x, double bon,
-                                             double boff, double &a0, double &a1, double &a2) {
-    double p0, p1, p2;
-    if (first) {
-        p0 = pi[0]; p1 = pi[1]; p2 = pi[2];
-    } else {
-        p0 = a0 * A[0] + a1 * A[3] + a2 * A[6];
-        p1 = a0 * A[1] + a1 * A[4] + a2 * A[7];
-        p2 = a0 * A[2] + a1 * A[5] + a2 * A[8];
+// Extended-range values: v = m 2^e with m in [0.5, 1) and an exponent of
+// its own, an integer held in a double (exact to 2^53; -inf for m = 0).
+// Forward messages and transfer-matrix products are kept this way so that
+// no weight underflows: a state whose weight is e^-10000 relative to
+// another keeps it, as in the reference's log-space forward algorithm
+// (forward_algorithm.h), and sums drop a term only when it is below
+// 2^-1074 of the largest, as logsumexp's exp() does.
+__device__ __forceinline__ void xsplit(double v, double e0, double &m, double &e) {
+    int k;
+    m = frexp(v, &k);
+    e = v > 0.0 ? e0 + (double)k : -__builtin_inf();
+}
+
+// 2^(e - E) m for e <= E (E the largest exponent; -inf - -inf gives 0 for a
+// zero m)
+__device__ __forceinline__ double xalign(double m, double e, double E) {
+    return ldexp(m, (int)fmin(fmax(e - E, -2000.0), 0.0));
+}
+
+// m0 2^e0 + m1 2^e1 + m2 2^e2, the terms aligned to the largest exponent
+__device__ __forceinline__ void xsum3(double m0, double e0, double m1, double e1, double m2, double e2, double &m,
+                                      double &e) {
+    const double E = fmax(e0, fmax(e1, e2));
+    double s = xalign(m0, e0, E);
+    s += xalign(m1, e1, E);
+    s += xalign(m2, e2, E);
+    xsplit(s, E, m, e);
+}
+
+// e^y for y <= 0: y = n ln2 + r with |r| <= ln2 / 2 (Cody and Waite's
+// reduction, ln 2 in two parts, each product with n subtracted by a fused
+// multiply-add), e^r by exp().
+__device__ __forceinline__ void xexp(double y, double &m, double &e) {
+    const double n = rint(y * 1.4426950408889634074);
+    const double r = fma(-n, 1.90821492927058770002e-10, fma(-n, 6.93147180369123816490e-01, y));
+    xsplit(exp(r), n, m, e);
+}
+
+// Subtract the largest exponent of the nonzero entries from all of them.
+template <int N>
+__device__ __forceinline__ void xshift(double (&e)[N]) {
+    double E = -__builtin_inf();
+#pragma unroll
+    for (int i = 0; i < N; i++) E = fmax(E, e[i]);
+    if (E > -__builtin_inf()) {
+#pragma unroll
+        for (int i = 0; i < N; i++) e[i] -= E;
     }
-    const double n0 = p0 * bon, n1 = p1 * boff, n2 = p2 * boff;
-    const double c = n0 + n1 + n2;
-    if (c > 1e-250) {
-        // any positive scale will do: the hardware reciprocal
-        const double ic = __builtin_amdgcn_rcp(c);
-        a0 = n0 * ic; a1 = n1 * ic; a2 = n2 * ic;
-    } else {
-        const double ninf = -__builtin_inf();
-        const double l0 = (p0 > 0 ? log(p0) : ninf) + fmin(x, 0.0);
-        const double l1 = (p1 > 0 ? log(p1) : ninf) + fmin(-x, 0.0);
-        const double l2 = (p2 > 0 ? log(p2) : ninf) + fmin(-x, 0.0);
-        const double mx = fmax(l0, fmax(l1, l2));
-        if (mx == ninf) {
-            a0 = a1 = a2 = 0.0;
-        } else {
-            const double e0 = exp(l0 - mx), e1 = exp(l1 - mx), e2 = exp(l2 - mx);
-            const double is = 1.0 / (e0 + e1 + e2);
-            a0 = e0 * is; a1 = e1 * is; a2 = e2 * is;
+}
+
+// Prefix products of the frames' 3 x 3 transfer matrices across the wave,
+// in extended range: on entry (Mm, Me) is lane l's matrix M_l (row: state
+// in its frame, column: state in the previous frame); on return it is
+// M_l M_{l-1} ... M_0, each partial product shifted so that its largest
+// entry has exponent 0..1.
+__device__ __forceinline__ void prefix_products(double (&Mm)[9], double (&Me)[9], int lane) {
+#pragma unroll
+    for (int d = 1; d < kSweepThreads; d <<= 1) {
+        double Qm[9], Qe[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            Qm[q] = __shfl_up(Mm[q], d);
+            Qe[q] = __shfl_up(Me[q], d);
+        }
+        if (lane >= d) {
+            double Rm[9], Re[9];
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+                    xsum3(Mm[3 * j] * Qm[i], Me[3 * j] + Qe[i], Mm[3 * j + 1] * Qm[3 + i], Me[3 * j + 1] + Qe[3 + i],
+                          Mm[3 * j + 2] * Qm[6 + i], Me[3 * j + 2] + Qe[6 + i], Rm[3 * j + i], Re[3 * j + i]);
+            xshift(Re);
+#pragma unroll
+            for (int q = 0; q < 9; q++) {
+                Mm[q] = Rm[q];
+                Me[q] = Re[q];
+            }
         }
     }
 }
 
-// Prefix products of the frames' 3 x 3 transfer matrices across the wave:
-// on entry S is lane l's matrix M_l (row: state in its frame, column: state
-// in the previous frame); on return S = M_l M_{l-1} ... M_0, every partial
-// product scaled to sum 1 (any positive factor per frame will do).  Returns
-// false where a partial product summed to 1e-250 or less, i.e. where
-// entries may have underflowed.
-__device__ __forceinline__ bool prefix_products(double (&S)[9], int lane) {
-    bool ok = true;
-#pragma unroll
-    for (int d = 1; d < kSweepThreads; d <<= 1) {
-        double Q[9];
-#pragma unroll
-        for (int e = 0; e < 9; e++) Q[e] = __shfl_up(S[e], d);
-        if (lane >= d) {
-            double R[9], sum = 0.0;
-#pragma unroll
-            for (int j = 0; j < 3; j++)
-#pragma unroll
-                for (int i = 0; i < 3; i++) R[3 * j + i] = S[3 * j] * Q[i] + S[3 * j + 1] * Q[3 + i] + S[3 * j + 2] * Q[6 + i];
-#pragma unroll
-            for (int e = 0; e < 9; e++) sum += R[e];
-            const bool big = sum > 1e-250;
-            ok = ok && big;
-            const double is = big ? __builtin_amdgcn_rcp(sum) : 1.0;
-#pragma unroll
-            for (int e = 0; e < 9; e++) S[e] = R[e] * is;
-        }
-    }
-    return ok;
+// select_random_element (forward_algorithm.h:502-513) over the extended
+// weights w_i = m_i 2^e_i, aligned to the largest as sample_unscaled_log
+// scales its weights by the largest (:520-532).
+__device__ __forceinline__ int xselect3(double m0, double e0, double m1, double e1, double m2, double e2, double u) {
+    const double E = fmax(m0 > 0.0 ? e0 : -__builtin_inf(), fmax(m1 > 0.0 ? e1 : -__builtin_inf(),
+                                                                  m2 > 0.0 ? e2 : -__builtin_inf()));
+    return select3(xalign(m0, e0, E), xalign(m1, e1, E), xalign(m2, e2, E), u);
 }
 
 // Maps of {0, 1, 2} into itself, 2 bits per argument (n -> bits 2n..2n+1);
@@ -476,12 +497,14 @@ __global__ __launch_bounds__(kStatThreads) void b3_gibbs_reduce(const StatDev *c
 // the sweeps: one 64-thread workgroup per chain
 //
 // Per spot update, frames are taken 64 at a time, one per lane:
-// 1. each lane forms its frame's x, b_on, b_off and transfer matrix
-//    M_t[j][i] = b_j A[i][j] (b_j pi_j in frame 0); a 6-step scan across the
+// 1. each lane forms its frame's x = B_on - B_off, b_on = e^min(x,0),
+//    b_off = e^min(-x,0) and transfer matrix M_t[j][i] = b_j A[i][j]
+//    (b_j pi_j in frame 0), in extended range; a 6-step scan across the
 //    wave forms the prefix products, which applied to the message of the
-//    frame before the group give every frame's forward message.  Where a
-//    partial product underflowed, the group is redone one frame at a time
-//    with forward_step (which has the log-space step);
+//    frame before the group give every frame's forward message (forward_
+//    algorithm_delta2's delta_t, forward_algorithm.h:344-382, up to a
+//    per-frame factor, which leaves sample_unscaled_log's distribution
+//    unchanged);
 // 2. backward sampling (forward_algorithm.h:542-566) draws the last frame
 //    from alpha_{T-1} and frame t from alpha_t(i) A[i][s_{t+1}] with draw
 //    u_t; each lane evaluates select_random_element for its frame under
@@ -498,8 +521,9 @@ __global__ __launch_bounds__(kSweepThreads) void b3_gibbs(GibbsParams G) {
     const int T = c.T, K = c.K, Tc = G.Tcap;
     const int RC = Tc + kMtN;  // draw ring capacity
 
-    double *alpha = (double *)lds;          // [Tc][3]
-    int *chg = (int *)(alpha + 3 * Tc);     // [Tc] frames whose on/off status changed
+    double *am = (double *)lds;             // [Tc][3] forward messages, extended range:
+    double *ae = am + 3 * Tc;               // [Tc][3]   am 2^ae
+    int *chg = (int *)(ae + 3 * Tc);        // [Tc] frames whose on/off status changed
     uint32_t *mt = (uint32_t *)(chg + Tc);  // [kMtN]
     uint32_t *ring = mt + kMtN;             // [RC]
     const double *A = G.A, *pi = G.pi;
@@ -535,54 +559,54 @@ __global__ __launch_bounds__(kSweepThreads) void b3_gibbs(GibbsParams G) {
                 }
                 // 1. forward messages: per group of 64 frames, prefix products
                 //    of the transfer matrices applied to the message of the
-                //    frame before the group (the carry; any for the first
-                //    group, whose first matrix ignores it); the group is
-                //    redone one frame at a time where a product underflowed
-                double a0 = 1.0, a1 = 0.0, a2 = 0.0;
+                //    frame before the group (the carry; (1, 0, 0) for the
+                //    first group, whose first matrix ignores it)
+                const double ninf = -__builtin_inf();
+                double c0m = 0.5, c1m = 0.0, c2m = 0.0;
+                double c0e = 1.0, c1e = ninf, c2e = ninf;
                 for (int t0 = 0; t0 < T; t0 += kSweepThreads) {
                     const int t = t0 + lane;
-                    double x = 0.0, bon = 1.0, boff = 1.0;
+                    double Mm[9], Me[9];
                     if (t < T) {
                         const double d = S2 - 2.0 * (Ck[t] - Ak[t]);
-                        x = -d * i2v;
-                        bon = x >= 0.0 ? 1.0 : exp(x);
-                        boff = x >= 0.0 ? exp(-x) : 1.0;
-                    }
-                    double M[9];
+                        const double x = -d * i2v;
+                        double bm[2], be[2];
+                        xexp(fmin(x, 0.0), bm[0], be[0]);   // b_on
+                        xexp(fmin(-x, 0.0), bm[1], be[1]);  // b_off
 #pragma unroll
-                    for (int j = 0; j < 3; j++) {
-                        const double bj = j == 0 ? bon : boff;
+                        for (int j = 0; j < 3; j++)
 #pragma unroll
-                        for (int i = 0; i < 3; i++)
-                            M[3 * j + i] = t >= T ? (i == j ? 1.0 : 0.0) : bj * (t == 0 ? pi[j] : A[3 * i + j]);
-                    }
-                    bool ok = prefix_products(M, lane);
-                    double w0 = M[0] * a0 + M[1] * a1 + M[2] * a2;
-                    double w1 = M[3] * a0 + M[4] * a1 + M[5] * a2;
-                    double w2 = M[6] * a0 + M[7] * a1 + M[8] * a2;
-                    const double sw = w0 + w1 + w2;
-                    ok = ok && sw > 1e-250;
-                    const int nt = min(kSweepThreads, T - t0);
-                    if (__all(ok || t >= T)) {
-                        const double is = __builtin_amdgcn_rcp(sw);
-                        w0 *= is; w1 *= is; w2 *= is;
+                            for (int i = 0; i < 3; i++)
+                                xsplit((t == 0 ? pi[j] : A[3 * i + j]) * bm[j == 0 ? 0 : 1], be[j == 0 ? 0 : 1],
+                                       Mm[3 * j + i], Me[3 * j + i]);
                     } else {
-                        for (int j = 0; j < nt; j++) {
-                            forward_step(A, pi, t0 + j == 0, read_lane(x, j), read_lane(bon, j), read_lane(boff, j),
-                                         a0, a1, a2);
-                            if (lane == j) {
-                                w0 = a0; w1 = a1; w2 = a2;
-                            }
+#pragma unroll
+                        for (int q = 0; q < 9; q++) {
+                            Mm[q] = q % 4 == 0 ? 0.5 : 0.0;
+                            Me[q] = q % 4 == 0 ? 1.0 : ninf;
                         }
                     }
-                    a0 = read_lane(w0, nt - 1);
-                    a1 = read_lane(w1, nt - 1);
-                    a2 = read_lane(w2, nt - 1);
+                    prefix_products(Mm, Me, lane);
+                    double wm[3], we[3];
+#pragma unroll
+                    for (int j = 0; j < 3; j++)
+                        xsum3(Mm[3 * j] * c0m, Me[3 * j] + c0e, Mm[3 * j + 1] * c1m, Me[3 * j + 1] + c1e,
+                              Mm[3 * j + 2] * c2m, Me[3 * j + 2] + c2e, wm[j], we[j]);
+                    xshift(we);
                     if (t < T) {
-                        alpha[3 * t] = w0;
-                        alpha[3 * t + 1] = w1;
-                        alpha[3 * t + 2] = w2;
+#pragma unroll
+                        for (int j = 0; j < 3; j++) {
+                            am[3 * t + j] = wm[j];
+                            ae[3 * t + j] = we[j];
+                        }
                     }
+                    const int last = min(kSweepThreads, T - t0) - 1;
+                    c0m = read_lane(wm[0], last);
+                    c1m = read_lane(wm[1], last);
+                    c2m = read_lane(wm[2], last);
+                    c0e = read_lane(we[0], last);
+                    c1e = read_lane(we[1], last);
+                    c2e = read_lane(we[2], last);
                 }
                 __syncthreads();
                 // 2. backward sampling, last frame first; frame t takes draw
@@ -595,12 +619,14 @@ __global__ __launch_bounds__(kSweepThreads) void b3_gibbs(GibbsParams G) {
                         int r = head + (T - 1 - t);
                         r = r >= RC ? r - RC : r;
                         const double u = (double)ring[r] * scale;
-                        const double v0 = alpha[3 * t], v1 = alpha[3 * t + 1], v2 = alpha[3 * t + 2];
+                        const double v0 = am[3 * t], v1 = am[3 * t + 1], v2 = am[3 * t + 2];
+                        const double e0 = ae[3 * t], e1 = ae[3 * t + 1], e2 = ae[3 * t + 2];
                         if (t == T - 1) {
-                            f = select3(v0, v1, v2, u) * 21;
+                            f = xselect3(v0, e0, v1, e1, v2, e2, u) * 21;
                         } else {
 #pragma unroll
-                            for (int n = 0; n < 3; n++) f |= select3(v0 * A[n], v1 * A[3 + n], v2 * A[6 + n], u) << (2 * n);
+                            for (int n = 0; n < 3; n++)
+                                f |= xselect3(v0 * A[n], e0, v1 * A[3 + n], e1, v2 * A[6 + n], e2, u) << (2 * n);
                         }
                     }
                     // suffix compositions f_t o f_{t+1} o ... over the group,
@@ -754,7 +780,7 @@ struct HostMt {
 };
 
 // LDS of the sweep kernel for frames up to Tc
-size_t gibbs_lds(int Tc) { return 8 * (3 * (size_t)Tc) + 4 * (size_t)Tc + 4 * ((size_t)kMtN + Tc + kMtN); }
+size_t gibbs_lds(int Tc) { return 16 * (3 * (size_t)Tc) + 4 * (size_t)Tc + 4 * ((size_t)kMtN + Tc + kMtN); }
 
 // LDS of the overlap kernel for up to K spots per chain
 size_t overlap_lds(int K) { return 8 * ((size_t)kRange + K); }

@@ -52,7 +52,7 @@ def chain_cases():
     return cases
 
 
-def oracle_chain(ch, m, steps, iters, variances):
+def oracle_chain(ch, m, steps, iters, variances, A=A_REF, pi=PI_REF):
     T = ch.data.shape[0]
     data = np.asarray(ch.data, np.float64).reshape(T, -1)
     P = data.shape[1]
@@ -65,8 +65,7 @@ def oracle_chain(ch, m, steps, iters, variances):
     intens = np.zeros((T, P))
     out, k_u = [], 0
     for s in range(steps):
-        k_u += bo.gibbs_next(data - ch.base, spot, off, idx, A_REF, PI_REF, variances[s], iters, sample, intens,
-                             u[k_u:])
+        k_u += bo.gibbs_next(data - ch.base, spot, off, idx, A, pi, variances[s], iters, sample, intens, u[k_u:])
         out.append(sample.copy())
     return np.array(out).reshape(steps, K, T), intens + ch.base, spot
 
@@ -96,6 +95,31 @@ def test_gibbs_matches_oracle_draw_for_draw(gpu):
         # every state occurs somewhere (the comparison is not vacuous)
         allst = np.concatenate([g.samples(i, steps).ravel() for i in range(len(cases))])
         assert {0, 1, 2} <= set(np.unique(allst).tolist())
+    finally:
+        g.close()
+
+
+def test_gibbs_weights_beyond_double_range_match_the_log_space_oracle(gpu):
+    # A model in which an off spot never turns on (A[1][0] = 0) and a tiny
+    # variance (|B_on - B_off| up to ~1e9), run on data drawn with the usual
+    # blink matrix: where the data turns a spot on after off frames, the
+    # path that stayed on (weight e^-1e9 relative) competes with paths that
+    # pay the same order of penalty elsewhere.  Messages in plain doubles
+    # lose such paths (the weight becomes 0); the reference's log-space
+    # forward algorithm and the device's extended-range messages keep them,
+    # so the draws must equal the oracle's.
+    A_Z = np.array([[0.9, 0.09, 0.01], [0.0, 0.98, 0.02], [0.0, 0.0, 1.0]])
+    m = b3.make_model(A_Z, PI_REF, 0.01, zfactor=2.0)
+    data, params, states = b3.synth_block((8, 8, 5), 150, 3, 31, A=A_REF, pi=PI_REF)
+    on = states == 0
+    assert np.any(~on[:, :-1] & on[:, 1:]), "no off -> on switch in the data"
+    ch = b3.ChainData(data.reshape(150, -1), b3.box_pixels((8, 8, 5)), params, base=100.0, seed=41, skip=2, slots=2)
+    g = b3.Gibbs([ch], m)
+    try:
+        g.run(2, 1)
+        smp, intens, _ = oracle_chain(ch, m, 2, 1, [0.01, 0.01], A=A_Z)
+        assert np.array_equal(g.samples(0, 2), smp)
+        assert np.allclose(g.sample_intensities(0), intens, rtol=0, atol=1e-9)
     finally:
         g.close()
 
