@@ -99,8 +99,9 @@ int ckpt_of(const rfba3b_model *m, int T) {
 
 size_t eval_lds(int Tmax, int XYmax, int Zmax, int ckptmax) {
     const int nseg = (Tmax + 0) / 1 + 1;  // bound: at most Tmax segments (ckpt >= 1)
+    // checkpoints (m, e) x 3 per segment; alpha_t(on) and c_t (m, e) per frame of a segment
     return 8 * 8 * (size_t)Tmax + 16 * (size_t)XYmax + 16 * (size_t)Zmax + 16 +
-           8 * (3 * (size_t)nseg + 4 * (size_t)ckptmax);
+           8 * (6 * (size_t)nseg + 4 * (size_t)ckptmax);
 }
 
 typedef void (*EvalFn)(EvalParams);
@@ -248,7 +249,7 @@ extern "C" int rfba3b_batch_create(const rfba3b_block *blocks, int n, const rfba
                                            (size_t)u.K * ((u.dims[2] + 7) & ~7) + 2 * (size_t)u.K);
         b->lds_resid = std::max(b->lds_resid, lr);
         const int ck = ckpt_of(model, u.T);
-        b->ckslots = std::max(b->ckslots, 3 * ((u.T + ck - 1) / ck) + 2 * ck);
+        b->ckslots = std::max(b->ckslots, 6 * ((u.T + ck - 1) / ck) + 4 * ck);
     }
     if (b->lds_resid > 64 * 1024) {
         batch_free(b);

@@ -177,56 +177,6 @@ __device__ __forceinline__ double read_lane(double v, int l) {
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
-// Extended-range values: v = m 2^e with m in [0.5, 1) and an exponent of
-// its own, an integer held in a double (exact to 2^53; -inf for m = 0).
-// Forward messages and transfer-matrix products are kept this way so that
-// no weight underflows: a state whose weight is e^-10000 relative to
-// another keeps it, as in the reference's log-space forward algorithm
-// (forward_algorithm.h), and sums drop a term only when it is below
-// 2^-1074 of the largest, as logsumexp's exp() does.
-__device__ __forceinline__ void xsplit(double v, double e0, double &m, double &e) {
-    int k;
-    m = frexp(v, &k);
-    e = v > 0.0 ? e0 + (double)k : -__builtin_inf();
-}
-
-// 2^(e - E) m for e <= E (E the largest exponent; -inf - -inf gives 0 for a
-// zero m)
-__device__ __forceinline__ double xalign(double m, double e, double E) {
-    return ldexp(m, (int)fmin(fmax(e - E, -2000.0), 0.0));
-}
-
-// m0 2^e0 + m1 2^e1 + m2 2^e2, the terms aligned to the largest exponent
-__device__ __forceinline__ void xsum3(double m0, double e0, double m1, double e1, double m2, double e2, double &m,
-                                      double &e) {
-    const double E = fmax(e0, fmax(e1, e2));
-    double s = xalign(m0, e0, E);
-    s += xalign(m1, e1, E);
-    s += xalign(m2, e2, E);
-    xsplit(s, E, m, e);
-}
-
-// e^y for y <= 0: y = n ln2 + r with |r| <= ln2 / 2 (Cody and Waite's
-// reduction, ln 2 in two parts, each product with n subtracted by a fused
-// multiply-add), e^r by exp().
-__device__ __forceinline__ void xexp(double y, double &m, double &e) {
-    const double n = rint(y * 1.4426950408889634074);
-    const double r = fma(-n, 1.90821492927058770002e-10, fma(-n, 6.93147180369123816490e-01, y));
-    xsplit(exp(r), n, m, e);
-}
-
-// Subtract the largest exponent of the nonzero entries from all of them.
-template <int N>
-__device__ __forceinline__ void xshift(double (&e)[N]) {
-    double E = -__builtin_inf();
-#pragma unroll
-    for (int i = 0; i < N; i++) E = fmax(E, e[i]);
-    if (E > -__builtin_inf()) {
-#pragma unroll
-        for (int i = 0; i < N; i++) e[i] -= E;
-    }
-}
-
 // Prefix products of the frames' 3 x 3 transfer matrices across the wave,
 // in extended range: on entry (Mm, Me) is lane l's matrix M_l (row: state
 // in its frame, column: state in the previous frame); on return it is
@@ -247,7 +197,7 @@ __device__ __forceinline__ void prefix_products(double (&Mm)[9], double (&Me)[9]
             for (int j = 0; j < 3; j++)
 #pragma unroll
                 for (int i = 0; i < 3; i++)
-                    xsum3(Mm[3 * j] * Qm[i], Me[3 * j] + Qe[i], Mm[3 * j + 1] * Qm[3 + i], Me[3 * j + 1] + Qe[3 + i],
+                    xsum3n(Mm[3 * j] * Qm[i], Me[3 * j] + Qe[i], Mm[3 * j + 1] * Qm[3 + i], Me[3 * j + 1] + Qe[3 + i],
                           Mm[3 * j + 2] * Qm[6 + i], Me[3 * j + 2] + Qe[6 + i], Rm[3 * j + i], Re[3 * j + i]);
             xshift(Re);
 #pragma unroll
@@ -590,7 +540,7 @@ __global__ __launch_bounds__(kSweepThreads) void b3_gibbs(GibbsParams G) {
                     double wm[3], we[3];
 #pragma unroll
                     for (int j = 0; j < 3; j++)
-                        xsum3(Mm[3 * j] * c0m, Me[3 * j] + c0e, Mm[3 * j + 1] * c1m, Me[3 * j + 1] + c1e,
+                        xsum3n(Mm[3 * j] * c0m, Me[3 * j] + c0e, Mm[3 * j + 1] * c1m, Me[3 * j + 1] + c1e,
                               Mm[3 * j + 2] * c2m, Me[3 * j + 2] + c2e, wm[j], we[j]);
                     xshift(we);
                     if (t < T) {

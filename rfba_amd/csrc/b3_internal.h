@@ -1,5 +1,6 @@
 // b3_internal.h -- shared host/device structures of librfba_3b.so.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include <mutex>
@@ -112,5 +113,70 @@ struct EvalParams {
     int Tmax, XYmax, Zmax;
     double *delta_out;      // MODE_DELTA: [item][T][3]
 };
+
+// ---------------------------------------------------------------------------
+// Extended-range values: v = m 2^e with m in [0.5, 1) and an exponent of
+// its own, an integer held in a double (exact to 2^53; -inf for m = 0).
+// Forward messages and transfer-matrix products are kept this way so that
+// no weight underflows: a state whose weight is e^-10000 relative to
+// another keeps it, as in the reference's log-space forward algorithm
+// (forward_algorithm.h), and sums drop a term only when it is below
+// 2^-1074 of the largest, as logsumexp's exp() does.
+__device__ __forceinline__ void xsplit(double v, double e0, double &m, double &e) {
+    int k;
+    m = frexp(v, &k);
+    e = v > 0.0 ? e0 + (double)k : -__builtin_inf();
+}
+
+// 2^(e - E) m for e <= E (E the largest exponent; -inf - -inf gives 0 for a
+// zero m)
+__device__ __forceinline__ double xalign(double m, double e, double E) {
+    return ldexp(m, (int)fmin(fmax(e - E, -2000.0), 0.0));
+}
+
+// m0 2^e0 + m1 2^e1 + m2 2^e2, the terms aligned to the largest exponent of
+// a nonzero term (a term m_i = 0 may carry any exponent, e.g. after a
+// product with a zero transition probability)
+__device__ __forceinline__ void xsum3(double m0, double e0, double m1, double e1, double m2, double e2, double &m,
+                                      double &e) {
+    const double ninf = -__builtin_inf();
+    const double E = fmax(m0 > 0.0 ? e0 : ninf, fmax(m1 > 0.0 ? e1 : ninf, m2 > 0.0 ? e2 : ninf));
+    double s = xalign(m0, e0, E);
+    s += xalign(m1, e1, E);
+    s += xalign(m2, e2, E);
+    xsplit(s, E, m, e);
+}
+
+// xsum3 for products of values that xsplit formed, where a zero term
+// always carries exponent -inf (no masking needed)
+__device__ __forceinline__ void xsum3n(double m0, double e0, double m1, double e1, double m2, double e2, double &m,
+                                       double &e) {
+    const double E = fmax(e0, fmax(e1, e2));
+    double s = xalign(m0, e0, E);
+    s += xalign(m1, e1, E);
+    s += xalign(m2, e2, E);
+    xsplit(s, E, m, e);
+}
+
+// e^y for y <= 0: y = n ln2 + r with |r| <= ln2 / 2 (Cody and Waite's
+// reduction, ln 2 in two parts, each product with n subtracted by a fused
+// multiply-add), e^r by exp().
+__device__ __forceinline__ void xexp(double y, double &m, double &e) {
+    const double n = rint(y * 1.4426950408889634074);
+    const double r = fma(-n, 1.90821492927058770002e-10, fma(-n, 6.93147180369123816490e-01, y));
+    xsplit(exp(r), n, m, e);
+}
+
+// Subtract the largest exponent of the nonzero entries from all of them.
+template <int N>
+__device__ __forceinline__ void xshift(double (&e)[N]) {
+    double E = -__builtin_inf();
+#pragma unroll
+    for (int i = 0; i < N; i++) E = fmax(E, e[i]);
+    if (E > -__builtin_inf()) {
+#pragma unroll
+        for (int i = 0; i < N; i++) e[i] -= E;
+    }
+}
 
 }  // namespace rfba3b

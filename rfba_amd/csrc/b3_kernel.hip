@@ -14,8 +14,9 @@
 // probabilities (spot_with_background.hh:127-147):
 //     B(off) = B(bleached) = -Soff/(2 var) - P ln(2 pi var)/2,
 //     B(on) = B(off) + D/(2 var),   d B(on)/d phi = diff / var.
-// One lane then runs the HMM in FP64: a scaled forward recursion
-// (forward_algorithm.h:192-237 in linear space, ln 0 -> 0 probability)
+// One lane then runs the HMM in FP64: a normalised forward recursion in
+// extended range, equivalent to the log-space one (forward_algorithm.h:
+// 192-237; ln 0 -> probability 0)
 // keeping the normalised messages only every `ckpt` frames, and the
 // adjoint: a backward recursion that recomputes each segment's forward
 // messages from its checkpoint and accumulates
@@ -132,55 +133,89 @@ __global__ __launch_bounds__(256) void b3_resid(const DevBlock *__restrict__ blo
 }
 
 // ---------------------------------------------------------------------------
-// HMM in FP64 on one lane.
+// HMM in FP64 on one lane, in extended range (a mantissa and an exponent
+// per value, xsplit): the reference runs the recursion in log space
+// (forward_algorithm.h:192-237), where a state whose weight is e^-10000
+// relative to another keeps it; normalised doubles would lose it.
 
 struct Hmm {
     double A[9], pi[3];
 };
 
-// emission ratios of frame t relative to the larger one: returns m and sets
-// b_on, b_off in (0, 1], one of them exactly 1.
-__device__ __forceinline__ void emis(double dlt, double &bon, double &boff, double &m) {
-    if (dlt >= 0.0) {
-        m = dlt;
-        bon = 1.0;
-        boff = exp(-dlt);
-    } else {
-        m = 0.0;
-        bon = exp(dlt);
-        boff = 1.0;
+// Emission ratios of a frame relative to the larger one: b_on =
+// e^min(dlt,0), b_off = e^min(-dlt,0); the larger log emission, max(dlt, 0),
+// is added to logL separately.
+struct Emis {
+    double onm, one, offm, offe;
+};
+
+__device__ __forceinline__ Emis xemis(double dlt) {
+    Emis b;
+    xexp(fmin(dlt, 0.0), b.onm, b.one);
+    xexp(fmin(-dlt, 0.0), b.offm, b.offe);
+    return b;
+}
+
+// The predicted message p = a A (pi in frame 0).
+__device__ __forceinline__ void xpred(const Hmm &h, const double am[3], const double ae[3], bool first, double pm[3],
+                                      double pe[3]) {
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+        if (first)
+            xsplit(h.pi[j], 0.0, pm[j], pe[j]);
+        else
+            xsum3(am[0] * h.A[j], ae[0], am[1] * h.A[3 + j], ae[1], am[2] * h.A[6 + j], ae[2], pm[j], pe[j]);
     }
 }
 
-// One normalised forward step: a <- normalise(b .* (a A)); returns log c.
-__device__ __forceinline__ double fstep(const Hmm &h, double a[3], double bon, double boff, bool first) {
-    double p0, p1, p2;
-    if (first) {
-        p0 = h.pi[0]; p1 = h.pi[1]; p2 = h.pi[2];
-    } else {
-        p0 = a[0] * h.A[0] + a[1] * h.A[3] + a[2] * h.A[6];
-        p1 = a[0] * h.A[1] + a[1] * h.A[4] + a[2] * h.A[7];
-        p2 = a[0] * h.A[2] + a[1] * h.A[5] + a[2] * h.A[8];
-    }
-    double n0 = p0 * bon, n1 = p1 * boff, n2 = p2 * boff;
-    double c = n0 + n1 + n2;
-    if (!(c > 1e-250)) {
-        // underflow guard: the same step in log space
-        const double l0 = log(p0) + log(bon), l1 = log(p1) + log(boff), l2 = log(p2) + log(boff);
-        double mx = fmax(l0, fmax(l1, l2));
-        if (mx == -__builtin_inf()) {
-            a[0] = a[1] = a[2] = 0.0;
-            return -__builtin_inf();
+// a <- (b .* p) / c with c = sum(b .* p), returned as (cm, ce); returns ln c
+// (-inf when every state has probability 0).
+__device__ __forceinline__ double xupdate(const double pm[3], const double pe[3], const Emis &b, double am[3],
+                                          double ae[3], double &cm, double &ce) {
+    double nm[3], ne[3];
+    xsplit(pm[0] * b.onm, pe[0] + b.one, nm[0], ne[0]);
+    xsplit(pm[1] * b.offm, pe[1] + b.offe, nm[1], ne[1]);
+    xsplit(pm[2] * b.offm, pe[2] + b.offe, nm[2], ne[2]);
+    xsum3(nm[0], ne[0], nm[1], ne[1], nm[2], ne[2], cm, ce);
+    if (!(cm > 0.0)) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            am[j] = 0.0;
+            ae[j] = -__builtin_inf();
         }
-        const double e0 = exp(l0 - mx), e1 = exp(l1 - mx), e2 = exp(l2 - mx);
-        const double s = e0 + e1 + e2;
-        a[0] = e0 / s; a[1] = e1 / s; a[2] = e2 / s;
-        return mx + log(s);
+        return -__builtin_inf();
     }
-    const double ic = 1.0 / c;
-    a[0] = n0 * ic; a[1] = n1 * ic; a[2] = n2 * ic;
-    return log(c);
+    const double ic = 1.0 / cm;
+#pragma unroll
+    for (int j = 0; j < 3; j++) xsplit(nm[j] * ic, ne[j] - ce, am[j], ae[j]);
+    return log(cm) + ce * M_LN2;
 }
+
+// One normalised forward step; returns ln c.
+__device__ __forceinline__ double xfstep(const Hmm &h, double am[3], double ae[3], const Emis &b, bool first,
+                                         double &cm, double &ce) {
+    double pm[3], pe[3];
+    xpred(h, am, ae, first, pm, pe);
+    return xupdate(pm, pe, b, am, ae, cm, ce);
+}
+
+// beta_t(i) = sum_j A_ij b_j(t+1) beta_{t+1}(j) / c_{t+1}, from the next
+// frame's emissions b and normaliser c.
+__device__ __forceinline__ void xbeta(const Hmm &h, double bm[3], double be[3], const Emis &b, double cm, double ce) {
+    const double wm0 = b.onm * bm[0], we0 = b.one + be[0];
+    const double wm1 = b.offm * bm[1], we1 = b.offe + be[1];
+    const double wm2 = b.offm * bm[2], we2 = b.offe + be[2];
+    const double ic = cm > 0.0 ? 1.0 / cm : 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        double nm, ne;
+        xsum3(h.A[3 * i] * wm0, we0, h.A[3 * i + 1] * wm1, we1, h.A[3 * i + 2] * wm2, we2, nm, ne);
+        xsplit(nm * ic, ne - ce, bm[i], be[i]);
+    }
+}
+
+// m 2^e as a double (a posterior weight, at most 1 up to rounding)
+__device__ __forceinline__ double xvalue(double m, double e) { return ldexp(m, (int)fmin(fmax(e, -2000.0), 64.0)); }
 
 // ---------------------------------------------------------------------------
 // kernel B: per-item emissions, logL, adjoint gradient
@@ -305,78 +340,71 @@ __global__ __launch_bounds__(256) void b3_eval(EvalParams E) {
             const double i2v = 0.5 / md.variance;
             const double cst = (double)b.npix * log(2.0 * M_PI * md.variance) / 2.0;
             const int ckpt = E.ckpt > 0 ? min(E.ckpt, T) : (int)ceil(sqrt((double)T));
-            double a[3] = {0, 0, 0};
+            const double ninf = -__builtin_inf();
+            double am[3] = {0.0, 0.0, 0.0}, ae[3] = {ninf, ninf, ninf};
             double L = 0.0;
             double *dout = (MODE == MODE_DELTA) ? E.delta_out + (size_t)it * 3 * T : nullptr;
             for (int t = 0; t < T; t++) {
                 const double e1 = -em[8 * t] * i2v - cst;
-                double bon, boff, m;
-                emis(em[8 * t + 1] * i2v, bon, boff, m);
+                const double dlt = em[8 * t + 1] * i2v;
                 if (t % ckpt == 0) {  // message entering frame t
-                    ck[3 * (t / ckpt)] = a[0];
-                    ck[3 * (t / ckpt) + 1] = a[1];
-                    ck[3 * (t / ckpt) + 2] = a[2];
-                }
-                if (MODE == MODE_DELTA) {
-                    // delta_t(j) = ln alpha_t(j) = ln pred_j + B(j, t) + logL_{t-1}
-                    // (forward_algorithm.h:344-382), evaluated in log space so
-                    // that states whose scaled weight underflows keep their
-                    // value; ln 0 = -1e100 as in the reference (:34-40).
-                    double pr[3];
-                    if (t == 0) {
-                        pr[0] = h.pi[0]; pr[1] = h.pi[1]; pr[2] = h.pi[2];
-                    } else {
-                        pr[0] = a[0] * h.A[0] + a[1] * h.A[3] + a[2] * h.A[6];
-                        pr[1] = a[0] * h.A[1] + a[1] * h.A[4] + a[2] * h.A[7];
-                        pr[2] = a[0] * h.A[2] + a[1] * h.A[5] + a[2] * h.A[8];
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        ck[6 * (t / ckpt) + j] = am[j];
+                        ck[6 * (t / ckpt) + 3 + j] = ae[j];
                     }
-                    const double Bon = e1 + em[8 * t + 1] * i2v;
-                    dout[3 * t] = (pr[0] > 0 ? log(pr[0]) : -1e100) + Bon + L;
-                    dout[3 * t + 1] = (pr[1] > 0 ? log(pr[1]) : -1e100) + e1 + L;
-                    dout[3 * t + 2] = (pr[2] > 0 ? log(pr[2]) : -1e100) + e1 + L;
                 }
-                L += fstep(h, a, bon, boff, t == 0) + e1 + m;
+                double pm[3], pe[3];
+                xpred(h, am, ae, t == 0, pm, pe);
+                if (MODE == MODE_DELTA) {
+                    // delta_t(j) = ln pred_j + B(j, t) + logL_{t-1}
+                    // (forward_algorithm.h:344-382); ln 0 = -1e100 as in the
+                    // reference (:34-40).
+                    const double Bon = e1 + dlt;
+                    dout[3 * t] = (pm[0] > 0 ? log(pm[0]) + pe[0] * M_LN2 : -1e100) + Bon + L;
+                    dout[3 * t + 1] = (pm[1] > 0 ? log(pm[1]) + pe[1] * M_LN2 : -1e100) + e1 + L;
+                    dout[3 * t + 2] = (pm[2] > 0 ? log(pm[2]) + pe[2] * M_LN2 : -1e100) + e1 + L;
+                }
+                double cm, ce;
+                L += xupdate(pm, pe, xemis(dlt), am, ae, cm, ce) + e1 + fmax(dlt, 0.0);
             }
             const size_t oi = ((size_t)item.q * K + item.k) * b.S + item.s;
             b.logL[oi] = L;
             if (MODE == MODE_GRAD) {
                 // adjoint: beta_t = A (b_{t+1} .* beta_{t+1}) / c_{t+1}; segments
                 // from the last, each re-running the forward steps from its
-                // checkpoint to recover alpha_t and c_t.
+                // checkpoint to recover alpha_t(on) and c_t.
                 double g[5] = {0, 0, 0, 0, 0};
-                double beta[3] = {1.0, 1.0, 1.0};
-                double *sa = ck + 3 * ((T + ckpt - 1) / ckpt);  // segment alphas [ckpt][3]
-                double *sc = sa + 3 * ckpt;                    // segment log c   [ckpt]
-                double c_next = 0.0;                           // log c_{t1}
-                double bon_n = 0, boff_n = 0;                  // b(t1)
+                double bm[3] = {0.5, 0.5, 0.5}, be[3] = {1.0, 1.0, 1.0};  // beta_{T-1} = 1
                 const int nseg = (T + ckpt - 1) / ckpt;
+                double *sa = ck + 6 * nseg;  // segment alpha_t(on) [ckpt][2] (m, e)
+                double *sc = sa + 2 * ckpt;  // segment c_t        [ckpt][2]
+                Emis bn = xemis(0.0);        // b(t+1)
+                double cnm = 1.0, cne = 0.0; // c_{t+1}
                 for (int sgi = nseg - 1; sgi >= 0; sgi--) {
                     const int t0 = sgi * ckpt, t1 = min(T, t0 + ckpt);
-                    double ar[3] = {ck[3 * sgi], ck[3 * sgi + 1], ck[3 * sgi + 2]};
+                    double arm[3], are[3];
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        arm[j] = ck[6 * sgi + j];
+                        are[j] = ck[6 * sgi + 3 + j];
+                    }
                     for (int t = t0; t < t1; t++) {
-                        double bon, boff, m;
-                        emis(em[8 * t + 1] * i2v, bon, boff, m);
-                        sc[t - t0] = fstep(h, ar, bon, boff, t == 0);
-                        sa[3 * (t - t0)] = ar[0];
-                        sa[3 * (t - t0) + 1] = ar[1];
-                        sa[3 * (t - t0) + 2] = ar[2];
+                        double cm, ce;
+                        xfstep(h, arm, are, xemis(em[8 * t + 1] * i2v), t == 0, cm, ce);
+                        sa[2 * (t - t0)] = arm[0];
+                        sa[2 * (t - t0) + 1] = are[0];
+                        sc[2 * (t - t0)] = cm;
+                        sc[2 * (t - t0) + 1] = ce;
                     }
                     for (int t = t1 - 1; t >= t0; t--) {
-                        if (t < T - 1) {
-                            // beta_t(i) = sum_j A_ij b_j(t+1) beta_{t+1}(j) / c_{t+1}
-                            const double ic = exp(-c_next);
-                            const double w0 = bon_n * beta[0], w1 = boff_n * beta[1], w2 = boff_n * beta[2];
-                            const double n0 = (h.A[0] * w0 + h.A[1] * w1 + h.A[2] * w2) * ic;
-                            const double n1 = (h.A[3] * w0 + h.A[4] * w1 + h.A[5] * w2) * ic;
-                            const double n2 = (h.A[6] * w0 + h.A[7] * w1 + h.A[8] * w2) * ic;
-                            beta[0] = n0; beta[1] = n1; beta[2] = n2;
-                        }
-                        const double gam = sa[3 * (t - t0)] * beta[0];
+                        if (t < T - 1) xbeta(h, bm, be, bn, cnm, cne);
+                        const double gam = xvalue(sa[2 * (t - t0)] * bm[0], sa[2 * (t - t0) + 1] + be[0]);
                         const double *mm = em + 8 * t;
                         for (int n = 0; n < 5; n++) g[n] += gam * mm[2 + n];
-                        double m;
-                        emis(mm[1] * i2v, bon_n, boff_n, m);
-                        c_next = sc[t - t0];  // log c_t (emission shift m excluded)
+                        bn = xemis(mm[1] * i2v);
+                        cnm = sc[2 * (t - t0)];
+                        cne = sc[2 * (t - t0) + 1];
                     }
                 }
                 const double iv = 1.0 / md.variance;
@@ -676,52 +704,56 @@ __global__ __launch_bounds__(256) void b3_hmm(HmmParams H) {
     const size_t NI = (size_t)H.NI;
     double *ck = H.ck + li;  // slot u at ck[u * NI]
     auto emr = [&](int t) { return H.em + ((size_t)t * NI + li) * 8; };
-    double a[3] = {0, 0, 0};
+    const double ninf = -__builtin_inf();
+    double am[3] = {0.0, 0.0, 0.0}, ae[3] = {ninf, ninf, ninf};
     double L = 0.0;
     for (int t = 0; t < T; t++) {
         const double *m8 = emr(t);
         const double e1 = -m8[0] * i2v - cst;
-        double bon, boff, m;
-        emis(m8[1] * i2v, bon, boff, m);
+        const double dlt = m8[1] * i2v;
         if (t % ckpt == 0) {
-            const int u = 3 * (t / ckpt);
-            ck[u * NI] = a[0];
-            ck[(u + 1) * NI] = a[1];
-            ck[(u + 2) * NI] = a[2];
+            const int u = 6 * (t / ckpt);
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                ck[(u + j) * NI] = am[j];
+                ck[(u + 3 + j) * NI] = ae[j];
+            }
         }
-        L += fstep(h, a, bon, boff, t == 0) + e1 + m;
+        double cm, ce;
+        L += xfstep(h, am, ae, xemis(dlt), t == 0, cm, ce) + e1 + fmax(dlt, 0.0);
     }
     const size_t oi = ((size_t)it.q * b.K + it.k) * b.S + it.s;
     b.logL[oi] = L;
     if (!GRAD) return;
     double g[5] = {0, 0, 0, 0, 0};
-    double beta[3] = {1.0, 1.0, 1.0};
-    const int sa0 = 3 * nseg, sc0 = sa0 + ckpt;  // slots: checkpoints | alpha_t(on) | log c_t
-    double c_next = 0.0, bon_n = 0.0, boff_n = 0.0;
+    double bm[3] = {0.5, 0.5, 0.5}, be[3] = {1.0, 1.0, 1.0};  // beta_{T-1} = 1
+    const int sa0 = 6 * nseg, sc0 = sa0 + 2 * ckpt;  // slots: checkpoints | alpha_t(on) | c_t, (m, e) each
+    Emis bn = xemis(0.0);
+    double cnm = 1.0, cne = 0.0;
     for (int sgi = nseg - 1; sgi >= 0; sgi--) {
         const int t0 = sgi * ckpt, t1 = min(T, t0 + ckpt);
-        double ar[3] = {ck[(3 * sgi) * NI], ck[(3 * sgi + 1) * NI], ck[(3 * sgi + 2) * NI]};
+        double arm[3], are[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            arm[j] = ck[(6 * sgi + j) * NI];
+            are[j] = ck[(6 * sgi + 3 + j) * NI];
+        }
         for (int t = t0; t < t1; t++) {
-            double bon, boff, m;
-            emis(emr(t)[1] * i2v, bon, boff, m);
-            ck[(sc0 + t - t0) * NI] = fstep(h, ar, bon, boff, t == 0);
-            ck[(sa0 + t - t0) * NI] = ar[0];
+            double cm, ce;
+            xfstep(h, arm, are, xemis(emr(t)[1] * i2v), t == 0, cm, ce);
+            ck[(sa0 + 2 * (t - t0)) * NI] = arm[0];
+            ck[(sa0 + 2 * (t - t0) + 1) * NI] = are[0];
+            ck[(sc0 + 2 * (t - t0)) * NI] = cm;
+            ck[(sc0 + 2 * (t - t0) + 1) * NI] = ce;
         }
         for (int t = t1 - 1; t >= t0; t--) {
-            if (t < T - 1) {
-                const double ic = exp(-c_next);
-                const double w0 = bon_n * beta[0], w1 = boff_n * beta[1], w2 = boff_n * beta[2];
-                const double n0 = (h.A[0] * w0 + h.A[1] * w1 + h.A[2] * w2) * ic;
-                const double n1 = (h.A[3] * w0 + h.A[4] * w1 + h.A[5] * w2) * ic;
-                const double n2 = (h.A[6] * w0 + h.A[7] * w1 + h.A[8] * w2) * ic;
-                beta[0] = n0; beta[1] = n1; beta[2] = n2;
-            }
+            if (t < T - 1) xbeta(h, bm, be, bn, cnm, cne);
             const double *m8 = emr(t);
-            const double gam = ck[(sa0 + t - t0) * NI] * beta[0];
+            const double gam = xvalue(ck[(sa0 + 2 * (t - t0)) * NI] * bm[0], ck[(sa0 + 2 * (t - t0) + 1) * NI] + be[0]);
             for (int n = 0; n < 5; n++) g[n] += gam * m8[2 + n];
-            double m;
-            emis(m8[1] * i2v, bon_n, boff_n, m);
-            c_next = ck[(sc0 + t - t0) * NI];
+            bn = xemis(m8[1] * i2v);
+            cnm = ck[(sc0 + 2 * (t - t0)) * NI];
+            cne = ck[(sc0 + 2 * (t - t0) + 1) * NI];
         }
     }
     const double iv = 1.0 / md.variance;

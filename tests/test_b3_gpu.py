@@ -159,6 +159,40 @@ def test_functor_and_forward_messages(gpu):
     assert L == pytest.approx(Lo, rel=LOGL_RTOL)
 
 
+def test_sharp_likelihood_with_zero_transitions(gpu):
+    # A model in which an off spot never turns on (A[1][0] = 0) with a
+    # variance of 0.01, on data drawn with the default blink matrix: the
+    # per-frame log emission ratios reach ~1e6 and state weights leave the
+    # double range.  The reference's log-space recursion keeps them (a path
+    # that stayed on through off frames competes where the data turns the
+    # spot on again); normalised doubles would lose them.
+    A_Z = np.array([[0.9, 0.09, 0.01], [0.0, 0.98, 0.02], [0.0, 0.0, 1.0]])
+    m = model(A=A_Z, variance=0.01)
+    data, params, states = b3.synth_block((10, 9, 8), 60, 4, 61)
+    on = states == 0
+    assert np.any(~on[:, :-1] & on[:, 1:]), "no off -> on switch in the data"
+    P = b3.perturb(params, 3, 62)
+    st = np.stack([states, np.roll(states, 5, axis=1)])
+    (L, G), = run_batch([b3.BlockData(data, P, st, 100.0)], m)
+    Lo, Go = bo.model_eval(data, 100.0, m, P, st)
+    compare(L, G, Lo, Go, "sharp, zero transitions")
+    # forward messages of one spot (FFBS input), finite where the oracle's are
+    xyz = grid(10, 9, 8)
+    d2 = data.reshape(60, -1)
+    bg = np.full(d2.shape, 100.0)
+    for j in range(1, 4):
+        bg += (states[j] == 0)[:, None] * bo.spot(xyz, params[j], 2.0)[0][None]
+    delta, Ld = b3.forward_messages(xyz, d2.astype(np.float32), bg.astype(np.float32), params[0], m)
+    mu, dmu = bo.spot(xyz, params[0], 2.0)
+    on_l, off_l, _ = bo.emissions(d2.astype(np.float32).astype(np.float64), bg.astype(np.float32).astype(np.float64),
+                                  mu, dmu, 0.01)
+    do, Lo1 = bo.forward_delta(A_Z, (0.5, 0.5, 0.0), on_l, off_l)
+    fin = do > -1e99
+    assert np.array_equal(fin, delta > -1e99)
+    np.testing.assert_allclose(delta[fin], do[fin], rtol=LOGL_RTOL)
+    assert Ld == pytest.approx(Lo1, rel=LOGL_RTOL)
+
+
 def test_config4_shape_subset(gpu):
     # 29^3 block, T = 100, 40 spots; two of the perturbed parameter sets
     m = model(checkpoint=0)
