@@ -130,12 +130,17 @@ def bench_stack(args, rec):
     K = smp.n
     alg_flops = float(T) * n_sr * (K * 430 + 128 * conf.IntensityW + 6)
     tflops = alg_flops / (kernel_ms * 1e-3) / 1e12
-    roof = hbm_roof(alg_bytes, kernel_ms)
     traffic = args.traffic if args.traffic is not None else (HEADLINE_TRAFFIC_BYTES if T == 200 else None)
-    roof.update({"traffic": traffic, "kernel": "srrf_fused (uint16 input)", "kernel_ms": kernel_ms,
-                 "algorithmic_bytes_per_launch": alg_bytes,
-                 "valu_fp32": {"achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                               "frac": tflops / FP32_PEAK_TFLOPS, "algorithmic_flops_per_launch": alg_flops}})
+    # The kernel is bound by FP32 arithmetic on the vector units (it issues no
+    # matrix instructions); MI355X's FP32 peak is the same 157.3 TFLOP/s for
+    # the vector units and the dense matrix cores, so that is the roof.  The
+    # HBM figure is reported beside it.
+    roof = {"bound": "mfma", "achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": tflops / FP32_PEAK_TFLOPS, "traffic": traffic, "kernel": "srrf_fused (uint16 input)",
+            "kernel_ms": kernel_ms, "algorithmic_flops_per_launch": alg_flops,
+            "note": "FP32 VALU-bound (no matrix instructions; the FP32 peak is the same on the vector units and "
+                    "the matrix cores); algorithmic flops count the interpolation shared by a voxel's samples once",
+            "hbm": dict(hbm_roof(alg_bytes, kernel_ms), algorithmic_bytes_per_launch=alg_bytes)}
     rec.update({
         "value": n_sr / (ms_per_step * 1e-3),
         "ms_per_step": ms_per_step,
