@@ -177,11 +177,32 @@ __device__ __forceinline__ double read_lane(double v, int l) {
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
-// Prefix products of the frames' 3 x 3 transfer matrices across the wave,
-// in extended range: on entry (Mm, Me) is lane l's matrix M_l (row: state
-// in its frame, column: state in the previous frame); on return it is
-// M_l M_{l-1} ... M_0, each partial product shifted so that its largest
+// R = S Q for 3 x 3 matrices in extended range, shifted so that the largest
 // entry has exponent 0..1.
+__device__ __forceinline__ void xmatmul(const double (&Sm)[9], const double (&Se)[9], const double (&Qm)[9],
+                                        const double (&Qe)[9], double (&Rm)[9], double (&Re)[9]) {
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+            xsum3n(Sm[3 * j] * Qm[i], Se[3 * j] + Qe[i], Sm[3 * j + 1] * Qm[3 + i], Se[3 * j + 1] + Qe[3 + i],
+                   Sm[3 * j + 2] * Qm[6 + i], Se[3 * j + 2] + Qe[6 + i], Rm[3 * j + i], Re[3 * j + i]);
+    xshift(Re);
+}
+
+// w = M v in extended range, shifted so that its largest entry has
+// exponent 0..1.
+__device__ __forceinline__ void xmatvec(const double (&Mm)[9], const double (&Me)[9], const double (&vm)[3],
+                                        const double (&ve)[3], double (&wm)[3], double (&we)[3]) {
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+        xsum3n(Mm[3 * j] * vm[0], Me[3 * j] + ve[0], Mm[3 * j + 1] * vm[1], Me[3 * j + 1] + ve[1],
+               Mm[3 * j + 2] * vm[2], Me[3 * j + 2] + ve[2], wm[j], we[j]);
+    xshift(we);
+}
+
+// Prefix products across the wave, in extended range: on entry (Mm, Me) is
+// lane l's matrix L_l; on return it is L_l L_{l-1} ... L_0.
 __device__ __forceinline__ void prefix_products(double (&Mm)[9], double (&Me)[9], int lane) {
 #pragma unroll
     for (int d = 1; d < kSweepThreads; d <<= 1) {
@@ -193,18 +214,38 @@ __device__ __forceinline__ void prefix_products(double (&Mm)[9], double (&Me)[9]
         }
         if (lane >= d) {
             double Rm[9], Re[9];
-#pragma unroll
-            for (int j = 0; j < 3; j++)
-#pragma unroll
-                for (int i = 0; i < 3; i++)
-                    xsum3n(Mm[3 * j] * Qm[i], Me[3 * j] + Qe[i], Mm[3 * j + 1] * Qm[3 + i], Me[3 * j + 1] + Qe[3 + i],
-                          Mm[3 * j + 2] * Qm[6 + i], Me[3 * j + 2] + Qe[6 + i], Rm[3 * j + i], Re[3 * j + i]);
-            xshift(Re);
+            xmatmul(Mm, Me, Qm, Qe, Rm, Re);
 #pragma unroll
             for (int q = 0; q < 9; q++) {
                 Mm[q] = Rm[q];
                 Me[q] = Re[q];
             }
+        }
+    }
+}
+
+// The transfer matrix of frame t in extended range: M_t[j][i] = b_j A[i][j]
+// (b_j pi_j in frame 0), b_on = e^min(x,0), b_off = e^min(-x,0) with
+// x = B_on - B_off = -(S2 - 2 (C_t - A_t)) / (2 var); the identity past T.
+__device__ __forceinline__ void frame_matrix(int t, int T, double S2, const double *Ck, const double *Ak, double i2v,
+                                             const double *A, const double *pi, double (&Mm)[9], double (&Me)[9]) {
+    if (t < T) {
+        const double d = S2 - 2.0 * (Ck[t] - Ak[t]);
+        const double x = -d * i2v;
+        double bm[2], be[2];
+        xexp(fmin(x, 0.0), bm[0], be[0]);   // b_on
+        xexp(fmin(-x, 0.0), bm[1], be[1]);  // b_off
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                xsplit((t == 0 ? pi[j] : A[3 * i + j]) * bm[j == 0 ? 0 : 1], be[j == 0 ? 0 : 1], Mm[3 * j + i],
+                       Me[3 * j + i]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            Mm[q] = q % 4 == 0 ? 0.5 : 0.0;
+            Me[q] = q % 4 == 0 ? 1.0 : -__builtin_inf();
         }
     }
 }
@@ -446,16 +487,19 @@ __global__ __launch_bounds__(kStatThreads) void b3_gibbs_reduce(const StatDev *c
 // ---------------------------------------------------------------------------
 // the sweeps: one 64-thread workgroup per chain
 //
-// Per spot update, frames are taken 64 at a time, one per lane:
-// 1. each lane forms its frame's x = B_on - B_off, b_on = e^min(x,0),
-//    b_off = e^min(-x,0) and transfer matrix M_t[j][i] = b_j A[i][j]
-//    (b_j pi_j in frame 0), in extended range; a 6-step scan across the
-//    wave forms the prefix products, which applied to the message of the
-//    frame before the group give every frame's forward message (forward_
-//    algorithm_delta2's delta_t, forward_algorithm.h:344-382, up to a
-//    per-frame factor, which leaves sample_unscaled_log's distribution
+// Per spot update:
+// 1. forward messages, frames taken 128 at a time, two per lane: each lane
+//    forms its frames' x = B_on - B_off, b_on = e^min(x,0), b_off =
+//    e^min(-x,0) and transfer matrices M_t[j][i] = b_j A[i][j] (b_j pi_j in
+//    frame 0), in extended range, and their product; a 6-step scan across
+//    the wave forms the prefix products, and the message of the frame
+//    before the group carried through the products of the lanes before and
+//    the lane's own two matrices gives its frames' forward messages
+//    (forward_algorithm_delta2's delta_t, forward_algorithm.h:344-382, up to
+//    a per-frame factor, which leaves sample_unscaled_log's distribution
 //    unchanged);
-// 2. backward sampling (forward_algorithm.h:542-566) draws the last frame
+// 2. backward sampling, frames taken 64 at a time, one per lane
+//    (forward_algorithm.h:542-566): the last frame is drawn
 //    from alpha_{T-1} and frame t from alpha_t(i) A[i][s_{t+1}] with draw
 //    u_t; each lane evaluates select_random_element for its frame under
 //    each of the three possible s_{t+1} (the arithmetic of the sequential
@@ -507,56 +551,64 @@ __global__ __launch_bounds__(kSweepThreads) void b3_gibbs(GibbsParams G) {
                     }
                     cnt += kMtN;
                 }
-                // 1. forward messages: per group of 64 frames, prefix products
-                //    of the transfer matrices applied to the message of the
-                //    frame before the group (the carry; (1, 0, 0) for the
-                //    first group, whose first matrix ignores it)
+                // 1. forward messages: per group of 128 frames, two per lane:
+                //    the lane's product of its two transfer matrices, prefix
+                //    products of those across the wave, and the message of
+                //    the frame before the group (the carry; (1, 0, 0) for the
+                //    first group, whose first matrix ignores it) carried
+                //    through the lane's two frames
                 const double ninf = -__builtin_inf();
-                double c0m = 0.5, c1m = 0.0, c2m = 0.0;
-                double c0e = 1.0, c1e = ninf, c2e = ninf;
-                for (int t0 = 0; t0 < T; t0 += kSweepThreads) {
-                    const int t = t0 + lane;
-                    double Mm[9], Me[9];
-                    if (t < T) {
-                        const double d = S2 - 2.0 * (Ck[t] - Ak[t]);
-                        const double x = -d * i2v;
-                        double bm[2], be[2];
-                        xexp(fmin(x, 0.0), bm[0], be[0]);   // b_on
-                        xexp(fmin(-x, 0.0), bm[1], be[1]);  // b_off
-#pragma unroll
-                        for (int j = 0; j < 3; j++)
-#pragma unroll
-                            for (int i = 0; i < 3; i++)
-                                xsplit((t == 0 ? pi[j] : A[3 * i + j]) * bm[j == 0 ? 0 : 1], be[j == 0 ? 0 : 1],
-                                       Mm[3 * j + i], Me[3 * j + i]);
-                    } else {
-#pragma unroll
-                        for (int q = 0; q < 9; q++) {
-                            Mm[q] = q % 4 == 0 ? 0.5 : 0.0;
-                            Me[q] = q % 4 == 0 ? 1.0 : ninf;
-                        }
+                double cm[3] = {0.5, 0.0, 0.0}, ce[3] = {1.0, ninf, ninf};
+                for (int t0 = 0; t0 < T; t0 += 2 * kSweepThreads) {
+                    const int ta = t0 + 2 * lane, tb = ta + 1;
+                    double Pm[9], Pe[9];
+                    {
+                        double Mam[9], Mae[9], Mbm[9], Mbe[9];
+                        frame_matrix(ta, T, S2, Ck, Ak, i2v, A, pi, Mam, Mae);
+                        frame_matrix(tb, T, S2, Ck, Ak, i2v, A, pi, Mbm, Mbe);
+                        xmatmul(Mbm, Mbe, Mam, Mae, Pm, Pe);
                     }
-                    prefix_products(Mm, Me, lane);
-                    double wm[3], we[3];
+                    prefix_products(Pm, Pe, lane);
+                    // the product over the lanes before this one (identity for lane 0)
+                    double Em[9], Ee[9];
 #pragma unroll
-                    for (int j = 0; j < 3; j++)
-                        xsum3n(Mm[3 * j] * c0m, Me[3 * j] + c0e, Mm[3 * j + 1] * c1m, Me[3 * j + 1] + c1e,
-                              Mm[3 * j + 2] * c2m, Me[3 * j + 2] + c2e, wm[j], we[j]);
-                    xshift(we);
-                    if (t < T) {
+                    for (int q = 0; q < 9; q++) {
+                        const double m = __shfl_up(Pm[q], 1), e = __shfl_up(Pe[q], 1);
+                        Em[q] = lane == 0 ? (q % 4 == 0 ? 0.5 : 0.0) : m;
+                        Ee[q] = lane == 0 ? (q % 4 == 0 ? 1.0 : ninf) : e;
+                    }
+                    double vm[3], ve[3], wam[3], wae[3], wbm[3], wbe[3];
+                    xmatvec(Em, Ee, cm, ce, vm, ve);
+                    {
+                        double Mm[9], Me[9];
+                        frame_matrix(ta, T, S2, Ck, Ak, i2v, A, pi, Mm, Me);
+                        xmatvec(Mm, Me, vm, ve, wam, wae);
+                        frame_matrix(tb, T, S2, Ck, Ak, i2v, A, pi, Mm, Me);
+                        xmatvec(Mm, Me, wam, wae, wbm, wbe);
+                    }
+                    if (ta < T) {
 #pragma unroll
                         for (int j = 0; j < 3; j++) {
-                            am[3 * t + j] = wm[j];
-                            ae[3 * t + j] = we[j];
+                            am[3 * ta + j] = wam[j];
+                            ae[3 * ta + j] = wae[j];
                         }
                     }
-                    const int last = min(kSweepThreads, T - t0) - 1;
-                    c0m = read_lane(wm[0], last);
-                    c1m = read_lane(wm[1], last);
-                    c2m = read_lane(wm[2], last);
-                    c0e = read_lane(we[0], last);
-                    c1e = read_lane(we[1], last);
-                    c2e = read_lane(we[2], last);
+                    if (tb < T) {
+#pragma unroll
+                        for (int j = 0; j < 3; j++) {
+                            am[3 * tb + j] = wbm[j];
+                            ae[3 * tb + j] = wbe[j];
+                        }
+                    }
+                    // carry: the message of the group's last frame
+                    const int tl = min(T, t0 + 2 * kSweepThreads) - 1 - t0;
+                    const int ll = tl >> 1;
+                    const bool odd = tl & 1;
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        cm[j] = odd ? read_lane(wbm[j], ll) : read_lane(wam[j], ll);
+                        ce[j] = odd ? read_lane(wbe[j], ll) : read_lane(wae[j], ll);
+                    }
                 }
                 __syncthreads();
                 // 2. backward sampling, last frame first; frame t takes draw
