@@ -135,6 +135,41 @@ def test_forward_vs_brute_force(T):
                 assert delta[t, j] == pytest.approx(ref, rel=1e-12, abs=1e-10)
 
 
+@pytest.mark.parametrize("T", [4, 7])
+def test_forward_vs_brute_force_beyond_double_range(T):
+    # Log emissions of order 1e6 and a transition matrix in which off never
+    # turns on: state weights of e^-1e6 relative must be kept (log space),
+    # since the path that stayed on through off frames is the only way to be
+    # on again.  The oracle is pinned here for the regime in which the
+    # device's extended-range recursion is compared with it.
+    rng = np.random.default_rng(100 + T)
+    A = np.array([[0.9, 0.09, 0.01], [0.0, 0.98, 0.02], [0.0, 0.0, 1.0]])
+    pi = np.array([0.5, 0.5, 0.0])
+    sign = np.where(np.arange(T) % 3 == 1, -1.0, 1.0)  # on, off, on, on, off, ...
+    on = -1e6 * rng.random(T) * (sign < 0)
+    off = -1e6 * rng.random(T) * (sign > 0)
+    ref = brute_force(A, pi, on, off)
+    L, _ = bo.forward_deriv(A, pi, on, off, np.zeros((T, 5)))
+    assert np.isfinite(ref) and L == pytest.approx(ref, rel=1e-13)
+    delta, L2 = bo.forward_delta(A, pi, on, off)
+    assert L2 == pytest.approx(ref, rel=1e-13)
+    # the last frame's messages: every state's log weight, however small
+    for j in range(3):
+        refj = -np.inf
+        for path in itertools.product(range(3), repeat=T):
+            if path[-1] != j or pi[path[0]] == 0:
+                continue
+            if any(A[path[u - 1], path[u]] == 0 for u in range(1, T)):
+                continue
+            lp = np.log(pi[path[0]]) + sum(np.log(A[path[u - 1], path[u]]) for u in range(1, T))
+            lp += sum(on[u] if st == 0 else off[u] for u, st in enumerate(path))
+            refj = np.logaddexp(refj, lp)
+        if refj == -np.inf:
+            assert delta[T - 1, j] < -1e99
+        else:
+            assert delta[T - 1, j] == pytest.approx(refj, rel=1e-12)
+
+
 def test_forward_derivative_vs_fd():
     rng = np.random.default_rng(3)
     T = 9
