@@ -33,6 +33,8 @@ namespace rfba3b {
 
 constexpr int kSweepThreads = 64;   // one wave per chain
 constexpr int kStatThreads = 256;   // statistics kernels
+constexpr int kCpartThreads = 512;  // b3_gibbs_cpart: more waves per staged range (measured best)
+constexpr int kOpartThreads = 256;  // b3_gibbs_opart (measured best)
 constexpr int kRange = 1024;        // pixels per range of the statistics kernels
 constexpr int kMtN = 624;           // MT19937 state words (mersenne.cpp MERS_N)
 
@@ -341,7 +343,7 @@ __global__ __launch_bounds__(kStatThreads) void b3_gibbs_mask_values(const StatD
 // each wave takes list entries of the range and sums mu (data - base) over
 // the entry's segment for every frame, four mask entries per lane in
 // flight, into Cp[entry][t].
-__global__ __launch_bounds__(kStatThreads) void b3_gibbs_cpart(const StatDev *chains, const StatTask *tasks,
+__global__ __launch_bounds__(kCpartThreads) void b3_gibbs_cpart(const StatDev *chains, const StatTask *tasks,
                                                               int ntasks) {
     __shared__ float fr[kFB * kRange];
     const int ti = xcd_task(ntasks);
@@ -356,21 +358,21 @@ __global__ __launch_bounds__(kStatThreads) void b3_gibbs_cpart(const StatDev *ch
     {
         // every load of the group in flight at once; rows past T repeat the
         // last frame and are discarded
-        constexpr int kU = kRange / kStatThreads;
+        constexpr int kU = kRange / kCpartThreads;
         float v[kU][kFB];
 #pragma unroll
         for (int u = 0; u < kU; u++) {
-            const int i = tid + u * kStatThreads;
+            const int i = tid + u * kCpartThreads;
 #pragma unroll
             for (int f = 0; f < kFB; f++) v[u][f] = i < np ? c.data[(size_t)(t0 + min(f, nf - 1)) * P + p0 + i] : 0.0f;
         }
 #pragma unroll
         for (int u = 0; u < kU; u++)
 #pragma unroll
-            for (int f = 0; f < kFB; f++) fr[f * kRange + tid + u * kStatThreads] = v[u][f];
+            for (int f = 0; f < kFB; f++) fr[f * kRange + tid + u * kCpartThreads] = v[u][f];
     }
     __syncthreads();
-    for (int q = c.roff[r] + wave; q < c.roff[r + 1]; q += kStatThreads / 64) {
+    for (int q = c.roff[r] + wave; q < c.roff[r + 1]; q += kCpartThreads / 64) {
         const int e0 = c.lst[4 * q + 2], e1 = c.lst[4 * q + 3];
         double acc[kFB];
 #pragma unroll
@@ -406,7 +408,7 @@ __global__ __launch_bounds__(kStatThreads) void b3_gibbs_cpart(const StatDev *ch
 // (j, r) of the range and sums mu_j times the row over their segments,
 // four mask entries per lane in flight.  Op[q][j] for every j (0 for the
 // spots that do not meet the range).
-__global__ __launch_bounds__(kStatThreads) void b3_gibbs_opart(const StatDev *chains, const SpotTask *tasks,
+__global__ __launch_bounds__(kOpartThreads) void b3_gibbs_opart(const StatDev *chains, const SpotTask *tasks,
                                                               int ntasks) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int ti = xcd_task(ntasks);
@@ -420,10 +422,10 @@ __global__ __launch_bounds__(kStatThreads) void b3_gibbs_opart(const StatDev *ch
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     double *row = (double *)lds;  // [kRange]
     double *Ol = row + kRange;    // [K]
-    for (int i = tid; i < kRange; i += kStatThreads) row[i] = 0.0;
-    for (int j = tid; j < K; j += kStatThreads) Ol[j] = 0.0;
+    for (int i = tid; i < kRange; i += kOpartThreads) row[i] = 0.0;
+    for (int j = tid; j < K; j += kOpartThreads) Ol[j] = 0.0;
     __syncthreads();
-    for (int e = ke0 + tid; e < ke1; e += kStatThreads) row[c.midx[e] - p0] = c.mu[e];
+    for (int e = ke0 + tid; e < ke1; e += kOpartThreads) row[c.midx[e] - p0] = c.mu[e];
     __syncthreads();
     const int q0 = c.roff[r], nq = c.roff[r + 1] - q0;
     for (int b = 0; b < nq; b += 64) {
@@ -433,7 +435,7 @@ __global__ __launch_bounds__(kStatThreads) void b3_gibbs_opart(const StatDev *ch
             const int4 ent = *(const int4 *)(c.lst + 4 * (q0 + b + lane));
             lj = ent.x; l0 = ent.z; l1 = ent.w;
         }
-        for (int qi = wave; qi < min(64, nq - b); qi += kStatThreads / 64) {
+        for (int qi = wave; qi < min(64, nq - b); qi += kOpartThreads / 64) {
             const int j = __builtin_amdgcn_readlane(lj, qi), e0 = __builtin_amdgcn_readlane(l0, qi),
                       e1 = __builtin_amdgcn_readlane(l1, qi);
             if (j == k) continue;
@@ -459,7 +461,7 @@ __global__ __launch_bounds__(kStatThreads) void b3_gibbs_opart(const StatDev *ch
         }
     }
     __syncthreads();
-    for (int j = tid; j < K; j += kStatThreads) c.Op[(size_t)q * K + j] = Ol[j];
+    for (int j = tid; j < K; j += kOpartThreads) c.Op[(size_t)q * K + j] = Ol[j];
 }
 
 // C[k][t] and O[k][j] (task.k = k): the partial sums of spot k's list
@@ -1010,10 +1012,10 @@ static int gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_mo
             hipLaunchKernelGGL(b3_gibbs_mask_values, grid(ns), dim3(kStatThreads), 0, nullptr,
                                (const StatDev *)d_sdev, (const SpotTask *)d_st, ns);
             if (nc > 0)
-                hipLaunchKernelGGL(b3_gibbs_cpart, grid(nc), dim3(kStatThreads), 0, nullptr, (const StatDev *)d_sdev,
+                hipLaunchKernelGGL(b3_gibbs_cpart, grid(nc), dim3(kCpartThreads), 0, nullptr, (const StatDev *)d_sdev,
                                    (const StatTask *)d_ct, nc);
             if (ne > 0)
-                hipLaunchKernelGGL(b3_gibbs_opart, grid(ne), dim3(kStatThreads), olds, nullptr,
+                hipLaunchKernelGGL(b3_gibbs_opart, grid(ne), dim3(kOpartThreads), olds, nullptr,
                                    (const StatDev *)d_sdev, (const SpotTask *)d_et, ne);
             hipLaunchKernelGGL(b3_gibbs_reduce, grid(ns), dim3(kStatThreads), 0, nullptr, (const StatDev *)d_sdev,
                                (const SpotTask *)d_st, ns);
