@@ -17,10 +17,12 @@
  * sampled_multispot.h:681-709) would bind.
  *
  * Precision: spot images and residuals in FP32; per-frame sums, the
- * forward/backward recursions and all reductions in FP64.  The gradient is
- * the adjoint (reverse mode): forward messages are kept only every
- * `checkpoint` frames and recomputed segment by segment during the
- * backward pass.
+ * forward/backward recursions and all reductions in FP64.  The recursions
+ * keep every message value with an exponent of its own (extended range),
+ * so a state whose weight falls below the double range keeps it, as in
+ * the reference's log space.  The gradient is the adjoint (reverse mode):
+ * forward messages are kept only every `checkpoint` frames and recomputed
+ * segment by segment during the backward pass.
  *
  * Conventions: parameters phi = (I, sigma, x, y, z) (storm.h:71); states
  * 0 = on, 1 = off, 2 = bleached; pixel coordinates are integer voxel
@@ -165,7 +167,9 @@ typedef struct rfba3b_gibbs rfba3b_gibbs;  /* opaque plan */
 /* Validates the chains and their masks, computes S2, C and O, seeds the
  * generators; the caller's buffers stay caller-owned and must outlive the
  * plan.  Starts from reset.  setup_ms (may be NULL) gets the statistics'
- * kernel time. */
+ * kernel time.  Limits: at most 65535 chains, T <= 2836 frames per chain
+ * (a chain's forward messages are held in on-chip memory), K <= 19456
+ * spots; each mask strictly increasing within [0, P). */
 int rfba3b_gibbs_create(const rfba3b_gibbs_chain *chains, int n, const rfba3b_model *model, rfba3b_gibbs **plan,
                         float *setup_ms);
 /* GibbsSampler2::reset (sampled_multispot.h:430-438): every spot state 2
