@@ -404,10 +404,10 @@ __global__ __launch_bounds__(kCpartThreads) void b3_gibbs_cpart(const StatDev *c
 }
 
 // Per list entry q = (k, r): mu_k over its segment scattered into a dense
-// LDS row (0 elsewhere in the range); each wave takes the other entries
-// (j, r) of the range and sums mu_j times the row over their segments,
-// four mask entries per lane in flight.  Op[q][j] for every j (0 for the
-// spots that do not meet the range).
+// LDS row (0 elsewhere in the range); each wave takes the entries (j, r) of
+// the range with j > k (O is symmetric) and sums mu_j times the row over
+// their segments, four mask entries per lane in flight.  Op[q][j] for every
+// j (0 for j <= k and for the spots that do not meet the range).
 __global__ __launch_bounds__(kOpartThreads) void b3_gibbs_opart(const StatDev *chains, const SpotTask *tasks,
                                                               int ntasks) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -438,7 +438,7 @@ __global__ __launch_bounds__(kOpartThreads) void b3_gibbs_opart(const StatDev *c
         for (int qi = wave; qi < min(64, nq - b); qi += kOpartThreads / 64) {
             const int j = __builtin_amdgcn_readlane(lj, qi), e0 = __builtin_amdgcn_readlane(l0, qi),
                       e1 = __builtin_amdgcn_readlane(l1, qi);
-            if (j == k) continue;
+            if (j <= k) continue;  // O is symmetric: the pairs j > k only
             double a = 0.0;
             for (int e = e0 + lane; e < e1; e += 4 * 64) {
                 int x[4];
@@ -479,10 +479,14 @@ __global__ __launch_bounds__(kStatThreads) void b3_gibbs_reduce(const StatDev *c
         for (int q = q0; q < q1; q++) a += c.Cp[(size_t)c.kent[q] * T + t];
         c.C[(size_t)k * T + t] = a;
     }
-    for (int j = threadIdx.x; j < K; j += kStatThreads) {
+    // O_kj = O_jk: this spot writes both for j > k (the partial sums hold the
+    // pairs j > k only), and 0 on the diagonal
+    for (int j = k + threadIdx.x; j < K; j += kStatThreads) {
         double a = 0.0;
-        for (int q = q0; q < q1; q++) a += c.Op[(size_t)c.kent[q] * K + j];
+        if (j > k)
+            for (int q = q0; q < q1; q++) a += c.Op[(size_t)c.kent[q] * K + j];
         c.O[(size_t)k * K + j] = a;
+        c.O[(size_t)j * K + k] = a;
     }
 }
 
