@@ -18,6 +18,8 @@ import numpy as np
 
 from . import _lib
 from .srrf import DeviceBuffer
+from .srrf import _check as _srrf_check
+from .srrf import lib as _srrf_lib
 
 
 class B3Error(RuntimeError):
@@ -317,6 +319,27 @@ class ChainData:
     slots: int = 0           # S: sample copies kept by Gibbs.run (d_samples [S, K, T])
 
 
+class DeviceView:
+    """A byte range of a pooled DeviceBuffer; keeps the pool alive."""
+
+    def __init__(self, pool: DeviceBuffer, offset: int, nbytes: int):
+        self.pool = pool
+        self.nbytes = int(nbytes)
+        self.ptr = c_void_p(pool.ptr.value + int(offset))
+
+    def write(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _srrf_check(_srrf_lib().rfba_memcpy_h2d(self.ptr, a.ctypes.data_as(c_void_p), ctypes.c_size_t(a.nbytes)))
+
+    def to_array(self, shape, dtype) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        _srrf_check(_srrf_lib().rfba_memcpy_d2h(out.ctypes.data_as(c_void_p), self.ptr,
+                                                ctypes.c_size_t(out.nbytes)))
+        return out
+
+
 class Gibbs:
     """Device Gibbs sampler over chains (GibbsSampler2, one workgroup per
     chain).  Spot tables are computed on the device, masks on the host;
@@ -327,7 +350,11 @@ class Gibbs:
         self._bufs = []
         self.meta = []
         arr = (_GibbsChain * len(chains))()
-        for i, ch in enumerate(chains):
+        # host-side arrays per chain, then one device allocation per kind with
+        # every chain at an aligned offset (thousands of chains would
+        # otherwise cost thousands of allocations)
+        host = []
+        for ch in chains:
             T = ch.data.shape[0]
             data = np.ascontiguousarray(ch.data, np.float32).reshape(T, -1)
             P = data.shape[1]
@@ -336,25 +363,40 @@ class Gibbs:
             xyz = np.ascontiguousarray(ch.xyz, np.int32)
             assert xyz.shape == (P, 3)
             off, idx = spot_pixels(xyz, phi)
-            dd = DeviceBuffer.from_array(data)
-            dx = DeviceBuffer.from_array(xyz)
-            dphi = DeviceBuffer.from_array(phi) if K else None
-            dspot = DeviceBuffer(max(K * P, 1) * 8)
-            doff = DeviceBuffer.from_array(off)
-            didx = DeviceBuffer.from_array(idx if idx.size else np.zeros(1, np.int32))
-            dsmp = DeviceBuffer(max(K * T, 1))
-            dout = DeviceBuffer(max(ch.slots * K * T, 1)) if ch.slots else None
+            host.append((data, xyz, phi, off, idx if idx.size else np.zeros(1, np.int32), P, T, K, ch))
+        kinds = {
+            "data": lambda h: h[0].nbytes, "xyz": lambda h: h[1].nbytes, "phi": lambda h: max(h[2].nbytes, 8),
+            "spot": lambda h: max(h[7] * h[5], 1) * 8, "off": lambda h: h[3].nbytes, "idx": lambda h: h[4].nbytes,
+            "smp": lambda h: max(h[7] * h[6], 1), "out": lambda h: max(h[8].slots * h[7] * h[6], 1)}
+        views = [dict() for _ in host]
+        for kind, size in kinds.items():
+            offs, total = [], 0
+            for h in host:
+                offs.append(total)
+                total += (size(h) + 255) & ~255
+            pool = DeviceBuffer(max(total, 256))
+            self._bufs.append(pool)
+            for i, h in enumerate(host):
+                views[i][kind] = DeviceView(pool, offs[i], size(h))
+        for i, (data, xyz, phi, off, idx, P, T, K, ch) in enumerate(host):
+            v = views[i]
+            v["data"].write(data)
+            v["xyz"].write(xyz)
+            v["off"].write(off)
+            v["idx"].write(idx)
             if K:
-                _check(lib().rfba3b_spot_intensities_device(dx.ptr, P, dphi.ptr, K, model.zfactor, dspot.ptr, None))
-            self._bufs += [b for b in (dd, dx, dphi, dspot, doff, didx, dsmp, dout) if b is not None]
-            self.meta.append(dict(P=P, T=T, K=K, slots=ch.slots, spot=dspot, sample=dsmp, out=dout,
+                v["phi"].write(phi)
+                _check(lib().rfba3b_spot_intensities_device(v["xyz"].ptr, P, v["phi"].ptr, K, model.zfactor,
+                                                            v["spot"].ptr, None))
+            out = v["out"] if ch.slots else None
+            self.meta.append(dict(P=P, T=T, K=K, slots=ch.slots, spot=v["spot"], sample=v["smp"], out=out,
                                   offsets=off, idx=idx))
             a = arr[i]
             a.P, a.T, a.K = P, T, K
-            a.d_data, a.d_spot, a.d_mask_off, a.d_mask = dd.ptr, dspot.ptr, doff.ptr, didx.ptr
+            a.d_data, a.d_spot, a.d_mask_off, a.d_mask = v["data"].ptr, v["spot"].ptr, v["off"].ptr, v["idx"].ptr
             a.base = float(ch.base)
-            a.d_sample = dsmp.ptr
-            a.d_samples = dout.ptr if dout else None
+            a.d_sample = v["smp"].ptr
+            a.d_samples = out.ptr if out else None
             a.seed = int(ch.seed) & 0xFFFFFFFF
             a.skip = int(ch.skip)
         self.plan = c_void_p()

@@ -1159,6 +1159,20 @@ extern "C" int rfba3b_spot_pixels(const int *xyz, int P, const double *phi, int 
     for (int p = 1; p < P; p++)
         if (!less(xyz + 3 * (p - 1), xyz + 3 * p))
             return host_fail(-1, "pixel list is not strictly increasing in (z, y, x) at %d", p);
+    // A sorted list of P distinct pixels that fills its bounding box is the
+    // box: then a pixel's index follows from its coordinates.
+    int b0[3] = {0, 0, 0}, b1[3] = {-1, -1, -1};
+    bool box = false;
+    if (P > 0) {
+        for (int j = 0; j < 3; j++) b0[j] = b1[j] = xyz[j];
+        for (int p = 1; p < P; p++)
+            for (int j = 0; j < 3; j++) {
+                b0[j] = std::min(b0[j], xyz[3 * (size_t)p + j]);
+                b1[j] = std::max(b1[j], xyz[3 * (size_t)p + j]);
+            }
+        box = (long long)(b1[0] - b0[0] + 1) * (b1[1] - b0[1] + 1) * (b1[2] - b0[2] + 1) == (long long)P;
+    }
+    const long long nx = b1[0] - b0[0] + 1, ny = b1[1] - b0[1] + 1;
     size_t tot = 0;
     offsets[0] = 0;
     for (int k = 0; k < K; k++) {
@@ -1175,23 +1189,48 @@ extern "C" int rfba3b_spot_pixels(const int *xyz, int P, const double *phi, int 
             const double v = ph[2 + j];
             off[j] = static_cast<int>(v > 0.0 ? v + 0.5 : v - 0.5);
         }
+        // Ball points in (z, y, x) order, as get_ball lists them; the points
+        // of one (z, y) row are a contiguous run of x (the test is monotone
+        // in |x|), and the pixel list is sorted in (z, y, x), so each row
+        // takes one lower_bound and then a forward walk along x.
         for (int z = -r; z <= r; z++)
-            for (int y = -r; y <= r; y++)
-                for (int x = -r; x <= r; x++) {
-                    const unsigned int m2 = unsigned(x * x) + unsigned(y * y) + unsigned(z * z);
-                    if (!((float)m2 <= r2)) continue;
-                    const int q[3] = {x + off[0], y + off[1], z + off[2]};
-                    int lo = 0, hi = P;  // lower_bound
-                    while (lo < hi) {
-                        const int mid = lo + (hi - lo) / 2;
-                        if (less(xyz + 3 * (size_t)mid, q)) lo = mid + 1;
-                        else hi = mid;
+            for (int y = -r; y <= r; y++) {
+                // the largest |x| <= r with (float)(x^2 + y^2 + z^2) <= r2:
+                // an integer square root, then the exact test moves it
+                const unsigned int c = unsigned(y * y) + unsigned(z * z);
+                if (!((float)c <= r2)) continue;
+                int w = (int)std::sqrt(std::max(0.0, (double)r2 - (double)c));
+                while (w < r && (float)(unsigned(w + 1) * unsigned(w + 1) + c) <= r2) w++;
+                while (w > 0 && !((float)(unsigned(w) * unsigned(w) + c) <= r2)) w--;
+                w = std::min(w, r);
+                const int xl = -w, xh = w;
+                if (box) {
+                    const int zz = z + off[2], yy = y + off[1];
+                    if (zz < b0[2] || zz > b1[2] || yy < b0[1] || yy > b1[1]) continue;
+                    const int xa = std::max(xl + off[0], b0[0]), xb = std::min(xh + off[0], b1[0]);
+                    const long long row = ((long long)(zz - b0[2]) * ny + (yy - b0[1])) * nx - b0[0];
+                    for (int xx = xa; xx <= xb; xx++) {
+                        if (idx && tot < cap) idx[tot] = (int)(row + xx);
+                        tot++;
                     }
+                    continue;
+                }
+                const int q0[3] = {xl + off[0], y + off[1], z + off[2]};
+                int lo = 0, hi = P;  // lower_bound of the row's first point
+                while (lo < hi) {
+                    const int mid = lo + (hi - lo) / 2;
+                    if (less(xyz + 3 * (size_t)mid, q0)) lo = mid + 1;
+                    else hi = mid;
+                }
+                for (int x = xl; x <= xh; x++) {
+                    const int q[3] = {x + off[0], y + off[1], z + off[2]};
+                    while (lo < P && less(xyz + 3 * (size_t)lo, q)) lo++;
                     if (lo < P && !less(q, xyz + 3 * (size_t)lo)) {
                         if (idx && tot < cap) idx[tot] = lo;
                         tot++;
                     }
                 }
+            }
         if (tot > (size_t)INT_MAX) return host_fail(-1, "too many mask entries");
         offsets[k + 1] = (int)tot;
     }
