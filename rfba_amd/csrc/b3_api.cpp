@@ -201,10 +201,11 @@ static CorrFn corr_fn(bool grad) { return grad ? b3_corr_mfma<true> : b3_corr_mf
 static HmmFn hmm_fn(bool grad) { return grad ? b3_hmm<true> : b3_hmm<false>; }
 
 static size_t corr_lds(bool grad, int naxmax) {
-    // Q/H | residual chunk [128][36] | basis chunk [32][16-column tiles] | axis tables
+    // Q/H [8][6] | per-wave Q/H [4][8][6] | residual chunk [128][36] |
+    // basis chunk [32][16-column tiles] | axis tables [3][nax][8]
     const size_t bmat = (size_t)kCorrPix * (((grad ? 48 : 8) + 15) / 16 * 16);
-    return sizeof(double) * kCorrSpots * 6 +
-           sizeof(float) * ((size_t)kCorrFrames * kCorrRow + bmat + (size_t)kCorrSpots * naxmax * 4);
+    return sizeof(double) * kCorrSpots * 6 * 5 +
+           sizeof(float) * ((size_t)kCorrFrames * kCorrRow + bmat + (size_t)3 * kCorrSpots * naxmax);
 }
 
 extern "C" int rfba3b_batch_create(const rfba3b_block *blocks, int n, const rfba3b_model *model,

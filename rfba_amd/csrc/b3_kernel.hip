@@ -457,14 +457,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     const DevModel &md = *C.model;
 
     double *qh = ldsc;                                   // [8][6] Q, H_0..4
-    float *Rs = (float *)(qh + kCorrSpots * 6);          // [kCorrFrames][kCorrRow]
+    double *qw = qh + kCorrSpots * 6;                    // [4 waves][8][6] partial Q, H
+    float *Rs = (float *)(qw + 4 * kCorrSpots * 6);      // [kCorrFrames][kCorrRow]
     float *Bs = Rs + kCorrFrames * kCorrRow;             // [kCorrPix][BSTR]
-    float *ax = Bs + kCorrPix * BSTR;                    // [8][nax][4]
+    float *ax = Bs + kCorrPix * BSTR;                    // [3][nax][8]
 
+    // Axis tables, component c of axis entry a (x, then y, then z) for spot j
+    // at ax[(c nax + a) 8 + j]: the 8 spots x 8 consecutive pixels a wave
+    // reads are 64 consecutive words.
     const float zF = (float)md.zfactor;
     const float zdiv = md.z_exact ? zF * zF : 1.0f;
     for (int e = tid; e < nk * nax; e += 256) {
-        const int j = e / nax, a = e - j * nax;
+        const int a = e / nk, j = e - a * nk;
         const double *ph = b.params + ((size_t)pr.q * K + tk.k0 + j) * 5;
         const double sg = ph[1];
         const float i2s2 = (float)(1.0 / (2.0 * sg * sg));
@@ -484,10 +488,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
             sv = -(dzs * dzs) * i2s2;
             dI = dz * Iis2 / zdiv;
         }
-        float *q = ax + 4 * (j * nax + a);
-        q[0] = expf(sv);
-        q[1] = sv;
-        q[2] = dI;
+        ax[a * 8 + j] = expf(sv);
+        ax[(nax + a) * 8 + j] = sv;
+        ax[(2 * nax + a) * 8 + j] = dI;
     }
     // columns past the spots' basis stay zero
     for (int e = tid; BSTR > NCOL && e < kCorrPix * (BSTR - NCOL); e += 256) {
@@ -495,65 +498,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
         Bs[c * BSTR + NCOL + o] = 0.0f;
     }
 
-    // basis roles: spot jb, pixel lane cb.  The Q/H pre-pass reduces over
-    // the 32 lanes of a half-wave (jb = tid >> 5); the chunk loop writes the
-    // 8 spots of a pixel from neighbouring lanes (jb = tid & 7), which keeps
-    // its LDS stores nearly conflict-free.
-    int jb = tid >> 5, cb = tid & 31;
+    // basis role: spot jb, pixel lane cb (pixels p0 + cb of each chunk); the
+    // 8 spots of a pixel come from neighbouring lanes, which keeps the basis
+    // stores nearly conflict-free.
+    const int jb = tid & 7, cb = tid >> 3;
     float Isb = 0.0f, invb = 0.0f, msigb = 0.0f;
-    auto spot_consts = [&]() {
-        Isb = invb = msigb = 0.0f;
-        if (jb < nk) {
-            const double *ph = b.params + ((size_t)pr.q * K + tk.k0 + jb) * 5;
-            Isb = (float)ph[0];
-            invb = (float)(1.0 / (ph[1] * 2.5066282746310002));
-            msigb = -Isb / (float)ph[1];
-        }
-    };
-    spot_consts();
-    // Q = sum mu^2 and H_n = sum dmu_n mu over the block (frame independent):
-    // spot jb, pixels cb, cb + 32, ...
+    if (jb < nk) {
+        const double *ph = b.params + ((size_t)pr.q * K + tk.k0 + jb) * 5;
+        Isb = (float)ph[0];
+        invb = (float)(1.0 / (ph[1] * 2.5066282746310002));
+        msigb = -Isb / (float)ph[1];
+    }
     auto basis = [&](int x, int y, int z, float *v) {
-        const float *qx = ax + 4 * (jb * nax + x);
-        const float *qy = ax + 4 * (jb * nax + X + y);
-        const float *qz = ax + 4 * (jb * nax + X + Y + z);
-        const float E = (qx[0] * qy[0] * invb) * qz[0];
+        const float *ex = ax + jb;       // exp(-d^2 / 2 sigma^2)
+        const float *sx = ex + 8 * nax;  // -d^2 / 2 sigma^2
+        const float *dx = sx + 8 * nax;  // d I / sigma^2 (z: over zfactor^2 if exact)
+        const int ix = 8 * x, iy = 8 * (X + y), iz = 8 * (X + Y + z);
+        const float E = (ex[ix] * ex[iy] * invb) * ex[iz];
         v[0] = Isb * E;
         if (GRAD) {
-            const float sv = (qx[1] + qy[1]) + qz[1];
+            const float sv = (sx[ix] + sx[iy]) + sx[iz];
             v[NB > 1 ? 1 : 0] = E;
             v[NB > 2 ? 2 : 0] = E * msigb * (1.0f + 2.0f * sv);
-            v[NB > 3 ? 3 : 0] = E * qx[2];
-            v[NB > 4 ? 4 : 0] = E * qy[2];
-            v[NB > 5 ? 5 : 0] = E * qz[2];
+            v[NB > 3 ? 3 : 0] = E * dx[ix];
+            v[NB > 4 ? 4 : 0] = E * dx[iy];
+            v[NB > 5 ? 5 : 0] = E * dx[iz];
         }
     };
-    __syncthreads();  // axis tables
-    {
-        double qacc[NB];
+    // Q = sum mu^2 and H_n = sum dmu_n mu over the block (frame independent),
+    // accumulated from the chunk loop's basis values: this lane's part is
+    // spot jb over the pixels p = cb mod 32.
+    double qacc[NB];
 #pragma unroll
-        for (int n = 0; n < NB; n++) qacc[n] = 0.0;
-        if (jb < nk)
-            for (int p = cb; p < P; p += 32) {
-                const int z = p / XY, rr = p - z * XY;
-                const int y = rr / X, x = rr - y * X;
-                float v[NB];
-                basis(x, y, z, v);
-#pragma unroll
-                for (int n = 0; n < NB; n++) qacc[n] += (double)v[n] * (double)v[0];
-            }
-#pragma unroll
-        for (int n = 0; n < NB; n++) {
-            double v = qacc[n];
-#pragma unroll
-            for (int o = 16; o > 0; o >>= 1) v += __shfl_xor(v, o);
-            if (cb == 0) qh[jb * 6 + n] = v;
-        }
-    }
+    for (int n = 0; n < NB; n++) qacc[n] = 0.0;
 
-    jb = tid & 7;
-    cb = tid >> 3;
-    spot_consts();
     const float *Rp = C.R + pr.rslot;
     const int lr = lane & 15, lk = lane >> 4;
     for (int t0 = 0; t0 < T; t0 += kCorrFrames) {
@@ -608,6 +586,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                 if (jb < nk && p < P) basis(bx, by, bz, v);
 #pragma unroll
                 for (int n = 0; n < NB; n++) Bs[cb * BSTR + jb * NB + n] = v[n];
+                // every pass sees the same values; only the first pass's sums are used
+#pragma unroll
+                for (int n = 0; n < NB; n++) qacc[n] += (double)v[n] * (double)v[0];
                 // advance this lane's pixel by kCorrPix
                 bx += kCorrPix;
                 while (bx >= X) {
@@ -646,6 +627,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
             }
         }
         __syncthreads();
+        if (t0 == 0) {
+            // lanes l, l ^ 8, l ^ 16, l ^ 32 share a spot; then the 4 waves in order
+#pragma unroll
+            for (int n = 0; n < NB; n++) {
+                double v = qacc[n];
+                v += __shfl_xor(v, 8);
+                v += __shfl_xor(v, 16);
+                v += __shfl_xor(v, 32);
+                if (lane < kCorrSpots) qw[(wave * kCorrSpots + lane) * 6 + n] = v;
+            }
+            __syncthreads();
+            if (tid < kCorrSpots * NB) {
+                const int j = tid / NB, n = tid - j * NB;
+                const double *w = qw + j * 6 + n;
+                qh[j * 6 + n] = ((w[0] + w[kCorrSpots * 6]) + w[2 * kCorrSpots * 6]) + w[3 * kCorrSpots * 6];
+            }
+            __syncthreads();
+        }
 #pragma unroll
         for (int f2 = 0; f2 < 2; f2++)
 #pragma unroll
