@@ -31,9 +31,10 @@ Beside it, in "workloads" of the same JSON line:
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only c2,c3,c4,c5,gibbs]
 
 Prints one JSON line.  value = SR voxels / s of the headline workload;
-roofline = algorithmic HBM bytes per launch / measured kernel time against
-8 TB/s, with the FP32-VALU roof beside it; cpu_baseline = the multithreaded
-oracle on a bounded crop of the same series.
+roofline = algorithmic FP32 flops per launch / measured kernel time against
+MI355X's FP32 peak (the kernel is VALU-bound), with the HBM roof (algorithmic
+bytes against 8 TB/s) beside it; cpu_baseline = the multithreaded oracle on a
+bounded crop of the same series.
 """
 from __future__ import annotations
 
