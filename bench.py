@@ -75,16 +75,18 @@ def hbm_roof(alg_bytes, kernel_ms):
     return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
 
 
-def srrf_flops_per_voxel_frame(n_samples, intensity_w):
+def srrf_flops_per_voxel_frame(n_samples, intensity_w, trac):
     """Algorithmic FP32 flops of one SR voxel-frame of SRRF: 430 per sample
     direction (the gradient interpolation and the radiality term), 128 for the
-    intensity weight's interpolation when it is on, 6 for the temporal
-    statistic (counted for every temporal mode)."""
-    return n_samples * 430 + 128 * intensity_w + 6
+    intensity weight's interpolation when it is on, and the temporal
+    statistic: 6 for TRAC, 4 for the mean (TRA)."""
+    return n_samples * 430 + 128 * intensity_w + (6 if trac else 4)
 
 
 SRRF_ROOF_NOTE = ("FP32 VALU-bound (no matrix instructions; the FP32 peak is the same on the vector units and "
-                  "the matrix cores); algorithmic flops count the interpolation shared by a voxel's samples once")
+                  "the matrix cores); algorithmic flops count every SR voxel-frame, with the interpolation shared "
+                  "by a voxel's samples once; the kernel skips voxels whose value is NaN by definition (a sample "
+                  "out of range), the fraction given beside")
 
 
 # ---------------------------------------------------------------- headline: one full-volume stack
@@ -141,9 +143,7 @@ def bench_stack(args, rec):
 
     alg_bytes = 2.0 * n_lr * T + 4.0 * n_sr
     K = smp.n
-    # voxels whose value is NaN by definition (a sample out of range) need no
-    # sample work and are not counted
-    alg_flops = float(T) * n_sr * (1.0 - nan_frac) * srrf_flops_per_voxel_frame(K, conf.IntensityW)
+    alg_flops = float(T) * n_sr * srrf_flops_per_voxel_frame(K, conf.IntensityW, conf.Method == 3)
     tflops = alg_flops / (kernel_ms * 1e-3) / 1e12
     traffic = args.traffic if args.traffic is not None else (HEADLINE_TRAFFIC_BYTES if T == 200 else None)
     # The kernel is bound by FP32 arithmetic on the vector units (it issues no
@@ -153,7 +153,7 @@ def bench_stack(args, rec):
     roof = {"bound": "mfma", "achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": tflops / FP32_PEAK_TFLOPS, "traffic": traffic, "kernel": "srrf_fused (uint16 input)",
             "kernel_ms": kernel_ms, "algorithmic_flops_per_launch": alg_flops,
-            "note": SRRF_ROOF_NOTE + "; only voxels whose value is not NaN by definition are counted",
+            "note": SRRF_ROOF_NOTE, "nan_by_definition_fraction": nan_frac,
             "hbm": dict(hbm_roof(alg_bytes, kernel_ms), algorithmic_bytes_per_launch=alg_bytes)}
     rec.update({
         "value": n_sr / (ms_per_step * 1e-3),
@@ -259,24 +259,24 @@ def bench_srrf_batched(args):
     wall_pb = (time.perf_counter() - t0) * 1e3
     log("c3: per-block %.1f ms" % ms_pb)
     same = all(np.array_equal(a, b, equal_nan=True) for a, b in zip(ref, snap()))
-    # A voxel with a sample out of range on any axis is NaN in every frame by
-    # definition and needs no sample work; the non-NaN outputs of each block
-    # (a lower bound on the voxels computed) times its frames give the work.
+    # Voxels with a sample out of range on any axis are NaN in every frame by
+    # definition, and the kernel skips them: their share of the voxel-frames
+    # is reported beside the roofline.
     host = np.empty(n_sr, np.float32)
     srrf._check(srrf.lib().rfba_memcpy_d2h(host.ctypes.data_as(ctypes.c_void_p), d_out.ptr, host.nbytes))
     valid = np.add.reduceat((~np.isnan(host)).view(np.uint8), (offs // 4).astype(np.int64), dtype=np.int64)
     del host
-    work_frames = float(sum(int(v) * g[4] for v, g in zip(valid, geo)))
+    valid_frames = float(sum(int(v) * g[4] for v, g in zip(valid, geo)))
     bt.close()
     for b in [d_vol, d_out] + d_extra:
         b.close()
     kb = float(np.mean(kms))
-    fl = work_frames * srrf_flops_per_voxel_frame(smp.n, conf.IntensityW)
+    fl = sr_frames * srrf_flops_per_voxel_frame(smp.n, conf.IntensityW, conf.Method == 3)
     tf = fl / (kb * 1e-3) / 1e12
     roof = {"bound": "mfma", "achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": tf / FP32_PEAK_TFLOPS, "kernel": "srrf_fused persistent queue (uint16 input)", "kernel_ms": kb,
-            "algorithmic_flops_per_launch": fl, "computed_sr_voxel_frames": work_frames,
-            "note": SRRF_ROOF_NOTE + "; only voxels whose value is not NaN by definition are counted",
+            "algorithmic_flops_per_launch": fl, "note": SRRF_ROOF_NOTE,
+            "nan_by_definition_fraction": 1.0 - valid_frames / sr_frames,
             "hbm": dict(hbm_roof(alg_bytes, kb), algorithmic_bytes_per_launch=alg_bytes)}
     res = {
         "workload": "batched 3D SRRF: 512x512x128 T=120 uint16 (8000 emitters, seed 3) split by doSplit.m into "
