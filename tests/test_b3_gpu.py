@@ -61,6 +61,20 @@ def test_batched_vs_oracle(gpu, ckpt):
     compare(L, G, Lo, Go, "ckpt=%d" % ckpt)
 
 
+def test_series_longer_than_one_pass(gpu):
+    # T = 300: three 128-frame passes of the correlation kernel, the last one
+    # partial (the frame-independent sums come from the first pass); K = 10
+    # gives a full and a partial group of 8 spots; P = 504 ends in a partial
+    # pixel chunk; slow bleaching keeps spots blinking through all passes
+    m = model(checkpoint=0)
+    data, params, states = b3.synth_block((9, 8, 7), 300, 10, 31, A=b3.blink_matrix(0.2, 0.1, 0.001))
+    P = b3.perturb(params, 2, 32)
+    st = np.stack([states, np.roll(states, 5, axis=1)])
+    (L, G), = run_batch([b3.BlockData(data, P, st, 100.0)], m)
+    Lo, Go = bo.model_eval(data, 100.0, m, P, st)
+    compare(L, G, Lo, Go, "T=300 K=10")
+
+
 def test_checkpoint_stride_does_not_change_results(gpu):
     data, params, states = b3.synth_block((10, 9, 8), 40, 4, 23)
     P = b3.perturb(params, 2, 24)
