@@ -551,8 +551,9 @@ def bench_b3_gibbs(args):
     roof = hbm_roof(by, kb)
     roof.update({"kernel": "b3_gibbs (one 64-thread workgroup per chain)", "kernel_ms": kb,
                  "algorithmic_bytes_per_launch": by,
-                 "note": "bound by the serial forward recursion and backward sampling of each spot update "
-                         "(T dependent FP64 steps on one lane), not by memory"})
+                 "note": "bound by the latency of each spot update, not by memory: the spots of a chain are "
+                         "updated in order, each by a 64-lane scan of extended-range FP64 3x3 products and a "
+                         "backward sampling scan, one wave per chain; the longest chain sets the time"})
     # plain form: one launch per block (a plan per chain) on the first blocks
     nsub = min(20, len(chains))
     subs = [b3.Gibbs([chains[i]], m) for i in range(nsub)]
