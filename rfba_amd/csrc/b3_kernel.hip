@@ -528,9 +528,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     // Q = sum mu^2 and H_n = sum dmu_n mu over the block (frame independent),
     // accumulated from the chunk loop's basis values: this lane's part is
     // spot jb over the pixels p = cb mod 32.
+    // FP32 over four chunks (as the matrix sums), then FP64
     double qacc[NB];
+    float q32[NB];
 #pragma unroll
-    for (int n = 0; n < NB; n++) qacc[n] = 0.0;
+    for (int n = 0; n < NB; n++) {
+        qacc[n] = 0.0;
+        q32[n] = 0.0f;
+    }
 
     const float *Rp = C.R + pr.rslot;
     const int lr = lane & 15, lk = lane >> 4;
@@ -588,7 +593,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                 for (int n = 0; n < NB; n++) Bs[cb * BSTR + jb * NB + n] = v[n];
                 // every pass sees the same values; only the first pass's sums are used
 #pragma unroll
-                for (int n = 0; n < NB; n++) qacc[n] += (double)v[n] * (double)v[0];
+                for (int n = 0; n < NB; n++) q32[n] += v[n] * v[0];
                 // advance this lane's pixel by kCorrPix
                 bx += kCorrPix;
                 while (bx >= X) {
@@ -624,6 +629,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                         for (int r = 0; r < 4; r++) a64[f2][ct][r] += (double)acc[f2][ct][r];
                         acc[f2][ct] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
                     }
+#pragma unroll
+                for (int n = 0; n < NB; n++) {
+                    qacc[n] += (double)q32[n];
+                    q32[n] = 0.0f;
+                }
             }
         }
         __syncthreads();
