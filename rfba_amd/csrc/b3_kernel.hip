@@ -560,6 +560,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
         float4 pre[4];
         auto fetch = [&](int p0) {
             const int p = p0 + pr4;
+            if (p0 + kCorrPix <= P) {
+                // whole chunk inside the plane: rows past the pass are read
+                // from its last row and zeroed
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int r = prow + 32 * i;
+                    float4 v;
+                    __builtin_memcpy(&v, Rp + (size_t)(t0 + min(r, tn - 1)) * P + p, sizeof v);
+                    pre[i] = r < tn ? v : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
+                return;
+            }
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int r = prow + 32 * i;
