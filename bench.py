@@ -43,6 +43,7 @@ import ctypes
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -90,6 +91,20 @@ SRRF_ROOF_NOTE = ("FP32 VALU-bound (no matrix instructions; the FP32 peak is the
 
 
 # ---------------------------------------------------------------- headline: one full-volume stack
+def host_cpu(threads):
+    """What the host offers and what a CPU leg used: nproc (os.cpu_count),
+    the affinity mask's size, std::thread::hardware_concurrency and the
+    threads used (at most 16: the CPU share one GPU's job is given on the
+    measured box)."""
+    from oracle import oracle
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except Exception:
+        aff = None
+    return {"nproc": os.cpu_count(), "affinity": aff,
+            "hardware_concurrency": int(oracle.lib().oracle_hardware_threads()), "threads_used": threads}
+
+
 def cpu_baseline(series, conf, smp, crop, threads):
     """Time the oracle on a crop [T, cz, cy, cx] of the same series."""
     from oracle import oracle
@@ -101,12 +116,44 @@ def cpu_baseline(series, conf, smp, crop, threads):
     out = oracle.srrf_stack(sub, conf, smp.xyz, smp.divisor, nthreads=threads)
     dt = time.perf_counter() - t0
     n_sr = out.size
-    return {"value": n_sr / dt, "unit": "SR-voxels/s", "cores": threads, "kind": "port",
+    return {"value": n_sr / dt, "unit": "SR-voxels/s", "cores": threads, "kind": "port", "host": host_cpu(threads),
             "sample": "oracle (FP32 transcription, %d threads) on a %dx%dx%d low-res crop x T=%d of the same series "
                       "(%d SR voxels, %.1f s); rate per SR voxel at full T" % (threads, cx, cy, cz, T, n_sr, dt)}
 
 
-def bench_stack(args, rec):
+def srrf_accuracy(series, conf, smp, crop, frames, threads):
+    """Accuracy of the GPU kernel on a crop [frames, cz, cy, cx] of the series
+    (its first frames), run as a stack of its own: against the FP32 oracle (the bar: bit-identical,
+    NaN masks equal) and against the FP64 re-evaluation (max-normalised error
+    over finite voxels), and the number of (voxel, frame, sample) triples
+    whose converging/diverging test differs between FP32 and FP64."""
+    from oracle import oracle
+    from rfba_amd import srrf
+    cx, cy, cz = crop
+    T, Z, Y, X = series.shape
+    z0, y0, x0 = (Z - cz) // 2 + 3, (Y - cy) // 2 + 5, (X - cx) // 2 + 7
+    sub = np.ascontiguousarray(series[:frames, z0:z0 + cz, y0:y0 + cy, x0:x0 + cx])
+    T = sub.shape[0]
+    g = srrf.srrf_stack(sub, conf, smp)
+    o32 = oracle.srrf_stack(sub.astype(np.float32), conf, smp.xyz, smp.divisor, nthreads=threads)
+    o64 = oracle.srrf_stack(sub.astype(np.float64), conf, smp.xyz.astype(np.float64), smp.divisor, f64=True,
+                            nthreads=threads)
+    flips = oracle.gdotr_flips(sub, conf, smp.xyz, nthreads=threads)
+    fin = np.isfinite(o32)
+    scale = float(np.abs(o64[fin]).max()) if fin.any() else 0.0
+    same_nan = bool(np.array_equal(np.isnan(g), np.isnan(o32)))
+    rel32 = float(np.max(np.abs(g[fin].astype(np.float64) - o32[fin]) / np.maximum(np.abs(o32[fin]), 1e-30))) \
+        if fin.any() else 0.0
+    return {"crop_lowres_xyz": [cx, cy, cz], "frames": T, "sr_voxels": int(g.size), "finite_voxels": int(fin.sum()),
+            "nan_masks_equal": same_nan, "bit_identical_to_fp32_oracle": bool(np.array_equal(g, o32, equal_nan=True)),
+            "max_rel_err_vs_fp32_oracle": rel32,
+            "gpu_err_vs_fp64_max_normalised": float(np.abs(g[fin] - o64[fin]).max() / scale) if scale else 0.0,
+            "fp32_oracle_err_vs_fp64_max_normalised": float(np.abs(o32[fin] - o64[fin]).max() / scale)
+            if scale else 0.0,
+            "gdotr_sign_flips_fp32_vs_fp64": flips, "gdotr_tests": int(g.size) * T * smp.n}
+
+
+def bench_stack(args, rec, lock):
     from rfba_amd import srrf, synth
     X, Y, Z, T = 256, 256, 96, args.frames
     n_em, seed = 2000, 2
@@ -155,7 +202,7 @@ def bench_stack(args, rec):
             "kernel_ms": kernel_ms, "algorithmic_flops_per_launch": alg_flops,
             "note": SRRF_ROOF_NOTE, "nan_by_definition_fraction": nan_frac,
             "hbm": dict(hbm_roof(alg_bytes, kernel_ms), algorithmic_bytes_per_launch=alg_bytes)}
-    rec.update({
+    upd = {
         "value": n_sr / (ms_per_step * 1e-3),
         "ms_per_step": ms_per_step,
         "config": {"workload": "plain 3D SRRF, one stack: %dx%dx%d uint16, T=%d, %d emitters, M=%d -> %dx%dx%d SR, "
@@ -163,10 +210,17 @@ def bench_stack(args, rec):
                                % (X, Y, Z, T, n_em, M, X * M, Y * M, Z * M),
                    "sr_voxels": n_sr, "frames": T, "nan_fraction": nan_frac},
         "roofline": roof,
-    })
+    }
+    with lock:
+        rec.update(upd)
     if not args.no_cpu:
         crop = tuple(int(v) for v in args.cpu_crop.split(","))
-        rec["cpu_baseline"] = cpu_baseline(series, conf, smp, crop, synth.default_threads())
+        cb = cpu_baseline(series, conf, smp, crop, synth.default_threads())
+        with lock:
+            rec["cpu_baseline"] = cb
+        acc = srrf_accuracy(series, conf, smp, (8, 8, 6), 50, synth.default_threads())
+        with lock:
+            rec["accuracy"] = acc
 
 
 # ---------------------------------------------------------------- batched SRRF over uneven blocks
@@ -230,16 +284,16 @@ def bench_srrf_batched(args):
     groups = bt.groups
     log("c3: autotune %.1f s -> groups (M, tile depth) %s; largest group's sample ms %s"
         % (time.perf_counter() - t0, groups, tune))
-    for i in range(args.warmup):
+    for i in range(args.aux_warmup):
         log("c3: warmup batched %.1f ms" % bt.run(use_graph=True))
     srrf.lib().rfba_device_synchronize()
     kms = []
     t0 = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(args.aux_steps):
         kms.append(bt.run(use_graph=True))
         log("c3: batched step %d: %.1f ms" % (i, kms[-1]))
     srrf.lib().rfba_device_synchronize()
-    wall_b = (time.perf_counter() - t0) * 1e3 / args.steps
+    wall_b = (time.perf_counter() - t0) * 1e3 / args.aux_steps
     probe = [0, len(split) // 2, len(split) - 1, len(blocks) - 1]
 
     def snap():
@@ -303,6 +357,7 @@ def bench_srrf_batched(args):
         out = oracle.srrf_stack(sub[:, :, :12, :], c, smp.xyz, smp.divisor, nthreads=th)
         dt = time.perf_counter() - t0
         res["cpu_baseline"] = {"value": out.size / dt, "unit": "SR-voxels/s", "cores": th, "kind": "port",
+                               "host": host_cpu(th),
                                "sample": "oracle on a 12x12x29 x T=120 part of block 0 (M=%d), %.1f s"
                                          % (int(mags[i]), dt)}
     return res
@@ -351,11 +406,11 @@ def bench_b3_single(args):
     P = b3.perturb(params, 256, 7)
     blk = b3.BlockData(data, P, states[None], 100.0)
     bt = b3.Batch([blk], m)
-    for i in range(max(1, args.warmup)):
+    for i in range(max(1, args.aux_warmup)):
         log("c4: warmup batched %.2f ms" % bt.run(use_graph=True))
     kms = []
     t0 = time.perf_counter()
-    for i in range(max(3, args.steps)):
+    for i in range(max(1, args.aux_steps)):
         kms.append(bt.run(use_graph=True))
     wall_b = (time.perf_counter() - t0) * 1e3 / len(kms)
     L, G = bt.results(0)
@@ -394,6 +449,7 @@ def bench_b3_single(args):
                                        "zero_gradient_items": int((~nz).sum()),
                                        "zero_gradient_items_exact": bool(np.all(G[:q][~nz] == 0))}
         res["cpu_baseline"] = {"value": q / dt, "unit": "block-evals/s", "cores": th, "kind": "port",
+                               "host": host_cpu(th),
                                "sample": "FP64 oracle, %d parameter sets x 40 spots, %.1f s" % (q, dt)}
     return res
 
@@ -432,7 +488,7 @@ def bench_b3_batched(args):
     log("c5: warmup batched %.1f ms" % bt.run(use_graph=True))
     kms = []
     t0 = time.perf_counter()
-    for i in range(max(1, args.steps)):
+    for i in range(max(1, args.aux_steps)):
         kms.append(bt.run(use_graph=True))
         log("c5: batched step %d %.1f ms" % (i, kms[-1]))
     wall_b = (time.perf_counter() - t0) * 1e3 / len(kms)
@@ -485,6 +541,7 @@ def bench_b3_batched(args):
         bo.model_eval(b.data, 100.0, m, b.params[:2], b.states, threads=th)
         dt = time.perf_counter() - t0
         res["cpu_baseline"] = {"value": 2 / dt, "unit": "block-evals/s", "cores": th, "kind": "port",
+                               "host": host_cpu(th),
                                "sample": "FP64 oracle, block 0 (K=%d), 2 parameter sets, %.1f s"
                                          % (b.params.shape[1], dt)}
     return res
@@ -523,7 +580,7 @@ def bench_b3_gibbs(args):
     # first samples of every block, for the oracle's check (a bounded CPU sample: one sweep per block)
     first = [g.samples(i, 1)[0].copy() for i in range(len(chains))] if not args.no_cpu else []
     kms, walls = [], []
-    for i in range(max(1, args.steps)):
+    for i in range(max(1, args.aux_steps)):
         prev = [g.sample(c) for c in range(len(chains))]
         t1 = time.perf_counter()
         kms.append(g.run(S, 1))
@@ -611,7 +668,7 @@ def bench_b3_gibbs(args):
         neq = sum(bool(np.array_equal(o[1], first[i])) for i, o in enumerate(out))
         res["first_sample_equals_oracle"] = {"blocks": n, "equal_blocks": neq, "equal": neq == n}
         res["cpu_baseline"] = {"value": sum(o[2] for o in out) / dt, "unit": "spot-updates/s", "cores": th,
-                               "kind": "port",
+                               "kind": "port", "host": host_cpu(th),
                                "sample": "oracle GibbsSampler2::next, first sweep of every block (%d blocks), %d "
                                          "threads, %.1f s" % (n, th, dt)}
     return res
@@ -632,7 +689,7 @@ def bench_prep(args):
     stats, ms = prep.normalize_device(d_raw, series.dtype, series.shape, c, d_out)
     log("prep: warmup %.1f ms" % ms)
     kms, walls = [], []
-    for i in range(max(3, args.steps)):
+    for i in range(max(1, args.aux_steps)):
         t0 = time.perf_counter()
         stats, ms = prep.normalize_device(d_raw, series.dtype, series.shape, c, d_out)
         walls.append((time.perf_counter() - t0) * 1e3)
@@ -691,22 +748,38 @@ def bench_prep(args):
                                                                          np.array_equal(gst, ost))}
         del oo, go
         res["cpu_baseline"] = {"value": sub.size / dt, "unit": "voxel-frames/s", "cores": min(th, nf), "kind": "port",
+                               "host": host_cpu(min(th, nf)),
                                "sample": "oracle on frames 0-%d of the same series (%d voxel-frames), %.1f s; "
                                          "high-pass one thread per frame, statistics and scaling on one thread"
                                          % (nf - 1, sub.size, dt)}
     return res
 
 
+# Estimated wall seconds of each auxiliary workload at one warm-up and two
+# timed passes, CPU legs included (measured on MI355X with 16 host threads,
+# rounded up).  A workload starts only if it fits the remaining budget.
+EST_S = {"c4": 25, "c5": 70, "gibbs": 60, "c3": 150, "prep": 45}
+C3_SERIES_S = 35  # generating the 512x512x128 T=120 series shared by c3 and prep
+
+
 def main():
+    t_start = time.perf_counter()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2, help="timed passes of the headline workload")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed passes of the headline workload")
+    ap.add_argument("--aux-steps", type=int, default=2, help="timed passes of every other workload")
+    ap.add_argument("--aux-warmup", type=int, default=1, help="untimed passes of every other workload")
+    ap.add_argument("--budget", type=float, default=470.0,
+                    help="seconds from start after which no further workload is started")
+    ap.add_argument("--deadline", type=float, default=565.0,
+                    help="seconds from start at which the JSON line is printed with what has been measured, "
+                         "and the process ends")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-crop", type=str, default="12,12,8")
     ap.add_argument("--frames", type=int, default=200)
-    ap.add_argument("--only", type=str, default="c2,c3,c4,c5,gibbs,prep",
-                    help="workloads to run: c2 (headline stack), c3 (batched SRRF), c4, c5 (3B), "
+    ap.add_argument("--only", type=str, default="c2,c4,c5,gibbs,c3,prep",
+                    help="workloads to run, in this order: c2 (headline stack), c3 (batched SRRF), c4, c5 (3B), "
                          "gibbs (3B blink-state sampling), prep (image normalisation)")
     ap.add_argument("--c5-plain-blocks", type=int, default=100)
     ap.add_argument("--traffic", type=float, default=None,
@@ -724,27 +797,63 @@ def main():
         print("bench.py measures one GPU only; multi-GPU runs are not part of this build "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
-    only = set(args.only.split(","))
+    order = [w for w in args.only.split(",") if w]
     rec = {
         "metric": METRIC, "value": None, "unit": "SR-voxels/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded emitters, Markov blinking, Poisson / Gaussian noise; generated on the host)",
         "config": None, "roofline": None, "cpu_baseline": None, "workloads": {},
+        "passes": {"headline": {"warmup": args.warmup, "timed": args.steps},
+                   "other_workloads": {"warmup": args.aux_warmup, "timed": args.aux_steps}},
     }
-    if "c2" in only:
-        bench_stack(args, rec)
-    if "c3" in only:
-        rec["workloads"]["srrf_batched"] = bench_srrf_batched(args)
-    if "c4" in only:
-        rec["workloads"]["b3_single"] = bench_b3_single(args)
-    if "c5" in only:
-        rec["workloads"]["b3_batched"] = bench_b3_batched(args)
-    if "gibbs" in only:
-        rec["workloads"]["b3_gibbs"] = bench_b3_gibbs(args)
-    if "prep" in only:
-        rec["workloads"]["prep"] = bench_prep(args)
-    print(json.dumps(rec), flush=True)
+    lock = threading.Lock()
+    printed = []
+
+    def emit(reason=None):
+        with lock:
+            if printed:
+                return
+            printed.append(True)
+            rec["wall_s"] = time.perf_counter() - t_start
+            if reason:
+                rec["truncated"] = reason
+            print(json.dumps(rec), flush=True)
+
+    def on_deadline():
+        emit("deadline of %.0f s reached while a workload ran; what it had not finished is missing" % args.deadline)
+        os._exit(0)
+
+    watchdog = threading.Timer(max(1.0, args.deadline - (time.perf_counter() - t_start)), on_deadline)
+    watchdog.daemon = True
+    watchdog.start()
+    runners = {"c3": ("srrf_batched", bench_srrf_batched), "c4": ("b3_single", bench_b3_single),
+               "c5": ("b3_batched", bench_b3_batched), "gibbs": ("b3_gibbs", bench_b3_gibbs),
+               "prep": ("prep", bench_prep)}
+    try:
+        for w in order:
+            elapsed = time.perf_counter() - t_start
+            if w == "c2":
+                bench_stack(args, rec, lock)
+                log("c2 done at %.0f s" % (time.perf_counter() - t_start))
+                continue
+            name, fn = runners[w]
+            need = EST_S[w] + (C3_SERIES_S if w in ("c3", "prep") and not _C3 else 0)
+            if elapsed + need > args.budget:
+                with lock:
+                    rec["workloads"][name] = {"skipped": "budget", "elapsed_s": elapsed, "estimated_s": need,
+                                              "budget_s": args.budget}
+                log("%s skipped: %.0f s elapsed + %.0f s estimated > budget %.0f s" % (w, elapsed, need, args.budget))
+                continue
+            t0 = time.perf_counter()
+            res = fn(args)
+            res["wall_s"] = time.perf_counter() - t0
+            with lock:
+                rec["workloads"][name] = res
+            log("%s done in %.0f s (at %.0f s)" % (w, res["wall_s"], time.perf_counter() - t_start))
+    finally:
+        watchdog.cancel()
+        emit()
 
 
 if __name__ == "__main__":

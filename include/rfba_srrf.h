@@ -161,12 +161,20 @@ int rfba_srrf_batch_create(const rfba_srrf_block *blocks, int n, int dtype,
                            rfba_srrf_batch **plan);
 /* Run: one persistent launch per magnification group over all its blocks
  * (captured into a hipGraph on the first call when use_graph != 0, replayed
- * afterwards). */
+ * afterwards).  With kernel_ms != NULL the call waits for the launches,
+ * stores their time and returns the kernel's status; with kernel_ms == NULL
+ * it returns after enqueueing, and rfba_srrf_batch_status reports how the
+ * run ended. */
 int rfba_srrf_batch_run(rfba_srrf_batch *plan, void *stream, int use_graph,
                         float *kernel_ms);
-/* Reference form for comparison: one launch per block on the same stream. */
+/* Reference form for comparison: one launch per block on the same stream
+ * (kernel_ms as for rfba_srrf_batch_run). */
 int rfba_srrf_batch_run_per_block(rfba_srrf_batch *plan, void *stream,
                                   float *kernel_ms);
+/* Waits for the plan's last run and returns RFBA_OK, or RFBA_EDEVICE when a
+ * kernel of that run found a tile larger than the planned LDS extents (its
+ * outputs are then incomplete). */
+int rfba_srrf_batch_status(rfba_srrf_batch *plan);
 /* Autotuner (no reference counterpart): per magnification group, times
  * every feasible tile depth (4, 8, 16 SR voxels along z) on a sub-plan of
  * every sample_stride-th block of the group (0: about 32 blocks), keeps the

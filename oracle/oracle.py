@@ -137,6 +137,24 @@ def srrf_stack_box(series, conf, xyz, divisor, box_xyz0, box_shape_xyz, nthreads
     return out
 
 
+def gdotr_flips(series, conf, xyz, nthreads=None):
+    """Number of (SR voxel, frame, sample) triples whose converging/diverging
+    test differs between the FP32 arithmetic and its FP64 re-evaluation."""
+    L = lib()
+    L.oracle_gdotr_flips.argtypes = [POINTER(c_float), POINTER(c_double), POINTER(c_int), POINTER(type(conf)),
+                                     POINTER(c_float), POINTER(c_double), c_int, c_int]
+    L.oracle_gdotr_flips.restype = ctypes.c_longlong
+    a32 = np.ascontiguousarray(series, np.float32)
+    a64 = np.ascontiguousarray(series, np.float64)
+    r32 = np.ascontiguousarray(xyz, np.float32)
+    r64 = np.ascontiguousarray(xyz, np.float64)
+    n = L.oracle_gdotr_flips(_p(a32, c_float), _p(a64, c_double), _dims(a32.shape), ctypes.byref(conf),
+                             _p(r32, c_float), _p(r64, c_double), len(r32), nthreads or threads())
+    if n < 0:
+        raise ValueError("oracle rejected the input")
+    return int(n)
+
+
 def temporal(series, rad, conf, f64=False):
     """Temporal reduction alone on a given radiality series rad [T, ZM, YM, XM]."""
     dt, ct = (np.float64, c_double) if f64 else (np.float32, c_float)

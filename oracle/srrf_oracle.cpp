@@ -492,6 +492,71 @@ int oracle_srrf_stack_box_f32(const float *img, const int *dims, const SRRFConf 
     return stack_impl<float>(img, dims, conf, ring, n, divisor, out, nullptr, threads, box);
 }
 
+// Over every (SR voxel, frame, sample) of a stack: the number of samples
+// whose converging/diverging test (GdotR > 0 || isnan(GdotR),
+// kernel.cu:227-231) comes out differently in the FP32 arithmetic and in its
+// FP64 re-evaluation on the same input.  A flip moves that sample's term
+// from +Dk to -Dk (or back).  Returns the count, or -1 on invalid input.
+long long oracle_gdotr_flips(const float *img32, const double *img64, const int *dims,
+                             const SRRFConf *conf, const float *ring32, const double *ring64, int n,
+                             int threads) {
+    const int W = dims[0], H = dims[1], D = dims[2], T = dims[3], M = conf->Magnification;
+    if (W <= 0 || H <= 0 || D <= 0 || T <= 0 || M <= 0 || n <= 0) return -1;
+    const size_t N = (size_t)W * H * D, NT = N * T;
+    std::vector<float> g32(3 * NT);
+    std::vector<double> g64(3 * NT);
+    parallel_for(NT, threads, [&](size_t a, size_t b) {
+        for (size_t i = a; i < b; i++) {
+            int x = (int)(i % W), y = (int)((i / W) % H), z = (int)((i / ((size_t)W * H)) % D),
+                t = (int)(i / N);
+            grad_at(img32, x, y, z, t, W, H, D, *conf, g32[i], g32[NT + i], g32[2 * NT + i]);
+            grad_at(img64, x, y, z, t, W, H, D, *conf, g64[i], g64[NT + i], g64[2 * NT + i]);
+        }
+    });
+    const size_t NSR = N * (size_t)M * M * M;
+    const int WM = W * M, HM = H * M;
+    std::vector<long long> part(std::max(threads, 1) + 1, 0);
+    const int nth = std::max(threads, 1);
+    const size_t chunk = (NSR + nth - 1) / nth;
+    auto classify = [&](auto Xc, auto Yc, auto Zc, auto rx, auto ry, auto rz, int t, auto radius,
+                        const auto *g) {
+        using R = decltype(Xc);
+        R x0 = Xc + rx, y0 = Yc + ry, z0 = Zc + rz;
+        R Gx = interp<R>(x0, y0, z0, t, M, W, H, D, g);
+        R Gy = interp<R>(x0, y0, z0, t, M, W, H, D, g + NT);
+        R Gz = interp<R>(x0, y0, z0, t, M, W, H, D, g + 2 * NT);
+        R GMag = std::sqrt(Gx * Gx + Gy * Gy + Gz * Gz);
+        R GdotR = (Gx * rx + Gy * ry + Gz * rz) / (GMag * radius);
+        return GdotR > (R)0 || GdotR != GdotR;
+    };
+    std::vector<std::thread> th;
+    for (int k = 0; k < nth; k++) {
+        const size_t v0 = k * chunk, v1 = std::min(NSR, v0 + chunk);
+        if (v0 >= v1) break;
+        th.emplace_back([&, k, v0, v1] {
+            long long c = 0;
+            for (size_t v = v0; v < v1; v++) {
+                const int X = (int)(v % WM), Y = (int)((v / WM) % HM), Z = (int)(v / ((size_t)WM * HM));
+                for (int t = 0; t < T; t++)
+                    for (int s = 0; s < n; s++) {
+                        bool d32 = classify((float)X + 0.5f, (float)Y + 0.5f, (float)Z + 0.5f, ring32[3 * s],
+                                            ring32[3 * s + 1], ring32[3 * s + 2], t, conf->RingRadius,
+                                            g32.data());
+                        bool d64 = classify((double)X + 0.5, (double)Y + 0.5, (double)Z + 0.5, ring64[3 * s],
+                                            ring64[3 * s + 1], ring64[3 * s + 2], t,
+                                            (double)conf->RingRadius, g64.data());
+                        c += d32 != d64;
+                    }
+            }
+            part[k] = c;
+        });
+    }
+    for (auto &x : th) x.join();
+    long long total = 0;
+    for (long long c : part) total += c;
+    return total;
+}
+
 // Temporal reduction alone on a given radiality series rad[T][N] with
 // intensities from img (kernel.cu:291-555); dims are the low-res dims.
 int oracle_temporal_f32(const float *img, const float *rad, const int *dims, const SRRFConf *conf,

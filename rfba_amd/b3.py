@@ -534,23 +534,3 @@ def perturb(params, Q, seed, pos=0.5, rel_int=0.2):
     out[:, :, 2:5] += rng.uniform(-pos, pos, (Q, K, 3))
     out[:, :, 0] *= 1 + rng.uniform(-rel_int, rel_int, (Q, K))
     return out
-
-
-def smoke_check():
-    """One small block through the batched adjoint kernel vs the FP64 oracle."""
-    from oracle import b3oracle
-    A = blink_matrix()
-    m = make_model(A, (0.5, 0.5, 0.0), 100.0, zfactor=2.0, checkpoint=4)
-    data, params, states = synth_block((9, 8, 7), 12, 3, 5)
-    P = perturb(params, 2, 6)
-    st = states[None]
-    b = Batch([BlockData(data, P, st, 100.0)], m)
-    b.run(use_graph=False)
-    L, G = b.results(0)
-    b.close()
-    Lo, Go = b3oracle.model_eval(data, 100.0, m, P, st)
-    errL = float(np.max(np.abs(L - Lo) / np.abs(Lo)))
-    errG = float(np.max(np.abs(G - Go)) / np.max(np.abs(Go)))
-    if not (errL < 1e-6 and errG < 1e-4):
-        raise AssertionError("3B smoke: logL rel err %g, grad err %g" % (errL, errG))
-    print("smoke 3b: logL rel err %.3g, grad err %.3g (max-normalised)" % (errL, errG), flush=True)

@@ -49,7 +49,8 @@ def _hip(out, srcs, contract="off"):
 
 TARGETS = {
     "srrf": lambda: _hip(os.path.join(PKG, "librfba_srrf.so"),
-                         [os.path.join(CSRC, "srrf_kernel.hip"), os.path.join(CSRC, "srrf_api.cpp")]),
+                         [os.path.join(CSRC, "srrf_kernel.hip"), os.path.join(CSRC, "srrf_api.cpp"),
+                          os.path.join(CSRC, "srrf_cxx_linkage.cpp")]),
     # 3B: FP32 pixel terms checked against an FP64 oracle with a tolerance;
     # contraction to FMA is allowed there.
     "b3": lambda: _hip(os.path.join(PKG, "librfba_3b.so"),
@@ -82,13 +83,6 @@ def build(targets=None, verbose=True):
             f.result()
             if verbose:
                 print("built", n, flush=True)
-    # oracle/_ref: the reference's own 3B headers, compiled from
-    # /root/reference when that tree is present (this container only).
-    ref_make = os.path.join(ORACLE, "ref", "build_ref.py")
-    if os.path.exists(ref_make) and os.path.isdir("/root/reference"):
-        _run([sys.executable, ref_make])
-        if verbose:
-            print("built oracle/_ref", flush=True)
     return names
 
 

@@ -1022,6 +1022,13 @@ extern "C" int rfba_srrf_batch_run_per_block(rfba_srrf_batch *b, void *stream, f
     return RFBA_OK;
 }
 
+extern "C" int rfba_srrf_batch_status(rfba_srrf_batch *b) {
+    if (!b) return fail(RFBA_EINVAL, "plan is NULL");
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    HIP_TRY(hipEventSynchronize(b->ev1));
+    return batch_check(b);
+}
+
 // Autotuner: for each magnification group, each feasible tile depth
 // (geometry planned over all of the group's blocks, so that the LDS layout
 // is the one the full run uses) is timed on a one-group sub-plan of every

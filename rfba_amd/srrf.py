@@ -111,6 +111,8 @@ def lib() -> ctypes.CDLL:
         L.rfba_srrf_batch_run.restype = c_int
         L.rfba_srrf_batch_run_per_block.argtypes = [c_void_p, c_void_p, POINTER(c_float)]
         L.rfba_srrf_batch_run_per_block.restype = c_int
+        L.rfba_srrf_batch_status.argtypes = [c_void_p]
+        L.rfba_srrf_batch_status.restype = c_int
         L.rfba_srrf_batch_autotune.argtypes = [c_void_p, c_void_p, c_int, POINTER(c_int), POINTER(c_float)]
         L.rfba_srrf_batch_autotune.restype = c_int
         L.rfba_srrf_batch_tile_depth.argtypes = [c_void_p]
@@ -363,6 +365,14 @@ class Batch:
         ms = c_float(0)
         _check(lib().rfba_srrf_batch_run_per_block(self.plan, None, ctypes.byref(ms)))
         return ms.value
+
+    def run_async(self, use_graph: bool = True):
+        """Enqueue a run and return at once; status() waits for it."""
+        _check(lib().rfba_srrf_batch_run(self.plan, None, int(use_graph), None))
+
+    def status(self):
+        """Wait for the last run; raises if a kernel flagged an error."""
+        _check(lib().rfba_srrf_batch_status(self.plan))
 
     def autotune(self, sample_stride: int = 0):
         """Per magnification group, time tile depths 4/8/16 on a sample of its

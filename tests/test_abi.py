@@ -122,3 +122,27 @@ def test_invalid_arguments_rejected_on_host():
         srrf.directions(5, 0.5)
     with pytest.raises(srrf.SRRFError):
         srrf.ring(0, 0.5)
+
+
+def test_cxx_linkage_entry_points(tmp_path):
+    # GradientCalc.h:4-5 declares both entry points with C++ linkage, and the
+    # callers import them so (multispot5_place_methods.cc:48-49,
+    # multispot5.cc:77): the library exports the mangled names too, and a
+    # caller compiled from that declaration (dllimport removed) links and runs.
+    syms = exported(_lib.lib_path("rfba_srrf"))
+    assert {"_Z16SRRFGradientCalcPfPiS_", "_Z10SRRFGetMagv", "SRRFGradientCalc", "SRRFGetMag",
+            "SRRFGradientCalcGPU"} <= syms
+    src = tmp_path / "caller.cpp"
+    src.write_text("#include <cstdio>\n"
+                   "int SRRFGradientCalc(float * inImage, int * imageDim, float * outImage);\n"
+                   "int SRRFGetMag();\n"
+                   "int main() { std::printf(\"%d\\n\", SRRFGetMag());"
+                   " return SRRFGradientCalc == nullptr; }\n")
+    exe = tmp_path / "caller"
+    libdir = os.path.dirname(_lib.lib_path("rfba_srrf"))
+    r = subprocess.run(["g++", str(src), "-o", str(exe), "-L" + libdir, "-l:librfba_srrf.so",
+                        "-Wl,-rpath," + libdir], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    (tmp_path / "SRRF.ini").write_text("[SRRF]\nMagnification=3\n")
+    r = subprocess.run([str(exe)], capture_output=True, text=True, cwd=str(tmp_path))
+    assert r.returncode == 0 and r.stdout.strip() == "3", (r.returncode, r.stdout, r.stderr)

@@ -160,11 +160,14 @@ def test_batched_matches_single_stacks(gpu):
         sub = src[:frames, org[2]:org[2] + dims[2], org[1]:org[1] + dims[1], org[0]:org[0] + dims[0]]
         refs.append(srrf.srrf_stack(np.ascontiguousarray(sub), c.replace(Magnification=mag), smp))
     b = srrf.Batch(blocks, srrf.RFBA_U16, c, smp)
-    for mode in ("graph", "plain", "per_block", "graph"):
+    for mode in ("graph", "plain", "per_block", "graph", "async"):
         for bl in blocks:
             bl.out.zero()
         if mode == "per_block":
             b.run_per_block()
+        elif mode == "async":
+            b.run_async(use_graph=True)
+            b.status()
         else:
             b.run(use_graph=(mode == "graph"))
         for i, bl in enumerate(blocks):
