@@ -758,7 +758,7 @@ def bench_prep(args):
 # Estimated wall seconds of each auxiliary workload at one warm-up and two
 # timed passes, CPU legs included (measured on MI355X with 16 host threads,
 # rounded up).  A workload starts only if it fits the remaining budget.
-EST_S = {"c4": 25, "c5": 70, "gibbs": 60, "c3": 150, "prep": 45}
+EST_S = {"c4": 10, "c5": 25, "gibbs": 20, "c3": 100, "prep": 15}
 C3_SERIES_S = 35  # generating the 512x512x128 T=120 series shared by c3 and prep
 
 

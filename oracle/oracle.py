@@ -6,6 +6,7 @@ function restates (reference file:line).
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 import threading
@@ -20,36 +21,54 @@ _L = None
 LOCK = threading.RLock()
 
 
+def _load(path):
+    if not os.path.exists(path):
+        raise ImportError("oracle not built: run `python -m rfba_amd.build oracle`")
+    L = ctypes.CDLL(path)
+    fp, dp, ip = POINTER(c_float), POINTER(c_double), POINTER(c_int)
+    from rfba_amd.srrf import SRRFConf
+    cp = POINTER(SRRFConf)
+    L.oracle_build_ring_f32.argtypes = [c_int, c_float, fp]
+    L.oracle_build_ring_f64.argtypes = [c_int, c_double, dp]
+    L.oracle_cubic_f32.argtypes = [c_float]
+    L.oracle_cubic_f32.restype = c_float
+    L.oracle_cubic_f64.argtypes = [c_double]
+    L.oracle_cubic_f64.restype = c_double
+    L.oracle_gradient_f32.argtypes = [fp, ip, cp, fp, fp, fp]
+    L.oracle_interp_f32.argtypes = [fp, ip, c_int, c_int, fp, c_int, fp]
+    L.oracle_interp_f64.argtypes = [dp, ip, c_int, c_int, dp, c_int, dp]
+    L.oracle_srrf_stack_f32.argtypes = [fp, ip, cp, fp, c_int, c_float, fp, fp, c_int]
+    L.oracle_srrf_stack_f64.argtypes = [dp, ip, cp, dp, c_int, c_double, dp, dp, c_int]
+    L.oracle_srrf_stack_box_f32.argtypes = [fp, ip, cp, fp, c_int, c_float, ip, fp, c_int]
+    L.oracle_temporal_f32.argtypes = [fp, fp, ip, cp, fp]
+    L.oracle_temporal_f64.argtypes = [dp, dp, ip, cp, dp]
+    L.oracle_srrf_tiled_f32.argtypes = [fp, ip, cp, fp, c_int]
+    L.oracle_hardware_threads.restype = c_int
+    return L
+
+
 def lib():
     global _L
     if _L is None:
         with LOCK:
             if _L is None:
-                path = os.path.join(HERE, "liboracle.so")
-                if not os.path.exists(path):
-                    raise ImportError("oracle not built: run `python -m rfba_amd.build oracle`")
-                L = ctypes.CDLL(path)
-                fp, dp, ip = POINTER(c_float), POINTER(c_double), POINTER(c_int)
-                from rfba_amd.srrf import SRRFConf
-                cp = POINTER(SRRFConf)
-                L.oracle_build_ring_f32.argtypes = [c_int, c_float, fp]
-                L.oracle_build_ring_f64.argtypes = [c_int, c_double, dp]
-                L.oracle_cubic_f32.argtypes = [c_float]
-                L.oracle_cubic_f32.restype = c_float
-                L.oracle_cubic_f64.argtypes = [c_double]
-                L.oracle_cubic_f64.restype = c_double
-                L.oracle_gradient_f32.argtypes = [fp, ip, cp, fp, fp, fp]
-                L.oracle_interp_f32.argtypes = [fp, ip, c_int, c_int, fp, c_int, fp]
-                L.oracle_interp_f64.argtypes = [dp, ip, c_int, c_int, dp, c_int, dp]
-                L.oracle_srrf_stack_f32.argtypes = [fp, ip, cp, fp, c_int, c_float, fp, fp, c_int]
-                L.oracle_srrf_stack_f64.argtypes = [dp, ip, cp, dp, c_int, c_double, dp, dp, c_int]
-                L.oracle_srrf_stack_box_f32.argtypes = [fp, ip, cp, fp, c_int, c_float, ip, fp, c_int]
-                L.oracle_temporal_f32.argtypes = [fp, fp, ip, cp, fp]
-                L.oracle_temporal_f64.argtypes = [dp, dp, ip, cp, dp]
-                L.oracle_srrf_tiled_f32.argtypes = [fp, ip, cp, fp, c_int]
-                L.oracle_hardware_threads.restype = c_int
-                _L = L
+                _L = _load(os.path.join(HERE, "liboracle.so"))
     return _L
+
+
+@contextlib.contextmanager
+def using(path):
+    """Route this module's SRRF calls to another build of the oracle (a
+    deliberately broken copy compiled by a test) inside the block."""
+    global _L
+    with LOCK:
+        prev = lib()
+        _L = _load(path)
+    try:
+        yield
+    finally:
+        with LOCK:
+            _L = prev
 
 
 def _p(a, t):
