@@ -120,6 +120,10 @@ int rfba3b_batch_run(rfba3b_batch *plan, void *stream, int use_graph, float *ker
 /* Plain form for comparison, same kernels and results: one launch sequence
  * per block (per_param_set = 0) or per (block, parameter set) (1). */
 int rfba3b_batch_run_per_block(rfba3b_batch *plan, void *stream, int per_param_set, float *kernel_ms);
+/* Number of memory chunks the plan's residual planes are split into (each
+ * chunk's planes fit the plane budget: RFBA3B_PLANE_BUDGET_MB in the
+ * environment at rfba3b_batch_create, default 16384 MiB). */
+int rfba3b_batch_chunks(const rfba3b_batch *plan);
 void rfba3b_batch_destroy(rfba3b_batch *plan);
 
 /* ---- Gibbs sampling of the blink states --------------------------------

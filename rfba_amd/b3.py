@@ -107,6 +107,8 @@ def lib():
         L.rfba3b_batch_run.restype = c_int
         L.rfba3b_batch_run_per_block.argtypes = [c_void_p, c_void_p, c_int, POINTER(c_float)]
         L.rfba3b_batch_run_per_block.restype = c_int
+        L.rfba3b_batch_chunks.argtypes = [c_void_p]
+        L.rfba3b_batch_chunks.restype = c_int
         L.rfba3b_batch_destroy.argtypes = [c_void_p]
         L.rfba3b_batch_destroy.restype = None
         L.rfba3b_gibbs_create.argtypes = [POINTER(_GibbsChain), c_int, POINTER(Model), POINTER(c_void_p),
@@ -263,6 +265,11 @@ class Batch:
         ms = c_float(0)
         _check(lib().rfba3b_batch_run_per_block(self.plan, None, int(per_param_set), ctypes.byref(ms)))
         return ms.value
+
+    @property
+    def chunks(self) -> int:
+        """Memory chunks of the plan (RFBA3B_PLANE_BUDGET_MB at creation)."""
+        return lib().rfba3b_batch_chunks(self.plan)
 
     def results(self, i):
         dl, dg, (Q, K, S) = self.out[i]

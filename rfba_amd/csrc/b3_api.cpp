@@ -475,6 +475,11 @@ extern "C" int rfba3b_batch_run_per_block(rfba3b_batch *b, void *stream, int per
     return 0;
 }
 
+extern "C" int rfba3b_batch_chunks(const rfba3b_batch *b) {
+    if (!b) return fail(-1, "plan is NULL");
+    return (int)b->chunks.size();
+}
+
 extern "C" void rfba3b_batch_destroy(rfba3b_batch *b) {
     std::lock_guard<std::mutex> lk(g_mu);
     batch_free(b);

@@ -262,3 +262,45 @@ def test_short_series_and_single_spot(gpu, T, K):
     (L, G), = run_batch([b3.BlockData(data, P, states[None], 100.0)], m)
     Lo, Go = bo.model_eval(data, 100.0, m, P, states[None])
     compare(L, G, Lo, Go, "T=%d K=%d" % (T, K))
+
+
+def test_multi_chunk_plan_matches_single_chunk_and_oracle(gpu, monkeypatch):
+    # The residual planes of a plan are split into memory chunks when they
+    # exceed the plane budget (RFBA3B_PLANE_BUDGET_MB); chunk-local plane,
+    # item and correlation offsets restart in every chunk.  A budget of 1 MiB
+    # over uneven blocks of 0.1-0.55 MiB of planes each gives three chunks of
+    # two or three blocks; results must equal the single-chunk plan bit for bit
+    # and the oracle within the bars, in every launch form.
+    m = model(checkpoint=11)
+    rng = np.random.default_rng(17)
+    blocks = []
+    for i in range(7):
+        dims = tuple(int(v) for v in rng.integers(7, 14, 3))
+        T = int(rng.integers(20, 41))
+        K = int(rng.integers(3, 12))
+        data, params, states = b3.synth_block(dims, T, K, 300 + i)
+        S = int(rng.integers(1, 3))
+        st = np.stack([np.roll(states, 2 * s, axis=1) for s in range(S)])
+        blocks.append(b3.BlockData(data, b3.perturb(params, int(rng.integers(1, 4)), 400 + i), st, 100.0))
+    monkeypatch.delenv("RFBA3B_PLANE_BUDGET_MB", raising=False)
+    one = b3.Batch(blocks, m)
+    assert one.chunks == 1
+    one.run(use_graph=False)
+    ref = [one.results(i) for i in range(len(blocks))]
+    one.close()
+    monkeypatch.setenv("RFBA3B_PLANE_BUDGET_MB", "1")
+    bt = b3.Batch(blocks, m)
+    print("chunks at 1 MiB: %d" % bt.chunks)
+    assert bt.chunks >= 3
+    for mode in ("plain", "graph", "per_block", "per_param_set", "graph2"):
+        if mode.startswith("per_"):
+            bt.run_per_block(per_param_set=(mode == "per_param_set"))
+        else:
+            bt.run(use_graph=mode.startswith("graph"))
+        for i in range(len(blocks)):
+            L, G = bt.results(i)
+            assert np.array_equal(L, ref[i][0]) and np.array_equal(G, ref[i][1]), (mode, i)
+    bt.close()
+    for b, (L, G) in zip(blocks, ref):
+        Lo, Go = bo.model_eval(b.data, 100.0, m, b.params, b.states)
+        compare(L, G, Lo, Go, "chunked block %s S=%d" % (b.data.shape, b.states.shape[0]))

@@ -264,3 +264,69 @@ def test_float_input_with_nan_and_inf(gpu, method, poscon):
     o = oracle.srrf_stack(s, c, smp.xyz, smp.divisor)
     assert np.array_equal(g, o, equal_nan=True)
     print("method %d: %d of %d outputs NaN" % (method, np.isnan(o).sum(), o.size))
+
+
+def _c3_blocks(src, specs):
+    """Blocks (origin, dims, frames, mag) of a source series, with their
+    device outputs."""
+    d_src = srrf.DeviceBuffer.from_array(src)
+    blocks = []
+    for org, dims, frames, mag in specs:
+        out = srrf.DeviceBuffer(dims[0] * dims[1] * dims[2] * mag ** 3 * 4)
+        blocks.append(srrf.Block(d_src, src.shape, org, dims, frames, mag, out))
+    return d_src, blocks
+
+
+def _boxes(block):
+    """Three 16^3 SR boxes of a block: low corner, centre, high corner."""
+    X, Y, Z = (d * block.mag for d in block.dims_xyz)
+    n = [min(16, v) for v in (X, Y, Z)]
+    lo = (0, 0, 0)
+    mid = tuple(max(0, (v - k) // 2) for v, k in zip((X, Y, Z), n))
+    hi = tuple(v - k for v, k in zip((X, Y, Z), n))
+    return [lo, mid, hi], n
+
+
+@pytest.mark.parametrize("method", ["tra", "trac"])
+def test_c3_shaped_batch_vs_oracle(gpu, method):
+    # Blocks shaped like the batched workload: 29^3 doSplit blocks at M = 2,
+    # 3, 4, a 29 x 29 x 13 z-edge block, an 8^3 extra stack and uneven ones,
+    # frame counts 30-200; TRA with 6 axis directions and intensity
+    # weighting (TRAC order 2 at T = 200 for the series scratch).  The
+    # autotuned batched plan is compared, box by box, with the FP32 oracle
+    # evaluated on each block as its own stack.
+    src = series((200, 48, 60, 60), 40, 33)
+    if method == "tra":
+        specs = [((0, 0, 0), (29, 29, 29), 120, 2), ((26, 26, 0), (29, 29, 29), 30, 3),
+                 ((31, 5, 19), (29, 29, 29), 200, 4), ((3, 29, 35), (29, 29, 13), 120, 3),
+                 ((50, 50, 40), (8, 8, 8), 57, 2), ((40, 11, 2), (17, 23, 21), 163, 4)]
+        c = conf(Method=1, IntensityW=1)
+    else:
+        specs = [((5, 7, 9), (29, 29, 29), 200, 2), ((30, 30, 19), (29, 29, 29), 200, 4),
+                 ((50, 44, 40), (8, 9, 8), 200, 3)]
+        c = conf(Method=3, ITC=0, TRACOrder=2, IntensityW=1)
+    smp = srrf.directions(6, 0.5)
+    d_src, blocks = _c3_blocks(src, specs)
+    b = srrf.Batch(blocks, srrf.RFBA_U16, c, smp)
+    tz, ms = b.autotune()
+    print("%s: groups %s, tuned depth %d" % (method, b.groups, tz))
+    for bl in blocks:
+        bl.out.zero()
+    b.run(use_graph=True)
+    nbox = 0
+    for bl in blocks:
+        (ox, oy, oz), (dx, dy, dz) = bl.origin_xyz, bl.dims_xyz
+        M = bl.mag
+        got = bl.out.to_array((dz * M, dy * M, dx * M), np.float32)
+        sub = np.ascontiguousarray(src[:bl.frames, oz:oz + dz, oy:oy + dy, ox:ox + dx]).astype(np.float32)
+        starts, n = _boxes(bl)
+        cm = c.replace(Magnification=M)
+        for x0, y0, z0 in starts:
+            o = oracle.srrf_stack_box(sub, cm, smp.xyz, smp.divisor, (x0, y0, z0), tuple(n))
+            parity(got[z0:z0 + n[2], y0:y0 + n[1], x0:x0 + n[0]], o,
+                   "%s block %s+%s T=%d M=%d box %s" % (method, bl.origin_xyz, bl.dims_xyz, bl.frames, M,
+                                                         (x0, y0, z0)))
+            nbox += 1
+    b.close()
+    d_src.close()
+    assert nbox == 3 * len(blocks)
