@@ -62,6 +62,29 @@ int rfba3b_neg_grad_sampled(const int *xyz, int P, int T, const float *data, con
                             const double *phi, int n, const rfba3b_model *model, double *grad,
                             double *logL);
 
+/* Resident form of the same functor, created once per optimised spot as the
+ * reference builds SampledBackgroundData (multispot5.cc:538-575) and then
+ * called by ConjugateGradientOnly<5> at least twice per iteration
+ * (conjugate_gradient_only.h:92, 136, 180, 207).  Host inputs, as the
+ * reference's: xyz [P][3] (pixels), data [T][P] (pixel_intensities) and
+ * bg [S][T][P] (sample_intensities_without_spot), all FP64.  Creation forms
+ * each residual data - bg_s in FP64, rounds it once to FP32 and keeps it on
+ * the device with its per-frame FP64 sum of squares; max_candidates bounds
+ * n of every evaluation (<= 4096, as T and S).  The handle owns its device
+ * buffers. */
+typedef struct rfba3b_functor rfba3b_functor;  /* opaque handle */
+int rfba3b_functor_create(const int *xyz, int P, int T, const double *data, const double *bg, int S,
+                          const rfba3b_model *model, int max_candidates, rfba3b_functor **handle);
+/* operator()(phi) for n <= max_candidates candidate vectors phi [n][5]:
+ * grad [n][5] as rfba3b_neg_grad_sampled (NaN rows for I <= 0 or
+ * sigma <= 0), logL [n][S] (may be NULL).  Allocates nothing: two kernel
+ * launches on `stream` and copies through the handle's pinned staging
+ * buffer; returns when the results are on the host.  Calls on one handle
+ * are serialised. */
+int rfba3b_functor_eval(rfba3b_functor *handle, const double *phi, int n, double *grad, double *logL,
+                        void *stream);
+void rfba3b_functor_destroy(rfba3b_functor *handle);
+
 /* Forward messages delta_t(i) (forward_algorithm.h:344-382) of one spot over
  * one background, for forward filtering / backward sampling.
  * Host buffers; delta [T][3]; returns logL in *logL (may be NULL). */

@@ -92,6 +92,13 @@ def lib():
         ip, fp, dp = POINTER(c_int), POINTER(c_float), POINTER(c_double)
         L.rfba3b_neg_grad_sampled.argtypes = [ip, c_int, c_int, fp, fp, c_int, dp, c_int, POINTER(Model), dp, dp]
         L.rfba3b_neg_grad_sampled.restype = c_int
+        L.rfba3b_functor_create.argtypes = [ip, c_int, c_int, dp, dp, c_int, POINTER(Model), c_int,
+                                            POINTER(c_void_p)]
+        L.rfba3b_functor_create.restype = c_int
+        L.rfba3b_functor_eval.argtypes = [c_void_p, dp, c_int, dp, dp, c_void_p]
+        L.rfba3b_functor_eval.restype = c_int
+        L.rfba3b_functor_destroy.argtypes = [c_void_p]
+        L.rfba3b_functor_destroy.restype = None
         L.rfba3b_forward_messages.argtypes = [ip, c_int, c_int, fp, fp, dp, POINTER(Model), dp, dp]
         L.rfba3b_forward_messages.restype = c_int
         L.rfba3b_ffbs.argtypes = [ip, c_int, c_int, fp, fp, dp, POINTER(Model), dp, POINTER(c_uint8), dp, dp]
@@ -159,6 +166,45 @@ def neg_grad_sampled(xyz, data, bg, phi, model: Model):
     _check(lib().rfba3b_neg_grad_sampled(_p(xyz, c_int), P, T, _p(data, c_float), _p(bg, c_float), S,
                                          _p(phi, c_double), n, ctypes.byref(model), _p(g, c_double), _p(L, c_double)))
     return g, L
+
+
+class Functor:
+    """SpotNegProbabilityDiffWithSampledBackground kept on the device
+    (rfba3b_functor_*): created once from the pixel list, the data [T, P]
+    and the S background samples [S, T, P] (FP64), then evaluated at
+    candidate parameter vectors without further allocation."""
+
+    def __init__(self, xyz, data, bg, model: Model, max_candidates=1):
+        xyz = np.ascontiguousarray(xyz, np.int32)
+        data = np.ascontiguousarray(data, np.float64)
+        bg = np.ascontiguousarray(bg, np.float64)
+        S, T, P = bg.shape
+        assert data.shape == (T, P) and xyz.shape == (P, 3)
+        self.S, self.nmax = S, max_candidates
+        self.model = model
+        self.h = c_void_p()
+        _check(lib().rfba3b_functor_create(_p(xyz, c_int), P, T, _p(data, c_double), _p(bg, c_double), S,
+                                           ctypes.byref(model), max_candidates, ctypes.byref(self.h)))
+
+    def __call__(self, phi):
+        """grad [n, 5] (NaN rows for invalid candidates) and logL [n, S]."""
+        phi = np.ascontiguousarray(np.atleast_2d(phi), np.float64)
+        n = phi.shape[0]
+        g = np.empty((n, 5))
+        L = np.empty((n, self.S))
+        _check(lib().rfba3b_functor_eval(self.h, _p(phi, c_double), n, _p(g, c_double), _p(L, c_double), None))
+        return g, L
+
+    def close(self):
+        if self.h:
+            lib().rfba3b_functor_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def forward_messages(xyz, data, bg, phi, model: Model):

@@ -652,6 +652,262 @@ extern "C" int rfba3b_forward_messages(const int *xyz, int P, int T, const float
 }
 
 // ---------------------------------------------------------------------------
+// Resident CG functor: SampledBackgroundData (multispot5.cc:538-575) kept on
+// the device, evaluated by the batched correlation and HMM kernels.
+//
+// Per background sample s the residual data - bg_s is formed at creation in
+// FP64 and rounded once to FP32 (one plane [T][box], zero outside the
+// region), with its per-frame sum of squares S0_s(t) in FP64.  A candidate
+// spot is then an item (spot k, sample s) of one virtual block whose states
+// are all "off": the correlation kernel's off-state sums are exactly the
+// terms of spot_with_background.hh:127-137 with e = R - mu
+// (Soff = S0, D = 2C - Q, diff = G - H), and the HMM kernel gives logL and
+// its gradient.  An evaluation uploads the candidates and the task lists
+// into buffers sized at creation, launches two kernels and reads back
+// [n][S] values: it allocates nothing.
+
+struct rfba3b_functor {
+    rfba3b_model model;
+    int P = 0, T = 0, S = 0, nmax = 0, X = 0, Y = 0, Z = 0, ckslots = 0;
+    size_t lds_corr = 0;
+    DevModel *d_model = nullptr;
+    DevBlock *d_block = nullptr;
+    PlaneRef *d_planes = nullptr;
+    CorrTask *d_corr = nullptr;
+    Item *d_items = nullptr;
+    float *d_R = nullptr;
+    uint8_t *d_states = nullptr, *d_mask = nullptr;
+    double *d_S0 = nullptr, *d_em = nullptr, *d_ck = nullptr, *d_params = nullptr, *d_logL = nullptr,
+           *d_grad = nullptr;
+    // pinned staging: params [nmax][5] | logL [nmax][S] | grad [nmax][S][5] |
+    // correlation tasks | items
+    unsigned char *h_stage = nullptr;
+    size_t o_logL = 0, o_grad = 0, o_corr = 0, o_items = 0;
+    std::mutex mu;
+};
+
+static void functor_free(rfba3b_functor *f) {
+    if (!f) return;
+    hipFree(f->d_model); hipFree(f->d_block); hipFree(f->d_planes); hipFree(f->d_corr); hipFree(f->d_items);
+    hipFree(f->d_R); hipFree(f->d_states); hipFree(f->d_mask); hipFree(f->d_S0); hipFree(f->d_em);
+    hipFree(f->d_ck); hipFree(f->d_params); hipFree(f->d_logL); hipFree(f->d_grad);
+    if (f->h_stage) hipHostFree(f->h_stage);
+    delete f;
+}
+
+extern "C" int rfba3b_functor_create(const int *xyz, int P, int T, const double *data, const double *bg, int S,
+                                     const rfba3b_model *model, int max_candidates, rfba3b_functor **out) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc;
+    if (!out) return fail(-1, "handle pointer is NULL");
+    *out = nullptr;
+    if ((rc = check_model(model)) || (rc = dev_init())) return rc;
+    if (!xyz || !data || !bg || P <= 0 || T <= 0 || S <= 0 || max_candidates <= 0)
+        return fail(-1, "invalid arguments");
+    if (T > 4096 || max_candidates > 4096 || S > 4096) return fail(-1, "T, S and max_candidates must be <= 4096");
+    int lo[3] = {xyz[0], xyz[1], xyz[2]}, hi[3] = {xyz[0], xyz[1], xyz[2]};
+    for (int p = 0; p < P; p++)
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::min(lo[a], xyz[3 * p + a]);
+            hi[a] = std::max(hi[a], xyz[3 * p + a]);
+        }
+    const long long Xl = (long long)hi[0] - lo[0] + 1, Yl = (long long)hi[1] - lo[1] + 1,
+                    Zl = (long long)hi[2] - lo[2] + 1;
+    if (Xl + Yl + Zl > 1024 || Xl * Yl * Zl > (1 << 24)) return fail(-1, "pixel region too large");
+    const int X = (int)Xl, Y = (int)Yl, Z = (int)Zl;
+    const size_t Pd = (size_t)X * Y * Z;
+    std::vector<uint8_t> mask(Pd, 0);
+    std::vector<size_t> map(P);
+    for (int p = 0; p < P; p++) {
+        const size_t d = ((size_t)(xyz[3 * p + 2] - lo[2]) * Y + (xyz[3 * p + 1] - lo[1])) * X + (xyz[3 * p] - lo[0]);
+        if (mask[d]) return fail(-1, "duplicate pixel in the region");
+        mask[d] = 1;
+        map[p] = d;
+    }
+    const bool full = (size_t)P == Pd;
+    // residual planes and their per-frame sums of squares
+    std::vector<float> R((size_t)S * T * Pd, 0.0f);
+    std::vector<double> S0((size_t)S * T, 0.0);
+    for (int s = 0; s < S; s++)
+        for (int t = 0; t < T; t++) {
+            float *row = R.data() + ((size_t)s * T + t) * Pd;
+            const double *dr = data + (size_t)t * P, *br = bg + ((size_t)s * T + t) * P;
+            for (int p = 0; p < P; p++) row[map[p]] = (float)(dr[p] - br[p]);
+            double acc = 0.0;
+            for (size_t d = 0; d < Pd; d++) acc += (double)row[d] * (double)row[d];
+            S0[(size_t)s * T + t] = acc;
+        }
+
+    rfba3b_functor *f = new rfba3b_functor();
+    f->model = *model;
+    f->P = P; f->T = T; f->S = S; f->nmax = max_candidates;
+    f->X = X; f->Y = Y; f->Z = Z;
+    const int ck = ckpt_of(model, T);
+    f->ckslots = 6 * ((T + ck - 1) / ck) + 4 * ck;
+    f->lds_corr = corr_lds(true, X + Y + Z);
+    if (f->lds_corr > 160 * 1024) {
+        functor_free(f);
+        return fail(-1, "region edges too long for the correlation kernel");
+    }
+    const size_t NI = (size_t)S * max_candidates;
+    const size_t ncorr = (size_t)S * ((max_candidates + kCorrSpots - 1) / kCorrSpots);
+    f->o_logL = (size_t)max_candidates * 5 * sizeof(double);
+    f->o_grad = f->o_logL + NI * sizeof(double);
+    f->o_corr = f->o_grad + NI * 5 * sizeof(double);
+    f->o_items = f->o_corr + ncorr * sizeof(CorrTask);
+    const size_t stage = f->o_items + NI * sizeof(Item);
+
+    hipError_t e = hipSuccess;
+    auto al = [&](void **p, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc(p, std::max<size_t>(bytes, 16));
+    };
+    al((void **)&f->d_model, sizeof(DevModel));
+    al((void **)&f->d_block, sizeof(DevBlock));
+    al((void **)&f->d_planes, S * sizeof(PlaneRef));
+    al((void **)&f->d_corr, ncorr * sizeof(CorrTask));
+    al((void **)&f->d_items, NI * sizeof(Item));
+    al((void **)&f->d_R, R.size() * sizeof(float));
+    al((void **)&f->d_states, NI * (size_t)T);
+    if (!full) al((void **)&f->d_mask, Pd);
+    al((void **)&f->d_S0, S0.size() * sizeof(double));
+    al((void **)&f->d_em, NI * (size_t)T * 8 * sizeof(double));
+    al((void **)&f->d_ck, NI * (size_t)f->ckslots * sizeof(double));
+    al((void **)&f->d_params, (size_t)max_candidates * 5 * sizeof(double));
+    al((void **)&f->d_logL, NI * sizeof(double));
+    al((void **)&f->d_grad, NI * 5 * sizeof(double));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&f->h_stage, stage, hipHostMallocDefault);
+    DevBlock d;
+    memset(&d, 0, sizeof d);
+    d.X = X; d.Y = Y; d.Z = Z; d.T = T; d.K = max_candidates; d.Q = 1; d.S = S;
+    d.ox = lo[0]; d.oy = lo[1]; d.oz = lo[2];
+    d.npix = P;
+    d.params = f->d_params;
+    d.states = f->d_states;
+    d.mask = f->d_mask;
+    d.logL = f->d_logL;
+    d.grad = f->d_grad;
+    std::vector<PlaneRef> planes(S);
+    for (int s = 0; s < S; s++) planes[s] = {0, 0, s, 0, (long long)s * T * (long long)Pd};
+    const DevModel dm = dev_model(model);
+    auto up = [&](void *dst, const void *src, size_t bytes) {
+        if (e == hipSuccess && bytes) e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    };
+    up(f->d_model, &dm, sizeof dm);
+    up(f->d_block, &d, sizeof d);
+    up(f->d_planes, planes.data(), S * sizeof(PlaneRef));
+    up(f->d_R, R.data(), R.size() * sizeof(float));
+    up(f->d_S0, S0.data(), S0.size() * sizeof(double));
+    if (!full) up(f->d_mask, mask.data(), Pd);
+    if (e == hipSuccess) e = hipMemset(f->d_states, 1, NI * (size_t)T);  // every candidate off in every frame
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess && f->lds_corr > 64 * 1024)
+        e = hipFuncSetAttribute((const void *)corr_fn(true), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)f->lds_corr);
+    if (e != hipSuccess) {
+        const int code = e == hipErrorOutOfMemory ? -2 : -3;
+        functor_free(f);
+        return fail(code, "functor setup: %s", hipGetErrorString(e));
+    }
+    *out = f;
+    return 0;
+}
+
+extern "C" int rfba3b_functor_eval(rfba3b_functor *f, const double *phi, int n, double *grad, double *logL,
+                                   void *stream) {
+    if (!f) return fail(-1, "handle is NULL");
+    if (!phi || !grad || n <= 0) return fail(-1, "invalid arguments");
+    if (n > f->nmax) return fail(-1, "%d candidates exceed the handle's maximum of %d", n, f->nmax);
+    std::lock_guard<std::mutex> hl(f->mu);
+    const int S = f->S, T = f->T;
+    hipStream_t st = (hipStream_t)stream;
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    // multispot5.cc:618-619: invalid parameters give a NaN vector
+    double *sp = (double *)f->h_stage;
+    int idx[4096];
+    int m = 0;
+    for (int i = 0; i < n; i++) {
+        for (int c = 0; c < 5; c++) grad[5 * i + c] = nan;
+        if (logL)
+            for (int s = 0; s < S; s++) logL[(size_t)i * S + s] = nan;
+        if (phi[5 * i] > 0 && phi[5 * i + 1] > 0) {
+            for (int c = 0; c < 5; c++) sp[5 * m + c] = phi[5 * i + c];
+            idx[m++] = i;
+        }
+    }
+    if (m == 0) return 0;
+    CorrTask *ct = (CorrTask *)(f->h_stage + f->o_corr);
+    Item *it = (Item *)(f->h_stage + f->o_items);
+    const long long plane = (long long)T * f->X * f->Y * f->Z;
+    int nc = 0;
+    for (int s = 0; s < S; s++) {
+        for (int k0 = 0; k0 < m; k0 += kCorrSpots) ct[nc++] = {s, k0, std::min(kCorrSpots, m - k0), s * m + k0};
+        for (int k = 0; k < m; k++) it[s * m + k] = {0, 0, k, s, s * plane};
+    }
+    const int NI = S * m;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        HIP_TRY(hipMemcpyAsync(f->d_params, sp, (size_t)m * 5 * sizeof(double), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(f->d_corr, ct, nc * sizeof(CorrTask), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(f->d_items, it, NI * sizeof(Item), hipMemcpyHostToDevice, st));
+        CorrParams C;
+        memset(&C, 0, sizeof C);
+        C.blocks = f->d_block;
+        C.planes = f->d_planes;
+        C.tasks = f->d_corr;
+        C.ntasks = nc;
+        C.naxmax = f->X + f->Y + f->Z;
+        C.Tmax = T;
+        C.NI = NI;
+        C.R = f->d_R;
+        C.S0 = f->d_S0;
+        C.model = f->d_model;
+        C.em = f->d_em;
+        hipLaunchKernelGGL(corr_fn(true), dim3((nc + 7) / 8 * 8), dim3(256), f->lds_corr, st, C);
+        HIP_TRY(hipGetLastError());
+        HmmParams H;
+        memset(&H, 0, sizeof H);
+        H.blocks = f->d_block;
+        H.items = f->d_items;
+        H.lo = 0;
+        H.n = NI;
+        H.NI = NI;
+        H.ckpt = f->model.checkpoint;
+        H.ckslots = f->ckslots;
+        H.em = f->d_em;
+        H.ck = f->d_ck;
+        H.model = f->d_model;
+        hipLaunchKernelGGL(hmm_fn(true), dim3((NI + 255) / 256), dim3(256), 0, st, H);
+        HIP_TRY(hipGetLastError());
+        // outputs [k][s] of the first m candidates (the block's K is nmax)
+        HIP_TRY(hipMemcpyAsync(f->h_stage + f->o_logL, f->d_logL, (size_t)NI * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(f->h_stage + f->o_grad, f->d_grad, (size_t)NI * 5 * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipStreamSynchronize(st));
+    const double *L = (const double *)(f->h_stage + f->o_logL);
+    const double *G = (const double *)(f->h_stage + f->o_grad);
+    for (int c = 0; c < m; c++) {
+        const int i = idx[c];
+        double sum[5] = {0, 0, 0, 0, 0};
+        for (int s = 0; s < S; s++) {
+            for (int k = 0; k < 5; k++) sum[k] += G[((size_t)c * S + s) * 5 + k];
+            if (logL) logL[(size_t)i * S + s] = L[(size_t)c * S + s];
+        }
+        const double *p = phi + 5 * i;
+        const double prior[5] = {dlln(p[0], f->model.mu_brightness, f->model.sigma_brightness),
+                                 dlln(p[1], f->model.mu_blur, f->model.sigma_blur), 0, 0, 0};
+        for (int k = 0; k < 5; k++) grad[5 * i + k] = -(sum[k] / S + prior[k]);
+    }
+    return 0;
+}
+
+extern "C" void rfba3b_functor_destroy(rfba3b_functor *f) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    functor_free(f);
+}
+
+// ---------------------------------------------------------------------------
 // forward filtering / backward sampling
 
 static LogA log_a(const rfba3b_model *m) {

@@ -600,7 +600,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                 float v[NB];
 #pragma unroll
                 for (int n = 0; n < NB; n++) v[n] = 0.0f;
-                if (jb < nk && p < P) basis(bx, by, bz, v);
+                // a region given as a box and a mask: pixels outside it
+                // carry no spot value (their residuals are zero as well)
+                if (jb < nk && p < P && (!b.mask || b.mask[p])) basis(bx, by, bz, v);
 #pragma unroll
                 for (int n = 0; n < NB; n++) Bs[cb * BSTR + jb * NB + n] = v[n];
                 // every pass sees the same values; only the first pass's sums are used

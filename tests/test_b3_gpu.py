@@ -304,3 +304,53 @@ def test_multi_chunk_plan_matches_single_chunk_and_oracle(gpu, monkeypatch):
     for b, (L, G) in zip(blocks, ref):
         Lo, Go = bo.model_eval(b.data, 100.0, m, b.params, b.states)
         compare(L, G, Lo, Go, "chunked block %s S=%d" % (b.data.shape, b.states.shape[0]))
+
+
+def _functor_case(dims, T, K, S, seed, region="ball"):
+    """Data [T, P], backgrounds [S, T, P] (FP64: b0 + the other spots at
+    shifted sampled states) and the pixel list of a region around spot 0."""
+    data, params, states = b3.synth_block(dims, T, K, seed)
+    xyz_all = grid(*dims)
+    if region == "ball":
+        keep = np.linalg.norm((xyz_all - params[0, 2:5]) * [1, 1, 0.5], axis=1) < 4.5
+    else:
+        keep = np.ones(len(xyz_all), bool)
+    xyz = xyz_all[keep]
+    d2 = data.reshape(T, -1)[:, keep].astype(np.float64)
+    spots = [bo.spot(xyz, params[j], 2.0)[0] for j in range(K)]
+    bgs = np.empty((S, T, len(xyz)))
+    for s in range(S):
+        bgs[s] = 100.0
+        for j in range(1, K):
+            bgs[s] += (np.roll(states[j], 3 * s) == 0)[:, None] * spots[j][None]
+    return xyz, d2, bgs, params
+
+
+@pytest.mark.parametrize("region,S", [("ball", 3), ("box", 10)])
+def test_resident_functor_vs_oracle(gpu, region, S):
+    # SampledBackgroundData created once, then the CG functor called with
+    # changing candidate lists: FP64 data and backgrounds, the residual
+    # rounded once to FP32 at creation; against the FP64 oracle of
+    # multispot5.cc:616-669 at the batched form's bars
+    m = model()
+    xyz, d2, bgs, params = _functor_case((11, 10, 9), 24, 4, S, 81, region)
+    f = b3.Functor(xyz, d2, bgs, m, max_candidates=12)
+    rng = np.random.default_rng(5)
+    base = params[0]
+    cands = base + np.concatenate([[np.zeros(5)], rng.uniform(-0.3, 0.3, (10, 5)) * [300, 0.2, 1, 1, 1]])
+    calls = [cands[:1], np.concatenate([cands[1:4], [[0.0, 1.3, 3, 3, 3], [900, -1, 3, 3, 3]]]), cands[1:11],
+             cands[5:6]]
+    for c, phis in enumerate(calls):
+        g, L = f(phis)
+        for i, phi in enumerate(phis):
+            if not (phi[0] > 0 and phi[1] > 0):
+                assert np.isnan(g[i]).all() and np.isnan(L[i]).all()
+                continue
+            go, Lo = bo.neg_grad(xyz, d2, bgs, phi, m)
+            eg = np.linalg.norm(g[i] - go) / np.linalg.norm(go)
+            el = np.max(np.abs(L[i] - Lo) / np.abs(Lo))
+            assert eg <= GRAD_RTOL and el <= LOGL_RTOL, (c, i, eg, el)
+        print("%s S=%d call %d (n=%d): ok" % (region, S, c, len(phis)))
+    with pytest.raises(b3.B3Error):
+        f(np.tile(base, (13, 1)))  # more candidates than the handle holds
+    f.close()
