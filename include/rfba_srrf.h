@@ -132,6 +132,24 @@ int rfba_srrf_stack_device(const void *d_in, int dtype, const int dims[4],
                            const rfba_srrf_samples *samples, float *d_out,
                            void *stream, float *kernel_ms);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc).  Same
+ * inputs, outputs and results as rfba_srrf_stack_device, but the run's
+ * sample and tile tables, queue counter and TRAC / TRPPM frame series live
+ * in a static 256 MiB device array of the library (one per GPU, held until
+ * the process ends): no hipMalloc or hipFree, also on the first call.
+ * Limits: at most 1,048,576 SR tiles (16 x 16 x 4..16 voxels each), and a
+ * TRAC / TRPPM series area of 239.9 MiB, of which each concurrently running
+ * workgroup takes frames x 4..16 x 256 floats: when the occupancy grid needs
+ * more, fewer workgroups run (same results, longer time); sizes beyond
+ * either limit return RFBA_EINVAL before any device work.  Synchronous on
+ * `stream`; calls are serialised within the process (a second process has
+ * its own array), and the array keeps the last call's contents. */
+int rfba_srrf_stack_device_noalloc(const void *d_in, int dtype,
+                                   const int dims[4], const SRRFConf *conf,
+                                   const rfba_srrf_samples *samples,
+                                   float *d_out, void *stream);
+
 /* ---- batched form over uneven blocks -----------------------------------
  * A block is a sub-volume (origin, extent) of a source series with its own
  * magnification, or a whole stack; every block is computed exactly as if it

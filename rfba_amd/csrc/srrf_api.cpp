@@ -490,13 +490,13 @@ KernelFn pick_kernel(int dtype, int TZ, const DevSamples &s) {
     return s.r_pow2 ? pick_kernel_r<true>(dtype, TZ) : pick_kernel_r<false>(dtype, TZ);
 }
 
-int occupancy_grid(KernelFn fn, size_t lds, int ntiles, int &grid) {
+int occupancy_grid(KernelFn fn, size_t lds, int ntiles, int &grid, int ncu = 0) {
     if (lds > 64 * 1024)
         HIP_TRY(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
     HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, kThreads, lds));
     if (per_cu <= 0) return fail(RFBA_EINVAL, "kernel does not fit on a CU (LDS %zu)", lds);
-    grid = std::max(1, std::min(ntiles, per_cu * g_ctx.ncu));
+    grid = std::max(1, std::min(ntiles, per_cu * (ncu > 0 ? ncu : g_ctx.ncu)));
     return RFBA_OK;
 }
 
@@ -521,9 +521,28 @@ LaunchParams make_params(const SRRFConf &c, const Plan &pl) {
 
 bool needs_series(const SRRFConf &c) { return c.Method == 2 || c.Method == 3; }
 
+// Device buffers one run of the fused kernel uses besides its inputs and
+// outputs: the sample tables, the queue counter and error flag, the item
+// and tile tables, and the per-workgroup frame series of TRAC / TRPPM.
+// Either the context's cached allocations (grown on demand) or fixed
+// regions of the library's static scratch array (never grown: a series
+// area too small for the occupancy grid reduces the grid instead).
+struct RunBufs {
+    bool fixed = false;
+    int ncu = 0;  // CUs of the current device (0: the context's)
+    DevSamples *smp = nullptr;
+    int *ctl = nullptr;
+    DevItem *items = nullptr;
+    size_t items_cap = 0;
+    DevTile *tiles = nullptr;
+    size_t tiles_cap = 0;
+    float *series = nullptr;
+    size_t series_bytes = 0;
+};
+
 // Run items through the fused kernel (single queue).  Synchronous.
-int run_items(const std::vector<DevItem> &items, int dtype, const SRRFConf &c, const DevSamples &s,
-              hipStream_t stream, float *ms) {
+int run_items_with(RunBufs &B, const std::vector<DevItem> &items, int dtype, const SRRFConf &c,
+                   const DevSamples &s, hipStream_t stream, float *ms) {
     if (items.empty()) return RFBA_OK;
     Plan pl;
     const char *tz_env = getenv("RFBA_SRRF_TZ");  // tuning override: 8 or 16
@@ -533,39 +552,64 @@ int run_items(const std::vector<DevItem> &items, int dtype, const SRRFConf &c, c
     if (pl.tiles.empty()) return RFBA_OK;
     KernelFn fn = pick_kernel(dtype, pl.TZ, s);
     int grid = 0;
-    if ((rc = occupancy_grid(fn, pl.lds_bytes, (int)pl.tiles.size(), grid))) return rc;
-    if ((rc = ensure(g_ctx.d_items, g_ctx.items_cap, items.size() * sizeof(DevItem)))) return rc;
-    if ((rc = ensure(g_ctx.d_tiles, g_ctx.tiles_cap, pl.tiles.size() * sizeof(DevTile)))) return rc;
+    if ((rc = occupancy_grid(fn, pl.lds_bytes, (int)pl.tiles.size(), grid, B.ncu))) return rc;
+    const size_t items_bytes = items.size() * sizeof(DevItem), tiles_bytes = pl.tiles.size() * sizeof(DevTile);
+    if (B.fixed) {
+        if (items_bytes > B.items_cap) return fail(RFBA_EINVAL, "%zu items exceed the scratch table", items.size());
+        if (tiles_bytes > B.tiles_cap)
+            return fail(RFBA_EINVAL, "%zu SR tiles exceed the scratch tile table of %zu", pl.tiles.size(),
+                        B.tiles_cap / sizeof(DevTile));
+    } else {
+        if ((rc = ensure(g_ctx.d_items, g_ctx.items_cap, items_bytes))) return rc;
+        if ((rc = ensure(g_ctx.d_tiles, g_ctx.tiles_cap, tiles_bytes))) return rc;
+        B.items = g_ctx.d_items;
+        B.tiles = g_ctx.d_tiles;
+    }
     LaunchParams P = make_params(c, pl);
     if (needs_series(c)) {
         const size_t stride = (size_t)pl.Tmax * pl.TZ * kThreads;
-        if ((rc = ensure(g_ctx.d_scratch, g_ctx.scratch_bytes, stride * grid * sizeof(float)))) return rc;
-        P.scratch = g_ctx.d_scratch;
+        if (B.fixed) {
+            const size_t fit = B.series_bytes / (stride * sizeof(float));
+            if (fit < 1)
+                return fail(RFBA_EINVAL, "a frame series of %d frames does not fit the scratch series area",
+                            pl.Tmax);
+            grid = (int)std::min<size_t>((size_t)grid, fit);
+        } else {
+            if ((rc = ensure(g_ctx.d_scratch, g_ctx.scratch_bytes, stride * grid * sizeof(float)))) return rc;
+            B.series = g_ctx.d_scratch;
+        }
+        P.scratch = B.series;
         P.scratch_stride = (long long)stride;
     }
-    HIP_TRY(hipMemcpyAsync(g_ctx.d_smp, &s, sizeof(DevSamples), hipMemcpyHostToDevice, stream));
-    HIP_TRY(hipMemcpyAsync(g_ctx.d_items, items.data(), items.size() * sizeof(DevItem), hipMemcpyHostToDevice,
-                           stream));
-    HIP_TRY(hipMemcpyAsync(g_ctx.d_tiles, pl.tiles.data(), pl.tiles.size() * sizeof(DevTile),
-                           hipMemcpyHostToDevice, stream));
-    P.smp = g_ctx.d_smp;
-    P.items = g_ctx.d_items;
-    P.tiles = g_ctx.d_tiles;
+    HIP_TRY(hipMemcpyAsync(B.smp, &s, sizeof(DevSamples), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(B.items, items.data(), items_bytes, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(B.tiles, pl.tiles.data(), tiles_bytes, hipMemcpyHostToDevice, stream));
+    P.smp = B.smp;
+    P.items = B.items;
+    P.tiles = B.tiles;
     P.ntiles = (int)pl.tiles.size();
-    P.counter = g_ctx.d_ctl;
-    P.err = g_ctx.d_ctl + 1;
-    HIP_TRY(hipEventRecord(g_ctx.ev0, stream));
+    P.counter = B.ctl;
+    P.err = B.ctl + 1;
+    if (ms) HIP_TRY(hipEventRecord(g_ctx.ev0, stream));
     hipLaunchKernelGGL(srrf_reset, dim3(1), dim3(64), 0, stream, P.counter, P.err);
     HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(fn, dim3(grid), dim3(kThreads), pl.lds_bytes, stream, P);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(g_ctx.ev1, stream));
+    if (ms) HIP_TRY(hipEventRecord(g_ctx.ev1, stream));
     HIP_TRY(hipStreamSynchronize(stream));
     int ctl[2] = {0, 0};
-    HIP_TRY(hipMemcpy(ctl, g_ctx.d_ctl, sizeof ctl, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(ctl, B.ctl, sizeof ctl, hipMemcpyDeviceToHost));
     if (ctl[1]) return fail(RFBA_EDEVICE, "tile geometry exceeded the planned LDS extents");
     if (ms) HIP_TRY(hipEventElapsedTime(ms, g_ctx.ev0, g_ctx.ev1));
     return RFBA_OK;
+}
+
+int run_items(const std::vector<DevItem> &items, int dtype, const SRRFConf &c, const DevSamples &s,
+              hipStream_t stream, float *ms) {
+    RunBufs B;
+    B.smp = g_ctx.d_smp;
+    B.ctl = g_ctx.d_ctl;
+    return run_items_with(B, items, dtype, c, s, stream, ms);
 }
 
 DevItem stack_item(const void *d_in, const int dims[4], int M, float *d_out) {
@@ -641,6 +685,85 @@ extern "C" int rfba_srrf_stack_device(const void *d_in, int dtype, const int dim
                                       float *kernel_ms) {
     std::lock_guard<std::mutex> lk(g_ctx.mu);
     return stack_device_locked(d_in, dtype, dims, conf, samples, d_out, (hipStream_t)stream, kernel_ms);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free form of rfba_srrf_stack_device.  Everything the run needs
+// besides the caller's input and output lives in one file-scope device array
+// (each GPU holds its own copy until the process ends):
+//   [0, 256)           queue counter and error flag
+//   [256, 64 KiB+256)  sample tables (DevSamples)
+//   next 256 bytes     the stack's item
+//   next 16 MiB        SR tile table (up to 1,048,576 tiles)
+//   the rest           TRAC / TRPPM frame series, one slot per workgroup
+// Host code never names the array directly: its address on each device comes
+// from hipGetSymbolAddress at the first call with that device current.
+
+constexpr size_t kNaBytes = (size_t)256 << 20;
+constexpr size_t kNaCtl = 256, kNaSmp = 64 * 1024, kNaItem = 256, kNaTiles = (size_t)16 << 20;
+constexpr size_t kNaSeries = kNaBytes - kNaCtl - kNaSmp - kNaItem - kNaTiles;
+static_assert(sizeof(DevSamples) <= kNaSmp, "sample tables exceed their scratch region");
+static_assert(sizeof(DevItem) <= kNaItem, "item exceeds its scratch region");
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+constexpr int kNaMaxDev = 64;
+struct NaDevice {
+    bool ready = false;
+    int ncu = 0;
+    unsigned char *ptr = nullptr;  // device address of this device's scratch
+};
+NaDevice g_na[kNaMaxDev];
+
+int noalloc_device(NaDevice *&out) {
+    int n = 0, dev = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(RFBA_ENODEV, "no HIP device");
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNaMaxDev) return fail(RFBA_ENODEV, "device %d beyond the %d supported", dev, kNaMaxDev);
+    NaDevice &d = g_na[dev];
+    if (!d.ready) {
+        hipDeviceProp_t prop;
+        HIP_TRY(hipGetDeviceProperties(&prop, dev));
+        if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            return fail(RFBA_ENODEV, "device %s is not gfx950", prop.gcnArchName);
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail(RFBA_EDEVICE, "scratch address: %s", hipGetErrorString(e));
+        d.ncu = prop.multiProcessorCount;
+        d.ptr = (unsigned char *)p;
+        d.ready = true;
+    }
+    out = &d;
+    return RFBA_OK;
+}
+}  // namespace
+
+extern "C" int rfba_srrf_stack_device_noalloc(const void *d_in, int dtype, const int dims[4], const SRRFConf *conf,
+                                              const rfba_srrf_samples *samples, float *d_out, void *stream) {
+    std::lock_guard<std::mutex> lk(g_ctx.mu);
+    int rc;
+    if ((rc = check_conf(conf)) || (rc = check_dims(dims))) return rc;
+    if (!dtype_size(dtype)) return fail(RFBA_EINVAL, "unknown dtype %d", dtype);
+    if (!d_in || !d_out) return fail(RFBA_EINVAL, "NULL buffer");
+    NaDevice *nd = nullptr;
+    if ((rc = noalloc_device(nd))) return rc;
+    DevSamples s;
+    if ((rc = samples_from(conf, samples, s))) return rc;
+    RunBufs B;
+    B.fixed = true;
+    B.ncu = nd->ncu;
+    unsigned char *base = nd->ptr;
+    B.ctl = (int *)base;
+    B.smp = (DevSamples *)(base + kNaCtl);
+    B.items = (DevItem *)(base + kNaCtl + kNaSmp);
+    B.items_cap = kNaItem;
+    B.tiles = (DevTile *)(base + kNaCtl + kNaSmp + kNaItem);
+    B.tiles_cap = kNaTiles;
+    B.series = (float *)(base + kNaCtl + kNaSmp + kNaItem + kNaTiles);
+    B.series_bytes = kNaSeries;
+    std::vector<DevItem> items{stack_item(d_in, dims, conf->Magnification, d_out)};
+    return run_items_with(B, items, dtype, *conf, s, (hipStream_t)stream, nullptr);
 }
 
 extern "C" int SRRFGradientCalcGPU(float *inImage, int *imageDim, float *outImage, SRRFConf *conf) {

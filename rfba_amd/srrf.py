@@ -104,6 +104,9 @@ def lib() -> ctypes.CDLL:
         L.rfba_srrf_stack_device.argtypes = [c_void_p, c_int, POINTER(c_int), POINTER(SRRFConf),
                                              POINTER(_Samples), c_void_p, c_void_p, POINTER(c_float)]
         L.rfba_srrf_stack_device.restype = c_int
+        L.rfba_srrf_stack_device_noalloc.argtypes = [c_void_p, c_int, POINTER(c_int), POINTER(SRRFConf),
+                                                     POINTER(_Samples), c_void_p, c_void_p]
+        L.rfba_srrf_stack_device_noalloc.restype = c_int
         L.rfba_srrf_batch_create.argtypes = [POINTER(_Block), c_int, c_int, POINTER(SRRFConf), POINTER(_Samples),
                                              POINTER(c_void_p)]
         L.rfba_srrf_batch_create.restype = c_int
