@@ -186,6 +186,68 @@ template <typename F> void parallel_items(int n, int threads, F f) {
     for (auto &t : th) t.join();
 }
 
+// hess_log_log_normal, storm.h:446-449
+inline double hess_log_log_normal(double x, double mu, double sigma) {
+    return (1 + (ln(x) - mu - 1) / sq(sigma)) / sq(x);
+}
+
+// spot_shape_hess_position, storm.h:86-150: value, derivative and Hessian
+// (row-major [5][5]) of the spot at one pixel.  z_exact = 0 keeps the
+// reference's z terms, which use x[2] - phi[4] without the 1/zF^2 of the
+// exact derivative (as the derivative does, storm.h:78), and takes the entry
+// [4][3] as 0: the reference assigns that entry to itself (storm.h:143), so
+// its value there is whatever the uninitialised matrix held.  z_exact = 1
+// gives the exact second derivatives (symmetric): with s = -(dx^2 + dy^2 +
+// dz^2/zF^2)/(2 sigma^2) and w = (dx, dy, dz/zF^2), d mu/d phi_i =
+// I e^s w_i/(sigma^3 sqrt(2 pi)) and d2 mu/d phi_i d phi_j =
+// I e^s (w_i w_j - sigma^2 c_i delta_ij)/(sigma^5 sqrt(2 pi)), c = (1, 1, 1/zF^2).
+void spot_hess(const int *x, const double *phi, double zF, int z_exact, double &prob, double *deriv, double *hess) {
+    double d0 = x[0] - phi[2], d1 = x[1] - phi[3], d2 = x[2] - phi[4];
+    const double d2s = d2 / zF;
+    const double s = -(d0 * d0 + d1 * d1 + d2s * d2s) / (2 * phi[1] * phi[1]);
+    const double r_2_pi = std::sqrt(2 * M_PI);
+    const double es = std::exp(s);
+    const double I = phi[0], sg = phi[1];
+    prob = es * I / (sg * r_2_pi);
+    const double w[3] = {d0, d1, z_exact ? d2 / (zF * zF) : d2};
+    const double c[3] = {1.0, 1.0, z_exact ? 1.0 / (zF * zF) : 1.0};
+    const double f = es / (sg * r_2_pi);
+    deriv[0] = f * 1;
+    deriv[1] = f * (-I * (1 + 2 * s) / sg);
+    for (int i = 0; i < 3; i++) deriv[2 + i] = f * (w[i] * (I / sq(sg)));
+    double *h = hess;
+    h[0] = 0;
+    h[1] = h[5] = -es * (1 + 2 * s) / (sg * sg * r_2_pi);
+    for (int i = 0; i < 3; i++) h[2 + i] = h[5 * (2 + i)] = es * w[i] / (std::pow(sg, 3) * r_2_pi);
+    h[6] = 2 * I * es * (1 + 5 * s + 2 * s * s) / (std::pow(sg, 3) * r_2_pi);
+    for (int i = 0; i < 3; i++)
+        h[7 + i] = h[5 * (2 + i) + 1] = -I * es * (3 + 2 * s) * w[i] / (std::pow(sg, 4) * r_2_pi);
+    for (int i = 0; i < 3; i++)
+        for (int j = i; j < 3; j++)
+            h[5 * (2 + i) + 2 + j] = h[5 * (2 + j) + 2 + i] =
+                I * es * (w[i] * w[j] - (i == j ? sq(sg) * c[i] : 0.0)) / (r_2_pi * std::pow(sg, 5));
+    if (!z_exact) h[5 * 4 + 3] = 0.0;
+}
+
+// The per-frame parts of hess_handler (multispot5.cc:839-855) for one
+// background: hess_t = sum_p e H_p - d_p d_p^T and deriv_t = sum_p e d_p with
+// e = data - (bg + mu), in pixel order.  hp [T][25], dp [T][5].
+void hess_parts(const std::vector<double> &mu, const std::vector<double> &dmu, const std::vector<double> &hmu,
+                const double *data, const double *bg, int T, int P, double *hp, double *dp) {
+    for (int t = 0; t < T; t++) {
+        double H[25] = {0}, D[5] = {0};
+        for (int p = 0; p < P; p++) {
+            const double e = data[(size_t)t * P + p] - (bg[(size_t)t * P + p] + mu[p]);
+            const double *d = &dmu[5 * (size_t)p], *h = &hmu[25 * (size_t)p];
+            for (int i = 0; i < 5; i++)
+                for (int j = 0; j < 5; j++) H[5 * i + j] += e * h[5 * i + j] - d[i] * d[j];
+            for (int i = 0; i < 5; i++) D[i] += e * d[i];
+        }
+        std::copy(H, H + 25, hp + 25 * (size_t)t);
+        std::copy(D, D + 5, dp + 5 * (size_t)t);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -288,6 +350,89 @@ int oracle_b3_model_eval(const int *dims, int T, const double *data, double b0, 
         }
     });
     return 0;
+}
+
+int oracle_b3_spot_hess(const int *xyz, int P, const double *phi, double zF, int z_exact, double *mu, double *dmu,
+                        double *hess) {
+    for (int p = 0; p < P; p++)
+        spot_hess(xyz + 3 * p, phi, zF, z_exact, mu[p], dmu + 5 * (size_t)p, hess + 25 * (size_t)p);
+    return 0;
+}
+
+double oracle_b3_hess_log_log_normal(double x, double mu, double sigma) { return hess_log_log_normal(x, mu, sigma); }
+
+// sampled_background_spot_hessian_ffbs (multispot5.cc:902-982) with the state
+// paths given: paths [S][n][T] (0 = on) with weights [S][n].  Per background s
+// and path, H = sum over on frames of hess_t and D = sum of deriv_t
+// (hess_handler, :870-887); sum_hess = sum w (H + D D^T), sum_deriv = sum w D;
+// then sum_hess / (S var) - (sum_deriv / (S var))(sum_deriv / (S var))^T plus
+// the log-normal priors' second derivatives on [0][0] and [1][1]
+// (:960-975).  The reference's paths come from backward sampling, each with
+// weight 1 / bs_iterations; the exact posterior's weights give the limit.
+int oracle_b3_hessian_paths(const int *xyz, int P, int T, const double *data, const double *bg, int S,
+                            const double *phi, const rfba3b_model *m, int n, const uint8_t *paths,
+                            const double *weights, double *hess_out, double *deriv_out) {
+    std::vector<double> mu(P), dmu(5 * (size_t)P), hmu(25 * (size_t)P);
+    for (int p = 0; p < P; p++)
+        spot_hess(xyz + 3 * p, phi, m->zfactor, m->z_exact, mu[p], &dmu[5 * (size_t)p], &hmu[25 * (size_t)p]);
+    std::vector<double> hp(25 * (size_t)T), dp(5 * (size_t)T);
+    double sh[25] = {0}, sd[5] = {0};
+    for (int s = 0; s < S; s++) {
+        hess_parts(mu, dmu, hmu, data, bg + (size_t)s * T * P, T, P, hp.data(), dp.data());
+        for (int k = 0; k < n; k++) {
+            const uint8_t *path = paths + ((size_t)s * n + k) * T;
+            const double w = weights[(size_t)s * n + k];
+            double H[25] = {0}, D[5] = {0};
+            for (int t = 0; t < T; t++)
+                if (path[t] == 0) {
+                    for (int i = 0; i < 25; i++) H[i] += hp[25 * (size_t)t + i];
+                    for (int i = 0; i < 5; i++) D[i] += dp[5 * (size_t)t + i];
+                }
+            for (int i = 0; i < 5; i++)
+                for (int j = 0; j < 5; j++) sh[5 * i + j] += w * (H[5 * i + j] + D[i] * D[j]);
+            for (int i = 0; i < 5; i++) sd[i] += w * D[i];
+        }
+    }
+    const double norm = (double)S * m->variance;
+    for (int i = 0; i < 25; i++) sh[i] /= norm;
+    for (int i = 0; i < 5; i++) sd[i] /= norm;
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) sh[5 * i + j] -= sd[i] * sd[j];
+    sh[0] += hess_log_log_normal(phi[0], m->mu_brightness, m->sigma_brightness);
+    sh[6] += hess_log_log_normal(phi[1], m->mu_blur, m->sigma_blur);
+    std::copy(sh, sh + 25, hess_out);
+    if (deriv_out) {
+        sd[0] += diff_log_log_normal(phi[0], m->mu_brightness, m->sigma_brightness);
+        sd[1] += diff_log_log_normal(phi[1], m->mu_blur, m->sigma_blur);
+        std::copy(sd, sd + 5, deriv_out);
+    }
+    return 0;
+}
+
+// The whole of sampled_background_spot_hessian_ffbs: per background s the
+// forward messages of the spot (SpotWithBackground with its values only,
+// forward_algorithm_delta), then bs backward samplings, draw k of iteration
+// it being uniforms[(s bs + it) T + k] (the last frame first).  states
+// [S][bs][T] (may be NULL) receives the paths.
+int oracle_b3_hessian_ffbs(const int *xyz, int P, int T, const double *data, const double *bg, int S,
+                           const double *phi, const rfba3b_model *m, int bs, const double *uniforms, double *hess_out,
+                           double *deriv_out, uint8_t *states) {
+    std::vector<double> mu, dmu, on(T), off(T), diff(5 * (size_t)T), delta(3 * (size_t)T);
+    spot_tables(xyz, P, phi, *m, mu, dmu);
+    std::vector<uint8_t> paths((size_t)S * bs * T);
+    std::vector<int> st(T);
+    for (int s = 0; s < S; s++) {
+        emissions(data, bg + (size_t)s * T * P, T, P, mu.data(), dmu.data(), m->variance, on.data(), off.data(),
+                  diff.data());
+        forward_delta(m->A, m->pi, on.data(), off.data(), T, delta.data());
+        for (int it = 0; it < bs; it++) {
+            backward_sampling(m->A, delta.data(), T, uniforms + ((size_t)s * bs + it) * T, st.data());
+            for (int t = 0; t < T; t++) paths[((size_t)s * bs + it) * T + t] = (uint8_t)st[t];
+        }
+    }
+    if (states) std::copy(paths.begin(), paths.end(), states);
+    std::vector<double> w((size_t)S * bs, 1.0 / bs);
+    return oracle_b3_hessian_paths(xyz, P, T, data, bg, S, phi, m, bs, paths.data(), w.data(), hess_out, deriv_out);
 }
 
 }  // extern "C"

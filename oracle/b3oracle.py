@@ -39,6 +39,12 @@ def lib():
                 L.oracle_b3_gibbs_next.argtypes = [c_int, c_int, c_int, dp, dp, ip, ip, dp, dp, c_double, c_int,
                                                    POINTER(c_uint8), dp, dp, POINTER(ctypes.c_size_t)]
                 L.oracle_b3_gibbs_next.restype = c_int
+                up = POINTER(c_uint8)
+                L.oracle_b3_spot_hess.argtypes = [ip, c_int, dp, c_double, c_int, dp, dp, dp]
+                L.oracle_b3_hess_log_log_normal.argtypes = [c_double, c_double, c_double]
+                L.oracle_b3_hess_log_log_normal.restype = c_double
+                L.oracle_b3_hessian_paths.argtypes = [ip, c_int, c_int, dp, dp, c_int, dp, mp, c_int, up, dp, dp, dp]
+                L.oracle_b3_hessian_ffbs.argtypes = [ip, c_int, c_int, dp, dp, c_int, dp, mp, c_int, dp, dp, dp, up]
                 _sig = True
     return L
 
@@ -166,3 +172,45 @@ def gibbs_next(data, spot, offsets, idx, A, pi, var, iters, sample, intens, u):
     lib().oracle_b3_gibbs_next(P, T, K, _p(data), _p(spot), _p(offsets, c_int), _p(idx, c_int), _p(_d(A)),
                                _p(_d(pi)), var, iters, _p(sample, c_uint8), _p(intens), _p(u), ctypes.byref(used))
     return used.value
+
+
+def spot_hess(xyz, phi, zfactor=1.0, z_exact=0):
+    """spot_shape_hess_position at every pixel: mu [P], dmu [P, 5], hess [P, 5, 5]."""
+    xyz = np.ascontiguousarray(xyz, np.int32)
+    phi = _d(phi)
+    P = len(xyz)
+    mu, dmu, h = np.empty(P), np.empty((P, 5)), np.empty((P, 5, 5))
+    lib().oracle_b3_spot_hess(_p(xyz, c_int), P, _p(phi), zfactor, z_exact, _p(mu), _p(dmu), _p(h))
+    return mu, dmu, h
+
+
+def hess_log_log_normal(x, mu, sigma):
+    return lib().oracle_b3_hess_log_log_normal(x, mu, sigma)
+
+
+def hessian_paths(xyz, data, bg, phi, model, paths, weights):
+    """Hessian of sampled_background_spot_hessian_ffbs over given paths
+    [S, n, T] (0 = on) with weights [S, n]; returns (hess [5, 5], deriv [5])."""
+    xyz = np.ascontiguousarray(xyz, np.int32)
+    data, bg, phi, w = _d(data), _d(bg), _d(phi), _d(weights)
+    S, T, P = bg.shape
+    paths = np.ascontiguousarray(paths, np.uint8)
+    n = paths.shape[1]
+    h, d = np.empty((5, 5)), np.empty(5)
+    lib().oracle_b3_hessian_paths(_p(xyz, c_int), P, T, _p(data), _p(bg), S, _p(phi), ctypes.byref(model), n,
+                                  _p(paths, c_uint8), _p(w), _p(h), _p(d))
+    return h, d
+
+
+def hessian_ffbs(xyz, data, bg, phi, model, bs, uniforms):
+    """sampled_background_spot_hessian_ffbs with the draws uniforms [S, bs, T];
+    returns (hess [5, 5], deriv [5], states [S, bs, T])."""
+    xyz = np.ascontiguousarray(xyz, np.int32)
+    data, bg, phi, u = _d(data), _d(bg), _d(phi), _d(uniforms)
+    S, T, P = bg.shape
+    assert u.shape == (S, bs, T)
+    h, d = np.empty((5, 5)), np.empty(5)
+    st = np.empty((S, bs, T), np.uint8)
+    lib().oracle_b3_hessian_ffbs(_p(xyz, c_int), P, T, _p(data), _p(bg), S, _p(phi), ctypes.byref(model), bs, _p(u),
+                                 _p(h), _p(d), _p(st, c_uint8))
+    return h, d, st
