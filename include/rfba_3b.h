@@ -83,6 +83,25 @@ int rfba3b_functor_create(const int *xyz, int P, int T, const double *data, cons
  * are serialised. */
 int rfba3b_functor_eval(rfba3b_functor *handle, const double *phi, int n, double *grad, double *logL,
                         void *stream);
+/* sampled_background_spot_hessian_ffbs (multispot5.cc:902-982), the
+ * Hessian the model-selection step decomposes (multispot5.cc:1981, 2074),
+ * for the handle's data at spot phi [5]: per background sample the spot's
+ * forward messages (forward_algorithm_delta), bs_iterations backward
+ * samplings (draw k of sampling (s, it) is uniforms[(s*bs + it)*T + k], the
+ * last frame first, as rfba3b_ffbs), and over each sampled path the sums of
+ * the per-frame parts sum_p e H - d d^T and sum_p e d of the on frames
+ * (spot_shape_hess_position, storm.h:86-150); hess [5][5] (row-major) gets
+ * sum (H + D D^T) / (bs S var) - mean(D) mean(D)^T plus the log-normal
+ * priors' second derivatives on [0][0] and [1][1].  deriv [5] (may be NULL)
+ * gets mean(D) plus the priors' derivatives; paths [S][bs][T] (may be NULL)
+ * the sampled states.  With z_exact = 0 the spot Hessian keeps the
+ * reference's z terms and takes its never-set entry [4][3] (storm.h:143)
+ * as 0; z_exact = 1 uses the exact second derivatives.  The first call, and
+ * a call with a larger S x bs_iterations, allocates the handle's Hessian
+ * buffers.  Error for I <= 0 or sigma <= 0. */
+int rfba3b_functor_hessian_ffbs(rfba3b_functor *handle, const double *phi, int bs_iterations,
+                                const double *uniforms, double *hess, double *deriv, uint8_t *paths,
+                                void *stream);
 void rfba3b_functor_destroy(rfba3b_functor *handle);
 
 /* Forward messages delta_t(i) (forward_algorithm.h:344-382) of one spot over

@@ -97,6 +97,8 @@ def lib():
         L.rfba3b_functor_create.restype = c_int
         L.rfba3b_functor_eval.argtypes = [c_void_p, dp, c_int, dp, dp, c_void_p]
         L.rfba3b_functor_eval.restype = c_int
+        L.rfba3b_functor_hessian_ffbs.argtypes = [c_void_p, dp, c_int, dp, dp, dp, POINTER(c_uint8), c_void_p]
+        L.rfba3b_functor_hessian_ffbs.restype = c_int
         L.rfba3b_functor_destroy.argtypes = [c_void_p]
         L.rfba3b_functor_destroy.restype = None
         L.rfba3b_forward_messages.argtypes = [ip, c_int, c_int, fp, fp, dp, POINTER(Model), dp, dp]
@@ -194,6 +196,18 @@ class Functor:
         L = np.empty((n, self.S))
         _check(lib().rfba3b_functor_eval(self.h, _p(phi, c_double), n, _p(g, c_double), _p(L, c_double), None))
         return g, L
+
+    def hessian_ffbs(self, phi, bs_iterations, uniforms):
+        """sampled_background_spot_hessian_ffbs at phi with the draws
+        uniforms [S, bs, T]: (hess [5, 5], deriv [5], paths [S, bs, T])."""
+        phi = np.ascontiguousarray(phi, np.float64)
+        u = np.ascontiguousarray(uniforms, np.float64)
+        assert u.shape[:2] == (self.S, bs_iterations)
+        h, d = np.empty((5, 5)), np.empty(5)
+        paths = np.empty(u.shape, np.uint8)
+        _check(lib().rfba3b_functor_hessian_ffbs(self.h, _p(phi, c_double), bs_iterations, _p(u, c_double),
+                                                 _p(h, c_double), _p(d, c_double), _p(paths, c_uint8), None))
+        return h, d, paths
 
     def close(self):
         if self.h:

@@ -27,8 +27,10 @@ template <bool GRAD> __global__ void b3_hmm(HmmParams H);
 struct LogA {
     double v[9];
 };
-__global__ void b3_backward_sample(const double *__restrict__ delta, int n, int T, LogA lA,
+__global__ void b3_backward_sample(const double *__restrict__ delta, int n, int per, int T, LogA lA,
                                    const double *__restrict__ u, uint8_t *__restrict__ states);
+__global__ void b3_hess_basis(HessParams Hp);
+__global__ void b3_hess_parts(HessParams Hp);
 }  // namespace rfba3b
 
 using namespace rfba3b;
@@ -684,10 +686,20 @@ struct rfba3b_functor {
     unsigned char *h_stage = nullptr;
     size_t o_logL = 0, o_grad = 0, o_corr = 0, o_items = 0;
     std::mutex mu;
+    // FFBS Hessian buffers, allocated by the first rfba3b_functor_hessian_ffbs
+    // and grown with bs_iterations: queue counter, spot table and its
+    // d d^T partial sums, per-frame parts, forward messages, draws, paths
+    int *d_counter = nullptr;
+    float *d_basis = nullptr;
+    double *d_fixed = nullptr, *d_parts = nullptr, *d_delta = nullptr, *d_unif = nullptr;
+    uint8_t *d_paths = nullptr;
+    size_t paths_cap = 0;  // chains (S x bs) the draw and path buffers hold
 };
 
 static void functor_free(rfba3b_functor *f) {
     if (!f) return;
+    hipFree(f->d_counter); hipFree(f->d_basis); hipFree(f->d_fixed); hipFree(f->d_parts); hipFree(f->d_delta);
+    hipFree(f->d_unif); hipFree(f->d_paths);
     hipFree(f->d_model); hipFree(f->d_block); hipFree(f->d_planes); hipFree(f->d_corr); hipFree(f->d_items);
     hipFree(f->d_R); hipFree(f->d_states); hipFree(f->d_mask); hipFree(f->d_S0); hipFree(f->d_em);
     hipFree(f->d_ck); hipFree(f->d_params); hipFree(f->d_logL); hipFree(f->d_grad);
@@ -923,8 +935,8 @@ extern "C" int rfba3b_backward_sample_device(const double *d_delta, int n, int T
     if ((rc = check_model(model)) || (rc = dev_init())) return rc;
     if (n < 0 || T <= 0 || (n > 0 && (!d_delta || !d_uniforms || !d_states))) return fail(-1, "invalid arguments");
     if (n == 0) return 0;
-    hipLaunchKernelGGL(b3_backward_sample, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, d_delta, n, T,
-                       log_a(model), d_uniforms, d_states);
+    hipLaunchKernelGGL(b3_backward_sample, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, d_delta, n, 1,
+                       T, log_a(model), d_uniforms, d_states);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -972,5 +984,144 @@ extern "C" int rfba3b_mt19937_uniforms(int seed, unsigned long long skip, size_t
     std::mt19937 g((uint32_t)seed);
     g.discard(37 + skip);
     for (size_t i = 0; i < n; i++) out[i] = (double)g() * (1.0 / (65536.0 * 65536.0));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// FFBS Hessian of the resident functor: sampled_background_spot_hessian_ffbs
+// (multispot5.cc:902-982) for the handle's spot data.
+
+extern "C" int rfba3b_functor_hessian_ffbs(rfba3b_functor *f, const double *phi, int bs, const double *uniforms,
+                                           double *hess, double *deriv, uint8_t *paths, void *stream) {
+    if (!f) return fail(-1, "handle is NULL");
+    if (!phi || !uniforms || !hess || bs <= 0) return fail(-1, "invalid arguments");
+    if (!(phi[0] > 0 && phi[1] > 0)) return fail(-1, "invalid spot parameters");
+    std::lock_guard<std::mutex> hl(f->mu);
+    const int S = f->S, T = f->T, X = f->X, Y = f->Y, Z = f->Z;
+    const size_t Pd = (size_t)X * Y * Z;
+    const size_t nchain = (size_t)S * bs;
+    if (nchain > (size_t)1 << 24 || nchain * T > (size_t)1 << 31) return fail(-1, "S x bs_iterations x T too large");
+    const int nbas = (int)((Pd + 255) / 256);
+    const int ngrp = (T + kHessFrames - 1) / kHessFrames;
+    const int ck = ckpt_of(&f->model, T);
+    const size_t lds = eval_lds(T, X * Y, Z, ck);
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<double> parts((size_t)S * T * kHessParts), fixed((size_t)nbas * kHessUpper);
+    std::vector<uint8_t> path(nchain * T);
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        hipError_t e = hipSuccess;
+        auto al = [&](void **p, size_t bytes) {
+            if (e == hipSuccess && !*p) e = hipMalloc(p, std::max<size_t>(bytes, 16));
+        };
+        al((void **)&f->d_counter, sizeof(int));
+        al((void **)&f->d_basis, Pd * kHessBasis * sizeof(float));
+        al((void **)&f->d_fixed, (size_t)nbas * kHessUpper * sizeof(double));
+        al((void **)&f->d_parts, (size_t)S * T * kHessParts * sizeof(double));
+        al((void **)&f->d_delta, (size_t)S * T * 3 * sizeof(double));
+        if (e == hipSuccess && nchain > f->paths_cap) {
+            hipFree(f->d_unif);
+            hipFree(f->d_paths);
+            f->d_unif = nullptr;
+            f->d_paths = nullptr;
+            f->paths_cap = 0;
+            al((void **)&f->d_unif, nchain * T * sizeof(double));
+            al((void **)&f->d_paths, nchain * T);
+            if (e == hipSuccess) f->paths_cap = nchain;
+        }
+        if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? -2 : -3, "Hessian buffers: %s", hipGetErrorString(e));
+        EvalFn fn = eval_fn(MODE_DELTA);
+        int grid = 0, rc;
+        if ((rc = eval_grid(fn, lds, S, grid))) return rc;
+        // the spot and one item per background sample (staging as in eval)
+        double *sp = (double *)f->h_stage;
+        for (int c = 0; c < 5; c++) sp[c] = phi[c];
+        Item *it = (Item *)(f->h_stage + f->o_items);
+        const long long plane = (long long)T * (long long)Pd;
+        for (int s = 0; s < S; s++) it[s] = {0, 0, 0, s, s * plane};
+        HIP_TRY(hipMemcpyAsync(f->d_params, sp, 5 * sizeof(double), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(f->d_items, it, S * sizeof(Item), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(f->d_unif, uniforms, nchain * T * sizeof(double), hipMemcpyHostToDevice, st));
+        // forward messages of the spot over each background (rmode 1: the
+        // planes are data - bg), then bs backward samplings per background
+        EvalParams E;
+        memset(&E, 0, sizeof E);
+        E.blocks = f->d_block; E.items = f->d_items; E.nitems = S; E.counter = f->d_counter; E.R = f->d_R;
+        E.model = f->d_model; E.rmode = 1; E.ckpt = f->model.checkpoint; E.Tmax = T; E.XYmax = X * Y; E.Zmax = Z;
+        E.delta_out = f->d_delta;
+        hipLaunchKernelGGL(b3_reset, dim3(1), dim3(64), 0, st, f->d_counter);
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(256), lds, st, E);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(b3_backward_sample, dim3((unsigned)((nchain + 255) / 256)), dim3(256), 0, st, f->d_delta,
+                           (int)nchain, bs, T, log_a(&f->model), f->d_unif, f->d_paths);
+        HIP_TRY(hipGetLastError());
+        // spot table and the per-frame parts
+        HessParams Hp;
+        memset(&Hp, 0, sizeof Hp);
+        Hp.block = f->d_block; Hp.model = f->d_model; Hp.basis = f->d_basis; Hp.fixed = f->d_fixed; Hp.R = f->d_R;
+        Hp.S = S; Hp.T = T; Hp.parts = f->d_parts;
+        hipLaunchKernelGGL(b3_hess_basis, dim3(nbas), dim3(256), 0, st, Hp);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(b3_hess_parts, dim3(S * ngrp), dim3(256), 0, st, Hp);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(parts.data(), f->d_parts, parts.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(fixed.data(), f->d_fixed, fixed.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(path.data(), f->d_paths, path.size(), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    if (paths) memcpy(paths, path.data(), path.size());
+    // Frame-independent sums of d_i d_j, in workgroup order; the index of
+    // entry (i, j), i <= j, in the upper-triangle order.
+    double F[kHessUpper] = {0};
+    for (int w = 0; w < nbas; w++)
+        for (int k = 0; k < kHessUpper; k++) F[k] += fixed[(size_t)w * kHessUpper + k];
+    int U[5][5];
+    for (int i = 0, k = 0; i < 5; i++)
+        for (int j = i; j < 5; j++) U[i][j] = U[j][i] = k++;
+    // hess_t = sum e H - d d^T (the reference's entry [4][3] of H is never
+    // set, storm.h:143: taken as 0 in reference mode), deriv_t = sum e d;
+    // per path H = sum over on frames, D likewise; sum (H + D D^T) and sum D
+    // over backgrounds and paths (hess_handler, multispot5.cc:839-887).
+    std::vector<double> Ht(25 * (size_t)T), Dt(5 * (size_t)T);
+    double sh[25] = {0}, sd[5] = {0};
+    for (int s = 0; s < S; s++) {
+        for (int t = 0; t < T; t++) {
+            const double *pt = &parts[((size_t)s * T + t) * kHessParts];
+            for (int i = 0; i < 5; i++)
+                for (int j = 0; j < 5; j++) {
+                    const double eH = (i == 4 && j == 3 && !f->model.z_exact) ? 0.0 : pt[U[i][j]];
+                    Ht[25 * (size_t)t + 5 * i + j] = eH - F[U[i][j]];
+                }
+            for (int i = 0; i < 5; i++) Dt[5 * (size_t)t + i] = pt[kHessUpper + i];
+        }
+        for (int r = 0; r < bs; r++) {
+            const uint8_t *pa = &path[((size_t)s * bs + r) * T];
+            double H[25] = {0}, D[5] = {0};
+            for (int t = 0; t < T; t++)
+                if (pa[t] == 0) {
+                    for (int i = 0; i < 25; i++) H[i] += Ht[25 * (size_t)t + i];
+                    for (int i = 0; i < 5; i++) D[i] += Dt[5 * (size_t)t + i];
+                }
+            for (int i = 0; i < 5; i++)
+                for (int j = 0; j < 5; j++) sh[5 * i + j] += H[5 * i + j] + D[i] * D[j];
+            for (int i = 0; i < 5; i++) sd[i] += D[i];
+        }
+    }
+    // :960-975: normalise, subtract the mean outer product, add the priors
+    const double norm = (double)bs * S * f->model.variance;
+    for (int i = 0; i < 25; i++) sh[i] /= norm;
+    for (int i = 0; i < 5; i++) sd[i] /= norm;
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++) sh[5 * i + j] -= sd[i] * sd[j];
+    const rfba3b_model &m = f->model;
+    auto hlln = [](double x, double mu, double sg) { return (1 + (ln0(x) - mu - 1) / (sg * sg)) / (x * x); };
+    sh[0] += hlln(phi[0], m.mu_brightness, m.sigma_brightness);
+    sh[6] += hlln(phi[1], m.mu_blur, m.sigma_blur);
+    memcpy(hess, sh, sizeof sh);
+    if (deriv) {
+        sd[0] += dlln(phi[0], m.mu_brightness, m.sigma_brightness);
+        sd[1] += dlln(phi[1], m.mu_blur, m.sigma_blur);
+        memcpy(deriv, sd, sizeof sd);
+    }
     return 0;
 }

@@ -114,6 +114,24 @@ struct EvalParams {
     double *delta_out;      // MODE_DELTA: [item][T][3]
 };
 
+// FFBS Hessian (sampled_background_spot_hessian_ffbs): per-pixel table of
+// the spot's value, the 15 upper entries of its Hessian ((0,0), (0,1), ...,
+// (0,4), (1,1), ..., (4,4)) and its 5 derivatives, FP32, padded to 24.
+constexpr int kHessBasis = 24;
+constexpr int kHessUpper = 15;
+constexpr int kHessParts = 20;   // per frame: sum e H (15), sum e d (5)
+constexpr int kHessFrames = 4;   // frames per parts workgroup
+
+struct HessParams {
+    const DevBlock *block;  // region box and mask; params[0] = the spot
+    const DevModel *model;
+    float *basis;           // [box pixels][kHessBasis]
+    double *fixed;          // [basis workgroups][15]: sums of d_i d_j (i <= j)
+    const float *R;         // residual planes [S][T][box pixels]
+    int S, T;
+    double *parts;          // [S][T][kHessParts]
+};
+
 // ---------------------------------------------------------------------------
 // Extended-range values: v = m 2^e with m in [0.5, 1) and an exponent of
 // its own, an integer held in a double (exact to 2^53; -inf for m = 0).

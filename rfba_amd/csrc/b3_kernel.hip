@@ -787,6 +787,139 @@ template __global__ void b3_hmm<false>(HmmParams);
 template __global__ void b3_hmm<true>(HmmParams);
 
 // ---------------------------------------------------------------------------
+// FFBS Hessian, sampled_background_spot_hessian_ffbs (multispot5.cc:902-982).
+//
+// b3_hess_basis: one thread per pixel of the region's box evaluates
+// spot_shape_hess_position (storm.h:86-150) in FP64 and stores the value,
+// the upper Hessian and the derivative in FP32; per workgroup it also sums
+// d_i d_j (i <= j) in FP64 (the frame-independent part of hess_t).
+// z_exact = 0 keeps the reference's z terms (x[2] - phi[4] without 1/zF^2,
+// as the derivative) and leaves [4][3], which storm.h:143 never sets, to the
+// host (as 0); z_exact = 1 gives the exact second derivatives.
+__global__ __launch_bounds__(256) void b3_hess_basis(HessParams Hp) {
+    __shared__ double red[4][kHessUpper];
+    const DevBlock &b = *Hp.block;
+    const DevModel &md = *Hp.model;
+    const int X = b.X, Y = b.Y, XY = X * Y, P = XY * b.Z;
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double dd[kHessUpper];
+#pragma unroll
+    for (int k = 0; k < kHessUpper; k++) dd[k] = 0.0;
+    if (p < P) {
+        float *out = Hp.basis + (size_t)p * kHessBasis;
+        float v[kHessBasis];
+#pragma unroll
+        for (int k = 0; k < kHessBasis; k++) v[k] = 0.0f;
+        if (!b.mask || b.mask[p]) {
+            const double *ph = b.params;
+            const int z = p / XY, y = (p - z * XY) / X, x = p - z * XY - y * X;
+            const double zF = md.zfactor;
+            const double d0 = (double)(x + b.ox) - ph[2], d1 = (double)(y + b.oy) - ph[3];
+            const double d2 = (double)(z + b.oz) - ph[4];
+            const double d2s = d2 / zF;
+            const double I = ph[0], sg = ph[1];
+            const double s = -(d0 * d0 + d1 * d1 + d2s * d2s) / (2 * sg * sg);
+            const double r2p = 2.5066282746310002;
+            const double es = exp(s);
+            const double w[3] = {d0, d1, md.z_exact ? d2 / (zF * zF) : d2};
+            const double c[3] = {1.0, 1.0, md.z_exact ? 1.0 / (zF * zF) : 1.0};
+            const double sg2 = sg * sg, sg3 = sg2 * sg;
+            double d[5], h[kHessUpper];
+            const double f = es / (sg * r2p);
+            d[0] = f;
+            d[1] = f * (-I * (1 + 2 * s) / sg);
+            for (int i = 0; i < 3; i++) d[2 + i] = f * (w[i] * (I / sg2));
+            h[0] = 0.0;
+            h[1] = -es * (1 + 2 * s) / (sg2 * r2p);
+            for (int i = 0; i < 3; i++) h[2 + i] = es * w[i] / (sg3 * r2p);
+            h[5] = 2 * I * es * (1 + 5 * s + 2 * s * s) / (sg3 * r2p);
+            for (int i = 0; i < 3; i++) h[6 + i] = -I * es * (3 + 2 * s) * w[i] / (sg3 * sg * r2p);
+            // (2,2) (2,3) (2,4) (3,3) (3,4) (4,4)
+            int k = 9;
+            for (int i = 0; i < 3; i++)
+                for (int j = i; j < 3; j++)
+                    h[k++] = I * es * (w[i] * w[j] - (i == j ? sg2 * c[i] : 0.0)) / (r2p * sg3 * sg2);
+            v[0] = (float)(es * I / (sg * r2p));
+            for (int k2 = 0; k2 < kHessUpper; k2++) v[1 + k2] = (float)h[k2];
+            for (int i = 0; i < 5; i++) v[1 + kHessUpper + i] = (float)d[i];
+            int u = 0;
+            for (int i = 0; i < 5; i++)
+                for (int j = i; j < 5; j++) dd[u++] = d[i] * d[j];
+        }
+#pragma unroll
+        for (int k = 0; k < kHessBasis; k += 4)
+            *(float4 *)(out + k) = make_float4(v[k], v[k + 1], v[k + 2], v[k + 3]);
+    }
+#pragma unroll
+    for (int k = 0; k < kHessUpper; k++) {
+        double a = dd[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+        if (lane == 0) red[wave][k] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < kHessUpper) {
+        const int k = threadIdx.x;
+        Hp.fixed[(size_t)blockIdx.x * kHessUpper + k] = ((red[0][k] + red[1][k]) + red[2][k]) + red[3][k];
+    }
+}
+
+// b3_hess_parts: workgroup (s, 4 frames) sums, over the region's pixels,
+// e H_k (15 upper entries) and e d_i (5) with e = R - mu for its frames
+// (hess_handler's per-frame parts, multispot5.cc:839-855, without the
+// frame-independent d d^T).  Each thread sums its pixels in FP32, then the
+// 256 partials are added in FP64 (lanes by exchange, then the 4 waves in
+// order).
+__global__ __launch_bounds__(256) void b3_hess_parts(HessParams Hp) {
+    __shared__ double red[4][kHessFrames * kHessParts];
+    const DevBlock &b = *Hp.block;
+    const int P = b.X * b.Y * b.Z, T = Hp.T;
+    const int ng = (T + kHessFrames - 1) / kHessFrames;
+    const int s = blockIdx.x / ng, t0 = (blockIdx.x - s * ng) * kHessFrames;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nf = min(kHessFrames, T - t0);
+    const float *Rs = Hp.R + ((size_t)s * T + t0) * P;
+    float acc[kHessFrames][kHessParts];
+#pragma unroll
+    for (int f = 0; f < kHessFrames; f++)
+#pragma unroll
+        for (int k = 0; k < kHessParts; k++) acc[f][k] = 0.0f;
+    for (int p = threadIdx.x; p < P; p += 256) {
+        const float4 *bp = (const float4 *)(Hp.basis + (size_t)p * kHessBasis);
+        float v[kHessBasis];
+#pragma unroll
+        for (int k = 0; k < kHessBasis / 4; k++) {
+            const float4 q = bp[k];
+            v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
+        }
+#pragma unroll
+        for (int f = 0; f < kHessFrames; f++) {
+            if (f < nf) {
+                const float e = Rs[(size_t)f * P + p] - v[0];
+#pragma unroll
+                for (int k = 0; k < kHessParts; k++) acc[f][k] += e * v[1 + k];
+            }
+        }
+    }
+#pragma unroll
+    for (int f = 0; f < kHessFrames; f++)
+#pragma unroll
+        for (int k = 0; k < kHessParts; k++) {
+            double a = (double)acc[f][k];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+            if (lane == 0) red[wave][f * kHessParts + k] = a;
+        }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kHessFrames * kHessParts; i += 256) {
+        const int f = i / kHessParts, k = i - f * kHessParts;
+        if (f < nf)
+            Hp.parts[((size_t)s * T + t0 + f) * kHessParts + k] = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Backward sampling of forward filtering / backward sampling, one lane per
 // chain (forward_algorithm.h:502-566): the last frame's state is drawn from
 // its forward message, then each earlier frame's from its message plus
@@ -811,11 +944,13 @@ __device__ __forceinline__ int sample_unscaled_log3(double a0, double a1, double
     return 2;  // either the third running sum reaches choice or the last element is returned
 }
 
-__global__ void b3_backward_sample(const double *__restrict__ delta, int n, int T, LogA lA,
+// Chain c samples from the messages delta[c / per] (per chains share one
+// set of messages: repeated backward samplings of one forward pass).
+__global__ void b3_backward_sample(const double *__restrict__ delta, int n, int per, int T, LogA lA,
                                    const double *__restrict__ u, uint8_t *__restrict__ states) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    const double *d = delta + (size_t)c * T * 3;
+    const double *d = delta + (size_t)(c / per) * T * 3;
     const double *uc = u + (size_t)c * T;
     uint8_t *sc = states + (size_t)c * T;
     int k = 0;

@@ -354,3 +354,40 @@ def test_resident_functor_vs_oracle(gpu, region, S):
     with pytest.raises(b3.B3Error):
         f(np.tile(base, (13, 1)))  # more candidates than the handle holds
     f.close()
+
+
+HESS_RTOL = 1e-4
+
+
+@pytest.mark.parametrize("region,S,bs,z_exact", [("ball", 3, 5, 0), ("box", 2, 3, 1)])
+def test_functor_hessian_ffbs_vs_oracle(gpu, region, S, bs, z_exact):
+    # sampled_background_spot_hessian_ffbs for the handle's data: the paths
+    # drawn on the device from the device's forward messages equal the
+    # oracle's for the same draws; the Hessian matches the FP64 oracle within
+    # 1e-4 of its largest entry (FP32 residuals and spot tables), in the
+    # reference's mode (z terms without 1/zF^2, entry [4][3] unset -> 0) and
+    # the exact one
+    m = model(z_exact=z_exact)
+    xyz, d2, bgs, params = _functor_case((11, 10, 9), 30, 4, S, 91, region)
+    f = b3.Functor(xyz, d2, bgs, m, max_candidates=2)
+    phi = params[0] + [40.0, 0.05, 0.2, -0.1, 0.15]
+    for call, nb in enumerate((bs, bs + 2)):  # the second call grows the path buffers
+        u = np.random.default_rng(30 + call).random((S, nb, 30))
+        H, D, paths = f.hessian_ffbs(phi, nb, u)
+        Ho, Do, po = bo.hessian_ffbs(xyz, d2, bgs, phi, m, nb, u)
+        assert np.array_equal(paths, po), "sampled paths differ"
+        scale = np.abs(Ho).max()
+        err = np.abs(H - Ho).max() / scale
+        errd = np.abs(D - Do).max() / np.abs(Do).max()
+        print("%s S=%d bs=%d z_exact=%d: Hessian err %.3g (max-normalised), deriv err %.3g, %d on-frames sampled"
+              % (region, S, nb, z_exact, err, errd, int((paths == 0).sum())))
+        assert err <= HESS_RTOL and errd <= HESS_RTOL
+        if not z_exact:
+            assert H[4, 3] != H[3, 4]
+        else:
+            np.testing.assert_allclose(H, H.T, rtol=0, atol=1e-12 * scale)
+    # the CG functor still runs on the same handle
+    g, L = f(phi)
+    go, Lo = bo.neg_grad(xyz, d2, bgs, phi, m)
+    assert np.linalg.norm(g[0] - go) / np.linalg.norm(go) <= GRAD_RTOL
+    f.close()
