@@ -30,6 +30,7 @@ struct LogA {
 __global__ void b3_backward_sample(const double *__restrict__ delta, int n, int per, int T, LogA lA,
                                    const double *__restrict__ u, uint8_t *__restrict__ states);
 __global__ void b3_hess_basis(HessParams Hp);
+__global__ void b3_corr_reduce(CorrReduceParams R);
 __global__ void b3_hess_parts(HessParams Hp);
 }  // namespace rfba3b
 
@@ -686,6 +687,10 @@ struct rfba3b_functor {
     unsigned char *h_stage = nullptr;
     size_t o_logL = 0, o_grad = 0, o_corr = 0, o_items = 0;
     std::mutex mu;
+    // pixel-split correlation sums (few candidates: each plane's pixels are
+    // spread over many workgroups)
+    double *d_part = nullptr, *d_part_qh = nullptr;
+    size_t part_doubles = 0, partqh_doubles = 0;
     // FFBS Hessian buffers, allocated by the first rfba3b_functor_hessian_ffbs
     // and grown with bs_iterations: queue counter, spot table and its
     // d d^T partial sums, per-frame parts, forward messages, draws, paths
@@ -698,6 +703,7 @@ struct rfba3b_functor {
 
 static void functor_free(rfba3b_functor *f) {
     if (!f) return;
+    hipFree(f->d_part); hipFree(f->d_part_qh);
     hipFree(f->d_counter); hipFree(f->d_basis); hipFree(f->d_fixed); hipFree(f->d_parts); hipFree(f->d_delta);
     hipFree(f->d_unif); hipFree(f->d_paths);
     hipFree(f->d_model); hipFree(f->d_block); hipFree(f->d_planes); hipFree(f->d_corr); hipFree(f->d_items);
@@ -705,6 +711,27 @@ static void functor_free(rfba3b_functor *f) {
     hipFree(f->d_ck); hipFree(f->d_params); hipFree(f->d_logL); hipFree(f->d_grad);
     if (f->h_stage) hipHostFree(f->h_stage);
     delete f;
+}
+
+// Pixel splits of the functor's correlation for m candidates: enough
+// (plane, spot group, pixel range) tasks for about three workgroups per CU,
+// each range at least 8 chunks of 32 pixels, and the partial sums within the
+// handle's buffer; 1 = whole planes.
+static void functor_split(const rfba3b_functor *f, int m, int &nsplit, int &pstep) {
+    const long long Pd = (long long)f->X * f->Y * f->Z;
+    const long long base = (long long)f->S * ((m + kCorrSpots - 1) / kCorrSpots);
+    const long long NI = (long long)f->S * m;
+    long long ns = std::min<long long>((3LL * g_ncu + base - 1) / base, (Pd + 8 * kCorrPix - 1) / (8 * kCorrPix));
+    if (f->part_doubles) ns = std::min<long long>(ns, (long long)(f->part_doubles / ((size_t)NI * f->T * 6)));
+    if (f->partqh_doubles) ns = std::min<long long>(ns, (long long)(f->partqh_doubles / ((size_t)NI * 6)));
+    if (ns < 2) {
+        nsplit = 1;
+        pstep = (int)Pd;
+        return;
+    }
+    const long long per = (Pd + ns - 1) / ns;
+    pstep = (int)((per + kCorrPix - 1) / kCorrPix * kCorrPix);
+    nsplit = (int)((Pd + pstep - 1) / pstep);
 }
 
 extern "C" int rfba3b_functor_create(const int *xyz, int P, int T, const double *data, const double *bg, int S,
@@ -768,6 +795,20 @@ extern "C" int rfba3b_functor_create(const int *xyz, int P, int T, const double 
     f->o_corr = f->o_grad + NI * 5 * sizeof(double);
     f->o_items = f->o_corr + ncorr * sizeof(CorrTask);
     const size_t stage = f->o_items + NI * sizeof(Item);
+    // the largest partial-sum buffer any candidate count needs, at most 256 MiB
+    {
+        size_t need = 0, needqh = 0;
+        const size_t cap = (size_t)32 << 20;  // doubles
+        for (int m = 1; m <= max_candidates; m++) {
+            int ns, ps;
+            functor_split(f, m, ns, ps);
+            if (ns < 2) continue;
+            need = std::max(need, (size_t)ns * S * m * T * 6);
+            needqh = std::max(needqh, (size_t)ns * S * m * 6);
+        }
+        f->part_doubles = std::min(need, cap);
+        f->partqh_doubles = std::min(needqh, cap);
+    }
 
     hipError_t e = hipSuccess;
     auto al = [&](void **p, size_t bytes) {
@@ -787,6 +828,10 @@ extern "C" int rfba3b_functor_create(const int *xyz, int P, int T, const double 
     al((void **)&f->d_params, (size_t)max_candidates * 5 * sizeof(double));
     al((void **)&f->d_logL, NI * sizeof(double));
     al((void **)&f->d_grad, NI * 5 * sizeof(double));
+    if (f->part_doubles) {
+        al((void **)&f->d_part, f->part_doubles * sizeof(double));
+        al((void **)&f->d_part_qh, f->partqh_doubles * sizeof(double));
+    }
     if (e == hipSuccess) e = hipHostMalloc((void **)&f->h_stage, stage, hipHostMallocDefault);
     DevBlock d;
     memset(&d, 0, sizeof d);
@@ -874,8 +919,31 @@ extern "C" int rfba3b_functor_eval(rfba3b_functor *f, const double *phi, int n, 
         C.S0 = f->d_S0;
         C.model = f->d_model;
         C.em = f->d_em;
-        hipLaunchKernelGGL(corr_fn(true), dim3((nc + 7) / 8 * 8), dim3(256), f->lds_corr, st, C);
+        int nsplit = 1, pstep = 0;
+        functor_split(f, m, nsplit, pstep);
+        if (nsplit > 1) {
+            C.nsplit = nsplit;
+            C.pstep = pstep;
+            C.part = f->d_part;
+            C.part_qh = f->d_part_qh;
+        }
+        hipLaunchKernelGGL(corr_fn(true), dim3((nc * nsplit + 7) / 8 * 8), dim3(256), f->lds_corr, st, C);
         HIP_TRY(hipGetLastError());
+        if (nsplit > 1) {
+            CorrReduceParams Rd;
+            Rd.block = f->d_block;
+            Rd.items = f->d_items;
+            Rd.NI = NI;
+            Rd.T = T;
+            Rd.nsplit = nsplit;
+            Rd.Tmax = T;
+            Rd.part = f->d_part;
+            Rd.part_qh = f->d_part_qh;
+            Rd.S0 = f->d_S0;
+            Rd.em = f->d_em;
+            hipLaunchKernelGGL(b3_corr_reduce, dim3((NI * T + 255) / 256), dim3(256), 0, st, Rd);
+            HIP_TRY(hipGetLastError());
+        }
         HmmParams H;
         memset(&H, 0, sizeof H);
         H.blocks = f->d_block;

@@ -81,12 +81,31 @@ struct CorrParams {
     const double *S0;       // [plane][Tmax]: sum over pixels of R^2
     const DevModel *model;
     double *em;             // [t][NI][8]: Soff, D, diff[5]
+    // Pixel-split form (nsplit > 1): virtual task v is task v / nsplit over
+    // pixels [split pstep, (split + 1) pstep) of its plane, split = v %
+    // nsplit; instead of em it writes its raw sums, which b3_corr_reduce adds
+    // in split order and turns into em.
+    int nsplit;             // 0 or 1: whole planes
+    int pstep;              // pixels per split (a multiple of kCorrPix)
+    double *part;           // [nsplit][Tmax][NI][6]: C, G_0..4
+    double *part_qh;        // [nsplit][NI][6]: Q, H_0..4
 };
+
 
 
 struct Item {
     int block, q, k, s;
     long long rslot;
+};
+
+// b3_corr_reduce: items of one block with Q = 1 whose plane index is the
+// item's sample s (the resident functor's layout).
+struct CorrReduceParams {
+    const DevBlock *block;
+    const Item *items;
+    int NI, T, nsplit, Tmax;
+    const double *part, *part_qh, *S0;
+    double *em;
 };
 
 struct HmmParams {

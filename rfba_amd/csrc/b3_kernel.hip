@@ -446,15 +446,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     // tasks so that the spot groups of one plane share that XCD's L2 (the
     // grid is padded to a multiple of 8).
     const int per_xcd = gridDim.x >> 3;
-    const int task = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    if (task >= C.ntasks) return;
-    const CorrTask tk = C.tasks[task];
+    const int vtask = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    const int nsplit = C.nsplit > 1 ? C.nsplit : 1;
+    if (vtask >= C.ntasks * nsplit) return;
+    const CorrTask tk = C.tasks[vtask / nsplit];
+    const int split = vtask - (vtask / nsplit) * nsplit;
     const PlaneRef pr = C.planes[tk.plane];
     const DevBlock &b = C.blocks[pr.block];
     const int X = b.X, Y = b.Y, Z = b.Z, T = b.T, K = b.K;
     const int XY = X * Y, P = XY * Z, nax = X + Y + Z;
     const int nk = tk.nk;
     const DevModel &md = *C.model;
+    // pixel range of this (split) task
+    const int pbeg = nsplit > 1 ? split * C.pstep : 0;
+    const int pend = nsplit > 1 ? min(P, pbeg + C.pstep) : P;
 
     double *qh = ldsc;                                   // [8][6] Q, H_0..4
     double *qw = qh + kCorrSpots * 6;                    // [4 waves][8][6] partial Q, H
@@ -552,7 +557,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                 for (int r = 0; r < 4; r++) a64[f2][ct][r] = 0.0;
             }
         // pixel p = p0 + cb of the basis role, tracked as (x, y, z)
-        int bx = cb % X, by = (cb / X) % Y, bz = cb / XY;
+        int bx = (pbeg + cb) % X, by = ((pbeg + cb) / X) % Y, bz = (pbeg + cb) / XY;
         // Residual chunk prefetch: thread owns 4 consecutive pixels
         // (4 (tid & 7)) of rows (tid >> 3) + 32 i; the next chunk's loads are
         // in flight during this chunk's matrix work.
@@ -589,9 +594,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                 pre[i] = v;
             }
         };
-        fetch(0);
+        fetch(pbeg);
         int nch = 0;
-        for (int p0 = 0; p0 < P; p0 += kCorrPix, nch++) {
+        for (int p0 = pbeg; p0 < pend; p0 += kCorrPix, nch++) {
             __syncthreads();
 #pragma unroll
             for (int i = 0; i < 4; i++) *(float4 *)(Rs + (prow + 32 * i) * kCorrRow + pr4) = pre[i];
@@ -619,7 +624,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                 }
             }
             __syncthreads();
-            if (p0 + kCorrPix < P) fetch(p0 + kCorrPix);
+            if (p0 + kCorrPix < pend) fetch(p0 + kCorrPix);
 #pragma unroll
             for (int k4 = 0; k4 < kCorrPix; k4 += 4) {
                 const int kk = k4 + lk;
@@ -634,7 +639,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                     for (int ct = 0; ct < NCT; ct++)
                         acc[f2][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[f2], bv[ct], acc[f2][ct], 0, 0, 0);
             }
-            if ((nch & 3) == 3 || p0 + kCorrPix >= P) {
+            if ((nch & 3) == 3 || p0 + kCorrPix >= pend) {
 #pragma unroll
                 for (int f2 = 0; f2 < 2; f2++)
 #pragma unroll
@@ -666,6 +671,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                 const int j = tid / NB, n = tid - j * NB;
                 const double *w = qw + j * 6 + n;
                 qh[j * 6 + n] = ((w[0] + w[kCorrSpots * 6]) + w[2 * kCorrSpots * 6]) + w[3 * kCorrSpots * 6];
+                if (nsplit > 1 && j < nk)
+                    C.part_qh[((size_t)split * C.NI + tk.item0 + j) * 6 + n] = qh[j * 6 + n];
             }
             __syncthreads();
         }
@@ -678,6 +685,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                 if (col >= NCOL || j >= nk) continue;
                 const int k = tk.k0 + j;
                 const int item = tk.item0 + j;
+                if (nsplit > 1) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int trel = (2 * wave + f2) * 16 + 4 * lk + r;
+                        if (trel < tn)
+                            C.part[(((size_t)split * C.Tmax + t0 + trel) * C.NI + item) * 6 + n] = a64[f2][ct][r];
+                    }
+                    continue;
+                }
                 const uint8_t *stk = b.states + ((size_t)pr.s * K + k) * T;
                 const double Q = qh[j * 6], Hn = qh[j * 6 + n];
 #pragma unroll
@@ -703,6 +719,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 
 template __global__ void b3_corr_mfma<false>(CorrParams);
 template __global__ void b3_corr_mfma<true>(CorrParams);
+
+// b3_corr_reduce: the pixel-split correlation sums of one (frame, item),
+// added over the splits in order in FP64, then the on/off forms of
+// b3_corr_mfma's epilogue (Soff = S0 + [on](2C + Q), D = 2C +- Q,
+// diff_n = G_n - [off] H_n).
+__global__ __launch_bounds__(256) void b3_corr_reduce(CorrReduceParams R) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= R.NI * R.T) return;
+    const int t = i / R.NI, item = i - t * R.NI;
+    const Item it = R.items[item];
+    const DevBlock &b = *R.block;
+    double v[6] = {0, 0, 0, 0, 0, 0}, qh[6] = {0, 0, 0, 0, 0, 0};
+    for (int sp = 0; sp < R.nsplit; sp++) {
+        const double *pv = R.part + (((size_t)sp * R.Tmax + t) * R.NI + item) * 6;
+        const double *pq = R.part_qh + ((size_t)sp * R.NI + item) * 6;
+#pragma unroll
+        for (int n = 0; n < 6; n++) {
+            v[n] += pv[n];
+            qh[n] += pq[n];
+        }
+    }
+    const bool on = b.states[((size_t)it.s * b.K + it.k) * b.T + t] == 0;
+    const double S0 = R.S0[(size_t)it.s * R.Tmax + t];
+    double *m = R.em + ((size_t)t * R.NI + item) * 8;
+    const double c2 = 2.0 * v[0], Q = qh[0];
+    m[0] = on ? S0 + (c2 + Q) : S0;
+    m[1] = on ? c2 + Q : c2 - Q;
+#pragma unroll
+    for (int n = 1; n < 6; n++) m[1 + n] = on ? v[n] : v[n] - qh[n];
+}
 
 // ---------------------------------------------------------------------------
 // Kernel D (b3_hmm): one lane per item runs the FP64 HMM of kernel B from
