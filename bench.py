@@ -21,6 +21,10 @@ Beside it, in "workloads" of the same JSON line:
                 clipped to 5-120, T=120, 32 parameter sets per block,
                 checkpoint stride 11: batched vs one launch per (block,
                 parameter set), the latter on the first 100 blocks;
+  b3_functor    the CG functor of one spot of that block with 10 background
+                samples, created once and called per conjugate-gradient step
+                (vs the per-call re-upload path), and its FFBS Hessian with
+                1000 inner samples;
   b3_gibbs      Gibbs sampling of the blink states of the same 1200 blocks
                 (true spot parameters): 10 samples of one sweep per run,
                 all blocks in one launch vs one launch per block (first 20);
@@ -28,7 +32,7 @@ Beside it, in "workloads" of the same JSON line:
                 z-slice high-pass, mean/variance, scaling), and the TIFF path
                 end to end on 8 of its frames.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only c2,c3,c4,c5,gibbs]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only c2,c4,functor,c5,gibbs,c3,prep]
 
 Prints one JSON line.  value = SR voxels / s of the headline workload;
 roofline = algorithmic FP32 flops per launch / measured kernel time against
@@ -454,6 +458,86 @@ def bench_b3_single(args):
     return res
 
 
+def bench_b3_functor(args):
+    """The CG functor and the FFBS Hessian of one spot, as the 3B fit calls
+    them: SampledBackgroundData built once (multispot5.cc:538-575) for spot 0
+    of the C4 block (29^3, T=100, 40 spots, seed 6) with S = 10 background
+    samples (main.gibbs.samples, multispot5.cfg:28: the other spots at 10
+    shifted sampled state paths), then operator() per conjugate-gradient call
+    (one candidate vector) and sampled_background_spot_hessian_ffbs with
+    1000 inner samples (add_remove.hessian.inner_samples, multispot5.cfg:43)."""
+    from rfba_amd import b3
+    from oracle import b3oracle as bo
+    m = b3_model(b3, 0)
+    T, S, bs = 100, 10, 1000
+    data, params, states = b3.synth_block((29, 29, 29), T, 40, 6)
+    xyz = b3.box_pixels((29, 29, 29))
+    P = xyz.shape[0]
+    d2 = data.reshape(T, -1).astype(np.float64)
+    spots = np.stack([bo.spot(xyz, q, m.zfactor)[0] for q in params])
+    bg = np.empty((S, T, P))
+    for smp in range(S):
+        on = (np.roll(states[1:], smp, axis=1) == 0).astype(np.float64)
+        bg[smp] = 100.0 + on.T @ spots[1:]
+    phi = params[0] * [1.05, 1.02, 1, 1, 1] + [0, 0, 0.2, -0.1, 0.1]
+    t0 = time.perf_counter()
+    f = b3.Functor(xyz, d2, bg, m, max_candidates=8)
+    create_ms = (time.perf_counter() - t0) * 1e3
+    for _ in range(3):
+        f(phi)
+    n_calls = 50
+    t0 = time.perf_counter()
+    for _ in range(n_calls):
+        g, L = f(phi)
+    call_ms = (time.perf_counter() - t0) * 1e3 / n_calls
+    b3.neg_grad_sampled(xyz, d2.astype(np.float32), bg.astype(np.float32), phi[None], m)
+    t0 = time.perf_counter()
+    b3.neg_grad_sampled(xyz, d2.astype(np.float32), bg.astype(np.float32), phi[None], m)
+    old_ms = (time.perf_counter() - t0) * 1e3
+    u = np.random.default_rng(11).random((S, bs, T))
+    f.hessian_ffbs(phi, bs, u)
+    t0 = time.perf_counter()
+    H, D, paths = f.hessian_ffbs(phi, bs, u)
+    hess_ms = (time.perf_counter() - t0) * 1e3
+    f.close()
+    # the plane is read once per call: S x T x P FP32 residuals
+    by = 4.0 * S * T * P
+    roof = hbm_roof(by, call_ms)
+    roof.update({"kernel": "b3_corr_mfma (pixel-split) + b3_corr_reduce + b3_hmm, per call incl. copies",
+                 "algorithmic_bytes_per_call": by,
+                 "note": "one candidate per call: latency-bound (launches, copies and the FP64 HMM's serial frames), "
+                         "not bandwidth-bound"})
+    res = {
+        "workload": "CG functor + FFBS Hessian of one spot: 29^3 block (P=24389), T=100, 40 spots (seed 6), "
+                    "S=10 background samples, one candidate per CG call; Hessian with 1000 inner samples",
+        "create_ms": create_ms,
+        "per_cg_call": {"ms": call_ms, "calls_per_s": 1e3 / call_ms},
+        "per_cg_call_host_reupload_path": {"ms": old_ms, "what": "rfba3b_neg_grad_sampled: residuals formed and "
+                                                                 "copied on every call, per-item kernel"},
+        "speedup_resident_vs_reupload": old_ms / call_ms,
+        "hessian_ffbs": {"ms": hess_ms, "inner_samples": bs, "backward_samplings": S * bs},
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        t0 = time.perf_counter()
+        go, Lo = bo.neg_grad(xyz, d2, bg, phi, m)
+        dt_g = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        Ho, Do, po = bo.hessian_ffbs(xyz, d2, bg, phi, m, bs, u)
+        dt_h = time.perf_counter() - t0
+        res["error_vs_fp64_oracle"] = {
+            "grad_rel": float(np.linalg.norm(g[0] - go) / np.linalg.norm(go)),
+            "logL_max_rel": float(np.max(np.abs(L[0] - Lo) / np.abs(Lo))),
+            "hessian_max_normalised": float(np.abs(H - Ho).max() / np.abs(Ho).max()),
+            "hessian_paths_equal": bool(np.array_equal(paths, po))}
+        res["cpu_baseline"] = {"value": 1.0 / dt_g, "unit": "CG calls/s", "cores": 1, "kind": "port",
+                               "host": host_cpu(1),
+                               "sample": "FP64 oracle functor, one call (%.2f s); its FFBS Hessian with the same "
+                                         "draws %.2f s" % (dt_g, dt_h),
+                               "hessian_s": dt_h}
+    return res
+
+
 _C5 = {}
 
 
@@ -758,7 +842,7 @@ def bench_prep(args):
 # Estimated wall seconds of each auxiliary workload at one warm-up and two
 # timed passes, CPU legs included (measured on MI355X with 16 host threads,
 # rounded up).  A workload starts only if it fits the remaining budget.
-EST_S = {"c4": 10, "c5": 25, "gibbs": 20, "c3": 100, "prep": 15}
+EST_S = {"c4": 10, "functor": 10, "c5": 25, "gibbs": 20, "c3": 100, "prep": 15}
 C3_SERIES_S = 35  # generating the 512x512x128 T=120 series shared by c3 and prep
 
 
@@ -778,9 +862,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-crop", type=str, default="12,12,8")
     ap.add_argument("--frames", type=int, default=200)
-    ap.add_argument("--only", type=str, default="c2,c4,c5,gibbs,c3,prep",
+    ap.add_argument("--only", type=str, default="c2,c4,functor,c5,gibbs,c3,prep",
                     help="workloads to run, in this order: c2 (headline stack), c3 (batched SRRF), c4, c5 (3B), "
-                         "gibbs (3B blink-state sampling), prep (image normalisation)")
+                         "functor (3B CG functor + FFBS Hessian per call), gibbs (3B blink-state sampling), "
+                         "prep (image normalisation)")
     ap.add_argument("--c5-plain-blocks", type=int, default=100)
     ap.add_argument("--traffic", type=float, default=None,
                     help="measured HBM bytes per launch of the headline kernel (rocprofv3 FETCH_SIZE+WRITE_SIZE)")
@@ -828,6 +913,7 @@ def main():
     watchdog.daemon = True
     watchdog.start()
     runners = {"c3": ("srrf_batched", bench_srrf_batched), "c4": ("b3_single", bench_b3_single),
+               "functor": ("b3_functor", bench_b3_functor),
                "c5": ("b3_batched", bench_b3_batched), "gibbs": ("b3_gibbs", bench_b3_gibbs),
                "prep": ("prep", bench_prep)}
     try:
