@@ -379,11 +379,18 @@ size_t lds_layout(LaunchGeom &g, int TZ, const DevSamples &s) {
     g.RU = g.NU + 2 * g.hx;
     g.RV = g.NV + 2 * g.hy;
     g.RW = g.NW + 2 * g.hz;
-    // X stride of the x-interpolated planes: rows of different v shifted by
-    // half the banks so two y rows of a half-wave do not collide.
+    // X stride of the x-interpolated planes: the y pass of a wave reads 8
+    // consecutive x values from several consecutive v rows, so consecutive
+    // rows are shifted by 8, 16 or 24 of the 32 banks (2 to 4 rows
+    // disjoint); the least padding that does so.
     g.TXP = kTX + 1;
-    for (int p = 0; p < 32; p++)
-        if ((g.NW * (kTX + p)) % 32 == 16) { g.TXP = kTX + p; break; }
+    for (int p = 0; p < 32; p++) {
+        const int sh = (g.NW * (kTX + p)) % 32;
+        if (sh == 8 || sh == 16 || sh == 24) {
+            g.TXP = kTX + p;
+            break;
+        }
+    }
     g.noz = noz;
     int off = 0;
     // The raw halo is read only before the sample passes (gradient and the
