@@ -392,11 +392,25 @@ size_t lds_layout(LaunchGeom &g, int TZ, const DevSamples &s) {
         }
     }
     g.noz = noz;
+    g.NWP = g.NW + (g.NW & 1);
+    g.VS = kTX * g.NWP + 16;
+    // Column layout when it costs no occupancy tier (workgroups per CU by
+    // LDS: budgets of 53, 80, 160 KiB) against the row layout.
+    auto tier = [](size_t words) {
+        const size_t b = words * sizeof(float) + 64;
+        return b <= 53 * 1024 ? 0 : (b <= 80 * 1024 ? 1 : 2);
+    };
+    const size_t rest = (size_t)3 * g.NU * g.NV * g.NW + 2 + std::max(3 * g.NW * kThreads, g.RU * g.RV * g.RW) +
+                        kTX * 6 + TZ * noz * 8 + TZ + TZ * 6 + TZ * kThreads + s.nox + s.noy + s.noz +
+                        3 * s.ngroups + 5 * s.nruns + 2 * s.n;
+    const size_t px_row = (size_t)3 * g.NV * g.NW * g.TXP, px_col = (size_t)3 * g.NV * g.VS;
+    g.pxv = tier(rest + px_col) <= tier(rest + px_row) ? 1 : 0;
     int off = 0;
     // The raw halo is read only before the sample passes (gradient and the
     // voxel-centre intensity); the y-interpolated columns reuse its space.
     g.off_g = off; off += 3 * g.NU * g.NV * g.NW;
-    g.off_px = off; off += 3 * g.NV * g.NW * g.TXP;
+    off += off & 1;  // 8-byte aligned planes (pairs of w in the column layout)
+    g.off_px = off; off += g.pxv ? (int)px_col : (int)px_row;
     g.off_py = off;
     g.off_raw = off; off += std::max(3 * g.NW * kThreads, g.RU * g.RV * g.RW);
     g.off_xtab = off; off += kTX * 6;

@@ -63,7 +63,12 @@ struct LaunchGeom {
     int NU, NV, NW;      // gradient-plane extents held in LDS
     int hx, hy, hz;      // gradient halo
     int RU, RV, RW;      // raw extents = N + 2h
-    int TXP;             // padded X stride of the x-interpolated planes
+    int TXP;             // padded X stride of the x-interpolated planes (row layout)
+    // Column layout of the x-interpolated planes (pxv = 1): Px[c][v][X][w],
+    // the w values of a column contiguous (NWP = NW rounded up to even) and
+    // planes of one v VS floats apart (VS = 16 NWP + 16: rows of v shifted by
+    // 16 banks), so a thread's y pass reads pairs of w from one base.
+    int pxv, NWP, VS;
     int noz;
     // float offsets into dynamic LDS
     int off_raw, off_g, off_px, off_py, off_xtab, off_ztab, off_zc, off_ctab, off_dsum, off_stab, off_total;

@@ -205,7 +205,7 @@ __global__ void copy_probe(const float4 *__restrict__ src, float4 *__restrict__ 
 
 template <typename Tin, int TZ, bool RP2>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void srrf_fused(LaunchParams P) {
-    extern __shared__ float lds[];
+    extern __shared__ __align__(16) float lds[];
     __shared__ int s_tile;
     const LaunchGeom &g = P.g;
     float *raw = lds + g.off_raw;
@@ -265,6 +265,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
     // end of sample-table setup
     float *scr = P.scratch + (long long)blockIdx.x * P.scratch_stride;
     const int NU = g.NU, NV = g.NV, NW = g.NW, TXP = g.TXP, noz = g.noz;
+    const bool pxv = g.pxv != 0;
+    const int NWP = g.NWP, VS = g.VS;
     const int RU = g.RU, RV = g.RV;
 
     for (;;) {
@@ -449,7 +451,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                     p = p + rr[1] * xt[2];
                     p = p + rr[2] * xt[3];
                     p = p + rr[3] * xt[4];
-                    Px[(v * NW + w) * TXP + X] = p;
+                    Px[pxv ? v * VS + X * NWP + w : (v * NW + w) * TXP + X] = p;
                 }
                 __syncthreads();
                 int vb, validy;
@@ -458,8 +460,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                 const int vrel = clampi(vb - vlo, 0, NV - 4);
                 const int validx = !((Xc / (float)M) < 1.5f || (Xc / (float)M) > (float)W - 1.5f);
                 for (int w = 0; w < nw; w++) {
-                    const float *pp = Px + (vrel * NW + w) * TXP + xl;
-                    const int vs = NW * TXP;
+                    const float *pp = Px + (pxv ? vrel * VS + xl * NWP + w : (vrel * NW + w) * TXP + xl);
+                    const int vs = pxv ? VS : NW * TXP;
                     float q = 0.0f;
                     q = q + pp[0] * wy0;
                     q = q + pp[vs] * wy1;
@@ -513,7 +515,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                             p = p + gg[1] * w1;
                             p = p + gg[2] * w2;
                             p = p + gg[3] * w3;
-                            Px[((c * NV + v) * NW + w) * TXP + X] = p;
+                            Px[pxv ? (c * NV + v) * VS + X * NWP + w : ((c * NV + v) * NW + w) * TXP + X] = p;
                         }
                     }
                 }
@@ -534,16 +536,49 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                     if (!validy) wy0 = wy1 = wy2 = wy3 = 0.0f;
                     const int vrel = clampi(vb - vlo, 0, NV - 4);
                     // ---- y pass: Py[w][c][tid] = sum_j Px[c][vrel+j][w][X] * wy_j
-                    for (int w = 0; w < nw; w++) {
+                    if (pxv) {
+                        // column layout: the 3 x 4 tap rows of two w values
+                        // are loaded as pairs first, then summed in order
+                        const float *pb = Px + vrel * VS + xl * NWP;
+#pragma unroll 4
+                        for (int w = 0; w < nw; w += 2) {
+                            float2 pv[3][4];
 #pragma unroll
-                        for (int c = 0; c < 3; c++) {
-                            const float *pp = Px + ((c * NV + vrel) * NW + w) * TXP + xl;
-                            const int vs = NW * TXP;
-                            float q = pp[0] * wy0;
-                            q = q + pp[vs] * wy1;
-                            q = q + pp[2 * vs] * wy2;
-                            q = q + pp[3 * vs] * wy3;
-                            Py[(w * 3 + c) * kThreads + tid] = q;
+                            for (int c = 0; c < 3; c++)
+#pragma unroll
+                                for (int j = 0; j < 4; j++)
+                                    pv[c][j] = *(const float2 *)(pb + (c * NV + j) * VS + w);
+#pragma unroll
+                            for (int c = 0; c < 3; c++) {
+                                float q = pv[c][0].x * wy0;
+                                q = q + pv[c][1].x * wy1;
+                                q = q + pv[c][2].x * wy2;
+                                q = q + pv[c][3].x * wy3;
+                                Py[(w * 3 + c) * kThreads + tid] = q;
+                            }
+                            if (w + 1 < nw) {
+#pragma unroll
+                                for (int c = 0; c < 3; c++) {
+                                    float q = pv[c][0].y * wy0;
+                                    q = q + pv[c][1].y * wy1;
+                                    q = q + pv[c][2].y * wy2;
+                                    q = q + pv[c][3].y * wy3;
+                                    Py[((w + 1) * 3 + c) * kThreads + tid] = q;
+                                }
+                            }
+                        }
+                    } else {
+                        for (int w = 0; w < nw; w++) {
+#pragma unroll
+                            for (int c = 0; c < 3; c++) {
+                                const float *pp = Px + ((c * NV + vrel) * NW + w) * TXP + xl;
+                                const int vs = NW * TXP;
+                                float q = pp[0] * wy0;
+                                q = q + pp[vs] * wy1;
+                                q = q + pp[2 * vs] * wy2;
+                                q = q + pp[3 * vs] * wy3;
+                                Py[(w * 3 + c) * kThreads + tid] = q;
+                            }
                         }
                     }
                     const int s0 = st_run[5 * r], ns = st_run[5 * r + 1];
