@@ -96,8 +96,12 @@ __device__ __forceinline__ float term_general(float Gx, float Gy, float Gz, cons
 // Branch-free form, bit-identical to term_general whenever `bad` stays
 // false: both sqrt arguments lie in [2^-96, inf) (there the compiler's sqrtf
 // reduces to v_sqrt_f32 plus the FMA-residual choice between neighbours) and
-// the sign of num / (GMag * R) is decided without dividing (finite positive
-// denominator, numerator <= 0 or >= den * 2^-64, or NaN).
+// the sign of num / (GMag * R) is decided without dividing.  With g2 in
+// [2^-96, 2^100] and R in [2^-20, 2^20] (checked on the host, else every
+// sample takes the general form), den = GMag * R lies in [2^-68, 2^70]; a
+// numerator >= 2^-60 then gives a quotient >= 2^-130, which rounds to a
+// positive value, a numerator <= 0 a quotient that is not > 0, and a NaN
+// numerator a NaN: so "diverging" is !(num <= 0) unless 0 < num < 2^-60.
 __device__ __forceinline__ float sqrt_core(float x) {
     const float s = __builtin_amdgcn_sqrtf(x);
     const float sd = __int_as_float(__float_as_int(s) - 1);
@@ -115,7 +119,6 @@ __device__ __forceinline__ float term_fast(float Gx, float Gy, float Gz, const T
     const float GMag = sqrt_core(g2);
     float num = Gx * a.rx + Gy * a.ry;
     num = num + Gz * a.rz;
-    const float den = GMag * R;
     const bool dv = !(num <= 0.0f);
     float tn = Gx * a.dx + Gy * a.dy;
     tn = tn + Gz * a.dz;
@@ -126,11 +129,8 @@ __device__ __forceinline__ float term_fast(float Gx, float Gy, float Gz, const T
     float Dk = sqrt_core(d2);
     Dk = 1.0f - (RP2 ? Dk * invR : Dk / R);
     Dk = Dk * Dk;
-    // With R in [2^-20, 2^20] (checked on the host, else every sample takes
-    // the general form) a finite g2 >= 2^-96 makes den finite and positive;
-    // infinite or NaN d2 come out of sqrt_core as sqrtf gives them.
-    bad = !(g2 >= 0x1p-96f && g2 < __builtin_inff()) || !(d2 >= 0x1p-96f) ||
-          (num > 0.0f && num < den * 0x1p-64f);
+    // infinite or NaN d2 come out of sqrt_core as sqrtf gives them
+    bad = !(g2 >= 0x1p-96f && g2 <= 0x1p100f) || !(d2 >= 0x1p-96f) || (num > 0.0f && num < 0x1p-60f);
     return dv ? -Dk : Dk;
 }
 
