@@ -22,7 +22,7 @@ __global__ void b3_resid(const DevBlock *__restrict__ blocks, const PlaneTask *_
                          float *__restrict__ R, double *__restrict__ S0, int s0_stride,
                          const DevModel *__restrict__ mdl);
 template <int MODE> __global__ void b3_eval(EvalParams E);
-template <bool GRAD> __global__ void b3_corr_mfma(CorrParams C);
+template <bool GRAD, bool FUNC> __global__ void b3_corr_mfma(CorrParams C);
 template <bool GRAD> __global__ void b3_hmm(HmmParams H);
 struct LogA {
     double v[9];
@@ -200,7 +200,9 @@ static void batch_free(rfba3b_batch *b) {
 
 typedef void (*CorrFn)(CorrParams);
 typedef void (*HmmFn)(HmmParams);
-static CorrFn corr_fn(bool grad) { return grad ? b3_corr_mfma<true> : b3_corr_mfma<false>; }
+static CorrFn corr_fn(bool grad) { return grad ? b3_corr_mfma<true, false> : b3_corr_mfma<false, false>; }
+// the resident functor's form: region mask and pixel splits
+static CorrFn corr_fn_functor() { return b3_corr_mfma<true, true>; }
 static HmmFn hmm_fn(bool grad) { return grad ? b3_hmm<true> : b3_hmm<false>; }
 
 static size_t corr_lds(bool grad, int naxmax) {
@@ -858,7 +860,7 @@ extern "C" int rfba3b_functor_create(const int *xyz, int P, int T, const double 
     if (e == hipSuccess) e = hipMemset(f->d_states, 1, NI * (size_t)T);  // every candidate off in every frame
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e == hipSuccess && f->lds_corr > 64 * 1024)
-        e = hipFuncSetAttribute((const void *)corr_fn(true), hipFuncAttributeMaxDynamicSharedMemorySize,
+        e = hipFuncSetAttribute((const void *)corr_fn_functor(), hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)f->lds_corr);
     if (e != hipSuccess) {
         const int code = e == hipErrorOutOfMemory ? -2 : -3;
@@ -927,7 +929,7 @@ extern "C" int rfba3b_functor_eval(rfba3b_functor *f, const double *phi, int n, 
             C.part = f->d_part;
             C.part_qh = f->d_part_qh;
         }
-        hipLaunchKernelGGL(corr_fn(true), dim3((nc * nsplit + 7) / 8 * 8), dim3(256), f->lds_corr, st, C);
+        hipLaunchKernelGGL(corr_fn_functor(), dim3((nc * nsplit + 7) / 8 * 8), dim3(256), f->lds_corr, st, C);
         HIP_TRY(hipGetLastError());
         if (nsplit > 1) {
             CorrReduceParams Rd;

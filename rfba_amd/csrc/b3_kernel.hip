@@ -432,7 +432,9 @@ __global__ __launch_bounds__(256) void b3_eval(EvalParams E) {
 // Each lane writes the em fields of its own (frame, column) results.
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-template <bool GRAD>
+// FUNC: the resident functor's form (a region mask, pixel splits with raw
+// partial sums); the batched form compiles without them.
+template <bool GRAD, bool FUNC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void b3_corr_mfma(CorrParams C) {
     extern __shared__ double ldsc[];
     constexpr int NB = GRAD ? 6 : 1;
@@ -447,7 +449,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     // grid is padded to a multiple of 8).
     const int per_xcd = gridDim.x >> 3;
     const int vtask = (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-    const int nsplit = C.nsplit > 1 ? C.nsplit : 1;
+    const int nsplit = FUNC && C.nsplit > 1 ? C.nsplit : 1;
     if (vtask >= C.ntasks * nsplit) return;
     const CorrTask tk = C.tasks[vtask / nsplit];
     const int split = vtask - (vtask / nsplit) * nsplit;
@@ -458,8 +460,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     const int nk = tk.nk;
     const DevModel &md = *C.model;
     // pixel range of this (split) task
-    const int pbeg = nsplit > 1 ? split * C.pstep : 0;
-    const int pend = nsplit > 1 ? min(P, pbeg + C.pstep) : P;
+    const int pbeg = FUNC && nsplit > 1 ? split * C.pstep : 0;
+    const int pend = FUNC && nsplit > 1 ? min(P, pbeg + C.pstep) : P;
 
     double *qh = ldsc;                                   // [8][6] Q, H_0..4
     double *qw = qh + kCorrSpots * 6;                    // [4 waves][8][6] partial Q, H
@@ -607,7 +609,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                 for (int n = 0; n < NB; n++) v[n] = 0.0f;
                 // a region given as a box and a mask: pixels outside it
                 // carry no spot value (their residuals are zero as well)
-                if (jb < nk && p < P && (!b.mask || b.mask[p])) basis(bx, by, bz, v);
+                if (jb < nk && p < P && (!FUNC || !b.mask || b.mask[p])) basis(bx, by, bz, v);
 #pragma unroll
                 for (int n = 0; n < NB; n++) Bs[cb * BSTR + jb * NB + n] = v[n];
                 // every pass sees the same values; only the first pass's sums are used
@@ -671,7 +673,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                 const int j = tid / NB, n = tid - j * NB;
                 const double *w = qw + j * 6 + n;
                 qh[j * 6 + n] = ((w[0] + w[kCorrSpots * 6]) + w[2 * kCorrSpots * 6]) + w[3 * kCorrSpots * 6];
-                if (nsplit > 1 && j < nk)
+                if (FUNC && nsplit > 1 && j < nk)
                     C.part_qh[((size_t)split * C.NI + tk.item0 + j) * 6 + n] = qh[j * 6 + n];
             }
             __syncthreads();
@@ -685,7 +687,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
                 if (col >= NCOL || j >= nk) continue;
                 const int k = tk.k0 + j;
                 const int item = tk.item0 + j;
-                if (nsplit > 1) {
+                if (FUNC && nsplit > 1) {
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
                         const int trel = (2 * wave + f2) * 16 + 4 * lk + r;
@@ -717,8 +719,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     }
 }
 
-template __global__ void b3_corr_mfma<false>(CorrParams);
-template __global__ void b3_corr_mfma<true>(CorrParams);
+template __global__ void b3_corr_mfma<false, false>(CorrParams);
+template __global__ void b3_corr_mfma<true, false>(CorrParams);
+template __global__ void b3_corr_mfma<true, true>(CorrParams);
 
 // b3_corr_reduce: the pixel-split correlation sums of one (frame, item),
 // added over the splits in order in FP64, then the on/off forms of
