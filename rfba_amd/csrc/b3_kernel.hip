@@ -774,15 +774,24 @@ __global__ __launch_bounds__(256) void b3_hmm(HmmParams H) {
     const int ckpt = H.ckpt > 0 ? min(H.ckpt, T) : (int)ceil(sqrt((double)T));
     const int nseg = (T + ckpt - 1) / ckpt;
     const size_t NI = (size_t)H.NI;
-    double *ck = H.ck + li;  // slot u at ck[u * NI]
-    auto emr = [&](int t) { return H.em + ((size_t)t * NI + li) * 8; };
+    // Each frame's sums are loaded one frame ahead of the recursion that
+    // needs them (the recursion is a serial chain per lane, so a load issued
+    // in the iteration that uses it would expose its whole latency).
+    double *__restrict__ ck = H.ck + li;               // slot u at ck[u * NI]
+    const double *__restrict__ emb = H.em + li * 8;    // frame t at emb[t * es]
+    const size_t es = NI * 8;
     const double ninf = -__builtin_inf();
     double am[3] = {0.0, 0.0, 0.0}, ae[3] = {ninf, ninf, ninf};
     double L = 0.0;
+    double n0 = emb[0], n1 = emb[1];
     for (int t = 0; t < T; t++) {
-        const double *m8 = emr(t);
-        const double e1 = -m8[0] * i2v - cst;
-        const double dlt = m8[1] * i2v;
+        const double c0 = n0, c1 = n1;
+        if (t + 1 < T) {
+            n0 = emb[(t + 1) * es];
+            n1 = emb[(t + 1) * es + 1];
+        }
+        const double e1 = -c0 * i2v - cst;
+        const double dlt = c1 * i2v;
         if (t % ckpt == 0) {
             const int u = 6 * (t / ckpt);
 #pragma unroll
@@ -810,22 +819,40 @@ __global__ __launch_bounds__(256) void b3_hmm(HmmParams H) {
             arm[j] = ck[(6 * sgi + j) * NI];
             are[j] = ck[(6 * sgi + 3 + j) * NI];
         }
+        double p1 = emb[t0 * es + 1];
         for (int t = t0; t < t1; t++) {
+            const double c1 = p1;
+            if (t + 1 < t1) p1 = emb[(t + 1) * es + 1];
             double cm, ce;
-            xfstep(h, arm, are, xemis(emr(t)[1] * i2v), t == 0, cm, ce);
+            xfstep(h, arm, are, xemis(c1 * i2v), t == 0, cm, ce);
             ck[(sa0 + 2 * (t - t0)) * NI] = arm[0];
             ck[(sa0 + 2 * (t - t0) + 1) * NI] = are[0];
             ck[(sc0 + 2 * (t - t0)) * NI] = cm;
             ck[(sc0 + 2 * (t - t0) + 1) * NI] = ce;
         }
+        // frame t's alpha_t(on), c_t and sums, loaded during frame t + 1
+        auto fetch = [&](int t, double (&v)[10]) {
+            const double *m8 = emb + t * es;
+#pragma unroll
+            for (int n = 0; n < 6; n++) v[n] = m8[1 + n];
+            v[6] = ck[(sa0 + 2 * (t - t0)) * NI];
+            v[7] = ck[(sa0 + 2 * (t - t0) + 1) * NI];
+            v[8] = ck[(sc0 + 2 * (t - t0)) * NI];
+            v[9] = ck[(sc0 + 2 * (t - t0) + 1) * NI];
+        };
+        double nx[10];
+        fetch(t1 - 1, nx);
         for (int t = t1 - 1; t >= t0; t--) {
+            double cur[10];
+#pragma unroll
+            for (int n = 0; n < 10; n++) cur[n] = nx[n];
+            if (t - 1 >= t0) fetch(t - 1, nx);
             if (t < T - 1) xbeta(h, bm, be, bn, cnm, cne);
-            const double *m8 = emr(t);
-            const double gam = xvalue(ck[(sa0 + 2 * (t - t0)) * NI] * bm[0], ck[(sa0 + 2 * (t - t0) + 1) * NI] + be[0]);
-            for (int n = 0; n < 5; n++) g[n] += gam * m8[2 + n];
-            bn = xemis(m8[1] * i2v);
-            cnm = ck[(sc0 + 2 * (t - t0)) * NI];
-            cne = ck[(sc0 + 2 * (t - t0) + 1) * NI];
+            const double gam = xvalue(cur[6] * bm[0], cur[7] + be[0]);
+            for (int n = 0; n < 5; n++) g[n] += gam * cur[1 + n];
+            bn = xemis(cur[0] * i2v);
+            cnm = cur[8];
+            cne = cur[9];
         }
     }
     const double iv = 1.0 / md.variance;
