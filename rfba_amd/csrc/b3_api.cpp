@@ -23,7 +23,8 @@ __global__ void b3_resid(const DevBlock *__restrict__ blocks, const PlaneTask *_
                          const DevModel *__restrict__ mdl);
 template <int MODE> __global__ void b3_eval(EvalParams E);
 template <bool GRAD, bool FUNC> __global__ void b3_corr_mfma(CorrParams C);
-template <bool GRAD> __global__ void b3_hmm(HmmParams H);
+template <int MODE> __global__ void b3_hmm(HmmParams H);
+template <int MODE> __global__ void b3_hmm_wave(HmmParams H);
 struct LogA {
     double v[9];
 };
@@ -203,7 +204,29 @@ typedef void (*HmmFn)(HmmParams);
 static CorrFn corr_fn(bool grad) { return grad ? b3_corr_mfma<true, false> : b3_corr_mfma<false, false>; }
 // the resident functor's form: region mask and pixel splits
 static CorrFn corr_fn_functor() { return b3_corr_mfma<true, true>; }
-static HmmFn hmm_fn(bool grad) { return grad ? b3_hmm<true> : b3_hmm<false>; }
+
+// The HMMs of a launch: one wave per item (b3_hmm_wave, the frames of an
+// item spread over the lanes) when every block's T fits the wave's
+// kWaveTmax frames, else one lane per item (b3_hmm, a serial chain of T
+// frames per item).  The choice depends on T only, so that a batched
+// launch and one launch per block or parameter set give the same bits.
+// Measured on MI355X (29^3 block, T = 100): a functor call of 10 items
+// takes 0.39 ms with the lane form and 0.15 ms with the wave form; one
+// launch of 10240 items costs the same in both forms.
+static int launch_hmm(int mode, const HmmParams &H, int Tmax, hipStream_t st) {
+    if (H.n <= 0) return 0;
+    if (Tmax <= kWaveTmax) {
+        const HmmFn fn = mode == MODE_GRAD    ? b3_hmm_wave<MODE_GRAD>
+                         : mode == MODE_DELTA ? b3_hmm_wave<MODE_DELTA>
+                                              : b3_hmm_wave<MODE_LOGL>;
+        hipLaunchKernelGGL(fn, dim3((H.n + 3) / 4), dim3(256), 0, st, H);
+    } else {
+        const HmmFn fn = mode == MODE_GRAD ? b3_hmm<MODE_GRAD> : mode == MODE_DELTA ? b3_hmm<MODE_DELTA> : b3_hmm<MODE_LOGL>;
+        hipLaunchKernelGGL(fn, dim3((H.n + 255) / 256), dim3(256), 0, st, H);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
 
 static size_t corr_lds(bool grad, int naxmax) {
     // Q/H [8][6] | per-wave Q/H [4][8][6] | residual chunk [128][36] |
@@ -407,8 +430,8 @@ static int enqueue_range(rfba3b_batch *b, hipStream_t st, const rfba3b_batch::Ch
         H.em = b->d_em;
         H.ck = b->d_ck;
         H.model = b->d_model;
-        hipLaunchKernelGGL(hmm_fn(b->grad), dim3((nitem + 255) / 256), dim3(256), 0, st, H);
-        HIP_TRY(hipGetLastError());
+        const int rc = launch_hmm(b->grad ? MODE_GRAD : MODE_LOGL, H, b->Tmax, st);
+        if (rc) return rc;
     }
     return 0;
 }
@@ -871,6 +894,82 @@ extern "C" int rfba3b_functor_create(const int *xyz, int P, int T, const double 
     return 0;
 }
 
+// The per-frame sums em of the m candidates staged at the start of h_stage
+// over the handle's S background samples (item s m + k: candidate k over
+// background s): correlation, then the reduction of the pixel splits.
+// Called with g_mu held.
+static int functor_sums(rfba3b_functor *f, int m, hipStream_t st) {
+    const int S = f->S, T = f->T;
+    CorrTask *ct = (CorrTask *)(f->h_stage + f->o_corr);
+    Item *it = (Item *)(f->h_stage + f->o_items);
+    const long long plane = (long long)T * f->X * f->Y * f->Z;
+    int nc = 0;
+    for (int s = 0; s < S; s++) {
+        for (int k0 = 0; k0 < m; k0 += kCorrSpots) ct[nc++] = {s, k0, std::min(kCorrSpots, m - k0), s * m + k0};
+        for (int k = 0; k < m; k++) it[s * m + k] = {0, 0, k, s, s * plane};
+    }
+    const int NI = S * m;
+    HIP_TRY(hipMemcpyAsync(f->d_params, f->h_stage, (size_t)m * 5 * sizeof(double), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(f->d_corr, ct, nc * sizeof(CorrTask), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(f->d_items, it, NI * sizeof(Item), hipMemcpyHostToDevice, st));
+    CorrParams C;
+    memset(&C, 0, sizeof C);
+    C.blocks = f->d_block;
+    C.planes = f->d_planes;
+    C.tasks = f->d_corr;
+    C.ntasks = nc;
+    C.naxmax = f->X + f->Y + f->Z;
+    C.Tmax = T;
+    C.NI = NI;
+    C.R = f->d_R;
+    C.S0 = f->d_S0;
+    C.model = f->d_model;
+    C.em = f->d_em;
+    int nsplit = 1, pstep = 0;
+    functor_split(f, m, nsplit, pstep);
+    if (nsplit > 1) {
+        C.nsplit = nsplit;
+        C.pstep = pstep;
+        C.part = f->d_part;
+        C.part_qh = f->d_part_qh;
+    }
+    hipLaunchKernelGGL(corr_fn_functor(), dim3((nc * nsplit + 7) / 8 * 8), dim3(256), f->lds_corr, st, C);
+    HIP_TRY(hipGetLastError());
+    if (nsplit > 1) {
+        CorrReduceParams Rd;
+        Rd.block = f->d_block;
+        Rd.items = f->d_items;
+        Rd.NI = NI;
+        Rd.T = T;
+        Rd.nsplit = nsplit;
+        Rd.Tmax = T;
+        Rd.part = f->d_part;
+        Rd.part_qh = f->d_part_qh;
+        Rd.S0 = f->d_S0;
+        Rd.em = f->d_em;
+        hipLaunchKernelGGL(b3_corr_reduce, dim3((NI * T + 255) / 256), dim3(256), 0, st, Rd);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+// The HMM launch over the NI items of functor_sums.
+static HmmParams functor_hmm(const rfba3b_functor *f, int NI) {
+    HmmParams H;
+    memset(&H, 0, sizeof H);
+    H.blocks = f->d_block;
+    H.items = f->d_items;
+    H.lo = 0;
+    H.n = NI;
+    H.NI = NI;
+    H.ckpt = f->model.checkpoint;
+    H.ckslots = f->ckslots;
+    H.em = f->d_em;
+    H.ck = f->d_ck;
+    H.model = f->d_model;
+    return H;
+}
+
 extern "C" int rfba3b_functor_eval(rfba3b_functor *f, const double *phi, int n, double *grad, double *logL,
                                    void *stream) {
     if (!f) return fail(-1, "handle is NULL");
@@ -894,72 +993,12 @@ extern "C" int rfba3b_functor_eval(rfba3b_functor *f, const double *phi, int n, 
         }
     }
     if (m == 0) return 0;
-    CorrTask *ct = (CorrTask *)(f->h_stage + f->o_corr);
-    Item *it = (Item *)(f->h_stage + f->o_items);
-    const long long plane = (long long)T * f->X * f->Y * f->Z;
-    int nc = 0;
-    for (int s = 0; s < S; s++) {
-        for (int k0 = 0; k0 < m; k0 += kCorrSpots) ct[nc++] = {s, k0, std::min(kCorrSpots, m - k0), s * m + k0};
-        for (int k = 0; k < m; k++) it[s * m + k] = {0, 0, k, s, s * plane};
-    }
     const int NI = S * m;
     {
         std::lock_guard<std::mutex> lk(g_mu);
-        HIP_TRY(hipMemcpyAsync(f->d_params, sp, (size_t)m * 5 * sizeof(double), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(f->d_corr, ct, nc * sizeof(CorrTask), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(f->d_items, it, NI * sizeof(Item), hipMemcpyHostToDevice, st));
-        CorrParams C;
-        memset(&C, 0, sizeof C);
-        C.blocks = f->d_block;
-        C.planes = f->d_planes;
-        C.tasks = f->d_corr;
-        C.ntasks = nc;
-        C.naxmax = f->X + f->Y + f->Z;
-        C.Tmax = T;
-        C.NI = NI;
-        C.R = f->d_R;
-        C.S0 = f->d_S0;
-        C.model = f->d_model;
-        C.em = f->d_em;
-        int nsplit = 1, pstep = 0;
-        functor_split(f, m, nsplit, pstep);
-        if (nsplit > 1) {
-            C.nsplit = nsplit;
-            C.pstep = pstep;
-            C.part = f->d_part;
-            C.part_qh = f->d_part_qh;
-        }
-        hipLaunchKernelGGL(corr_fn_functor(), dim3((nc * nsplit + 7) / 8 * 8), dim3(256), f->lds_corr, st, C);
-        HIP_TRY(hipGetLastError());
-        if (nsplit > 1) {
-            CorrReduceParams Rd;
-            Rd.block = f->d_block;
-            Rd.items = f->d_items;
-            Rd.NI = NI;
-            Rd.T = T;
-            Rd.nsplit = nsplit;
-            Rd.Tmax = T;
-            Rd.part = f->d_part;
-            Rd.part_qh = f->d_part_qh;
-            Rd.S0 = f->d_S0;
-            Rd.em = f->d_em;
-            hipLaunchKernelGGL(b3_corr_reduce, dim3((NI * T + 255) / 256), dim3(256), 0, st, Rd);
-            HIP_TRY(hipGetLastError());
-        }
-        HmmParams H;
-        memset(&H, 0, sizeof H);
-        H.blocks = f->d_block;
-        H.items = f->d_items;
-        H.lo = 0;
-        H.n = NI;
-        H.NI = NI;
-        H.ckpt = f->model.checkpoint;
-        H.ckslots = f->ckslots;
-        H.em = f->d_em;
-        H.ck = f->d_ck;
-        H.model = f->d_model;
-        hipLaunchKernelGGL(hmm_fn(true), dim3((NI + 255) / 256), dim3(256), 0, st, H);
-        HIP_TRY(hipGetLastError());
+        int rc = functor_sums(f, m, st);
+        if (!rc) rc = launch_hmm(MODE_GRAD, functor_hmm(f, NI), T, st);
+        if (rc) return rc;
         // outputs [k][s] of the first m candidates (the block's K is nmax)
         HIP_TRY(hipMemcpyAsync(f->h_stage + f->o_logL, f->d_logL, (size_t)NI * sizeof(double),
                                hipMemcpyDeviceToHost, st));
@@ -1073,8 +1112,6 @@ extern "C" int rfba3b_functor_hessian_ffbs(rfba3b_functor *f, const double *phi,
     if (nchain > (size_t)1 << 24 || nchain * T > (size_t)1 << 31) return fail(-1, "S x bs_iterations x T too large");
     const int nbas = (int)((Pd + 255) / 256);
     const int ngrp = (T + kHessFrames - 1) / kHessFrames;
-    const int ck = ckpt_of(&f->model, T);
-    const size_t lds = eval_lds(T, X * Y, Z, ck);
     hipStream_t st = (hipStream_t)stream;
     std::vector<double> parts((size_t)S * T * kHessParts), fixed((size_t)nbas * kHessUpper);
     std::vector<uint8_t> path(nchain * T);
@@ -1084,7 +1121,6 @@ extern "C" int rfba3b_functor_hessian_ffbs(rfba3b_functor *f, const double *phi,
         auto al = [&](void **p, size_t bytes) {
             if (e == hipSuccess && !*p) e = hipMalloc(p, std::max<size_t>(bytes, 16));
         };
-        al((void **)&f->d_counter, sizeof(int));
         al((void **)&f->d_basis, Pd * kHessBasis * sizeof(float));
         al((void **)&f->d_fixed, (size_t)nbas * kHessUpper * sizeof(double));
         al((void **)&f->d_parts, (size_t)S * T * kHessParts * sizeof(double));
@@ -1100,28 +1136,17 @@ extern "C" int rfba3b_functor_hessian_ffbs(rfba3b_functor *f, const double *phi,
             if (e == hipSuccess) f->paths_cap = nchain;
         }
         if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? -2 : -3, "Hessian buffers: %s", hipGetErrorString(e));
-        EvalFn fn = eval_fn(MODE_DELTA);
-        int grid = 0, rc;
-        if ((rc = eval_grid(fn, lds, S, grid))) return rc;
-        // the spot and one item per background sample (staging as in eval)
-        double *sp = (double *)f->h_stage;
-        for (int c = 0; c < 5; c++) sp[c] = phi[c];
-        Item *it = (Item *)(f->h_stage + f->o_items);
-        const long long plane = (long long)T * (long long)Pd;
-        for (int s = 0; s < S; s++) it[s] = {0, 0, 0, s, s * plane};
-        HIP_TRY(hipMemcpyAsync(f->d_params, sp, 5 * sizeof(double), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(f->d_items, it, S * sizeof(Item), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(f->d_unif, uniforms, nchain * T * sizeof(double), hipMemcpyHostToDevice, st));
-        // forward messages of the spot over each background (rmode 1: the
+        // forward messages of the spot over each background (the handle's
         // planes are data - bg), then bs backward samplings per background
-        EvalParams E;
-        memset(&E, 0, sizeof E);
-        E.blocks = f->d_block; E.items = f->d_items; E.nitems = S; E.counter = f->d_counter; E.R = f->d_R;
-        E.model = f->d_model; E.rmode = 1; E.ckpt = f->model.checkpoint; E.Tmax = T; E.XYmax = X * Y; E.Zmax = Z;
-        E.delta_out = f->d_delta;
-        hipLaunchKernelGGL(b3_reset, dim3(1), dim3(64), 0, st, f->d_counter);
-        hipLaunchKernelGGL(fn, dim3(grid), dim3(256), lds, st, E);
-        HIP_TRY(hipGetLastError());
+        for (int c = 0; c < 5; c++) ((double *)f->h_stage)[c] = phi[c];
+        int rc = functor_sums(f, 1, st);
+        if (!rc) {
+            HmmParams H = functor_hmm(f, S);
+            H.delta = f->d_delta;
+            rc = launch_hmm(MODE_DELTA, H, T, st);
+        }
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(f->d_unif, uniforms, nchain * T * sizeof(double), hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(b3_backward_sample, dim3((unsigned)((nchain + 255) / 256)), dim3(256), 0, st, f->d_delta,
                            (int)nchain, bs, T, log_a(&f->model), f->d_unif, f->d_paths);
         HIP_TRY(hipGetLastError());

@@ -28,6 +28,7 @@
 #include <vector>
 
 #include "b3_internal.h"
+#include "b3_wave.h"
 
 namespace rfba3b {
 
@@ -150,12 +151,6 @@ __device__ void mt_twist64(uint32_t *mt, int lane) {
     __syncthreads();
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
-}
-
 // select_random_element over weights w (forward_algorithm.h:502-513): the
 // first state whose running sum reaches u * (w0 + w1 + w2), else the last.
 __device__ __forceinline__ int select3(double w0, double w1, double w2, double u) {
@@ -171,85 +166,14 @@ __device__ __forceinline__ int select3(double w0, double w1, double w2, double u
     return 2;
 }
 
-// The value of lane l for the whole wave (v_readlane); l must be uniform.
-__device__ __forceinline__ double read_lane(double v, int l) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)b, l);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-
-// R = S Q for 3 x 3 matrices in extended range, shifted so that the largest
-// entry has exponent 0..1.
-__device__ __forceinline__ void xmatmul(const double (&Sm)[9], const double (&Se)[9], const double (&Qm)[9],
-                                        const double (&Qe)[9], double (&Rm)[9], double (&Re)[9]) {
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-            xsum3n(Sm[3 * j] * Qm[i], Se[3 * j] + Qe[i], Sm[3 * j + 1] * Qm[3 + i], Se[3 * j + 1] + Qe[3 + i],
-                   Sm[3 * j + 2] * Qm[6 + i], Se[3 * j + 2] + Qe[6 + i], Rm[3 * j + i], Re[3 * j + i]);
-    xshift(Re);
-}
-
-// w = M v in extended range, shifted so that its largest entry has
-// exponent 0..1.
-__device__ __forceinline__ void xmatvec(const double (&Mm)[9], const double (&Me)[9], const double (&vm)[3],
-                                        const double (&ve)[3], double (&wm)[3], double (&we)[3]) {
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-        xsum3n(Mm[3 * j] * vm[0], Me[3 * j] + ve[0], Mm[3 * j + 1] * vm[1], Me[3 * j + 1] + ve[1],
-               Mm[3 * j + 2] * vm[2], Me[3 * j + 2] + ve[2], wm[j], we[j]);
-    xshift(we);
-}
-
-// Prefix products across the wave, in extended range: on entry (Mm, Me) is
-// lane l's matrix L_l; on return it is L_l L_{l-1} ... L_0.
-__device__ __forceinline__ void prefix_products(double (&Mm)[9], double (&Me)[9], int lane) {
-#pragma unroll
-    for (int d = 1; d < kSweepThreads; d <<= 1) {
-        double Qm[9], Qe[9];
-#pragma unroll
-        for (int q = 0; q < 9; q++) {
-            Qm[q] = __shfl_up(Mm[q], d);
-            Qe[q] = __shfl_up(Me[q], d);
-        }
-        if (lane >= d) {
-            double Rm[9], Re[9];
-            xmatmul(Mm, Me, Qm, Qe, Rm, Re);
-#pragma unroll
-            for (int q = 0; q < 9; q++) {
-                Mm[q] = Rm[q];
-                Me[q] = Re[q];
-            }
-        }
-    }
-}
-
-// The transfer matrix of frame t in extended range: M_t[j][i] = b_j A[i][j]
-// (b_j pi_j in frame 0), b_on = e^min(x,0), b_off = e^min(-x,0) with
-// x = B_on - B_off = -(S2 - 2 (C_t - A_t)) / (2 var); the identity past T.
+// The transfer matrix of frame t (transfer_matrix) with x = B_on - B_off =
+// -(S2 - 2 (C_t - A_t)) / (2 var); the identity past T.
 __device__ __forceinline__ void frame_matrix(int t, int T, double S2, const double *Ck, const double *Ak, double i2v,
                                              const double *A, const double *pi, double (&Mm)[9], double (&Me)[9]) {
-    if (t < T) {
-        const double d = S2 - 2.0 * (Ck[t] - Ak[t]);
-        const double x = -d * i2v;
-        double bm[2], be[2];
-        xexp(fmin(x, 0.0), bm[0], be[0]);   // b_on
-        xexp(fmin(-x, 0.0), bm[1], be[1]);  // b_off
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-                xsplit((t == 0 ? pi[j] : A[3 * i + j]) * bm[j == 0 ? 0 : 1], be[j == 0 ? 0 : 1], Mm[3 * j + i],
-                       Me[3 * j + i]);
-    } else {
-#pragma unroll
-        for (int q = 0; q < 9; q++) {
-            Mm[q] = q % 4 == 0 ? 0.5 : 0.0;
-            Me[q] = q % 4 == 0 ? 1.0 : -__builtin_inf();
-        }
-    }
+    if (t < T)
+        transfer_matrix(-(S2 - 2.0 * (Ck[t] - Ak[t])) * i2v, t == 0, A, pi, Mm, Me);
+    else
+        xident(Mm, Me);
 }
 
 // select_random_element (forward_algorithm.h:502-513) over the extended
@@ -574,7 +498,7 @@ __global__ __launch_bounds__(kSweepThreads) void b3_gibbs(GibbsParams G) {
                         frame_matrix(tb, T, S2, Ck, Ak, i2v, A, pi, Mbm, Mbe);
                         xmatmul(Mbm, Mbe, Mam, Mae, Pm, Pe);
                     }
-                    prefix_products(Pm, Pe, lane);
+                    prefix_products<true>(Pm, Pe, lane);
                     // the product over the lanes before this one (identity for lane 0)
                     double Em[9], Ee[9];
 #pragma unroll

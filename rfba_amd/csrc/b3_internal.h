@@ -118,7 +118,12 @@ struct HmmParams {
     const double *em;
     double *ck;             // [slot][NI]
     const DevModel *model;
+    double *delta;          // b3_hmm_wave MODE_DELTA: [chunk-local item][T][3]
 };
+
+// b3_hmm_wave: one wave per item, up to kWaveFrames frames per lane.
+constexpr int kWaveFrames = 8;
+constexpr int kWaveTmax = 64 * kWaveFrames;
 
 struct EvalParams {
     const DevBlock *blocks;

@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include "b3_internal.h"
+#include "b3_wave.h"
 
 namespace rfba3b {
 
@@ -755,10 +756,12 @@ __global__ __launch_bounds__(256) void b3_corr_reduce(CorrReduceParams R) {
 
 // ---------------------------------------------------------------------------
 // Kernel D (b3_hmm): one lane per item runs the FP64 HMM of kernel B from
-// the per-frame sums em (forward with checkpoints, logL; for GRAD the
-// adjoint over segments recomputed from the checkpoints, held in ck).
-template <bool GRAD>
+// the per-frame sums em (forward with checkpoints, logL; for MODE_GRAD the
+// adjoint over segments recomputed from the checkpoints, held in ck; for
+// MODE_DELTA the forward messages as kernel B writes them).
+template <int MODE>
 __global__ __launch_bounds__(256) void b3_hmm(HmmParams H) {
+    constexpr bool GRAD = MODE == MODE_GRAD;
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= H.n) return;
     const int li = H.lo + i;
@@ -792,7 +795,7 @@ __global__ __launch_bounds__(256) void b3_hmm(HmmParams H) {
         }
         const double e1 = -c0 * i2v - cst;
         const double dlt = c1 * i2v;
-        if (t % ckpt == 0) {
+        if (GRAD && t % ckpt == 0) {
             const int u = 6 * (t / ckpt);
 #pragma unroll
             for (int j = 0; j < 3; j++) {
@@ -800,8 +803,17 @@ __global__ __launch_bounds__(256) void b3_hmm(HmmParams H) {
                 ck[(u + 3 + j) * NI] = ae[j];
             }
         }
-        double cm, ce;
-        L += xfstep(h, am, ae, xemis(dlt), t == 0, cm, ce) + e1 + fmax(dlt, 0.0);
+        double pm[3], pe[3], cm, ce;
+        xpred(h, am, ae, t == 0, pm, pe);
+        if (MODE == MODE_DELTA) {
+            // as kernel B: delta_t(j) = ln pred_j + B(j, t) + logL_{t-1}
+            double *dout = H.delta + ((size_t)li * T + t) * 3;
+            const double Bon = e1 + dlt;
+            dout[0] = (pm[0] > 0 ? log(pm[0]) + pe[0] * M_LN2 : -1e100) + Bon + L;
+            dout[1] = (pm[1] > 0 ? log(pm[1]) + pe[1] * M_LN2 : -1e100) + e1 + L;
+            dout[2] = (pm[2] > 0 ? log(pm[2]) + pe[2] * M_LN2 : -1e100) + e1 + L;
+        }
+        L += xupdate(pm, pe, xemis(dlt), am, ae, cm, ce) + e1 + fmax(dlt, 0.0);
     }
     const size_t oi = ((size_t)it.q * b.K + it.k) * b.S + it.s;
     b.logL[oi] = L;
@@ -859,8 +871,213 @@ __global__ __launch_bounds__(256) void b3_hmm(HmmParams H) {
     for (int n = 0; n < 5; n++) b.grad[oi * 5 + n] = g[n] * iv;
 }
 
-template __global__ void b3_hmm<false>(HmmParams);
-template __global__ void b3_hmm<true>(HmmParams);
+template __global__ void b3_hmm<MODE_LOGL>(HmmParams);
+template __global__ void b3_hmm<MODE_GRAD>(HmmParams);
+template __global__ void b3_hmm<MODE_DELTA>(HmmParams);
+
+// ---------------------------------------------------------------------------
+// Kernel D' (b3_hmm_wave): the HMM of kernel D with one wave per item, for
+// launches with few items, where one lane per item leaves the GPU idle
+// behind a serial chain of T frames.  Lane l takes frames [lF, lF + nf),
+// F = ceil(T / 64) <= kWaveFrames.  With the transfer matrix M_t of frame t
+// (transfer_matrix: M_t[j][i] = b_t(j) A[i][j], b_t(j) pi_j in frame 0) the
+// forward message is alpha_t = M_t ... M_0 e_0 and the backward one
+// beta_t^T = 1^T M_{T-1} ... M_{t+1}.  The wave forms the prefix and suffix
+// products of the lanes' matrices (6 exchanges each); each lane then runs
+// its own frames from the product that enters them and the one that leaves
+// them.  Nothing is normalised (the exponents keep the scale), and with the
+// emissions relative to e^max(dlt, 0) as in kernel D:
+//     logL = ln Z + K_{T-1},   Z = 1^T alpha_{T-1},
+//     d logL / d phi = sum_t alpha_t(on) beta_t(on) / Z diff_t / var,
+//     delta_t(j) = ln (A^T alpha_{t-1})_j + K_{t-1} + B(j, t)
+// where K_t is the running sum of e1 + max(dlt, 0) over frames 0..t.
+template <int MODE>
+__global__ __launch_bounds__(256) void b3_hmm_wave(HmmParams H) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= H.n) return;  // whole waves
+    const int li = H.lo + i;
+    const Item it = H.items[li];
+    const DevBlock &b = H.blocks[it.block];
+    const int T = b.T;
+    const DevModel &md = *H.model;
+    double A[9], pi[3];
+#pragma unroll
+    for (int u = 0; u < 9; u++) A[u] = md.A[u];
+#pragma unroll
+    for (int u = 0; u < 3; u++) pi[u] = md.pi[u];
+    const double i2v = 0.5 / md.variance;
+    const double cst = (double)b.npix * log(2.0 * M_PI * md.variance) / 2.0;
+    const int F = (T + 63) >> 6;             // at most kWaveFrames (the host checks T)
+    const int tb = lane * F;                 // the lane's first frame
+    const int nf = max(0, min(F, T - tb));   // and its number of frames
+    const double *__restrict__ emb = H.em + (size_t)li * 8;  // frame t at emb[t * es]
+    const size_t es = (size_t)H.NI * 8;
+    const double ninf = -__builtin_inf();
+
+    double dlt[kWaveFrames], e1[kWaveFrames];
+    double ks = 0.0;  // the lane's sum of e1 + max(dlt, 0)
+#pragma unroll
+    for (int f = 0; f < kWaveFrames; f++) {
+        dlt[f] = 0.0;
+        e1[f] = 0.0;
+        if (f < nf) {
+            const double *m8 = emb + (size_t)(tb + f) * es;
+            e1[f] = -m8[0] * i2v - cst;
+            dlt[f] = m8[1] * i2v;
+            ks += e1[f] + fmax(dlt[f], 0.0);
+        }
+    }
+    // the lane's product L_l = M_{tb+nf-1} ... M_tb (the identity without frames)
+    double Lm[9], Le[9];
+    xident(Lm, Le);
+#pragma unroll
+    for (int f = 0; f < kWaveFrames; f++)
+        if (f < nf) {
+            double Mm[9], Me[9];
+            transfer_matrix(dlt[f], tb + f == 0, A, pi, Mm, Me);
+            if (f == 0) {
+#pragma unroll
+                for (int q = 0; q < 9; q++) {
+                    Lm[q] = Mm[q];
+                    Le[q] = Me[q];
+                }
+            } else {
+                double Rm[9], Re[9];
+                xmatmul_t<false>(Mm, Me, Lm, Le, Rm, Re);
+#pragma unroll
+                for (int q = 0; q < 9; q++) {
+                    Lm[q] = Rm[q];
+                    Le[q] = Re[q];
+                }
+            }
+        }
+    // alpha entering the lane (column 0 of the previous lane's prefix, e_0 in
+    // lane 0) and Z (column 0 of the last lane's prefix, summed)
+    double am[3], ae[3], zm, ze;
+    {
+        double Pm[9], Pe[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            Pm[q] = Lm[q];
+            Pe[q] = Le[q];
+        }
+        prefix_products<false>(Pm, Pe, lane);
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            am[j] = __shfl_up(Pm[3 * j], 1);
+            ae[j] = __shfl_up(Pe[3 * j], 1);
+        }
+        if (lane == 0) {
+            am[0] = 0.5; ae[0] = 1.0;
+            am[1] = am[2] = 0.0;
+            ae[1] = ae[2] = ninf;
+        }
+        xsum3n(read_lane(Pm[0], 63), read_lane(Pe[0], 63), read_lane(Pm[3], 63), read_lane(Pe[3], 63),
+               read_lane(Pm[6], 63), read_lane(Pe[6], 63), zm, ze);
+    }
+    // K_{T-1} over the wave; K before the lane's first frame
+    double kin = ks;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const double v = __shfl_up(kin, d);
+        if (lane >= d) kin += v;
+    }
+    const double Ktot = read_lane(kin, 63);
+    kin = __shfl_up(kin, 1);
+    if (lane == 0) kin = 0.0;
+    const size_t oi = ((size_t)it.q * b.K + it.k) * b.S + it.s;
+    if (lane == 0) b.logL[oi] = zm > 0.0 ? log(zm) + ze * M_LN2 + Ktot : ninf;
+    if (MODE == MODE_LOGL) return;
+
+    // forward over the lane's frames: alpha_t(on) for the adjoint, or delta
+    double a0m[kWaveFrames], a0e[kWaveFrames];
+    double kr = kin;  // K_{t-1}
+#pragma unroll
+    for (int f = 0; f < kWaveFrames; f++) {
+        a0m[f] = 0.0;
+        a0e[f] = ninf;
+        if (f < nf) {
+            const int t = tb + f;
+            double pm[3], pe[3];  // A^T alpha_{t-1} (pi in frame 0)
+            if (t == 0) {
+#pragma unroll
+                for (int j = 0; j < 3; j++) xsplit(pi[j], 0.0, pm[j], pe[j]);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    xsum3(am[0] * A[j], ae[0], am[1] * A[3 + j], ae[1], am[2] * A[6 + j], ae[2], pm[j], pe[j]);
+            }
+            if (MODE == MODE_DELTA) {
+                double *dout = H.delta + ((size_t)li * T + t) * 3;
+                const double Bon = e1[f] + dlt[f];
+                dout[0] = (pm[0] > 0 ? log(pm[0]) + pe[0] * M_LN2 : -1e100) + Bon + kr;
+                dout[1] = (pm[1] > 0 ? log(pm[1]) + pe[1] * M_LN2 : -1e100) + e1[f] + kr;
+                dout[2] = (pm[2] > 0 ? log(pm[2]) + pe[2] * M_LN2 : -1e100) + e1[f] + kr;
+                kr += e1[f] + fmax(dlt[f], 0.0);
+            }
+            const Emis bt = xemis(dlt[f]);
+            xsplit(pm[0] * bt.onm, pe[0] + bt.one, am[0], ae[0]);
+            xsplit(pm[1] * bt.offm, pe[1] + bt.offe, am[1], ae[1]);
+            xsplit(pm[2] * bt.offm, pe[2] + bt.offe, am[2], ae[2]);
+            a0m[f] = am[0];
+            a0e[f] = ae[0];
+        }
+    }
+    if (MODE == MODE_DELTA) return;
+
+    // beta leaving the lane: 1^T (L_63 ... L_{l+1}), the column sums of the
+    // next lane's suffix product (1 in the last lane)
+    double bm[3], be[3];
+    {
+        suffix_products(Lm, Le, lane);
+        double wm[3], we[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) xsum3n(Lm[c], Le[c], Lm[3 + c], Le[3 + c], Lm[6 + c], Le[6 + c], wm[c], we[c]);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            bm[c] = __shfl_down(wm[c], 1);
+            be[c] = __shfl_down(we[c], 1);
+        }
+        if (lane == kWave - 1) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                bm[c] = 0.5;
+                be[c] = 1.0;
+            }
+        }
+    }
+    const double izm = zm > 0.0 ? 1.0 / zm : 0.0;
+    double g[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int f = kWaveFrames - 1; f >= 0; f--)
+        if (f < nf) {
+            const int t = tb + f;
+            const double gam = xvalue(a0m[f] * bm[0] * izm, a0e[f] + be[0] - ze);
+            const double *m8 = emb + (size_t)t * es;
+#pragma unroll
+            for (int n = 0; n < 5; n++) g[n] += gam * m8[2 + n];
+            if (f > 0) {  // beta_{t-1} = M_t^T beta_t
+                const Emis bt = xemis(dlt[f]);
+                const double wm0 = bt.onm * bm[0], we0 = bt.one + be[0];
+                const double wm1 = bt.offm * bm[1], we1 = bt.offe + be[1];
+                const double wm2 = bt.offm * bm[2], we2 = bt.offe + be[2];
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    xsum3(A[3 * c] * wm0, we0, A[3 * c + 1] * wm1, we1, A[3 * c + 2] * wm2, we2, bm[c], be[c]);
+            }
+        }
+    const double iv = 1.0 / md.variance;
+#pragma unroll
+    for (int n = 0; n < 5; n++) {
+        const double v = wave_sum(g[n]);
+        if (lane == 0) b.grad[oi * 5 + n] = v * iv;
+    }
+}
+
+template __global__ void b3_hmm_wave<MODE_LOGL>(HmmParams);
+template __global__ void b3_hmm_wave<MODE_GRAD>(HmmParams);
+template __global__ void b3_hmm_wave<MODE_DELTA>(HmmParams);
 
 // ---------------------------------------------------------------------------
 // FFBS Hessian, sampled_background_spot_hessian_ffbs (multispot5.cc:902-982).

@@ -391,3 +391,57 @@ def test_functor_hessian_ffbs_vs_oracle(gpu, region, S, bs, z_exact):
     go, Lo = bo.neg_grad(xyz, d2, bgs, phi, m)
     assert np.linalg.norm(g[0] - go) / np.linalg.norm(go) <= GRAD_RTOL
     f.close()
+
+
+def test_series_longer_than_wave_frames(gpu):
+    # T = 600 is past the 512 frames the one-wave-per-item HMM spreads over
+    # its lanes: the HMMs run one lane per item (serial chain over the frames)
+    m = model(checkpoint=0)
+    data, params, states = b3.synth_block((7, 6, 5), 600, 3, 91, A=b3.blink_matrix(0.2, 0.1, 0.001))
+    P = b3.perturb(params, 2, 92)
+    (L, G), = run_batch([b3.BlockData(data, P, states[None], 100.0)], m)
+    Lo, Go = bo.model_eval(data, 100.0, m, P, states[None])
+    compare(L, G, Lo, Go, "T=600 K=3")
+
+
+def test_many_items_one_launch(gpu):
+    # 4200 items (105 parameter sets x 40 spots) in one launch of the
+    # one-wave-per-item HMM, against one launch per parameter set (40 items)
+    # bit for bit, and the oracle on two parameter sets
+    m = model(checkpoint=0)
+    data, params, states = b3.synth_block((8, 8, 8), 130, 40, 93)
+    P = b3.perturb(params, 105, 94)
+    bt = b3.Batch([b3.BlockData(data, P, states[None], 100.0)], m)
+    bt.run(use_graph=False)
+    L1, G1 = bt.results(0)
+    bt.run_per_block(per_param_set=True)
+    L2, G2 = bt.results(0)
+    bt.close()
+    assert np.array_equal(L1, L2) and np.array_equal(G1, G2)
+    Lo, Go = bo.model_eval(data, 100.0, m, P[:2], states[None])
+    compare(L1[:2], G1[:2], Lo, Go, "8^3 T=130 K=40")
+
+
+@pytest.mark.parametrize("T", [130, 600])
+def test_functor_long_series(gpu, T):
+    # T = 130: three frames per lane in the wave HMM; T = 600: the lane form
+    # (gradient and the Hessian's forward messages)
+    m = model()
+    xyz, d2, bgs, params = _functor_case((9, 8, 7), T, 3, 2, 95, "ball")
+    f = b3.Functor(xyz, d2, bgs, m, max_candidates=2)
+    phi = params[0] + [40.0, 0.05, 0.2, -0.1, 0.15]
+    g, L = f(np.stack([params[0], phi]))
+    for i, p in enumerate((params[0], phi)):
+        go, Lo = bo.neg_grad(xyz, d2, bgs, p, m)
+        eg = np.linalg.norm(g[i] - go) / np.linalg.norm(go)
+        el = np.max(np.abs(L[i] - Lo) / np.abs(Lo))
+        print("T=%d candidate %d: grad err %.3g, logL err %.3g" % (T, i, eg, el))
+        assert eg <= GRAD_RTOL and el <= LOGL_RTOL
+    u = np.random.default_rng(96).random((2, 3, T))
+    H, D, paths = f.hessian_ffbs(phi, 3, u)
+    Ho, Do, po = bo.hessian_ffbs(xyz, d2, bgs, phi, m, 3, u)
+    assert np.array_equal(paths, po), "sampled paths differ"
+    err = np.abs(H - Ho).max() / np.abs(Ho).max()
+    print("T=%d Hessian err %.3g, %d on-frames sampled" % (T, err, int((paths == 0).sum())))
+    assert err <= HESS_RTOL
+    f.close()
