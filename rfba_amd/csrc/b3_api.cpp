@@ -701,16 +701,18 @@ struct rfba3b_functor {
     DevModel *d_model = nullptr;
     DevBlock *d_block = nullptr;
     PlaneRef *d_planes = nullptr;
-    CorrTask *d_corr = nullptr;
-    Item *d_items = nullptr;
+    CorrTask *d_corr = nullptr;  // in d_stage
+    Item *d_items = nullptr;     // in d_stage
     float *d_R = nullptr;
     uint8_t *d_states = nullptr, *d_mask = nullptr;
-    double *d_S0 = nullptr, *d_em = nullptr, *d_ck = nullptr, *d_params = nullptr, *d_logL = nullptr,
-           *d_grad = nullptr;
-    // pinned staging: params [nmax][5] | logL [nmax][S] | grad [nmax][S][5] |
-    // correlation tasks | items
-    unsigned char *h_stage = nullptr;
+    double *d_S0 = nullptr, *d_em = nullptr, *d_ck = nullptr;
+    // staging, pinned on the host and mirrored in one device allocation:
+    // params [nmax][5] | correlation tasks | items | logL [nmax][S] |
+    // grad [nmax][S][5], so that a call uploads its inputs in one copy and
+    // downloads its outputs in one copy
+    unsigned char *h_stage = nullptr, *d_stage = nullptr;
     size_t o_logL = 0, o_grad = 0, o_corr = 0, o_items = 0;
+    double *d_params = nullptr, *d_logL = nullptr, *d_grad = nullptr;  // in d_stage
     std::mutex mu;
     // pixel-split correlation sums (few candidates: each plane's pixels are
     // spread over many workgroups)
@@ -731,9 +733,9 @@ static void functor_free(rfba3b_functor *f) {
     hipFree(f->d_part); hipFree(f->d_part_qh);
     hipFree(f->d_counter); hipFree(f->d_basis); hipFree(f->d_fixed); hipFree(f->d_parts); hipFree(f->d_delta);
     hipFree(f->d_unif); hipFree(f->d_paths);
-    hipFree(f->d_model); hipFree(f->d_block); hipFree(f->d_planes); hipFree(f->d_corr); hipFree(f->d_items);
+    hipFree(f->d_model); hipFree(f->d_block); hipFree(f->d_planes); hipFree(f->d_stage);
     hipFree(f->d_R); hipFree(f->d_states); hipFree(f->d_mask); hipFree(f->d_S0); hipFree(f->d_em);
-    hipFree(f->d_ck); hipFree(f->d_params); hipFree(f->d_logL); hipFree(f->d_grad);
+    hipFree(f->d_ck);
     if (f->h_stage) hipHostFree(f->h_stage);
     delete f;
 }
@@ -815,11 +817,11 @@ extern "C" int rfba3b_functor_create(const int *xyz, int P, int T, const double 
     }
     const size_t NI = (size_t)S * max_candidates;
     const size_t ncorr = (size_t)S * ((max_candidates + kCorrSpots - 1) / kCorrSpots);
-    f->o_logL = (size_t)max_candidates * 5 * sizeof(double);
-    f->o_grad = f->o_logL + NI * sizeof(double);
-    f->o_corr = f->o_grad + NI * 5 * sizeof(double);
+    f->o_corr = (size_t)max_candidates * 5 * sizeof(double);
     f->o_items = f->o_corr + ncorr * sizeof(CorrTask);
-    const size_t stage = f->o_items + NI * sizeof(Item);
+    f->o_logL = (f->o_items + NI * sizeof(Item) + 7) / 8 * 8;
+    f->o_grad = f->o_logL + NI * sizeof(double);
+    const size_t stage = f->o_grad + NI * 5 * sizeof(double);
     // the largest partial-sum buffer any candidate count needs, at most 256 MiB
     {
         size_t need = 0, needqh = 0;
@@ -842,17 +844,20 @@ extern "C" int rfba3b_functor_create(const int *xyz, int P, int T, const double 
     al((void **)&f->d_model, sizeof(DevModel));
     al((void **)&f->d_block, sizeof(DevBlock));
     al((void **)&f->d_planes, S * sizeof(PlaneRef));
-    al((void **)&f->d_corr, ncorr * sizeof(CorrTask));
-    al((void **)&f->d_items, NI * sizeof(Item));
+    al((void **)&f->d_stage, stage);
     al((void **)&f->d_R, R.size() * sizeof(float));
     al((void **)&f->d_states, NI * (size_t)T);
     if (!full) al((void **)&f->d_mask, Pd);
     al((void **)&f->d_S0, S0.size() * sizeof(double));
     al((void **)&f->d_em, NI * (size_t)T * 8 * sizeof(double));
     al((void **)&f->d_ck, NI * (size_t)f->ckslots * sizeof(double));
-    al((void **)&f->d_params, (size_t)max_candidates * 5 * sizeof(double));
-    al((void **)&f->d_logL, NI * sizeof(double));
-    al((void **)&f->d_grad, NI * 5 * sizeof(double));
+    if (e == hipSuccess) {
+        f->d_params = (double *)f->d_stage;
+        f->d_corr = (CorrTask *)(f->d_stage + f->o_corr);
+        f->d_items = (Item *)(f->d_stage + f->o_items);
+        f->d_logL = (double *)(f->d_stage + f->o_logL);
+        f->d_grad = (double *)(f->d_stage + f->o_grad);
+    }
     if (f->part_doubles) {
         al((void **)&f->d_part, f->part_doubles * sizeof(double));
         al((void **)&f->d_part_qh, f->partqh_doubles * sizeof(double));
@@ -909,9 +914,8 @@ static int functor_sums(rfba3b_functor *f, int m, hipStream_t st) {
         for (int k = 0; k < m; k++) it[s * m + k] = {0, 0, k, s, s * plane};
     }
     const int NI = S * m;
-    HIP_TRY(hipMemcpyAsync(f->d_params, f->h_stage, (size_t)m * 5 * sizeof(double), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(f->d_corr, ct, nc * sizeof(CorrTask), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(f->d_items, it, NI * sizeof(Item), hipMemcpyHostToDevice, st));
+    // params, tasks and items in one copy (unused slots between them included)
+    HIP_TRY(hipMemcpyAsync(f->d_stage, f->h_stage, f->o_items + NI * sizeof(Item), hipMemcpyHostToDevice, st));
     CorrParams C;
     memset(&C, 0, sizeof C);
     C.blocks = f->d_block;
@@ -947,7 +951,7 @@ static int functor_sums(rfba3b_functor *f, int m, hipStream_t st) {
         Rd.part_qh = f->d_part_qh;
         Rd.S0 = f->d_S0;
         Rd.em = f->d_em;
-        hipLaunchKernelGGL(b3_corr_reduce, dim3((NI * T + 255) / 256), dim3(256), 0, st, Rd);
+        hipLaunchKernelGGL(b3_corr_reduce, dim3((NI * T * 6 + 255) / 256), dim3(256), 0, st, Rd);
         HIP_TRY(hipGetLastError());
     }
     return 0;
@@ -999,11 +1003,10 @@ extern "C" int rfba3b_functor_eval(rfba3b_functor *f, const double *phi, int n, 
         int rc = functor_sums(f, m, st);
         if (!rc) rc = launch_hmm(MODE_GRAD, functor_hmm(f, NI), T, st);
         if (rc) return rc;
-        // outputs [k][s] of the first m candidates (the block's K is nmax)
-        HIP_TRY(hipMemcpyAsync(f->h_stage + f->o_logL, f->d_logL, (size_t)NI * sizeof(double),
-                               hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipMemcpyAsync(f->h_stage + f->o_grad, f->d_grad, (size_t)NI * 5 * sizeof(double),
-                               hipMemcpyDeviceToHost, st));
+        // outputs [k][s] of the first m candidates (the block's K is nmax):
+        // logL and grad in one copy
+        HIP_TRY(hipMemcpyAsync(f->h_stage + f->o_logL, f->d_stage + f->o_logL,
+                               f->o_grad - f->o_logL + (size_t)NI * 5 * sizeof(double), hipMemcpyDeviceToHost, st));
     }
     HIP_TRY(hipStreamSynchronize(st));
     const double *L = (const double *)(f->h_stage + f->o_logL);

@@ -729,29 +729,33 @@ template __global__ void b3_corr_mfma<true, true>(CorrParams);
 // b3_corr_mfma's epilogue (Soff = S0 + [on](2C + Q), D = 2C +- Q,
 // diff_n = G_n - [off] H_n).
 __global__ __launch_bounds__(256) void b3_corr_reduce(CorrReduceParams R) {
+    // one thread per (frame, item, field n): field 0 gives Soff and D, field
+    // n >= 1 gives diff_{n-1}; the splits are added in split order
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= R.NI * R.T) return;
-    const int t = i / R.NI, item = i - t * R.NI;
+    if (i >= R.NI * R.T * 6) return;
+    const int n = i % 6, ti = i / 6;
+    const int t = ti / R.NI, item = ti - t * R.NI;
     const Item it = R.items[item];
     const DevBlock &b = *R.block;
-    double v[6] = {0, 0, 0, 0, 0, 0}, qh[6] = {0, 0, 0, 0, 0, 0};
+    const double *__restrict__ pv = R.part + ((size_t)t * R.NI + item) * 6 + n;
+    const double *__restrict__ pq = R.part_qh + (size_t)item * 6 + n;
+    const size_t sv = (size_t)R.Tmax * R.NI * 6, sq = (size_t)R.NI * 6;
+    double v = 0.0, qh = 0.0;
+#pragma unroll 8
     for (int sp = 0; sp < R.nsplit; sp++) {
-        const double *pv = R.part + (((size_t)sp * R.Tmax + t) * R.NI + item) * 6;
-        const double *pq = R.part_qh + ((size_t)sp * R.NI + item) * 6;
-#pragma unroll
-        for (int n = 0; n < 6; n++) {
-            v[n] += pv[n];
-            qh[n] += pq[n];
-        }
+        v += pv[sp * sv];
+        qh += pq[sp * sq];
     }
     const bool on = b.states[((size_t)it.s * b.K + it.k) * b.T + t] == 0;
-    const double S0 = R.S0[(size_t)it.s * R.Tmax + t];
     double *m = R.em + ((size_t)t * R.NI + item) * 8;
-    const double c2 = 2.0 * v[0], Q = qh[0];
-    m[0] = on ? S0 + (c2 + Q) : S0;
-    m[1] = on ? c2 + Q : c2 - Q;
-#pragma unroll
-    for (int n = 1; n < 6; n++) m[1 + n] = on ? v[n] : v[n] - qh[n];
+    if (n == 0) {
+        const double S0 = R.S0[(size_t)it.s * R.Tmax + t];
+        const double c2 = 2.0 * v;
+        m[0] = on ? S0 + (c2 + qh) : S0;
+        m[1] = on ? c2 + qh : c2 - qh;
+    } else {
+        m[1 + n] = on ? v : v - qh;
+    }
 }
 
 // ---------------------------------------------------------------------------
