@@ -311,12 +311,14 @@ def bench_srrf_batched(args):
         return out
 
     ref = snap()
-    t0 = time.perf_counter()
-    ms_pb = bt.run_per_block()
-    srrf.lib().rfba_device_synchronize()
-    wall_pb = (time.perf_counter() - t0) * 1e3
-    log("c3: per-block %.1f ms" % ms_pb)
-    same = all(np.array_equal(a, b, equal_nan=True) for a, b in zip(ref, snap()))
+    same, ms_pb, wall_pb = None, None, None
+    if not args.no_plain:
+        t0 = time.perf_counter()
+        ms_pb = bt.run_per_block()
+        srrf.lib().rfba_device_synchronize()
+        wall_pb = (time.perf_counter() - t0) * 1e3
+        log("c3: per-block %.1f ms" % ms_pb)
+        same = all(np.array_equal(a, b, equal_nan=True) for a, b in zip(ref, snap()))
     # Voxels with a sample out of range on any axis are NaN in every frame by
     # definition, and the kernel skips them: their share of the voxel-frames
     # is reported beside the roofline.
@@ -344,11 +346,12 @@ def bench_srrf_batched(args):
         "blocks": len(blocks), "edge_sizes": edges, "sr_voxels": n_sr, "sr_voxel_frames": sr_frames,
         "groups_mag_tile_depth": groups, "autotune_sample_ms_largest_group": {str(k): v for k, v in tune.items()},
         "batched": {"ms": wall_b, "kernel_ms": kb, "sr_voxels_per_s": n_sr / (wall_b * 1e-3)},
-        "per_block": {"ms": wall_pb, "kernel_ms": ms_pb, "sr_voxels_per_s": n_sr / (wall_pb * 1e-3)},
-        "speedup_batched_vs_per_block": wall_pb / wall_b,
-        "batched_equals_per_block": bool(same),
         "roofline": roof,
     }
+    if not args.no_plain:
+        res.update({"per_block": {"ms": wall_pb, "kernel_ms": ms_pb, "sr_voxels_per_s": n_sr / (wall_pb * 1e-3)},
+                    "speedup_batched_vs_per_block": wall_pb / wall_b,
+                    "batched_equals_per_block": bool(same)})
     if not args.no_cpu:
         from oracle import oracle
         i = 0
@@ -860,6 +863,9 @@ def main():
                     help="seconds from start at which the JSON line is printed with what has been measured, "
                          "and the process ends")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the batched SRRF workload's one-launch-per-block comparison leg (for profiling the "
+                         "batched dispatches, whose work is unchanged)")
     ap.add_argument("--cpu-crop", type=str, default="12,12,8")
     ap.add_argument("--frames", type=int, default=200)
     ap.add_argument("--only", type=str, default="c2,c4,functor,c5,gibbs,c3,prep",
