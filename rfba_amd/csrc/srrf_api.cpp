@@ -537,6 +537,17 @@ LaunchParams make_params(const SRRFConf &c, const Plan &pl) {
     P.zR = c.ZRange;
     P.g = pl.g;
     P.Tmax = pl.Tmax;
+    {
+        // terms of the widest gradient sum; values below 2^16 keep every
+        // partial sum of at most 256 terms below 2^24 (exact in FP32)
+        const long long m2 = 2LL * c.MainRange, x1 = 2LL * c.XRange + 1, y1 = 2LL * c.YRange + 1,
+                        z1 = 2LL * c.ZRange + 1;
+        const long long terms = std::max(m2 * y1 * z1, std::max(m2 * x1 * z1, m2 * x1 * y1));
+        const LaunchGeom &g = pl.g;
+        const long long px = g.pxv ? 3LL * g.NV * g.VS : 3LL * g.NV * g.NW * g.TXP;
+        const long long need = (long long)g.RU * g.RV * g.NW + (long long)g.RU * g.NV * g.RW;
+        P.sep = (c.GradientSmooth != 0 && terms <= 256 && need <= px) ? 1 : 0;
+    }
     return P;
 }
 

@@ -89,6 +89,9 @@ struct LaunchParams {
     // configuration shared by all items
     int method, itc, order, remove_con, renorm, smooth, iw;
     int mR, xR, yR, zR;
+    // smoothed gradient of integer input from shared box sums: every sum has
+    // at most 256 terms and the box sums fit in the x-plane area
+    int sep;
     LaunchGeom g;
 };
 

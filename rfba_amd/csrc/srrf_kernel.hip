@@ -62,6 +62,10 @@ __device__ __forceinline__ int first_tap(float c, int M) {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// uint16 input: raw values are integers below 2^16
+template <typename Tin> constexpr bool kIntInput = false;
+template <> constexpr bool kIntInput<uint16_t> = true;
+
 // Signed radiality term of one sample (kernel.cu:227-240 with _calculateDk
 // :174-183): -Dk when diverging, +Dk otherwise (dd - Dk == dd + (-Dk)).
 struct TermIn {
@@ -380,7 +384,72 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
             __syncthreads();
 
             // ---- gradient planes G[c][w][v][u] (only the sample passes read them)
-            for (int e = tid; e < (tile_skip ? 0 : nu * nv * nw); e += kThreads) {
+            // Integer input with sums of at most 256 terms (P.sep, set by the
+            // host): every partial sum is an integer below 2^24, exact in FP32
+            // in any order, so the smoothed gradient is formed from shared box
+            // sums -- Bz (z sums) for the x and y derivatives, By (y sums) for
+            // the z derivative -- and equals the term-by-term sum bit for bit.
+            // Bz and By live in the x-plane area, which the intensity and x
+            // passes overwrite only after the next barrier.
+            const bool sep = kIntInput<Tin> && P.sep && P.smooth != 0 && !tile_skip;
+            if (sep) {
+                float *Bz = Px;                  // [nw][rv_][ru]
+                float *By = Px + ru * rv_ * nw;  // [rw][nv][ru]
+                const int nbz = ru * rv_ * nw, nby = ru * nv * rw;
+                const int mR = P.mR, xR = P.xR, yR = P.yR, zR = P.zR;
+                for (int e = tid; e < nbz + nby; e += kThreads) {
+                    float s = 0.0f;
+                    if (e < nbz) {
+                        const int a = e % ru, b = (e / ru) % rv_, w = e / (ru * rv_);
+                        const float *r = raw + ((w + g.hz) * RV + b) * RU + a;
+                        for (int k = -zR; k <= zR; k++) s += r[k * RV * RU];
+                        Bz[e] = s;
+                    } else {
+                        const int f = e - nbz;
+                        const int a = f % ru, v = (f / ru) % nv, c = f / (ru * nv);
+                        const float *r = raw + (c * RV + v + g.hy) * RU + a;
+                        for (int j = -yR; j <= yR; j++) s += r[j * RU];
+                        By[f] = s;
+                    }
+                }
+                __syncthreads();
+                const int NP = NU * NV * NW;
+                for (int e = tid; e < nu * nv * nw; e += kThreads) {
+                    const int u = e % nu, v = (e / nu) % nv, w = e / (nu * nv);
+                    const int cu = u + g.hx, cv = v + g.hy, cw = w + g.hz;
+                    const float *bz = Bz + w * rv_ * ru;
+                    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+                    for (int i = 1; i <= mR; i++) {
+                        float sp = 0.0f, sm = 0.0f;
+                        for (int j = -yR; j <= yR; j++) {
+                            sp += bz[(cv + j) * ru + cu + i];
+                            sm += bz[(cv + j) * ru + cu - i];
+                        }
+                        gx += sp - sm;
+                    }
+                    for (int j = 1; j <= mR; j++) {
+                        float sp = 0.0f, sm = 0.0f;
+                        for (int i = -xR; i <= xR; i++) {
+                            sp += bz[(cv + j) * ru + cu + i];
+                            sm += bz[(cv - j) * ru + cu + i];
+                        }
+                        gy += sp - sm;
+                    }
+                    for (int k = 1; k <= mR; k++) {
+                        float sp = 0.0f, sm = 0.0f;
+                        for (int i = -xR; i <= xR; i++) {
+                            sp += By[((cw + k) * nv + v) * ru + cu + i];
+                            sm += By[((cw - k) * nv + v) * ru + cu + i];
+                        }
+                        gz += sp - sm;
+                    }
+                    const int gi3 = (w * NV + v) * NU + u;
+                    G[gi3] = gx;
+                    G[NP + gi3] = gy;
+                    G[2 * NP + gi3] = gz;
+                }
+            }
+            for (int e = tid; e < (tile_skip || sep ? 0 : nu * nv * nw); e += kThreads) {
                 const int u = e % nu, v = (e / nu) % nv, w = e / (nu * nv);
                 const int gi3 = (w * NV + v) * NU + u;
                 const int cu = u + g.hx, cv = v + g.hy, cw = w + g.hz;

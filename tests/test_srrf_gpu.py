@@ -120,6 +120,29 @@ def test_gradient_and_radiality_options(gpu, smooth, renorm, poscon, ranges):
     parity(g, o, "smooth %d renorm %d poscon %d ranges %s" % (smooth, renorm, poscon, ranges))
 
 
+@pytest.mark.parametrize("ranges", [(1, 1, 1, 1), (2, 3, 3, 3), (3, 3, 3, 3)])
+def test_integer_input_gradient_sums_exact(gpu, ranges):
+    # uint16 frames near full scale.  With at most 256 terms per smoothed
+    # gradient sum (the first two range sets: 12 and 196 terms) the kernel
+    # forms the gradient from shared box sums, exact for integers below
+    # 2^24; with more (294 terms) it sums term by term.  Either way the
+    # result equals the float-input path (term by term) and the oracle bit
+    # for bit.
+    rng = np.random.default_rng(19)
+    s = rng.integers(50000, 65536, (3, 12, 13, 14)).astype(np.uint16)
+    smp = srrf.directions(26, 0.5)
+    mR, xR, yR, zR = ranges
+    c = conf(Method=1, IntensityW=1, GradientSmooth=1, MainRange=mR, XRange=xR, YRange=yR, ZRange=zR,
+             Magnification=2)
+    g = srrf.srrf_stack(s, c, smp)
+    gf = srrf.srrf_stack(s.astype(np.float32), c, smp)
+    o = oracle.srrf_stack(s.astype(np.float32), c, smp.xyz, smp.divisor)
+    print("ranges %s: %d finite voxels" % (ranges, int(np.isfinite(o).sum())))
+    assert np.isfinite(o).any()
+    assert np.array_equal(g, gf, equal_nan=True)
+    assert np.array_equal(g, o, equal_nan=True)
+
+
 def test_float_input_and_drop_in_gpu_entry(gpu):
     # normalised float frames, as the reference caller passes them
     s = series((5, 10, 14, 16), 4, 17).astype(np.float32)
