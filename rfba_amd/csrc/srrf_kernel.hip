@@ -62,6 +62,14 @@ __device__ __forceinline__ int first_tap(float c, int M) {
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// e / d for 0 <= e < 2^16 and d >= 1, with rd the correctly rounded 1 / d:
+// (e + 0.5) / d lies at least 0.5 / d from an integer, and the two roundings
+// of (e + 0.5) rd move it by less than 2^-23 (e + 0.5) / d < 0.01 / d.  The
+// element loops of a frame use it for their index decompositions (LDS
+// extents: e < 40960), where integer division by a runtime divisor costs
+// about 25 instructions.
+__device__ __forceinline__ int qdiv(int e, float rd) { return (int)(((float)e + 0.5f) * rd); }
+
 // uint16 input: raw values are integers below 2^16
 template <typename Tin> constexpr bool kIntInput = false;
 template <> constexpr bool kIntInput<uint16_t> = true;
@@ -301,6 +309,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
         const int nu = uhi - ulo + 1, nv = vhi - vlo + 1, nw = whi - wlo + 1;
         const int ru = nu + 2 * g.hx, rv_ = nv + 2 * g.hy, rw = nw + 2 * g.hz;
         const int nr_raw = ru * rv_ * rw;
+        const float r_ru = 1.0f / (float)ru, r_rv = 1.0f / (float)rv_, r_nu = 1.0f / (float)nu;
+        const float r_nv = 1.0f / (float)nv, r_nw = 1.0f / (float)nw;
 
         // z tables (uniform per tile, frame independent).
         for (int e = tid; e < TZ * noz; e += kThreads) {
@@ -374,7 +384,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
             {
                 const Tin *srct = src + (long long)t * it.st;
                 for (int e = tid; e < nr_raw; e += kThreads) {
-                    const int a = e % ru, bb = (e / ru) % rv_, c = e / (ru * rv_);
+                    const int q = qdiv(e, r_ru), c = qdiv(q, r_rv);
+                    const int a = e - q * ru, bb = q - c * rv_;
                     const int x = clampi(rx0 + a, 0, W - 1) + it.ox;
                     const int y = clampi(ry0 + bb, 0, H - 1) + it.oy;
                     const int z = clampi(rz0 + c, 0, D - 1) + it.oz;
@@ -400,13 +411,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                 for (int e = tid; e < nbz + nby; e += kThreads) {
                     float s = 0.0f;
                     if (e < nbz) {
-                        const int a = e % ru, b = (e / ru) % rv_, w = e / (ru * rv_);
+                        const int q = qdiv(e, r_ru), w = qdiv(q, r_rv);
+                        const int a = e - q * ru, b = q - w * rv_;
                         const float *r = raw + ((w + g.hz) * RV + b) * RU + a;
                         for (int k = -zR; k <= zR; k++) s += r[k * RV * RU];
                         Bz[e] = s;
                     } else {
                         const int f = e - nbz;
-                        const int a = f % ru, v = (f / ru) % nv, c = f / (ru * nv);
+                        const int q = qdiv(f, r_ru), c = qdiv(q, r_nv);
+                        const int a = f - q * ru, v = q - c * nv;
                         const float *r = raw + (c * RV + v + g.hy) * RU + a;
                         for (int j = -yR; j <= yR; j++) s += r[j * RU];
                         By[f] = s;
@@ -415,7 +428,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                 __syncthreads();
                 const int NP = NU * NV * NW;
                 for (int e = tid; e < nu * nv * nw; e += kThreads) {
-                    const int u = e % nu, v = (e / nu) % nv, w = e / (nu * nv);
+                    const int q = qdiv(e, r_nu), w = qdiv(q, r_nv);
+                    const int u = e - q * nu, v = q - w * nv;
                     const int cu = u + g.hx, cv = v + g.hy, cw = w + g.hz;
                     const float *bz = Bz + w * rv_ * ru;
                     float gx = 0.0f, gy = 0.0f, gz = 0.0f;
@@ -450,7 +464,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                 }
             }
             for (int e = tid; e < (tile_skip || sep ? 0 : nu * nv * nw); e += kThreads) {
-                const int u = e % nu, v = (e / nu) % nv, w = e / (nu * nv);
+                const int q = qdiv(e, r_nu), w = qdiv(q, r_nv);
+                const int u = e - q * nu, v = q - w * nv;
                 const int gi3 = (w * NV + v) * NU + u;
                 const int cu = u + g.hx, cv = v + g.hy, cw = w + g.hz;
                 auto R3 = [&](int du, int dv, int dw) {
@@ -511,7 +526,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                 }
                 __syncthreads();
                 for (int e = tid; e < nv * nw * kTX; e += kThreads) {
-                    const int X = e % kTX, w = (e / kTX) % nw, v = e / (kTX * nw);
+                    const int q = e / kTX, v = qdiv(q, r_nw);
+                    const int X = e % kTX, w = q - v * nw;
                     const float *xt = xtab + X * 6;
                     const int ub = __float_as_int(xt[0]);
                     const float *rr = raw + ((w + g.hz) * RV + (v + g.hy)) * RU + ub + g.hx;
@@ -572,7 +588,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                 {
                     const int NP = NU * NV * NW;
                     for (int e = tid; e < nv * nw * kTX; e += kThreads) {
-                        const int X = e % kTX, w = (e / kTX) % nw, v = e / (kTX * nw);
+                        const int q = e / kTX, v = qdiv(q, r_nw);
+                        const int X = e % kTX, w = q - v * nw;
                         const float *xt = xtab + X * 6;
                         const int ub = __float_as_int(xt[0]);
                         const float w0 = xt[1], w1 = xt[2], w2 = xt[3], w3 = xt[4];
