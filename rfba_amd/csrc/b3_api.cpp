@@ -273,9 +273,11 @@ extern "C" int rfba3b_batch_create(const rfba3b_block *blocks, int n, const rfba
         b->Tmax = std::max(b->Tmax, u.T);
         b->naxmax = std::max(b->naxmax, u.dims[0] + u.dims[1] + u.dims[2]);
         // residual kernel tables: xy factors (padded to 4), z factors (Z
-        // rounded up to 8 per spot), amp, on
+        // rounded up to 8 per spot), amp, on-state masks (two words per 64
+        // spots)
         const size_t lr = sizeof(float) * ((((size_t)u.K * (u.dims[0] + u.dims[1]) + 3) & ~(size_t)3) +
-                                           (size_t)u.K * ((u.dims[2] + 7) & ~7) + 2 * (size_t)u.K);
+                                           (size_t)u.K * ((u.dims[2] + 7) & ~7) + (size_t)u.K +
+                                           2 * (((size_t)u.K + 63) / 64));
         b->lds_resid = std::max(b->lds_resid, lr);
         const int ck = ckpt_of(model, u.T);
         b->ckslots = std::max(b->ckslots, 6 * ((u.T + ck - 1) / ck) + 4 * ck);

@@ -48,22 +48,41 @@ __global__ __launch_bounds__(256) void b3_resid(const DevBlock *__restrict__ blo
         const DevBlock &b = blocks[tk.block];
         const int X = b.X, Y = b.Y, Z = b.Z, K = b.K, T = b.T;
         const int P = X * Y * Z;
-        // tables: gxy[K][X+Y] (x | y factors, padded to 4 floats), gz[K][Zp]
-        // (z factors, Zp = Z rounded up to 8, zero past Z: 16-byte aligned
-        // rows read as two float4 per 8-deep run), amp[K], on[K]
+        // tables: gxy[K][X+Y] (amp * x factor | y factor, padded to 4 floats),
+        // gz[K][Zp] (z factors, Zp = Z rounded up to 8, zero past Z: 16-byte
+        // aligned rows read as two float4 per 8-deep run), amp[K], and the
+        // on-states of this frame as 64-bit masks (two words each)
         const int nxy = X + Y, nxyp = (K * nxy + 3) & ~3, Zp = (Z + 7) & ~7;
         float *gxy = lds;
         float *gzt = gxy + nxyp;
         float *amp = gzt + K * Zp;
-        int *on = (int *)(amp + K);
+        unsigned *onm = (unsigned *)(amp + K);  // [ceil(K / 64)][2]
         __syncthreads();
         const double zF = mdl->zfactor;
+        for (int j = tid; j < K; j += 256) {
+            const double *ph = b.params + ((size_t)tk.q * K + j) * 5;
+            amp[j] = (float)(ph[0] / (ph[1] * 2.5066282746310002));
+        }
+        // on-state masks of this frame: wave 0, 64 spots per ballot
+        if (tid < 64) {
+            for (int j0 = 0; j0 < K; j0 += 64) {
+                const int j = j0 + tid;
+                const bool on = j < K && b.states[((size_t)tk.s * K + j) * T + tk.t] == 0;
+                const unsigned long long m = __ballot(on);
+                if (tid == 0) {
+                    onm[2 * (j0 >> 6)] = (unsigned)m;
+                    onm[2 * (j0 >> 6) + 1] = (unsigned)(m >> 32);
+                }
+            }
+        }
+        __syncthreads();
         for (int e = tid; e < K * nxy; e += 256) {
             const int j = e / nxy, a = e - j * nxy;
             const double *ph = b.params + ((size_t)tk.q * K + j) * 5;
             const float sig = (float)ph[1];
             const float d = a < X ? (float)(a + b.ox) - (float)ph[2] : (float)(a - X + b.oy) - (float)ph[3];
-            gxy[e] = expf(-(d * d) / (2.0f * sig * sig));
+            const float g = expf(-(d * d) / (2.0f * sig * sig));
+            gxy[e] = a < X ? amp[j] * g : g;
         }
         for (int e = tid; e < K * Zp; e += 256) {
             const int j = e / Zp, z = e - j * Zp;
@@ -75,11 +94,6 @@ __global__ __launch_bounds__(256) void b3_resid(const DevBlock *__restrict__ blo
                 v = expf(-(d * d) / (2.0f * sig * sig));
             }
             gzt[e] = v;
-        }
-        for (int j = tid; j < K; j += 256) {
-            const double *ph = b.params + ((size_t)tk.q * K + j) * 5;
-            amp[j] = (float)(ph[0] / (ph[1] * 2.5066282746310002));
-            on[j] = b.states[((size_t)tk.s * K + j) * T + tk.t] == 0;
         }
         __syncthreads();
         const float *dt = b.data + (size_t)tk.t * P;
@@ -104,14 +118,22 @@ __global__ __launch_bounds__(256) void b3_resid(const DevBlock *__restrict__ blo
                 dv[u] = dt[(size_t)min(z0 + u, Z - 1) * XY + col];
                 m[u] = 0.0f;
             }
-            for (int j = 0; j < K; j++) {
-                if (!on[j]) continue;
-                const float *gj = gxy + j * nxy;
-                const float a = amp[j] * gj[x] * gj[X + y];
-                const float4 *gz = (const float4 *)(gzt + j * Zp + z0);
-                const float4 g0 = gz[0], g1 = gz[1];
-                m[0] += a * g0.x; m[1] += a * g0.y; m[2] += a * g0.z; m[3] += a * g0.w;
-                m[4] += a * g1.x; m[5] += a * g1.y; m[6] += a * g1.z; m[7] += a * g1.w;
+            // the on-spots in increasing order, from the masks in scalar
+            // registers (the sum's order is that of the spots)
+            for (int j0 = 0; j0 < K; j0 += 64) {
+                unsigned long long msk =
+                    (unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)onm[2 * (j0 >> 6)]) |
+                    ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)onm[2 * (j0 >> 6) + 1]) << 32);
+                while (msk) {
+                    const int j = j0 + __builtin_ctzll(msk);
+                    msk &= msk - 1ull;
+                    const float *gj = gxy + j * nxy;
+                    const float a = gj[x] * gj[X + y];  // (amp gx) gy
+                    const float4 *gz = (const float4 *)(gzt + j * Zp + z0);
+                    const float4 g0 = gz[0], g1 = gz[1];
+                    m[0] += a * g0.x; m[1] += a * g0.y; m[2] += a * g0.z; m[3] += a * g0.w;
+                    m[4] += a * g1.x; m[5] += a * g1.y; m[6] += a * g1.z; m[7] += a * g1.w;
+                }
             }
 #pragma unroll
             for (int u = 0; u < RZ; u++)
