@@ -22,7 +22,7 @@ __global__ void b3_resid(const DevBlock *__restrict__ blocks, const PlaneTask *_
                          float *__restrict__ R, double *__restrict__ S0, int s0_stride,
                          const DevModel *__restrict__ mdl);
 template <int MODE> __global__ void b3_eval(EvalParams E);
-template <bool GRAD, bool FUNC> __global__ void b3_corr_mfma(CorrParams C);
+template <bool GRAD, bool FUNC, int NCU> __global__ void b3_corr_mfma(CorrParams C);
 template <int MODE> __global__ void b3_hmm(HmmParams H);
 template <int MODE> __global__ void b3_hmm_wave(HmmParams H);
 struct LogA {
@@ -201,9 +201,11 @@ static void batch_free(rfba3b_batch *b) {
 
 typedef void (*CorrFn)(CorrParams);
 typedef void (*HmmFn)(HmmParams);
-static CorrFn corr_fn(bool grad) { return grad ? b3_corr_mfma<true, false> : b3_corr_mfma<false, false>; }
-// the resident functor's form: region mask and pixel splits
-static CorrFn corr_fn_functor() { return b3_corr_mfma<true, true>; }
+static CorrFn corr_fn(bool grad) { return grad ? b3_corr_mfma<true, false, 0> : b3_corr_mfma<false, false, 0>; }
+// the resident functor's form: region mask and pixel splits; for one or two
+// candidates (at most 12 basis columns) matrix work on the first 16-column
+// tile only
+static CorrFn corr_fn_functor(int m) { return m <= 2 ? b3_corr_mfma<true, true, 1> : b3_corr_mfma<true, true, 0>; }
 
 // The HMMs of a launch: one wave per item (b3_hmm_wave, the frames of an
 // item spread over the lanes) when every block's T fits the wave's
@@ -890,8 +892,10 @@ extern "C" int rfba3b_functor_create(const int *xyz, int P, int T, const double 
     if (e == hipSuccess) e = hipMemset(f->d_states, 1, NI * (size_t)T);  // every candidate off in every frame
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e == hipSuccess && f->lds_corr > 64 * 1024)
-        e = hipFuncSetAttribute((const void *)corr_fn_functor(), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)f->lds_corr);
+        for (int m : {1, 3})
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute((const void *)corr_fn_functor(m), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)f->lds_corr);
     if (e != hipSuccess) {
         const int code = e == hipErrorOutOfMemory ? -2 : -3;
         functor_free(f);
@@ -939,7 +943,7 @@ static int functor_sums(rfba3b_functor *f, int m, hipStream_t st) {
         C.part = f->d_part;
         C.part_qh = f->d_part_qh;
     }
-    hipLaunchKernelGGL(corr_fn_functor(), dim3((nc * nsplit + 7) / 8 * 8), dim3(256), f->lds_corr, st, C);
+    hipLaunchKernelGGL(corr_fn_functor(m), dim3((nc * nsplit + 7) / 8 * 8), dim3(256), f->lds_corr, st, C);
     HIP_TRY(hipGetLastError());
     if (nsplit > 1) {
         CorrReduceParams Rd;

@@ -456,8 +456,11 @@ __global__ __launch_bounds__(256) void b3_eval(EvalParams E) {
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 // FUNC: the resident functor's form (a region mask, pixel splits with raw
-// partial sums); the batched form compiles without them.
-template <bool GRAD, bool FUNC>
+// partial sums); the batched form compiles without them.  NCU > 0: matrix
+// work on the first NCU 16-column tiles only (tasks whose spots' basis
+// columns all lie there: at most 2 spots for the gradient); the sums of
+// those columns are the same bits as with all tiles.
+template <bool GRAD, bool FUNC, int NCU>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void b3_corr_mfma(CorrParams C) {
     extern __shared__ double ldsc[];
     constexpr int NB = GRAD ? 6 : 1;
@@ -466,6 +469,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     // basis chunk row stride: a multiple of 16 that is 16 or 48 mod 64, so the
     // four pixel rows a B-operand read touches fall on disjoint bank groups
     constexpr int BSTR = NCT * 16;
+    constexpr int NCX = NCU > 0 && NCU < NCT ? NCU : NCT;  // column tiles with matrix work
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // Workgroup b runs on XCD b % 8: give each XCD a contiguous range of
     // tasks so that the spot groups of one plane share that XCD's L2 (the
@@ -571,12 +575,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     const int lr = lane & 15, lk = lane >> 4;
     for (int t0 = 0; t0 < T; t0 += kCorrFrames) {
         const int tn = min(kCorrFrames, T - t0);
-        v4f acc[2][NCT];
-        double a64[2][NCT][4];
+        v4f acc[2][NCX];
+        double a64[2][NCX][4];
 #pragma unroll
         for (int f2 = 0; f2 < 2; f2++)
 #pragma unroll
-            for (int ct = 0; ct < NCT; ct++) {
+            for (int ct = 0; ct < NCX; ct++) {
                 acc[f2][ct] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
                 for (int r = 0; r < 4; r++) a64[f2][ct][r] = 0.0;
@@ -653,22 +657,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 #pragma unroll
             for (int k4 = 0; k4 < kCorrPix; k4 += 4) {
                 const int kk = k4 + lk;
-                float av[2], bv[NCT];
+                float av[2], bv[NCX];
 #pragma unroll
                 for (int f2 = 0; f2 < 2; f2++) av[f2] = Rs[((2 * wave + f2) * 16 + lr) * kCorrRow + kk];
 #pragma unroll
-                for (int ct = 0; ct < NCT; ct++) bv[ct] = Bs[kk * BSTR + ct * 16 + lr];
+                for (int ct = 0; ct < NCX; ct++) bv[ct] = Bs[kk * BSTR + ct * 16 + lr];
 #pragma unroll
                 for (int f2 = 0; f2 < 2; f2++)
 #pragma unroll
-                    for (int ct = 0; ct < NCT; ct++)
+                    for (int ct = 0; ct < NCX; ct++)
                         acc[f2][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[f2], bv[ct], acc[f2][ct], 0, 0, 0);
             }
             if ((nch & 3) == 3 || p0 + kCorrPix >= pend) {
 #pragma unroll
                 for (int f2 = 0; f2 < 2; f2++)
 #pragma unroll
-                    for (int ct = 0; ct < NCT; ct++) {
+                    for (int ct = 0; ct < NCX; ct++) {
 #pragma unroll
                         for (int r = 0; r < 4; r++) a64[f2][ct][r] += (double)acc[f2][ct][r];
                         acc[f2][ct] = v4f{0.0f, 0.0f, 0.0f, 0.0f};
@@ -704,7 +708,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
 #pragma unroll
         for (int f2 = 0; f2 < 2; f2++)
 #pragma unroll
-            for (int ct = 0; ct < NCT; ct++) {
+            for (int ct = 0; ct < NCX; ct++) {
                 const int col = ct * 16 + lr;
                 const int j = col / NB, n = col - j * NB;
                 if (col >= NCOL || j >= nk) continue;
@@ -742,9 +746,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     }
 }
 
-template __global__ void b3_corr_mfma<false, false>(CorrParams);
-template __global__ void b3_corr_mfma<true, false>(CorrParams);
-template __global__ void b3_corr_mfma<true, true>(CorrParams);
+template __global__ void b3_corr_mfma<false, false, 0>(CorrParams);
+template __global__ void b3_corr_mfma<true, false, 0>(CorrParams);
+template __global__ void b3_corr_mfma<true, true, 0>(CorrParams);
+template __global__ void b3_corr_mfma<true, true, 1>(CorrParams);
 
 // b3_corr_reduce: the pixel-split correlation sums of one (frame, item),
 // added over the splits in order in FP64, then the on/off forms of
