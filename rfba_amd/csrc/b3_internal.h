@@ -189,14 +189,21 @@ __device__ __forceinline__ void xsum3(double m0, double e0, double m1, double e1
     xsplit(s, E, m, e);
 }
 
+// xalign for e <= E (E the largest of the exponents, zero terms included):
+// e - E is at most 0, or NaN for -inf - -inf (a zero term), which fmax
+// turns into -2000.
+__device__ __forceinline__ double xalign_le(double m, double e, double E) {
+    return ldexp(m, (int)fmax(e - E, -2000.0));
+}
+
 // xsum3 for products of values that xsplit formed, where a zero term
 // always carries exponent -inf (no masking needed)
 __device__ __forceinline__ void xsum3n(double m0, double e0, double m1, double e1, double m2, double e2, double &m,
                                        double &e) {
     const double E = fmax(e0, fmax(e1, e2));
-    double s = xalign(m0, e0, E);
-    s += xalign(m1, e1, E);
-    s += xalign(m2, e2, E);
+    double s = xalign_le(m0, e0, E);
+    s += xalign_le(m1, e1, E);
+    s += xalign_le(m2, e2, E);
     xsplit(s, E, m, e);
 }
 
