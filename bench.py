@@ -464,7 +464,7 @@ def bench_b3_single(args):
 def bench_b3_functor(args):
     """The CG functor and the FFBS Hessian of one spot, as the 3B fit calls
     them: SampledBackgroundData built once (multispot5.cc:538-575) for spot 0
-    of the C4 block (29^3, T=100, 40 spots, seed 6) with S = 10 background
+    of the b3_single block (29^3, T=100, 40 spots, seed 6) with S = 10 background
     samples (main.gibbs.samples, multispot5.cfg:28: the other spots at 10
     shifted sampled state paths), then operator() per conjugate-gradient call
     (one candidate vector) and sampled_background_spot_hessian_ffbs with
