@@ -19,6 +19,8 @@
 // reduces in a fixed order.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
@@ -224,10 +226,31 @@ __global__ __launch_bounds__(kThreads) void prep_highpass(const Tin *__restrict_
     const int x0 = tx * kTX, y0 = ty * kTY;
     const Tin *src = raw + slice * (long long)X * Y;
 
-    for (int e = tid; e < WX * WY; e += kThreads) {
-        const int r = e / WX, c = e - r * WX;
-        const int gy = min(max(y0 - ks + r, 0), Y - 1), gx = min(max(x0 - ks + c, 0), X - 1);
-        win[r * WS + c] = to_f(src[(size_t)gy * X + gx]);
+    // Interior uint16 tiles (no edge clamping, rows 8-byte aligned): each row
+    // of the window from aligned 4-value loads starting at x0 - ks - 1
+    // (x0 is a multiple of kTX, a multiple of 4, and ks = 15), 4 values
+    // per load instead of one.
+    constexpr bool kVec = std::is_same<Tin, uint16_t>::value && KS == 15;
+    const int xa = x0 - ks - 1;
+    if (kVec && xa >= 0 && xa + WX + 2 <= X && y0 - ks >= 0 && y0 + kTY + ks <= Y && (X & 3) == 0 &&
+        ((size_t)src & 7) == 0) {
+        constexpr int NV4 = (KS > 0 ? kTX + 2 * KS + 2 : 4) / 4;  // 4-value loads per row (96 values)
+        for (int e = tid; e < NV4 * WY; e += kThreads) {
+            const int r = e / NV4, v = e - r * NV4;
+            const ushort4 q = *(const ushort4 *)(src + (size_t)(y0 - ks + r) * X + xa + 4 * v);
+            const int c = 4 * v - 1;  // window column of q.x
+            float *wr = win + r * WS;
+            if (c >= 0) wr[c] = to_f((Tin)q.x);
+            wr[c + 1] = to_f((Tin)q.y);
+            wr[c + 2] = to_f((Tin)q.z);
+            if (c + 3 < WX) wr[c + 3] = to_f((Tin)q.w);
+        }
+    } else {
+        for (int e = tid; e < WX * WY; e += kThreads) {
+            const int r = e / WX, c = e - r * WX;
+            const int gy = min(max(y0 - ks + r, 0), Y - 1), gx = min(max(x0 - ks + c, 0), X - 1);
+            win[r * WS + c] = to_f(src[(size_t)gy * X + gx]);
+        }
     }
     __syncthreads();
     if (!skip) {

@@ -33,6 +33,8 @@ CASES = [
     ((1, 2, 80, 75), np.uint16, dict(lpf=12.0)),              # half-width 36: an 84 KB LDS window
     ((2, 2, 5, 7), np.uint16, dict(skip=1)),                  # no high-pass
     ((2, 2, 9, 11), np.uint16, dict(skip=1, fixed_scaling=1, subtract=100.0, divide=7.5)),
+    ((2, 2, 112, 208), np.uint16, dict(lpf=5.0)),             # interior tiles: 4-value window loads
+    ((1, 2, 100, 170), np.uint16, dict(lpf=5.0)),             # rows not 8-byte aligned: element loads
 ]
 
 
