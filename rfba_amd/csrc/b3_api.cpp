@@ -957,7 +957,7 @@ static int functor_sums(rfba3b_functor *f, int m, hipStream_t st) {
         Rd.part_qh = f->d_part_qh;
         Rd.S0 = f->d_S0;
         Rd.em = f->d_em;
-        hipLaunchKernelGGL(b3_corr_reduce, dim3((NI * T * 6 + 255) / 256), dim3(256), 0, st, Rd);
+        hipLaunchKernelGGL(b3_corr_reduce, dim3((NI * T * 6 * 8 + 255) / 256), dim3(256), 0, st, Rd);
         HIP_TRY(hipGetLastError());
     }
     return 0;

@@ -84,7 +84,7 @@ struct CorrParams {
     // Pixel-split form (nsplit > 1): virtual task v is task v / nsplit over
     // pixels [split pstep, (split + 1) pstep) of its plane, split = v %
     // nsplit; instead of em it writes its raw sums, which b3_corr_reduce adds
-    // in split order and turns into em.
+    // in a fixed order and turns into em.
     int nsplit;             // 0 or 1: whole planes
     int pstep;              // pixels per split (a multiple of kCorrPix)
     double *part;           // [nsplit][Tmax][NI][6]: C, G_0..4

@@ -756,23 +756,34 @@ template __global__ void b3_corr_mfma<true, true, 1>(CorrParams);
 // b3_corr_mfma's epilogue (Soff = S0 + [on](2C + Q), D = 2C +- Q,
 // diff_n = G_n - [off] H_n).
 __global__ __launch_bounds__(256) void b3_corr_reduce(CorrReduceParams R) {
-    // one thread per (frame, item, field n): field 0 gives Soff and D, field
-    // n >= 1 gives diff_{n-1}; the splits are added in split order
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= R.NI * R.T * 6) return;
-    const int n = i % 6, ti = i / 6;
+    // 8 lanes per (frame, item, field n): lane u adds splits u, u + 8, ...
+    // in order, then the 8 partial sums are combined by a fixed butterfly
+    // (a fixed order: the result does not depend on the launch).  Field 0
+    // gives Soff and D, field n >= 1 gives diff_{n-1}.
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    const int i = g >> 3, u = g & 7;
+    const bool live = i < R.NI * R.T * 6;
+    const int ii = live ? i : 0;
+    const int n = ii % 6, ti = ii / 6;
     const int t = ti / R.NI, item = ti - t * R.NI;
-    const Item it = R.items[item];
-    const DevBlock &b = *R.block;
     const double *__restrict__ pv = R.part + ((size_t)t * R.NI + item) * 6 + n;
     const double *__restrict__ pq = R.part_qh + (size_t)item * 6 + n;
     const size_t sv = (size_t)R.Tmax * R.NI * 6, sq = (size_t)R.NI * 6;
     double v = 0.0, qh = 0.0;
-#pragma unroll 8
-    for (int sp = 0; sp < R.nsplit; sp++) {
-        v += pv[sp * sv];
-        qh += pq[sp * sq];
+    if (live) {
+        for (int sp = u; sp < R.nsplit; sp += 8) {
+            v += pv[sp * sv];
+            qh += pq[sp * sq];
+        }
     }
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+        v += __shfl_xor(v, o);
+        qh += __shfl_xor(qh, o);
+    }
+    if (!live || u != 0) return;
+    const Item it = R.items[item];
+    const DevBlock &b = *R.block;
     const bool on = b.states[((size_t)it.s * b.K + it.k) * b.T + t] == 0;
     double *m = R.em + ((size_t)t * R.NI + item) * 8;
     if (n == 0) {
