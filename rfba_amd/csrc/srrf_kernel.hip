@@ -112,8 +112,13 @@ __device__ __forceinline__ float term_general(float Gx, float Gy, float Gz, cons
 // [2^-96, 2^100] and R in [2^-20, 2^20] (checked on the host, else every
 // sample takes the general form), den = GMag * R lies in [2^-68, 2^70]; a
 // numerator >= 2^-60 then gives a quotient >= 2^-130, which rounds to a
-// positive value, a numerator <= 0 a quotient that is not > 0, and a NaN
-// numerator a NaN: so "diverging" is !(num <= 0) unless 0 < num < 2^-60.
+// positive value, and a numerator <= 0 a quotient that is not > 0.  A term
+// not flagged has finite G (NaN or infinite G makes g2 NaN or infinite), so
+// a finite numerator: "diverging" is num > 0 unless 0 < num < 2^-60.
+// G2B: g2 <= 2^100 holds for every input (integer input: each gradient
+// component is at most 2 * 8 * 17 * 17 terms of at most 65535, times
+// tricubic weights whose absolute sums are below 2.2, so g2 < 2^61), and
+// the upper test is left out.
 __device__ __forceinline__ float sqrt_core(float x) {
     const float s = __builtin_amdgcn_sqrtf(x);
     const float sd = __int_as_float(__float_as_int(s) - 1);
@@ -123,7 +128,7 @@ __device__ __forceinline__ float sqrt_core(float x) {
     return r;
 }
 
-template <bool RP2>
+template <bool RP2, bool G2B>
 __device__ __forceinline__ float term_fast(float Gx, float Gy, float Gz, const TermIn &a, float R, float invR,
                                            bool &bad) {
     float g2 = Gx * Gx + Gy * Gy;
@@ -131,7 +136,7 @@ __device__ __forceinline__ float term_fast(float Gx, float Gy, float Gz, const T
     const float GMag = sqrt_core(g2);
     float num = Gx * a.rx + Gy * a.ry;
     num = num + Gz * a.rz;
-    const bool dv = !(num <= 0.0f);
+    const bool dv = num > 0.0f;
     float tn = Gx * a.dx + Gy * a.dy;
     tn = tn + Gz * a.dz;
     const float tt = tn / (GMag * GMag);
@@ -142,7 +147,7 @@ __device__ __forceinline__ float term_fast(float Gx, float Gy, float Gz, const T
     Dk = 1.0f - (RP2 ? Dk * invR : Dk / R);
     Dk = Dk * Dk;
     // infinite or NaN d2 come out of sqrt_core as sqrtf gives them
-    bad = !(g2 >= 0x1p-96f && g2 <= 0x1p100f) || !(d2 >= 0x1p-96f) || (num > 0.0f && num < 0x1p-60f);
+    bad = !(g2 >= 0x1p-96f) || (!G2B && !(g2 <= 0x1p100f)) || !(d2 >= 0x1p-96f) || (dv && num < 0x1p-60f);
     return dv ? -Dk : Dk;
 }
 
@@ -244,7 +249,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
     float *st_oz = st_ox + g.st_oz;
     int *st_grp = (int *)(st_ox + g.st_grp);  // ix, run0, nrun
     int *st_run = (int *)(st_ox + g.st_run);  // start, len, iy, rx, ry
-    int *st_smp = (int *)(st_ox + g.st_smp);  // iz, rz
+    int *st_smp = (int *)(st_ox + g.st_smp);  // 32 iz (byte offset of the z-table entry), rz
     const float sR = S->R, sInvR = S->invR, sDiv = S->divisor;
     const int ngroups = S->ngroups;
     const int n_ox = S->nox, n_oy = S->noy;
@@ -270,7 +275,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
             st_run[5 * e + 4] = __float_as_int(S->ry[s0]);
         }
         for (int e = tid; e < ns; e += kThreads) {
-            st_smp[2 * e] = S->siz[e];
+            st_smp[2 * e] = S->siz[e] * 8 * (int)sizeof(float);  // byte offset of its z-table entry
             st_smp[2 * e + 1] = __float_as_int(S->rz[e]);
         }
     }
@@ -326,7 +331,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
             // the general term, NaN).  Likewise for x and y below.
             if (!valid) w0 = w1 = w2 = w3 = 0.0f;
             float *zt = ztab + e * 8;
-            zt[0] = __int_as_float(clampi(b - wlo, 0, NW - 4));
+            // first tap row as the float offset of its y-interpolated
+            // columns, Py[(w 3 + c) 256 + tid]
+            zt[0] = __int_as_float(clampi(b - wlo, 0, NW - 4) * 3 * kThreads);
             zt[1] = w0; zt[2] = w1; zt[3] = w2; zt[4] = w3;
             zt[5] = __int_as_float(valid);
             zt[6] = z0;
@@ -680,11 +687,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                         a0.Xc = a1.Xc = Xc; a0.Yc = a1.Yc = Yc;
                         a0.Zc = zcen[Z];
                         a1.Zc = zcen[Z + 1];
+                        const float *zrow0 = ztab + Z * noz * 8, *zrow1 = zrow0 + noz * 8;
+                        const float *Pyt = Py + tid;
                         for (int s = s0; s < s0 + ns; s++) {
-                            const int iz = st_smp[2 * s];
+                            const int izb = st_smp[2 * s];
                             a0.rz = a1.rz = __int_as_float(st_smp[2 * s + 1]);
-                            const float *zt0 = ztab + (Z * noz + iz) * 8;
-                            const float *zt1 = ztab + ((Z + 1) * noz + iz) * 8;
+                            const float *zt0 = (const float *)((const char *)zrow0 + izb);
+                            const float *zt1 = (const float *)((const char *)zrow1 + izb);
                             const int wr0 = __float_as_int(zt0[0]), wr1 = __float_as_int(zt1[0]);
                             a0.z0 = zt0[6]; a0.dz = zt0[7];
                             a1.z0 = zt1[6]; a1.dz = zt1[7];
@@ -692,8 +701,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                             float G0[3], G1[3];
 #pragma unroll
                             for (int c = 0; c < 3; c++) {
-                                const float *q0 = Py + (wr0 * 3 + c) * kThreads + tid;
-                                const float *q1 = Py + (wr1 * 3 + c) * kThreads + tid;
+                                const float *q0 = Pyt + wr0 + c * kThreads;
+                                const float *q1 = Pyt + wr1 + c * kThreads;
                                 float s0v = q0[0] * zt0[1];
                                 float s1v = q1[0] * zt1[1];
                                 s0v = s0v + q0[3 * kThreads] * zt0[2];
@@ -706,8 +715,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                                 G1[c] = s1v;
                             }
                             bool b0, b1;
-                            float t0 = term_fast<RP2>(G0[0], G0[1], G0[2], a0, sR, sInvR, b0);
-                            float t1 = term_fast<RP2>(G1[0], G1[1], G1[2], a1, sR, sInvR, b1);
+                            float t0 = term_fast<RP2, kIntInput<Tin>>(G0[0], G0[1], G0[2], a0, sR, sInvR, b0);
+                            float t1 = term_fast<RP2, kIntInput<Tin>>(G1[0], G1[1], G1[2], a1, sR, sInvR, b1);
                             b0 = b0 || !fast_ok;
                             b1 = b1 || !fast_ok;
                             if (b0 || b1) {
