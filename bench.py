@@ -405,6 +405,43 @@ def b3_roof(by, fl, kernel_ms):
     return roof
 
 
+def b3_fd_check(data, phi, states, m, grad, spots=4, zfactor=2.0):
+    """Central finite differences (h = 1e-3, relative for I and sigma) of the
+    FP64 oracle's logL of spot k over the other spots' sampled images plus
+    b0, against the GPU gradient grad[k], for the first `spots` spots with a
+    nonzero gradient.  The model's z derivative is the reference's, which
+    lacks 1/zF^2 (storm.h:78): it is the exact one times zF^2, so the
+    difference quotient's z component is scaled by zF^2 before comparing."""
+    from oracle import b3oracle as bo
+    T, Z, Y, X = data.shape
+    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    xyz = np.stack([x.ravel(), y.ravel(), z.ravel()], 1).astype(np.int32)
+    d2 = data.reshape(T, -1).astype(np.float64)
+    K = phi.shape[0]
+    imgs = [bo.spot(xyz, phi[j], zfactor)[0] for j in range(K)]
+    on = states == 0  # [K, T]
+    errs = []
+    for k in range(K):
+        if len(errs) == spots:
+            break
+        if not np.linalg.norm(grad[k]) > 0:
+            continue
+        bg = np.full_like(d2, 100.0)
+        for j in range(K):
+            if j != k:
+                bg += on[j][:, None] * imgs[j][None]
+        fd = np.empty(5)
+        for n in range(5):
+            h = 1e-3 * (abs(phi[k, n]) if n < 2 else 1.0)
+            e = np.zeros(5)
+            e[n] = h
+            fd[n] = (bo.logl_grad(xyz, d2, bg, phi[k] + e, m)[0] - bo.logl_grad(xyz, d2, bg, phi[k] - e, m)[0]) / (2 * h)
+        fd[4] *= zfactor * zfactor
+        errs.append(float(np.linalg.norm(grad[k] - fd) / np.linalg.norm(fd)))
+    return {"grad_max_rel": max(errs) if errs else None, "spots": len(errs), "parameter_set": 0,
+            "h": "1e-3 (times |I|, |sigma| for those two)"}
+
+
 def bench_b3_single(args):
     from rfba_amd import b3
     from rfba_amd.synth import default_threads
@@ -455,6 +492,7 @@ def bench_b3_single(args):
         res["error_vs_fp64_oracle"] = {"logL_max_rel": el, "grad_max_rel": eg, "parameter_sets": q,
                                        "zero_gradient_items": int((~nz).sum()),
                                        "zero_gradient_items_exact": bool(np.all(G[:q][~nz] == 0))}
+        res["error_vs_central_fd"] = b3_fd_check(data, P[0], states, m, G[0, :, 0])
         res["cpu_baseline"] = {"value": q / dt, "unit": "block-evals/s", "cores": th, "kind": "port",
                                "host": host_cpu(th),
                                "sample": "FP64 oracle, %d parameter sets x 40 spots, %.1f s" % (q, dt)}
