@@ -19,8 +19,10 @@ Beside it, in "workloads" of the same JSON line:
                 batched vs one launch per parameter set;
   b3_batched    1200 blocks of the same split (seed 8), K ~ Poisson(40)
                 clipped to 5-120, T=120, 32 parameter sets per block,
-                checkpoint stride 11: batched vs one launch per (block,
-                parameter set), the latter on the first 100 blocks;
+                checkpoint stride 11 in the model (at T=120 the HMM runs
+                one wave per item and stores no messages): batched vs one
+                launch per block (first 100) and per (block, parameter
+                set) (first 10);
   b3_functor    the CG functor of one spot of that block with 10 background
                 samples, created once and called per conjugate-gradient step
                 (vs the per-call re-upload path), and its FFBS Hessian with
@@ -470,7 +472,7 @@ def bench_b3_single(args):
     roof = b3_roof(by, fl, kb)
     res = {
         "workload": "3B logL + adjoint, one 29^3 block, T=100, 40 spots (seed 6), 256 parameter sets (+-0.5 px, "
-                    "+-20%% intensity, seed 7), S=1, checkpoints every ceil(sqrt(T)) frames",
+                    "+-20%% intensity, seed 7), S=1, one-wave-per-item HMM (no stored messages)",
         "block_evals": 256, "spot_evals": 256 * 40,
         "batched": {"ms": wall_b, "kernel_ms": kb, "block_evals_per_s": 256 / (wall_b * 1e-3)},
         "per_param_set": {"ms": wall_pb, "kernel_ms": ms_pb, "block_evals_per_s": 256 / (wall_pb * 1e-3)},
@@ -645,7 +647,8 @@ def bench_b3_batched(args):
     res = {
         "workload": "batched 3B logL + adjoint: 1200 blocks of the 512x512x128 doSplit (29^3, 3-voxel overlap; "
                     "seed 8), K ~ Poisson(40) clipped 5-120, T=120, 32 parameter sets per block, S=1, "
-                    "checkpoint stride 11",
+                    "checkpoint stride 11 in the model (T <= 512 runs the one-wave-per-item HMM, whose lanes "
+                    "hold their frames' messages in registers and store none, so the stride is not used)",
         "blocks": len(blocks), "block_evals": n_ev, "spot_evals": int(K.sum()) * Q,
         "batched": {"ms": wall_b, "kernel_ms": kb, "block_evals_per_s": n_ev / (wall_b * 1e-3)},
         "plain_per_block": {"blocks": nsub, "ms": wall_pb, "kernel_ms": ms_pb,
