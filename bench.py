@@ -619,6 +619,20 @@ def bench_b3_batched(args):
         kms.append(bt.run(use_graph=True))
         log("c5: batched step %d %.1f ms" % (i, kms[-1]))
     wall_b = (time.perf_counter() - t0) * 1e3 / len(kms)
+    L0, G0 = bt.results(0)
+    # the same launch with the checkpointed lane-form HMM (forward messages
+    # kept every 11 frames, the rest recomputed in the adjoint)
+    bt.set_hmm_form(1)
+    bt.run(use_graph=True)
+    t0 = time.perf_counter()
+    ms_lane = bt.run(use_graph=True)
+    wall_lane = (time.perf_counter() - t0) * 1e3
+    L1, G1 = bt.results(0)
+    lane_dl = float(np.max(np.abs(L1 - L0) / np.abs(L0)))
+    ng = np.linalg.norm(G0, axis=-1)
+    nz = ng > 0
+    lane_dg = float(np.max(np.linalg.norm(G1 - G0, axis=-1)[nz] / ng[nz])) if nz.any() else 0.0
+    log("c5: batched with the checkpointed lane HMM %.1f ms" % wall_lane)
     bt.close()
     nsub = min(args.c5_plain_blocks, len(blocks))
     sub = b3.Batch(blocks[:nsub], m)
@@ -651,6 +665,10 @@ def bench_b3_batched(args):
                     "hold their frames' messages in registers and store none, so the stride is not used)",
         "blocks": len(blocks), "block_evals": n_ev, "spot_evals": int(K.sum()) * Q,
         "batched": {"ms": wall_b, "kernel_ms": kb, "block_evals_per_s": n_ev / (wall_b * 1e-3)},
+        "batched_checkpointed_lane_hmm": {"ms": wall_lane, "kernel_ms": ms_lane, "checkpoint_stride": 11,
+                                          "block_evals_per_s": n_ev / (wall_lane * 1e-3),
+                                          "block0_vs_wave_form": {"logL_max_rel": lane_dl,
+                                                                  "grad_max_rel": lane_dg}},
         "plain_per_block": {"blocks": nsub, "ms": wall_pb, "kernel_ms": ms_pb,
                             "block_evals_per_s": nsub * Q / (wall_pb * 1e-3),
                             "batched_ms_same_blocks": wall_sub,

@@ -166,6 +166,14 @@ int rfba3b_batch_run_per_block(rfba3b_batch *plan, void *stream, int per_param_s
  * chunk's planes fit the plane budget: RFBA3B_PLANE_BUDGET_MB in the
  * environment at rfba3b_batch_create, default 16384 MiB). */
 int rfba3b_batch_chunks(const rfba3b_batch *plan);
+/* HMM form of the plan's runs (forward_algorithm_deriv,
+ * forward_algorithm.h:157-332): 0 (default) chooses by the longest series,
+ * one wave per item for T <= 512 (each lane holds its frames' messages in
+ * registers; nothing is stored) and one lane per item above; 1 always runs
+ * the lane form, which keeps the forward messages every `checkpoint` frames
+ * of the model (ceil(sqrt(T)) for 0) and recomputes the rest in the adjoint.
+ * Returns the previous form, or a negative status. */
+int rfba3b_batch_set_hmm_form(rfba3b_batch *plan, int form);
 void rfba3b_batch_destroy(rfba3b_batch *plan);
 
 /* ---- Gibbs sampling of the blink states --------------------------------

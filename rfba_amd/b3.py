@@ -118,6 +118,8 @@ def lib():
         L.rfba3b_batch_run_per_block.restype = c_int
         L.rfba3b_batch_chunks.argtypes = [c_void_p]
         L.rfba3b_batch_chunks.restype = c_int
+        L.rfba3b_batch_set_hmm_form.argtypes = [c_void_p, c_int]
+        L.rfba3b_batch_set_hmm_form.restype = c_int
         L.rfba3b_batch_destroy.argtypes = [c_void_p]
         L.rfba3b_batch_destroy.restype = None
         L.rfba3b_gibbs_create.argtypes = [POINTER(_GibbsChain), c_int, POINTER(Model), POINTER(c_void_p),
@@ -325,6 +327,15 @@ class Batch:
         ms = c_float(0)
         _check(lib().rfba3b_batch_run_per_block(self.plan, None, int(per_param_set), ctypes.byref(ms)))
         return ms.value
+
+    def set_hmm_form(self, form: int) -> int:
+        """0: one wave per item for T <= 512 (the default), else one lane per
+        item; 1: always the lane form with the model's checkpoint stride.
+        Returns the previous form."""
+        rc = lib().rfba3b_batch_set_hmm_form(self.plan, int(form))
+        if rc < 0:
+            _check(rc)
+        return rc
 
     @property
     def chunks(self) -> int:

@@ -160,6 +160,7 @@ struct rfba3b_batch {
     };
     std::vector<Chunk> chunks;
     bool grad = true;
+    int hmm_form = 0;  // rfba3b_batch_set_hmm_form
     int Tmax = 1, naxmax = 1, ckslots = 1;
     size_t lds_resid = 0, lds_corr = 0;
     DevModel *d_model = nullptr;
@@ -215,9 +216,9 @@ static CorrFn corr_fn_functor(int m) { return m <= 2 ? b3_corr_mfma<true, true, 
 // Measured on MI355X (29^3 block, T = 100): a functor call of 10 items
 // takes 0.39 ms with the lane form and 0.15 ms with the wave form; one
 // launch of 10240 items costs the same in both forms.
-static int launch_hmm(int mode, const HmmParams &H, int Tmax, hipStream_t st) {
+static int launch_hmm(int mode, const HmmParams &H, int Tmax, hipStream_t st, bool lane_form = false) {
     if (H.n <= 0) return 0;
-    if (Tmax <= kWaveTmax) {
+    if (Tmax <= kWaveTmax && !lane_form) {
         const HmmFn fn = mode == MODE_GRAD    ? b3_hmm_wave<MODE_GRAD>
                          : mode == MODE_DELTA ? b3_hmm_wave<MODE_DELTA>
                                               : b3_hmm_wave<MODE_LOGL>;
@@ -434,7 +435,7 @@ static int enqueue_range(rfba3b_batch *b, hipStream_t st, const rfba3b_batch::Ch
         H.em = b->d_em;
         H.ck = b->d_ck;
         H.model = b->d_model;
-        const int rc = launch_hmm(b->grad ? MODE_GRAD : MODE_LOGL, H, b->Tmax, st);
+        const int rc = launch_hmm(b->grad ? MODE_GRAD : MODE_LOGL, H, b->Tmax, st, b->hmm_form == 1);
         if (rc) return rc;
     }
     return 0;
@@ -510,6 +511,22 @@ extern "C" int rfba3b_batch_run_per_block(rfba3b_batch *b, void *stream, int per
 extern "C" int rfba3b_batch_chunks(const rfba3b_batch *b) {
     if (!b) return fail(-1, "plan is NULL");
     return (int)b->chunks.size();
+}
+
+extern "C" int rfba3b_batch_set_hmm_form(rfba3b_batch *b, int form) {
+    if (!b) return fail(-1, "plan is NULL");
+    if (form != 0 && form != 1) return fail(-1, "HMM form %d: 0 (by series length) or 1 (lane form)", form);
+    std::lock_guard<std::mutex> lk(g_mu);
+    const int prev = b->hmm_form;
+    if (form != prev) {
+        // a captured graph holds the other form's launches
+        if (b->exec) hipGraphExecDestroy(b->exec);
+        if (b->graph) hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+        b->hmm_form = form;
+    }
+    return prev;
 }
 
 extern "C" void rfba3b_batch_destroy(rfba3b_batch *b) {

@@ -61,6 +61,37 @@ def test_batched_vs_oracle(gpu, ckpt):
     compare(L, G, Lo, Go, "ckpt=%d" % ckpt)
 
 
+@pytest.mark.parametrize("ckpt", [0, 1, 3, 11])
+def test_checkpointed_lane_hmm_vs_oracle(gpu, ckpt):
+    # the lane form of the HMM (forward messages kept every ckpt frames, the
+    # rest recomputed in the adjoint; stride 11 is the batched 3B config's),
+    # forced at T = 120 where the default is the one-wave form; a captured
+    # graph is rebuilt when the form changes
+    m = model(checkpoint=ckpt)
+    data, params, states = b3.synth_block((10, 9, 8), 120, 6, 41)
+    P = b3.perturb(params, 2, 42)
+    st = np.stack([states, np.roll(states, 7, axis=1)])
+    bt = b3.Batch([b3.BlockData(data, P, st, 100.0)], m)
+    bt.run(use_graph=True)
+    Lw, Gw = bt.results(0)
+    assert bt.set_hmm_form(1) == 0
+    bt.run(use_graph=True)
+    L, G = bt.results(0)
+    bt.run_per_block(per_param_set=True)
+    L2, G2 = bt.results(0)
+    assert bt.set_hmm_form(0) == 1
+    bt.run(use_graph=True)
+    Lw2, Gw2 = bt.results(0)
+    with pytest.raises(b3.B3Error):
+        bt.set_hmm_form(2)
+    bt.close()
+    assert np.array_equal(L, L2) and np.array_equal(G, G2)
+    assert np.array_equal(Lw, Lw2) and np.array_equal(Gw, Gw2)
+    Lo, Go = bo.model_eval(data, 100.0, m, P, st)
+    compare(L, G, Lo, Go, "lane form ckpt=%d" % ckpt)
+    compare(Lw, Gw, Lo, Go, "wave form")
+
+
 def test_series_longer_than_one_pass(gpu):
     # T = 300: three 128-frame passes of the correlation kernel, the last one
     # partial (the frame-independent sums come from the first pass); K = 10
