@@ -169,6 +169,10 @@ int read_pages(File &F, const char *path, std::vector<Page> &pages) {
         if (p.bits != 8 && p.bits != 16) return fail(-3, "%s: page %zu has %u bits per sample", path, pages.size(), p.bits);
         if (p.format == 3) return fail(-3, "%s: page %zu holds floating-point samples", path, pages.size());
         if (!p.width || !p.height) return fail(-3, "%s: page %zu has no size", path, pages.size());
+        // page sizes and byte counts stay far inside int and size_t
+        if (p.width > (1u << 20) || p.height > (1u << 20))
+            return fail(-3, "%s: page %zu is %u x %u pixels (at most 2^20 per side)", path, pages.size(), p.width,
+                        p.height);
         if (p.offsets.empty() || p.offsets.size() != p.counts.size())
             return fail(-3, "%s: page %zu has inconsistent strips", path, pages.size());
         pages.push_back(std::move(p));

@@ -217,6 +217,14 @@ def test_tiff_rejects_bad_files(tmp_path):
     q.write_bytes(bytes(raw))
     with pytest.raises(prep.PrepError, match="bits"):
         prep.tiff_load(q)
+    # a page width beyond 2^20 (a LONG value) is rejected before any size is formed
+    raw = bytearray(tiff_bytes(pages, "<", 16, 4))
+    i = raw.find(struct.pack("<HHI", 256, 3, 1))
+    raw[i:i + 12] = struct.pack("<HHII", 256, 4, 1, 1 << 21)
+    w = tmp_path / "wide.tif"
+    w.write_bytes(bytes(raw))
+    with pytest.raises(prep.PrepError, match="2\\^20"):
+        prep.tiff_info(w)
 
 
 REF_DATA = "/root/reference/Test Data"
