@@ -109,7 +109,9 @@ int rfba_srrf_build_ring(int axes_in_ring, float radius, float *xyz_out);
 
 /* Axis-aligned (kind 6) or 3x3x3-neighbourhood (kind 26) direction sets
  * scaled to `radius`; offsets are computed in double and rounded once.
- * Samples sharing an (x, y) offset are listed consecutively.  Returns n. */
+ * Listed by |x| (largest first), then x, y, z: samples sharing an x offset,
+ * and those sharing an (x, y) offset, are consecutive, and x = -radius and
+ * x = +radius are neighbours.  Returns n. */
 int rfba_srrf_build_dirs(int kind, float radius, float *xyz_out);
 
 /* ---- explicit single-stack entry points --------------------------------

@@ -196,7 +196,12 @@ extern "C" int rfba_srrf_build_dirs(int kind, float radius, float *xyz) {
                 const double len = std::sqrt((double)(dx * dx + dy * dy + dz * dz));
                 v.push_back({(float)(radius * dx / len), (float)(radius * dy / len), (float)(radius * dz / len)});
             }
+    // Order: by |x| (largest first), then x, y, z.  The samples of one x
+    // offset are consecutive (one group, runs of equal y), and the groups
+    // x = -R and x = +R are neighbours: at R = 1/2 the kernel serves both
+    // from one x pass (their sample positions differ by one SR column).
     std::stable_sort(v.begin(), v.end(), [](const P3 &a, const P3 &b) {
+        if (std::fabs(a.x) != std::fabs(b.x)) return std::fabs(a.x) > std::fabs(b.x);
         if (a.x != b.x) return a.x < b.x;
         if (a.y != b.y) return a.y < b.y;
         return a.z < b.z;
@@ -268,6 +273,32 @@ int analyse_samples(const float *xyz, int n, float divisor, float R, DevSamples 
         prev_ix = ix;
         prev_iy = iy;
     }
+    // Merge consecutive groups a, b = a + 1 (x offsets multiples of 1/2, so
+    // that (X + 1/2) + b == ((X + 1) + 1/2) + a exactly for every column X):
+    // b's runs read a's planes one column over.  The runs stay in list order.
+    int ng = 0;
+    for (int g = 0; g < s.ngroups; g++) {
+        const float a = s.ox[s.grp_ix[g]];
+        bool merge = false;
+        if (g + 1 < s.ngroups) {
+            const float b = s.ox[s.grp_ix[g + 1]];
+            merge = b - a == 1.0f && std::fabs(a) <= 0x1p20f && 2.0f * a == std::nearbyint(2.0f * a);
+        }
+        s.mgrp_ix[ng] = s.grp_ix[g];
+        s.mgrp_run0[ng] = s.grp_run0[g];
+        s.mgrp_nrun[ng] = s.grp_nrun[g];
+        s.mgrp_ext[ng] = 0;
+        for (int r = s.grp_run0[g]; r < s.grp_run0[g] + s.grp_nrun[g]; r++) s.run_shift[r] = 0;
+        if (merge) {
+            for (int r = s.grp_run0[g + 1]; r < s.grp_run0[g + 1] + s.grp_nrun[g + 1]; r++) s.run_shift[r] = 1;
+            s.mgrp_nrun[ng] += s.grp_nrun[g + 1];
+            s.mgrp_ext[ng] = 1;
+            s.any_ext = 1;
+            g++;
+        }
+        ng++;
+    }
+    s.nmgroups = ng;
     return RFBA_OK;
 }
 
@@ -358,6 +389,7 @@ int first_tap_h(float c, int M) { return (int)std::floor(c / (float)M - 0.5f) - 
 
 struct Plan {
     LaunchGeom g;
+    int merge = 0;  // merged sample groups (LaunchParams::merge)
     int TZ;
     size_t lds_bytes;
     int Tmax;
@@ -374,7 +406,7 @@ void axis_extent(int b0, int b1, int tw, float omin, float omax, int M, int &nee
     }
 }
 
-size_t lds_layout(LaunchGeom &g, int TZ, const DevSamples &s) {
+size_t lds_layout(LaunchGeom &g, int TZ, const DevSamples &s, bool merge) {
     const int noz = s.noz;
     g.RU = g.NU + 2 * g.hx;
     g.RV = g.NV + 2 * g.hy;
@@ -383,17 +415,19 @@ size_t lds_layout(LaunchGeom &g, int TZ, const DevSamples &s) {
     // consecutive x values from several consecutive v rows, so consecutive
     // rows are shifted by 8, 16 or 24 of the 32 banks (2 to 4 rows
     // disjoint); the least padding that does so.
-    g.TXP = kTX + 1;
+    // A merged group's x pass writes kTX + 1 columns.
+    const int xcols = kTX + (merge ? 1 : 0);
+    g.TXP = xcols;
     for (int p = 0; p < 32; p++) {
-        const int sh = (g.NW * (kTX + p)) % 32;
+        const int sh = (g.NW * (xcols + p)) % 32;
         if (sh == 8 || sh == 16 || sh == 24) {
-            g.TXP = kTX + p;
+            g.TXP = xcols + p;
             break;
         }
     }
     g.noz = noz;
     g.NWP = g.NW + (g.NW & 1);
-    g.VS = kTX * g.NWP + 16;
+    g.VS = (merge && g.NWP > 16 ? kTX + 1 : kTX) * g.NWP + 16;
     // Column layout when it costs no occupancy tier (workgroups per CU by
     // LDS: budgets of 53, 80, 160 KiB) against the row layout.
     auto tier = [](size_t words) {
@@ -401,8 +435,8 @@ size_t lds_layout(LaunchGeom &g, int TZ, const DevSamples &s) {
         return b <= 53 * 1024 ? 0 : (b <= 80 * 1024 ? 1 : 2);
     };
     const size_t rest = (size_t)3 * g.NU * g.NV * g.NW + 2 + std::max(3 * g.NW * kThreads, g.RU * g.RV * g.RW) +
-                        kTX * 6 + TZ * noz * 8 + TZ + TZ * 6 + TZ * kThreads + s.nox + s.noy + s.noz +
-                        3 * s.ngroups + 5 * s.nruns + 2 * s.n;
+                        xcols * 6 + TZ * noz * 8 + TZ + TZ * 6 + TZ * kThreads + s.nox + s.noy + s.noz +
+                        4 * s.ngroups + 6 * s.nruns + 2 * s.n;
     const size_t px_row = (size_t)3 * g.NV * g.NW * g.TXP, px_col = (size_t)3 * g.NV * g.VS;
     g.pxv = tier(rest + px_col) <= tier(rest + px_row) ? 1 : 0;
     int off = 0;
@@ -413,7 +447,7 @@ size_t lds_layout(LaunchGeom &g, int TZ, const DevSamples &s) {
     g.off_px = off; off += g.pxv ? (int)px_col : (int)px_row;
     g.off_py = off;
     g.off_raw = off; off += std::max(3 * g.NW * kThreads, g.RU * g.RV * g.RW);
-    g.off_xtab = off; off += kTX * 6;
+    g.off_xtab = off; off += xcols * 6;
     g.off_ztab = off; off += TZ * noz * 8;
     g.off_zc = off; off += TZ;
     g.off_ctab = off; off += TZ * 6;
@@ -423,8 +457,8 @@ size_t lds_layout(LaunchGeom &g, int TZ, const DevSamples &s) {
         int w = s.nox;
         g.st_oy = w; w += s.noy;
         g.st_oz = w; w += s.noz;
-        g.st_grp = w; w += 3 * s.ngroups;
-        g.st_run = w; w += 5 * s.nruns;
+        g.st_grp = w; w += 4 * s.ngroups;
+        g.st_run = w; w += 6 * s.nruns;
         g.st_smp = w; w += 2 * s.n;
         off += w;
     }
@@ -451,7 +485,20 @@ int plan_geometry(const std::vector<DevItem> &items, const DevSamples &s, const 
         Tmax = std::max(Tmax, it.T);
     }
     g.NU = NU; g.NV = NV; g.NW = NW;
-    pl.lds_bytes = lds_layout(g, TZ, s) + 64;
+    pl.lds_bytes = lds_layout(g, TZ, s, false) + 64;
+    pl.merge = 0;
+    if (s.any_ext) {
+        // merged groups save x passes but widen the planes by a column: use
+        // them only where that costs no workgroups per CU
+        LaunchGeom gm = g;
+        const size_t mb = lds_layout(gm, TZ, s, true) + 64;
+        auto wg = [](size_t b) { return b <= 53 * 1024 ? 3 : (b <= 80 * 1024 ? 2 : 1); };
+        if (wg(mb) >= wg(pl.lds_bytes)) {
+            g = gm;
+            pl.lds_bytes = mb;
+            pl.merge = 1;
+        }
+    }
     pl.g = g;
     pl.TZ = TZ;
     pl.Tmax = Tmax;
@@ -536,6 +583,7 @@ LaunchParams make_params(const SRRFConf &c, const Plan &pl) {
     P.yR = c.YRange;
     P.zR = c.ZRange;
     P.g = pl.g;
+    P.merge = pl.merge;
     P.Tmax = pl.Tmax;
     {
         // terms of the widest gradient sum; values below 2^16 keep every

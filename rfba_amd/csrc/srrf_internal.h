@@ -12,6 +12,10 @@ namespace rfba {
 //   groups: maximal consecutive stretches of runs with the same x offset
 // so that x-interpolated planes (per group) and y-interpolated columns (per
 // run) can be shared while every voxel still sums its samples in list order.
+// Two consecutive groups whose x offsets a and b = a + 1 are multiples of
+// 1/2 are merged: the sample x coordinate (X + 1/2) + b of SR column X is
+// exactly ((X + 1) + 1/2) + a, so the runs of b read the planes of a one
+// column over, and the merged group's x pass covers kTX + 1 columns.
 constexpr int kMaxSamples = RFBA_SRRF_MAX_SAMPLES;
 constexpr int kMaxOffsets = 256;  // distinct offsets per axis
 
@@ -29,10 +33,16 @@ struct DevSamples {
     // per sample (list order)
     float rx[kMaxSamples], ry[kMaxSamples], rz[kMaxSamples];
     int siz[kMaxSamples];
-    // runs
-    int run_start[kMaxSamples], run_len[kMaxSamples], run_iy[kMaxSamples];
+    // runs (shift 1, with merged groups: reads the group's x-interpolated
+    // planes one column over)
+    int run_start[kMaxSamples], run_len[kMaxSamples], run_iy[kMaxSamples], run_shift[kMaxSamples];
     // groups
     int grp_ix[kMaxSamples], grp_run0[kMaxSamples], grp_nrun[kMaxSamples];
+    // the groups with merges (ext 1: x pass over kTX + 1 columns); a launch
+    // uses them when LaunchParams::merge is set (the planner's choice: only
+    // where the wider planes cost no occupancy)
+    int mgrp_ix[kMaxSamples], mgrp_run0[kMaxSamples], mgrp_nrun[kMaxSamples], mgrp_ext[kMaxSamples];
+    int nmgroups, any_ext;
 };
 
 // One stack (or block of a source series) and the SR box of it to compute.
@@ -67,7 +77,9 @@ struct LaunchGeom {
     // Column layout of the x-interpolated planes (pxv = 1): Px[c][v][X][w],
     // the w values of a column contiguous (NWP = NW rounded up to even) and
     // planes of one v VS floats apart (VS = 16 NWP + 16: rows of v shifted by
-    // 16 banks), so a thread's y pass reads pairs of w from one base.
+    // 16 banks; column 16 of a merged group lies in those 16 when NWP <= 16,
+    // else VS = 17 NWP + 16), so a thread's y pass reads pairs of w from one
+    // base.
     int pxv, NWP, VS;
     int noz;
     // float offsets into dynamic LDS
@@ -92,6 +104,8 @@ struct LaunchParams {
     // smoothed gradient of integer input from shared box sums: every sum has
     // at most 256 terms and the box sums fit in the x-plane area
     int sep;
+    // the merged sample groups (DevSamples::mgrp_*) and run shifts apply
+    int merge;
     LaunchGeom g;
 };
 

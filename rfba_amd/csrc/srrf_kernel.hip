@@ -247,11 +247,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
     float *st_ox = lds + g.off_stab;
     float *st_oy = st_ox + g.st_oy;
     float *st_oz = st_ox + g.st_oz;
-    int *st_grp = (int *)(st_ox + g.st_grp);  // ix, run0, nrun
-    int *st_run = (int *)(st_ox + g.st_run);  // start, len, iy, rx, ry
+    int *st_grp = (int *)(st_ox + g.st_grp);  // ix, run0, nrun, ext
+    int *st_run = (int *)(st_ox + g.st_run);  // start, len, iy, rx, ry, shift
     int *st_smp = (int *)(st_ox + g.st_smp);  // 32 iz (byte offset of the z-table entry), rz
     const float sR = S->R, sInvR = S->invR, sDiv = S->divisor;
-    const int ngroups = S->ngroups;
+    // merged sample groups when the plan chose them (two x offsets one
+    // column apart served by one x pass), else one group per x offset run
+    const bool merge = P.merge != 0;
+    const int ngroups = merge ? S->nmgroups : S->ngroups;
     const int n_ox = S->nox, n_oy = S->noy;
     const bool fast_ok = S->fast_ok != 0;
     const float oxmin = S->oxmin, oxmax = S->oxmax, oymin = S->oymin, oymax = S->oymax;
@@ -262,17 +265,19 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
         for (int e = tid; e < noy; e += kThreads) st_oy[e] = S->oy[e];
         for (int e = tid; e < nzz; e += kThreads) st_oz[e] = S->oz[e];
         for (int e = tid; e < ngroups; e += kThreads) {
-            st_grp[3 * e] = S->grp_ix[e];
-            st_grp[3 * e + 1] = S->grp_run0[e];
-            st_grp[3 * e + 2] = S->grp_nrun[e];
+            st_grp[4 * e] = merge ? S->mgrp_ix[e] : S->grp_ix[e];
+            st_grp[4 * e + 1] = merge ? S->mgrp_run0[e] : S->grp_run0[e];
+            st_grp[4 * e + 2] = merge ? S->mgrp_nrun[e] : S->grp_nrun[e];
+            st_grp[4 * e + 3] = merge ? S->mgrp_ext[e] : 0;
         }
         for (int e = tid; e < nrun; e += kThreads) {
             const int s0 = S->run_start[e];
-            st_run[5 * e] = s0;
-            st_run[5 * e + 1] = S->run_len[e];
-            st_run[5 * e + 2] = S->run_iy[e];
-            st_run[5 * e + 3] = __float_as_int(S->rx[s0]);
-            st_run[5 * e + 4] = __float_as_int(S->ry[s0]);
+            st_run[6 * e] = s0;
+            st_run[6 * e + 1] = S->run_len[e];
+            st_run[6 * e + 2] = S->run_iy[e];
+            st_run[6 * e + 3] = __float_as_int(S->rx[s0]);
+            st_run[6 * e + 4] = __float_as_int(S->ry[s0]);
+            st_run[6 * e + 5] = merge ? S->run_shift[e] : 0;
         }
         for (int e = tid; e < ns; e += kThreads) {
             st_smp[2 * e] = S->siz[e] * 8 * (int)sizeof(float);  // byte offset of its z-table entry
@@ -579,9 +584,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
             for (int z = 0; z < TZ; z++) dsum[z * kThreads + tid] = 0.0f;
 
             for (int gi = 0; gi < ngroups && !tile_skip; gi++) {
-                const int ix = st_grp[3 * gi];
+                const int ix = st_grp[4 * gi];
                 const float oxv = st_ox[ix];
-                if (tid < kTX) {
+                // a merged group's planes have one more column (read by its
+                // shifted runs)
+                const int xw = kTX + st_grp[4 * gi + 3];
+                const float r_xw = 1.0f / (float)xw;
+                if (tid < xw) {
                     int b, valid;
                     float w0, w1, w2, w3;
                     axis_weights(((float)(X0 + tid) + 0.5f) + oxv, M, W, b, w0, w1, w2, w3, valid);
@@ -594,9 +603,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                 // ---- x pass: Px[c][v][w][X] = sum_i G[c][w][v][b+i] * wx_i ----
                 {
                     const int NP = NU * NV * NW;
-                    for (int e = tid; e < nv * nw * kTX; e += kThreads) {
-                        const int q = e / kTX, v = qdiv(q, r_nw);
-                        const int X = e % kTX, w = q - v * nw;
+                    // element e = (v, w, X): the constant divisor kTX for a
+                    // plain group, the reciprocal quotient for kTX + 1 columns
+                    auto xelem = [&](int X, int q) {
+                        const int v = qdiv(q, r_nw), w = q - v * nw;
                         const float *xt = xtab + X * 6;
                         const int ub = __float_as_int(xt[0]);
                         const float w0 = xt[1], w1 = xt[2], w2 = xt[3], w3 = xt[4];
@@ -610,17 +620,28 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                             p = p + gg[3] * w3;
                             Px[pxv ? (c * NV + v) * VS + X * NWP + w : ((c * NV + v) * NW + w) * TXP + X] = p;
                         }
+                    };
+                    if (xw == kTX) {
+                        for (int e = tid; e < nv * nw * kTX; e += kThreads) xelem(e % kTX, e / kTX);
+                    } else {
+                        for (int e = tid; e < nv * nw * xw; e += kThreads) {
+                            const int q = qdiv(e, r_xw);
+                            xelem(e - q * xw, q);
+                        }
                     }
                 }
                 __syncthreads();
 
-                const float x0 = Xc + oxv;
-                const float dxs = x0 - Xc;
-                const int r0 = st_grp[3 * gi + 1], nr = st_grp[3 * gi + 2];
+                const int r0 = st_grp[4 * gi + 1], nr = st_grp[4 * gi + 2];
                 // a wave whose voxels all skip does no y pass or samples
                 const bool wave_skip = __all(colbad || zmask == zall);
                 for (int r = r0; r < r0 + nr && !wave_skip; r++) {
-                    const int iy = st_run[5 * r + 2];
+                    const int iy = st_run[6 * r + 2];
+                    // the run's own x offset (a shifted run: the next plane column)
+                    const float rxs = __int_as_float(st_run[6 * r + 3]);
+                    const int xs = xl + st_run[6 * r + 5];
+                    const float x0 = Xc + rxs;
+                    const float dxs = x0 - Xc;
                     const float y0 = Yc + st_oy[iy];
                     const float dys = y0 - Yc;
                     int vb, validy;
@@ -632,7 +653,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                     if (pxv) {
                         // column layout: the 3 x 4 tap rows of two w values
                         // are loaded as pairs first, then summed in order
-                        const float *pb = Px + vrel * VS + xl * NWP;
+                        const float *pb = Px + vrel * VS + xs * NWP;
 #pragma unroll 4
                         for (int w = 0; w < nw; w += 2) {
                             float2 pv[3][4];
@@ -664,7 +685,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                         for (int w = 0; w < nw; w++) {
 #pragma unroll
                             for (int c = 0; c < 3; c++) {
-                                const float *pp = Px + ((c * NV + vrel) * NW + w) * TXP + xl;
+                                const float *pp = Px + ((c * NV + vrel) * NW + w) * TXP + xs;
                                 const int vs = NW * TXP;
                                 float q = pp[0] * wy0;
                                 q = q + pp[vs] * wy1;
@@ -674,8 +695,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
                             }
                         }
                     }
-                    const int s0 = st_run[5 * r], ns = st_run[5 * r + 1];
-                    const float rxs = __int_as_float(st_run[5 * r + 3]), rys = __int_as_float(st_run[5 * r + 4]);
+                    const int s0 = st_run[6 * r], ns = st_run[6 * r + 1];
+                    const float rys = __int_as_float(st_run[6 * r + 4]);
                     // Voxel pairs (Z, Z+1): two independent chains per sample.
                     for (int Z = 0; Z < TZ; Z += 2) {
                         if (__all(colbad || ((zmask >> Z) & 3u) == 3u)) continue;
