@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cctype>
 #include <cmath>
 #include <cstdarg>
@@ -396,14 +397,95 @@ struct Plan {
     std::vector<DevTile> tiles;             // queue order
     std::vector<std::pair<int, int>> item_tiles;  // per item [begin, count) in tiles_by_item
     std::vector<DevTile> tiles_by_item;
+    std::vector<std::array<int, 3>> phase;  // per item: tile phase k of x, y, z (origins b0 - k + j w)
 };
 
-void axis_extent(int b0, int b1, int tw, float omin, float omax, int M, int &need) {
-    for (int X0 = b0; X0 < b1; X0 += tw) {
+// Low-res taps spanned by the widest tile of one axis whose tiles of width tw
+// start at b0 - k + j tw (j = 0, 1, ... while below b1), the kernel's own
+// extent computation.
+void axis_extent(int b0, int b1, int k, int tw, float omin, float omax, int M, int &need) {
+    for (int X0 = b0 - k; X0 < b1; X0 += tw) {
         const int lo = first_tap_h(((float)X0 + 0.5f) + omin, M);
         const int hi = first_tap_h(((float)(X0 + tw - 1) + 0.5f) + omax, M) + 3;
         need = std::max(need, hi - lo + 1);
     }
+}
+
+// SR voxels [v0, v1) of [b0, b1) none of whose sample offsets along this axis
+// leaves [1.5, dim - 1.5] low-res pixels (the kernel's float test; monotone
+// in the coordinate).  The others are NaN on every frame and skip the sample
+// passes.  Empty: v0 >= v1.
+void axis_valid(int b0, int b1, const float *off, int noff, int M, int dim, int &v0, int &v1) {
+    v0 = b1;
+    v1 = b0;
+    for (int X = b0; X < b1; X++) {
+        const float Xc = (float)X + 0.5f;
+        bool ok = true;
+        for (int i = 0; i < noff && ok; i++) {
+            const float x = (Xc + off[i]) / (float)M;
+            ok = !(x < 1.5f || x > (float)dim - 1.5f);
+        }
+        if (ok) {
+            v0 = std::min(v0, X);
+            v1 = X + 1;
+        }
+    }
+}
+
+// Tile phases of one item.  Along each axis the tiles can start at any of tw
+// phases; the cost of a tile-frame is modelled from the measured phases of
+// the kernel: a part every tile pays (halo load, centre intensity, temporal
+// accumulation: 0.18), a part every tile with a voxel to sample pays (gradient
+// and x passes: 0.20) and the sample passes (0.62), which a wave (an 8 x 8
+// quadrant of the tile's columns) skips per pair of z slices when none of its
+// voxels has samples in range.  The costs are separable per axis, so each
+// axis contributes its tile count n, its tiles with a valid voxel e and its
+// active fraction a (quadrant halves for x and y, slice pairs for z).  Only
+// phases whose tiles span no more low-res taps than phase 0's are tried, so
+// the LDS plan never grows; ties keep phase 0.
+std::array<int, 3> tile_phases(const DevItem &it, const DevSamples &s, int TZ) {
+    struct Opt {
+        double n, e, a;
+        int k;
+    };
+    const int b0[3] = {it.bx0, it.by0, it.bz0}, b1[3] = {it.bx1, it.by1, it.bz1};
+    const int tw[3] = {kTX, kTY, TZ}, gran[3] = {8, 8, 2}, dim[3] = {it.W, it.H, it.D};
+    const float *off[3] = {s.ox, s.oy, s.oz};
+    const int noff[3] = {s.nox, s.noy, s.noz};
+    const float omin[3] = {s.oxmin, s.oymin, s.ozmin}, omax[3] = {s.oxmax, s.oymax, s.ozmax};
+    std::vector<Opt> opts[3];
+    for (int a = 0; a < 3; a++) {
+        int v0, v1, need0 = 0;
+        axis_valid(b0[a], b1[a], off[a], noff[a], it.M, dim[a], v0, v1);
+        axis_extent(b0[a], b1[a], 0, tw[a], omin[a], omax[a], it.M, need0);
+        for (int k = 0; k < tw[a]; k++) {
+            int need = 0;
+            axis_extent(b0[a], b1[a], k, tw[a], omin[a], omax[a], it.M, need);
+            if (need > need0) continue;
+            Opt o = {0.0, 0.0, 0.0, k};
+            for (int X0 = b0[a] - k; X0 < b1[a]; X0 += tw[a]) {
+                o.n += 1.0;
+                const int lo = std::max(X0, v0), hi = std::min(X0 + tw[a], v1);
+                if (lo < hi) {
+                    o.e += 1.0;
+                    o.a += (double)((hi - 1 - X0) / gran[a] - (lo - X0) / gran[a] + 1) / (double)(tw[a] / gran[a]);
+                }
+            }
+            opts[a].push_back(o);
+        }
+    }
+    std::array<int, 3> best = {0, 0, 0};
+    double bc = -1.0;
+    for (const auto &x : opts[0])
+        for (const auto &y : opts[1])
+            for (const auto &z : opts[2]) {
+                const double c = 0.18 * x.n * y.n * z.n + 0.20 * x.e * y.e * z.e + 0.62 * x.a * y.a * z.a;
+                if (bc < 0.0 || c < bc) {
+                    bc = c;
+                    best = {x.k, y.k, z.k};
+                }
+            }
+    return best;
 }
 
 size_t lds_layout(LaunchGeom &g, int TZ, const DevSamples &s, bool merge) {
@@ -478,10 +560,14 @@ int plan_geometry(const std::vector<DevItem> &items, const DevSamples &s, const 
         g.hz = std::max(c.MainRange, c.ZRange);
     }
     int NU = 4, NV = 4, NW = 4, Tmax = 1;
-    for (const auto &it : items) {
-        axis_extent(it.bx0, it.bx1, kTX, s.oxmin, s.oxmax, it.M, NU);
-        axis_extent(it.by0, it.by1, kTY, s.oymin, s.oymax, it.M, NV);
-        axis_extent(it.bz0, it.bz1, TZ, s.ozmin, s.ozmax, it.M, NW);
+    pl.phase.resize(items.size());
+    for (size_t i = 0; i < items.size(); i++) {
+        const auto &it = items[i];
+        const auto ph = tile_phases(it, s, TZ);
+        pl.phase[i] = ph;
+        axis_extent(it.bx0, it.bx1, ph[0], kTX, s.oxmin, s.oxmax, it.M, NU);
+        axis_extent(it.by0, it.by1, ph[1], kTY, s.oymin, s.oymax, it.M, NV);
+        axis_extent(it.bz0, it.bz1, ph[2], TZ, s.ozmin, s.ozmax, it.M, NW);
         Tmax = std::max(Tmax, it.T);
     }
     g.NU = NU; g.NV = NV; g.NW = NW;
@@ -507,15 +593,19 @@ int plan_geometry(const std::vector<DevItem> &items, const DevSamples &s, const 
     return RFBA_OK;
 }
 
+// Tiles of every item at the phases plan_geometry chose: they partition the
+// item's SR box (a tile may reach past it on the low side; the kernel stores
+// only voxels inside the box).
 void plan_tiles(const std::vector<DevItem> &items, int TZ, Plan &pl) {
     pl.tiles_by_item.clear();
     pl.item_tiles.clear();
     for (size_t i = 0; i < items.size(); i++) {
         const auto &it = items[i];
+        const auto &ph = pl.phase[i];
         const int begin = (int)pl.tiles_by_item.size();
-        for (int z = it.bz0; z < it.bz1; z += TZ)
-            for (int y = it.by0; y < it.by1; y += kTY)
-                for (int x = it.bx0; x < it.bx1; x += kTX) pl.tiles_by_item.push_back({(int)i, x, y, z});
+        for (int z = it.bz0 - ph[2]; z < it.bz1; z += TZ)
+            for (int y = it.by0 - ph[1]; y < it.by1; y += kTY)
+                for (int x = it.bx0 - ph[0]; x < it.bx1; x += kTX) pl.tiles_by_item.push_back({(int)i, x, y, z});
         pl.item_tiles.push_back({begin, (int)pl.tiles_by_item.size() - begin});
     }
     // Queue order: most expensive first (frames per tile), then spatial

@@ -289,7 +289,11 @@ class DeviceBuffer:
         return out
 
     def zero(self):
-        _check(lib().rfba_memset_device(self.ptr, 0, c_size_t(self.nbytes)))
+        self.fill_byte(0)
+
+    def fill_byte(self, value: int):
+        """Set every byte of the allocation to `value` (0-255)."""
+        _check(lib().rfba_memset_device(self.ptr, int(value), c_size_t(self.nbytes)))
 
     def close(self):
         if self.ptr:

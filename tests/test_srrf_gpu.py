@@ -333,9 +333,25 @@ def test_c3_shaped_batch_vs_oracle(gpu, method):
     b = srrf.Batch(blocks, srrf.RFBA_U16, c, smp)
     tz, ms = b.autotune()
     print("%s: groups %s, tuned depth %d" % (method, b.groups, tz))
-    for bl in blocks:
-        bl.out.zero()
-    b.run(use_graph=True)
+    # Every SR voxel of every block is written, by the batched and the
+    # per-block form alike: outputs start as 0x7f bytes (3.4e38, a value no
+    # output takes), and the two forms agree on the whole block.
+    per_block = []
+    for run in ("per_block", "batched"):
+        for bl in blocks:
+            bl.out.fill_byte(0x7F)
+        if run == "per_block":
+            b.run_per_block()
+        else:
+            b.run(use_graph=True)
+        for bl in blocks:
+            (dx, dy, dz), M = bl.dims_xyz, bl.mag
+            got = bl.out.to_array((dz * M, dy * M, dx * M), np.float32)
+            assert not np.any(got.view(np.uint32) == 0x7F7F7F7F), (run, bl.origin_xyz, bl.dims_xyz, M)
+            if run == "per_block":
+                per_block.append(got)
+            else:
+                assert np.array_equal(got, per_block.pop(0), equal_nan=True), (bl.origin_xyz, bl.dims_xyz, M)
     nbox = 0
     for bl in blocks:
         (ox, oy, oz), (dx, dy, dz) = bl.origin_xyz, bl.dims_xyz
