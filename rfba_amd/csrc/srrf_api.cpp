@@ -434,16 +434,18 @@ void axis_valid(int b0, int b1, const float *off, int noff, int M, int dim, int 
 
 // Tile phases of one item.  Along each axis the tiles can start at any of tw
 // phases; the cost of a tile-frame is modelled from the measured phases of
-// the kernel: a part every tile pays (halo load, centre intensity, temporal
-// accumulation: 0.18), a part every tile with a voxel to sample pays (gradient
-// and x passes: 0.20) and the sample passes (0.62), which a wave (an 8 x 8
-// quadrant of the tile's columns) skips per pair of z slices when none of its
-// voxels has samples in range.  The costs are separable per axis, so each
-// axis contributes its tile count n, its tiles with a valid voxel e and its
-// active fraction a (quadrant halves for x and y, slice pairs for z).  Only
-// phases whose tiles span no more low-res taps than phase 0's are tried, so
-// the LDS plan never grows; ties keep phase 0.
-std::array<int, 3> tile_phases(const DevItem &it, const DevSamples &s, int TZ) {
+// the kernel: a tile with a voxel to sample pays the halo load, the centre
+// intensity and the temporal accumulation (0.18), the gradient and the x
+// passes (0.20) and the sample passes (0.62), which a wave (an 8 x 8 quadrant
+// of the tile's columns) skips per pair of z slices when none of its voxels
+// has samples in range.  A tile without one (idle) pays idle_cost: 0.18, or
+// almost nothing where the kernel skips its halo and intensities (integer
+// input or no intensity weighting).  The costs are separable per axis, so
+// each axis contributes its tile count n, its tiles with a valid voxel e and
+// its active fraction a (quadrant halves for x and y, slice pairs for z).
+// Only phases whose tiles span no more low-res taps than phase 0's are
+// tried, so the LDS plan never grows; ties keep phase 0.
+std::array<int, 3> tile_phases(const DevItem &it, const DevSamples &s, int TZ, double idle_cost) {
     struct Opt {
         double n, e, a;
         int k;
@@ -479,7 +481,8 @@ std::array<int, 3> tile_phases(const DevItem &it, const DevSamples &s, int TZ) {
     for (const auto &x : opts[0])
         for (const auto &y : opts[1])
             for (const auto &z : opts[2]) {
-                const double c = 0.18 * x.n * y.n * z.n + 0.20 * x.e * y.e * z.e + 0.62 * x.a * y.a * z.a;
+                const double e = x.e * y.e * z.e;
+                const double c = idle_cost * (x.n * y.n * z.n - e) + 0.38 * e + 0.62 * x.a * y.a * z.a;
                 if (bc < 0.0 || c < bc) {
                     bc = c;
                     best = {x.k, y.k, z.k};
@@ -548,7 +551,7 @@ size_t lds_layout(LaunchGeom &g, int TZ, const DevSamples &s, bool merge) {
     return (size_t)off * sizeof(float);
 }
 
-int plan_geometry(const std::vector<DevItem> &items, const DevSamples &s, const SRRFConf &c, int TZ,
+int plan_geometry(const std::vector<DevItem> &items, const DevSamples &s, const SRRFConf &c, int dtype, int TZ,
                   Plan &pl) {
     LaunchGeom g;
     memset(&g, 0, sizeof g);
@@ -563,7 +566,7 @@ int plan_geometry(const std::vector<DevItem> &items, const DevSamples &s, const 
     pl.phase.resize(items.size());
     for (size_t i = 0; i < items.size(); i++) {
         const auto &it = items[i];
-        const auto ph = tile_phases(it, s, TZ);
+        const auto ph = tile_phases(it, s, TZ, (dtype == RFBA_U16 || c.IntensityW != 1) ? 0.02 : 0.18);
         pl.phase[i] = ph;
         axis_extent(it.bx0, it.bx1, ph[0], kTX, s.oxmin, s.oxmax, it.M, NU);
         axis_extent(it.by0, it.by1, ph[1], kTY, s.oymin, s.oymax, it.M, NV);
@@ -617,9 +620,9 @@ void plan_tiles(const std::vector<DevItem> &items, int TZ, Plan &pl) {
     });
 }
 
-int choose_tz(const std::vector<DevItem> &items, const DevSamples &s, const SRRFConf &c, Plan &pl,
+int choose_tz(const std::vector<DevItem> &items, const DevSamples &s, const SRRFConf &c, int dtype, Plan &pl,
               int force_tz) {
-    if (force_tz == 4 || force_tz == 8 || force_tz == 16) return plan_geometry(items, s, c, force_tz, pl);
+    if (force_tz == 4 || force_tz == 8 || force_tz == 16) return plan_geometry(items, s, c, dtype, force_tz, pl);
     // Largest voxel column that leaves room for two workgroups per CU, else
     // the largest that fits at all (small magnifications need wide halos).
     // Measured on MI355X: occupancy (workgroups per CU, set by LDS) matters
@@ -628,11 +631,11 @@ int choose_tz(const std::vector<DevItem> &items, const DevSamples &s, const SRRF
     Plan p;
     for (int budget : {53 * 1024, 80 * 1024, 160 * 1024})
         for (int tz : {8, 16, 4})
-            if (plan_geometry(items, s, c, tz, p) == RFBA_OK && p.lds_bytes <= (size_t)budget) {
+            if (plan_geometry(items, s, c, dtype, tz, p) == RFBA_OK && p.lds_bytes <= (size_t)budget) {
                 pl = p;
                 return RFBA_OK;
             }
-    return plan_geometry(items, s, c, 4, pl);
+    return plan_geometry(items, s, c, dtype, 4, pl);
 }
 
 typedef void (*KernelFn)(LaunchParams);
@@ -716,7 +719,7 @@ int run_items_with(RunBufs &B, const std::vector<DevItem> &items, int dtype, con
     if (items.empty()) return RFBA_OK;
     Plan pl;
     const char *tz_env = getenv("RFBA_SRRF_TZ");  // tuning override: 8 or 16
-    int rc = choose_tz(items, s, c, pl, tz_env ? atoi(tz_env) : 0);
+    int rc = choose_tz(items, s, c, dtype, pl, tz_env ? atoi(tz_env) : 0);
     if (rc) return rc;
     plan_tiles(items, pl.TZ, pl);
     if (pl.tiles.empty()) return RFBA_OK;
@@ -1198,7 +1201,7 @@ extern "C" int rfba_srrf_batch_create(const rfba_srrf_block *blocks, int n, int 
     batch_group(b);
     for (auto &g : b->groups) {
         Plan p;
-        if ((rc = choose_tz(g.gitems, s, *conf, p, 0)) || (rc = group_build(b, g, p))) {
+        if ((rc = choose_tz(g.gitems, s, *conf, b->dtype, p, 0)) || (rc = group_build(b, g, p))) {
             batch_free(b);
             return rc;
         }
@@ -1359,7 +1362,7 @@ extern "C" int rfba_srrf_batch_autotune(rfba_srrf_batch *b, void *stream, int sa
         for (int c = 0; c < 3 && rc == RFBA_OK; c++) {
             g.tune_ms[c] = 0.0f;
             Plan p;
-            if (plan_geometry(g.gitems, b->smp, b->conf, cand[c], p) != RFBA_OK) continue;
+            if (plan_geometry(g.gitems, b->smp, b->conf, b->dtype, cand[c], p) != RFBA_OK) continue;
             if ((rc = group_build(sub, sub->groups[0], p))) break;
             float ms = 0.0f;
             for (int rep = 0; rep < 2 && rc == RFBA_OK; rep++) {

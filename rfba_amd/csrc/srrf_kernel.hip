@@ -382,6 +382,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
         }
         const unsigned zall = (TZ >= 32) ? ~0u : ((1u << TZ) - 1u);
         const bool tile_skip = __syncthreads_and(colbad || zmask == zall) != 0;
+        // An idle tile: no voxel has its samples in range, so every frame's
+        // radiality is NaN (0 after RemoveCon).  With integer input (finite,
+        // non-negative centre intensities) or without intensity weighting
+        // (intensities stay 0) the temporal modes then give the same outputs
+        // as with all intensities 0: NaN * I = NaN, 0 * I = +0, and TRPPM's
+        // pair sum of zero series is +0 before its intensity factor.  Such a
+        // tile loads no halo and forms no intensities.
+        const bool idle = tile_skip && (kIntInput<Tin> || P.iw != 1);
 
         float acc0[TZ], acc1[TZ];
 #pragma unroll
@@ -395,7 +403,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
             __syncthreads();
             {
                 const Tin *srct = src + (long long)t * it.st;
-                for (int e = tid; e < nr_raw; e += kThreads) {
+                for (int e = tid; e < (idle ? 0 : nr_raw); e += kThreads) {
                     const int q = qdiv(e, r_ru), c = qdiv(q, r_rv);
                     const int a = e - q * ru, bb = q - c * rv_;
                     const int x = clampi(rx0 + a, 0, W - 1) + it.ox;
@@ -527,7 +535,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
             float Iv[TZ];
 #pragma unroll
             for (int Z = 0; Z < TZ; Z++) Iv[Z] = 0.0f;
-            if (P.iw == 1) {
+            if (P.iw == 1 && !idle) {
                 if (tid < kTX) {
                     int b, valid;
                     float w0, w1, w2, w3;

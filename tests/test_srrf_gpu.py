@@ -91,6 +91,33 @@ def test_temporal_modes(gpu, method, itc, order, iw):
     parity(g, o, "method %d itc %d ord %d iw %d" % (method, itc, order, iw))
 
 
+@pytest.mark.parametrize("dtype", ["u16", "f32"])
+@pytest.mark.parametrize("method,itc,order", [(0, 0, 2), (1, 0, 2), (2, 0, 2), (3, 0, 2), (3, 1, 3)])
+@pytest.mark.parametrize("poscon", [0, 1])
+def test_idle_tiles(gpu, dtype, method, itc, order, poscon):
+    # A tile none of whose voxels has its samples in range loads no halo and
+    # forms no centre intensities where that cannot change an output (integer
+    # input, or no intensity weighting).  Two stacks: one 3 voxels deep
+    # (every voxel NaN by definition, every tile idle) and one 29 wide at
+    # M = 3, whose tiles the planner shifts so that a column of idle tiles
+    # lies beside the computed ones.  Outputs equal the oracle's bit for bit
+    # (the sign of zero included).
+    smp = srrf.directions(26, 0.5)
+    for iw in ((0, 1) if dtype == "u16" else (0,)):
+        for shp, M in (((5, 3, 9, 11), 2), ((5, 6, 10, 29), 3)):
+            s = series(shp, 3, 17)
+            if dtype == "f32":
+                s = s.astype(np.float32)
+            c = conf(Method=method, ITC=itc, TRACOrder=order, IntensityW=iw, RemoveCon=poscon, Magnification=M)
+            g = srrf.srrf_stack(s, c, smp)
+            o = oracle.srrf_stack(s.astype(np.float32), c, smp.xyz, smp.divisor)
+            lab = "%s method %d itc %d order %d iw %d poscon %d shape %s M %d" % (
+                dtype, method, itc, order, iw, poscon, shp, M)
+            assert np.array_equal(np.isnan(g), np.isnan(o)), lab
+            m = ~np.isnan(o)
+            assert np.array_equal(g[m].view(np.uint32), o[m].view(np.uint32)), lab
+
+
 @pytest.mark.parametrize("mag,dirs", [(1, "d6"), (3, "d26"), (4, "d26"), (5, "ring6"), (2, "ring3")])
 def test_magnifications_and_sets(gpu, mag, dirs):
     s = series((4, 9, 13, 15), 4, 11)
