@@ -62,7 +62,7 @@ METRIC = "3D SRRF SR-voxels/s + %HBM roofline; 3B batched logL+grad evals/s vs p
 # rocprofv3 FETCH_SIZE and WRITE_SIZE scaled by the copy-kernel calibration
 # (profiles/srrf_fused_full_volume_256x256x96_T200.md).  Dominated by the TRAC
 # per-voxel series written and read back (8 B per SR voxel-frame).
-HEADLINE_TRAFFIC_BYTES = 9.142e11
+HEADLINE_TRAFFIC_BYTES = 9.482e11
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
