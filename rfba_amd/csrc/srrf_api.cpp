@@ -473,7 +473,10 @@ std::array<int, 3> tile_phases(const DevItem &it, const DevSamples &s, int TZ, d
                     o.a += (double)((hi - 1 - X0) / gran[a] - (lo - X0) / gran[a] + 1) / (double)(tw[a] / gran[a]);
                 }
             }
-            opts[a].push_back(o);
+            // phases with the same counts cost the same: keep the first
+            bool seen = false;
+            for (const auto &q : opts[a]) seen = seen || (q.n == o.n && q.e == o.e && q.a == o.a);
+            if (!seen) opts[a].push_back(o);
         }
     }
     std::array<int, 3> best = {0, 0, 0};
