@@ -388,7 +388,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
         // (intensities stay 0) the temporal modes then give the same outputs
         // as with all intensities 0: NaN * I = NaN, 0 * I = +0, and TRPPM's
         // pair sum of zero series is +0 before its intensity factor.  Such a
-        // tile loads no halo and forms no intensities.
+        // tile skips the frame body (halo, gradient, intensities, samples and
+        // their barriers) and only accumulates that constant frame value.
         const bool idle = tile_skip && (kIntInput<Tin> || P.iw != 1);
 
         float acc0[TZ], acc1[TZ];
@@ -400,375 +401,388 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3, 8))
         const int rx0 = ulo - g.hx, ry0 = vlo - g.hy, rz0 = wlo - g.hz;
 
         for (int t = 0; t < T; t++) {
-            __syncthreads();
-            {
-                const Tin *srct = src + (long long)t * it.st;
-                for (int e = tid; e < (idle ? 0 : nr_raw); e += kThreads) {
-                    const int q = qdiv(e, r_ru), c = qdiv(q, r_rv);
-                    const int a = e - q * ru, bb = q - c * rv_;
-                    const int x = clampi(rx0 + a, 0, W - 1) + it.ox;
-                    const int y = clampi(ry0 + bb, 0, H - 1) + it.oy;
-                    const int z = clampi(rz0 + c, 0, D - 1) + it.oz;
-                    raw[(c * RV + bb) * RU + a] = load_px<Tin>(srct, (long long)x + y * it.sy + z * it.sz);
+            float Iv[TZ], rv[TZ];
+            if (idle) {
+                // the frame value of every voxel (NaN, as for the marked voxels
+                // below) and intensities 0: no halo, gradient or samples, and no
+                // barrier (idle is the same for the whole workgroup)
+#pragma unroll
+                for (int Z = 0; Z < TZ; Z++) {
+                    float dd = __builtin_nanf("");
+                    if (P.renorm == 1) dd = 0.5f + (dd / 2.0f);
+                    rv[Z] = (P.remove_con == 1) ? fmaxf(dd, 0.0f) : dd;
+                    Iv[Z] = 0.0f;
                 }
-            }
-            __syncthreads();
-
-            // ---- gradient planes G[c][w][v][u] (only the sample passes read them)
-            // Integer input with sums of at most 256 terms (P.sep, set by the
-            // host): every partial sum is an integer below 2^24, exact in FP32
-            // in any order, so the smoothed gradient is formed from shared box
-            // sums -- Bz (z sums) for the x and y derivatives, By (y sums) for
-            // the z derivative -- and equals the term-by-term sum bit for bit.
-            // Bz and By live in the x-plane area, which the intensity and x
-            // passes overwrite only after the next barrier.
-            const bool sep = kIntInput<Tin> && P.sep && P.smooth != 0 && !tile_skip;
-            if (sep) {
-                float *Bz = Px;                  // [nw][rv_][ru]
-                float *By = Px + ru * rv_ * nw;  // [rw][nv][ru]
-                const int nbz = ru * rv_ * nw, nby = ru * nv * rw;
-                const int mR = P.mR, xR = P.xR, yR = P.yR, zR = P.zR;
-                for (int e = tid; e < nbz + nby; e += kThreads) {
-                    float s = 0.0f;
-                    if (e < nbz) {
-                        const int q = qdiv(e, r_ru), w = qdiv(q, r_rv);
-                        const int a = e - q * ru, b = q - w * rv_;
-                        const float *r = raw + ((w + g.hz) * RV + b) * RU + a;
-                        for (int k = -zR; k <= zR; k++) s += r[k * RV * RU];
-                        Bz[e] = s;
-                    } else {
-                        const int f = e - nbz;
-                        const int q = qdiv(f, r_ru), c = qdiv(q, r_nv);
-                        const int a = f - q * ru, v = q - c * nv;
-                        const float *r = raw + (c * RV + v + g.hy) * RU + a;
-                        for (int j = -yR; j <= yR; j++) s += r[j * RU];
-                        By[f] = s;
+            } else {
+                __syncthreads();
+                {
+                    const Tin *srct = src + (long long)t * it.st;
+                    for (int e = tid; e < nr_raw; e += kThreads) {
+                        const int q = qdiv(e, r_ru), c = qdiv(q, r_rv);
+                        const int a = e - q * ru, bb = q - c * rv_;
+                        const int x = clampi(rx0 + a, 0, W - 1) + it.ox;
+                        const int y = clampi(ry0 + bb, 0, H - 1) + it.oy;
+                        const int z = clampi(rz0 + c, 0, D - 1) + it.oz;
+                        raw[(c * RV + bb) * RU + a] = load_px<Tin>(srct, (long long)x + y * it.sy + z * it.sz);
                     }
                 }
                 __syncthreads();
-                const int NP = NU * NV * NW;
-                for (int e = tid; e < nu * nv * nw; e += kThreads) {
+
+                // ---- gradient planes G[c][w][v][u] (only the sample passes read them)
+                // Integer input with sums of at most 256 terms (P.sep, set by the
+                // host): every partial sum is an integer below 2^24, exact in FP32
+                // in any order, so the smoothed gradient is formed from shared box
+                // sums -- Bz (z sums) for the x and y derivatives, By (y sums) for
+                // the z derivative -- and equals the term-by-term sum bit for bit.
+                // Bz and By live in the x-plane area, which the intensity and x
+                // passes overwrite only after the next barrier.
+                const bool sep = kIntInput<Tin> && P.sep && P.smooth != 0 && !tile_skip;
+                if (sep) {
+                    float *Bz = Px;                  // [nw][rv_][ru]
+                    float *By = Px + ru * rv_ * nw;  // [rw][nv][ru]
+                    const int nbz = ru * rv_ * nw, nby = ru * nv * rw;
+                    const int mR = P.mR, xR = P.xR, yR = P.yR, zR = P.zR;
+                    for (int e = tid; e < nbz + nby; e += kThreads) {
+                        float s = 0.0f;
+                        if (e < nbz) {
+                            const int q = qdiv(e, r_ru), w = qdiv(q, r_rv);
+                            const int a = e - q * ru, b = q - w * rv_;
+                            const float *r = raw + ((w + g.hz) * RV + b) * RU + a;
+                            for (int k = -zR; k <= zR; k++) s += r[k * RV * RU];
+                            Bz[e] = s;
+                        } else {
+                            const int f = e - nbz;
+                            const int q = qdiv(f, r_ru), c = qdiv(q, r_nv);
+                            const int a = f - q * ru, v = q - c * nv;
+                            const float *r = raw + (c * RV + v + g.hy) * RU + a;
+                            for (int j = -yR; j <= yR; j++) s += r[j * RU];
+                            By[f] = s;
+                        }
+                    }
+                    __syncthreads();
+                    const int NP = NU * NV * NW;
+                    for (int e = tid; e < nu * nv * nw; e += kThreads) {
+                        const int q = qdiv(e, r_nu), w = qdiv(q, r_nv);
+                        const int u = e - q * nu, v = q - w * nv;
+                        const int cu = u + g.hx, cv = v + g.hy, cw = w + g.hz;
+                        const float *bz = Bz + w * rv_ * ru;
+                        float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+                        for (int i = 1; i <= mR; i++) {
+                            float sp = 0.0f, sm = 0.0f;
+                            for (int j = -yR; j <= yR; j++) {
+                                sp += bz[(cv + j) * ru + cu + i];
+                                sm += bz[(cv + j) * ru + cu - i];
+                            }
+                            gx += sp - sm;
+                        }
+                        for (int j = 1; j <= mR; j++) {
+                            float sp = 0.0f, sm = 0.0f;
+                            for (int i = -xR; i <= xR; i++) {
+                                sp += bz[(cv + j) * ru + cu + i];
+                                sm += bz[(cv - j) * ru + cu + i];
+                            }
+                            gy += sp - sm;
+                        }
+                        for (int k = 1; k <= mR; k++) {
+                            float sp = 0.0f, sm = 0.0f;
+                            for (int i = -xR; i <= xR; i++) {
+                                sp += By[((cw + k) * nv + v) * ru + cu + i];
+                                sm += By[((cw - k) * nv + v) * ru + cu + i];
+                            }
+                            gz += sp - sm;
+                        }
+                        const int gi3 = (w * NV + v) * NU + u;
+                        G[gi3] = gx;
+                        G[NP + gi3] = gy;
+                        G[2 * NP + gi3] = gz;
+                    }
+                }
+                for (int e = tid; e < (tile_skip || sep ? 0 : nu * nv * nw); e += kThreads) {
                     const int q = qdiv(e, r_nu), w = qdiv(q, r_nv);
                     const int u = e - q * nu, v = q - w * nv;
-                    const int cu = u + g.hx, cv = v + g.hy, cw = w + g.hz;
-                    const float *bz = Bz + w * rv_ * ru;
-                    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-                    for (int i = 1; i <= mR; i++) {
-                        float sp = 0.0f, sm = 0.0f;
-                        for (int j = -yR; j <= yR; j++) {
-                            sp += bz[(cv + j) * ru + cu + i];
-                            sm += bz[(cv + j) * ru + cu - i];
-                        }
-                        gx += sp - sm;
-                    }
-                    for (int j = 1; j <= mR; j++) {
-                        float sp = 0.0f, sm = 0.0f;
-                        for (int i = -xR; i <= xR; i++) {
-                            sp += bz[(cv + j) * ru + cu + i];
-                            sm += bz[(cv - j) * ru + cu + i];
-                        }
-                        gy += sp - sm;
-                    }
-                    for (int k = 1; k <= mR; k++) {
-                        float sp = 0.0f, sm = 0.0f;
-                        for (int i = -xR; i <= xR; i++) {
-                            sp += By[((cw + k) * nv + v) * ru + cu + i];
-                            sm += By[((cw - k) * nv + v) * ru + cu + i];
-                        }
-                        gz += sp - sm;
-                    }
                     const int gi3 = (w * NV + v) * NU + u;
+                    const int cu = u + g.hx, cv = v + g.hy, cw = w + g.hz;
+                    auto R3 = [&](int du, int dv, int dw) {
+                        return raw[((cw + dw) * RV + (cv + dv)) * RU + (cu + du)];
+                    };
+                    float gx, gy, gz;
+                    if (P.smooth == 0) {
+                        gx = -R3(-1, 0, 0) + R3(1, 0, 0);
+                        gy = -R3(0, -1, 0) + R3(0, 1, 0);
+                        gz = -R3(0, 0, -1) + R3(0, 0, 1);
+                    } else {
+                        const int mR = P.mR, xR = P.xR, yR = P.yR, zR = P.zR;
+                        float s = 0.0f;
+                        for (int i = -mR; i <= mR; i++)
+                            for (int j = -yR; j <= yR; j++)
+                                for (int k = -zR; k <= zR; k++) {
+                                    if (i < 0) s -= R3(i, j, k);
+                                    else if (i > 0) s += R3(i, j, k);
+                                }
+                        gx = s;
+                        s = 0.0f;
+                        for (int j = -mR; j <= mR; j++)
+                            for (int i = -xR; i <= xR; i++)
+                                for (int k = -zR; k <= zR; k++) {
+                                    if (j < 0) s -= R3(i, j, k);
+                                    else if (j > 0) s += R3(i, j, k);
+                                }
+                        gy = s;
+                        s = 0.0f;
+                        for (int k = -mR; k <= mR; k++)
+                            for (int i = -xR; i <= xR; i++)
+                                for (int j = -yR; j <= yR; j++) {
+                                    if (k < 0) s -= R3(i, j, k);
+                                    else if (k > 0) s += R3(i, j, k);
+                                }
+                        gz = s;
+                    }
+                    const int NP = NU * NV * NW;
                     G[gi3] = gx;
                     G[NP + gi3] = gy;
                     G[2 * NP + gi3] = gz;
                 }
-            }
-            for (int e = tid; e < (tile_skip || sep ? 0 : nu * nv * nw); e += kThreads) {
-                const int q = qdiv(e, r_nu), w = qdiv(q, r_nv);
-                const int u = e - q * nu, v = q - w * nv;
-                const int gi3 = (w * NV + v) * NU + u;
-                const int cu = u + g.hx, cv = v + g.hy, cw = w + g.hz;
-                auto R3 = [&](int du, int dv, int dw) {
-                    return raw[((cw + dw) * RV + (cv + dv)) * RU + (cu + du)];
-                };
-                float gx, gy, gz;
-                if (P.smooth == 0) {
-                    gx = -R3(-1, 0, 0) + R3(1, 0, 0);
-                    gy = -R3(0, -1, 0) + R3(0, 1, 0);
-                    gz = -R3(0, 0, -1) + R3(0, 0, 1);
-                } else {
-                    const int mR = P.mR, xR = P.xR, yR = P.yR, zR = P.zR;
-                    float s = 0.0f;
-                    for (int i = -mR; i <= mR; i++)
-                        for (int j = -yR; j <= yR; j++)
-                            for (int k = -zR; k <= zR; k++) {
-                                if (i < 0) s -= R3(i, j, k);
-                                else if (i > 0) s += R3(i, j, k);
-                            }
-                    gx = s;
-                    s = 0.0f;
-                    for (int j = -mR; j <= mR; j++)
-                        for (int i = -xR; i <= xR; i++)
-                            for (int k = -zR; k <= zR; k++) {
-                                if (j < 0) s -= R3(i, j, k);
-                                else if (j > 0) s += R3(i, j, k);
-                            }
-                    gy = s;
-                    s = 0.0f;
-                    for (int k = -mR; k <= mR; k++)
-                        for (int i = -xR; i <= xR; i++)
-                            for (int j = -yR; j <= yR; j++) {
-                                if (k < 0) s -= R3(i, j, k);
-                                else if (k > 0) s += R3(i, j, k);
-                            }
-                    gz = s;
-                }
-                const int NP = NU * NV * NW;
-                G[gi3] = gx;
-                G[NP + gi3] = gy;
-                G[2 * NP + gi3] = gz;
-            }
 
-            // ---- intensity at the voxel centre (interpolateIntensity :257-287),
-            // before the sample passes: the raw halo is dead afterwards and
-            // its LDS is reused by the y-interpolated columns.
-            float Iv[TZ];
+                // ---- intensity at the voxel centre (interpolateIntensity :257-287),
+                // before the sample passes: the raw halo is dead afterwards and
+                // its LDS is reused by the y-interpolated columns.
 #pragma unroll
-            for (int Z = 0; Z < TZ; Z++) Iv[Z] = 0.0f;
-            if (P.iw == 1 && !idle) {
-                if (tid < kTX) {
-                    int b, valid;
-                    float w0, w1, w2, w3;
-                    axis_weights((float)(X0 + tid) + 0.5f, M, W, b, w0, w1, w2, w3, valid);
-                    float *xt = xtab + tid * 6;
-                    xt[0] = __int_as_float(clampi(b - ulo, 0, NU - 4));
-                    xt[1] = w0; xt[2] = w1; xt[3] = w2; xt[4] = w3;
-                }
-                __syncthreads();
-                for (int e = tid; e < nv * nw * kTX; e += kThreads) {
-                    const int q = e / kTX, v = qdiv(q, r_nw);
-                    const int X = e % kTX, w = q - v * nw;
-                    const float *xt = xtab + X * 6;
-                    const int ub = __float_as_int(xt[0]);
-                    const float *rr = raw + ((w + g.hz) * RV + (v + g.hy)) * RU + ub + g.hx;
-                    float p = 0.0f;
-                    p = p + rr[0] * xt[1];
-                    p = p + rr[1] * xt[2];
-                    p = p + rr[2] * xt[3];
-                    p = p + rr[3] * xt[4];
-                    Px[pxv ? v * VS + X * NWP + w : (v * NW + w) * TXP + X] = p;
-                }
-                __syncthreads();
-                int vb, validy;
-                float wy0, wy1, wy2, wy3;
-                axis_weights(Yc, M, H, vb, wy0, wy1, wy2, wy3, validy);
-                const int vrel = clampi(vb - vlo, 0, NV - 4);
-                const int validx = !((Xc / (float)M) < 1.5f || (Xc / (float)M) > (float)W - 1.5f);
-                for (int w = 0; w < nw; w++) {
-                    const float *pp = Px + (pxv ? vrel * VS + xl * NWP + w : (vrel * NW + w) * TXP + xl);
-                    const int vs = pxv ? VS : NW * TXP;
-                    float q = 0.0f;
-                    q = q + pp[0] * wy0;
-                    q = q + pp[vs] * wy1;
-                    q = q + pp[2 * vs] * wy2;
-                    q = q + pp[3 * vs] * wy3;
-                    Py[w * kThreads + tid] = q;
-                }
-#pragma unroll
-                for (int Z = 0; Z < TZ; Z++) {
-                    const float *ct = ctab + Z * 6;
-                    const int wrel = __float_as_int(ct[0]);
-                    const float *qq = Py + wrel * kThreads + tid;
-                    float sv = 0.0f;
-                    sv = sv + qq[0] * ct[1];
-                    sv = sv + qq[kThreads] * ct[2];
-                    sv = sv + qq[2 * kThreads] * ct[3];
-                    sv = sv + qq[3 * kThreads] * ct[4];
-                    const int valid = validx & validy & __float_as_int(ct[5]);
-                    Iv[Z] = valid ? fmaxf(sv, 0.0f) : 0.0f;
-                }
-            }
-
-            for (int z = 0; z < TZ; z++) dsum[z * kThreads + tid] = 0.0f;
-
-            for (int gi = 0; gi < ngroups && !tile_skip; gi++) {
-                const int ix = st_grp[4 * gi];
-                const float oxv = st_ox[ix];
-                // a merged group's planes have one more column (read by its
-                // shifted runs)
-                const int xw = kTX + st_grp[4 * gi + 3];
-                const float r_xw = 1.0f / (float)xw;
-                if (tid < xw) {
-                    int b, valid;
-                    float w0, w1, w2, w3;
-                    axis_weights(((float)(X0 + tid) + 0.5f) + oxv, M, W, b, w0, w1, w2, w3, valid);
-                    if (!valid) w0 = w1 = w2 = w3 = 0.0f;
-                    float *xt = xtab + tid * 6;
-                    xt[0] = __int_as_float(clampi(b - ulo, 0, NU - 4));
-                    xt[1] = w0; xt[2] = w1; xt[3] = w2; xt[4] = w3;
-                }
-                __syncthreads();
-                // ---- x pass: Px[c][v][w][X] = sum_i G[c][w][v][b+i] * wx_i ----
-                {
-                    const int NP = NU * NV * NW;
-                    // element e = (v, w, X): the constant divisor kTX for a
-                    // plain group, the reciprocal quotient for kTX + 1 columns
-                    auto xelem = [&](int X, int q) {
-                        const int v = qdiv(q, r_nw), w = q - v * nw;
+                for (int Z = 0; Z < TZ; Z++) Iv[Z] = 0.0f;
+                if (P.iw == 1) {
+                    if (tid < kTX) {
+                        int b, valid;
+                        float w0, w1, w2, w3;
+                        axis_weights((float)(X0 + tid) + 0.5f, M, W, b, w0, w1, w2, w3, valid);
+                        float *xt = xtab + tid * 6;
+                        xt[0] = __int_as_float(clampi(b - ulo, 0, NU - 4));
+                        xt[1] = w0; xt[2] = w1; xt[3] = w2; xt[4] = w3;
+                    }
+                    __syncthreads();
+                    for (int e = tid; e < nv * nw * kTX; e += kThreads) {
+                        const int q = e / kTX, v = qdiv(q, r_nw);
+                        const int X = e % kTX, w = q - v * nw;
                         const float *xt = xtab + X * 6;
                         const int ub = __float_as_int(xt[0]);
-                        const float w0 = xt[1], w1 = xt[2], w2 = xt[3], w3 = xt[4];
-                        const int gb = (w * NV + v) * NU + ub;
-#pragma unroll
-                        for (int c = 0; c < 3; c++) {
-                            const float *gg = G + c * NP + gb;
-                            float p = gg[0] * w0;
-                            p = p + gg[1] * w1;
-                            p = p + gg[2] * w2;
-                            p = p + gg[3] * w3;
-                            Px[pxv ? (c * NV + v) * VS + X * NWP + w : ((c * NV + v) * NW + w) * TXP + X] = p;
-                        }
-                    };
-                    if (xw == kTX) {
-                        for (int e = tid; e < nv * nw * kTX; e += kThreads) xelem(e % kTX, e / kTX);
-                    } else {
-                        for (int e = tid; e < nv * nw * xw; e += kThreads) {
-                            const int q = qdiv(e, r_xw);
-                            xelem(e - q * xw, q);
-                        }
+                        const float *rr = raw + ((w + g.hz) * RV + (v + g.hy)) * RU + ub + g.hx;
+                        float p = 0.0f;
+                        p = p + rr[0] * xt[1];
+                        p = p + rr[1] * xt[2];
+                        p = p + rr[2] * xt[3];
+                        p = p + rr[3] * xt[4];
+                        Px[pxv ? v * VS + X * NWP + w : (v * NW + w) * TXP + X] = p;
                     }
-                }
-                __syncthreads();
-
-                const int r0 = st_grp[4 * gi + 1], nr = st_grp[4 * gi + 2];
-                // a wave whose voxels all skip does no y pass or samples
-                const bool wave_skip = __all(colbad || zmask == zall);
-                for (int r = r0; r < r0 + nr && !wave_skip; r++) {
-                    const int iy = st_run[6 * r + 2];
-                    // the run's own x offset (a shifted run: the next plane column)
-                    const float rxs = __int_as_float(st_run[6 * r + 3]);
-                    const int xs = xl + st_run[6 * r + 5];
-                    const float x0 = Xc + rxs;
-                    const float dxs = x0 - Xc;
-                    const float y0 = Yc + st_oy[iy];
-                    const float dys = y0 - Yc;
+                    __syncthreads();
                     int vb, validy;
                     float wy0, wy1, wy2, wy3;
-                    axis_weights(y0, M, H, vb, wy0, wy1, wy2, wy3, validy);
-                    if (!validy) wy0 = wy1 = wy2 = wy3 = 0.0f;
+                    axis_weights(Yc, M, H, vb, wy0, wy1, wy2, wy3, validy);
                     const int vrel = clampi(vb - vlo, 0, NV - 4);
-                    // ---- y pass: Py[w][c][tid] = sum_j Px[c][vrel+j][w][X] * wy_j
-                    if (pxv) {
-                        // column layout: the 3 x 4 tap rows of two w values
-                        // are loaded as pairs first, then summed in order
-                        const float *pb = Px + vrel * VS + xs * NWP;
-#pragma unroll 4
-                        for (int w = 0; w < nw; w += 2) {
-                            float2 pv[3][4];
+                    const int validx = !((Xc / (float)M) < 1.5f || (Xc / (float)M) > (float)W - 1.5f);
+                    for (int w = 0; w < nw; w++) {
+                        const float *pp = Px + (pxv ? vrel * VS + xl * NWP + w : (vrel * NW + w) * TXP + xl);
+                        const int vs = pxv ? VS : NW * TXP;
+                        float q = 0.0f;
+                        q = q + pp[0] * wy0;
+                        q = q + pp[vs] * wy1;
+                        q = q + pp[2 * vs] * wy2;
+                        q = q + pp[3 * vs] * wy3;
+                        Py[w * kThreads + tid] = q;
+                    }
 #pragma unroll
-                            for (int c = 0; c < 3; c++)
-#pragma unroll
-                                for (int j = 0; j < 4; j++)
-                                    pv[c][j] = *(const float2 *)(pb + (c * NV + j) * VS + w);
+                    for (int Z = 0; Z < TZ; Z++) {
+                        const float *ct = ctab + Z * 6;
+                        const int wrel = __float_as_int(ct[0]);
+                        const float *qq = Py + wrel * kThreads + tid;
+                        float sv = 0.0f;
+                        sv = sv + qq[0] * ct[1];
+                        sv = sv + qq[kThreads] * ct[2];
+                        sv = sv + qq[2 * kThreads] * ct[3];
+                        sv = sv + qq[3 * kThreads] * ct[4];
+                        const int valid = validx & validy & __float_as_int(ct[5]);
+                        Iv[Z] = valid ? fmaxf(sv, 0.0f) : 0.0f;
+                    }
+                }
+
+                for (int z = 0; z < TZ; z++) dsum[z * kThreads + tid] = 0.0f;
+
+                for (int gi = 0; gi < ngroups && !tile_skip; gi++) {
+                    const int ix = st_grp[4 * gi];
+                    const float oxv = st_ox[ix];
+                    // a merged group's planes have one more column (read by its
+                    // shifted runs)
+                    const int xw = kTX + st_grp[4 * gi + 3];
+                    const float r_xw = 1.0f / (float)xw;
+                    if (tid < xw) {
+                        int b, valid;
+                        float w0, w1, w2, w3;
+                        axis_weights(((float)(X0 + tid) + 0.5f) + oxv, M, W, b, w0, w1, w2, w3, valid);
+                        if (!valid) w0 = w1 = w2 = w3 = 0.0f;
+                        float *xt = xtab + tid * 6;
+                        xt[0] = __int_as_float(clampi(b - ulo, 0, NU - 4));
+                        xt[1] = w0; xt[2] = w1; xt[3] = w2; xt[4] = w3;
+                    }
+                    __syncthreads();
+                    // ---- x pass: Px[c][v][w][X] = sum_i G[c][w][v][b+i] * wx_i ----
+                    {
+                        const int NP = NU * NV * NW;
+                        // element e = (v, w, X): the constant divisor kTX for a
+                        // plain group, the reciprocal quotient for kTX + 1 columns
+                        auto xelem = [&](int X, int q) {
+                            const int v = qdiv(q, r_nw), w = q - v * nw;
+                            const float *xt = xtab + X * 6;
+                            const int ub = __float_as_int(xt[0]);
+                            const float w0 = xt[1], w1 = xt[2], w2 = xt[3], w3 = xt[4];
+                            const int gb = (w * NV + v) * NU + ub;
 #pragma unroll
                             for (int c = 0; c < 3; c++) {
-                                float q = pv[c][0].x * wy0;
-                                q = q + pv[c][1].x * wy1;
-                                q = q + pv[c][2].x * wy2;
-                                q = q + pv[c][3].x * wy3;
-                                Py[(w * 3 + c) * kThreads + tid] = q;
+                                const float *gg = G + c * NP + gb;
+                                float p = gg[0] * w0;
+                                p = p + gg[1] * w1;
+                                p = p + gg[2] * w2;
+                                p = p + gg[3] * w3;
+                                Px[pxv ? (c * NV + v) * VS + X * NWP + w : ((c * NV + v) * NW + w) * TXP + X] = p;
                             }
-                            if (w + 1 < nw) {
+                        };
+                        if (xw == kTX) {
+                            for (int e = tid; e < nv * nw * kTX; e += kThreads) xelem(e % kTX, e / kTX);
+                        } else {
+                            for (int e = tid; e < nv * nw * xw; e += kThreads) {
+                                const int q = qdiv(e, r_xw);
+                                xelem(e - q * xw, q);
+                            }
+                        }
+                    }
+                    __syncthreads();
+
+                    const int r0 = st_grp[4 * gi + 1], nr = st_grp[4 * gi + 2];
+                    // a wave whose voxels all skip does no y pass or samples
+                    const bool wave_skip = __all(colbad || zmask == zall);
+                    for (int r = r0; r < r0 + nr && !wave_skip; r++) {
+                        const int iy = st_run[6 * r + 2];
+                        // the run's own x offset (a shifted run: the next plane column)
+                        const float rxs = __int_as_float(st_run[6 * r + 3]);
+                        const int xs = xl + st_run[6 * r + 5];
+                        const float x0 = Xc + rxs;
+                        const float dxs = x0 - Xc;
+                        const float y0 = Yc + st_oy[iy];
+                        const float dys = y0 - Yc;
+                        int vb, validy;
+                        float wy0, wy1, wy2, wy3;
+                        axis_weights(y0, M, H, vb, wy0, wy1, wy2, wy3, validy);
+                        if (!validy) wy0 = wy1 = wy2 = wy3 = 0.0f;
+                        const int vrel = clampi(vb - vlo, 0, NV - 4);
+                        // ---- y pass: Py[w][c][tid] = sum_j Px[c][vrel+j][w][X] * wy_j
+                        if (pxv) {
+                            // column layout: the 3 x 4 tap rows of two w values
+                            // are loaded as pairs first, then summed in order
+                            const float *pb = Px + vrel * VS + xs * NWP;
+#pragma unroll 4
+                            for (int w = 0; w < nw; w += 2) {
+                                float2 pv[3][4];
+#pragma unroll
+                                for (int c = 0; c < 3; c++)
+#pragma unroll
+                                    for (int j = 0; j < 4; j++)
+                                        pv[c][j] = *(const float2 *)(pb + (c * NV + j) * VS + w);
 #pragma unroll
                                 for (int c = 0; c < 3; c++) {
-                                    float q = pv[c][0].y * wy0;
-                                    q = q + pv[c][1].y * wy1;
-                                    q = q + pv[c][2].y * wy2;
-                                    q = q + pv[c][3].y * wy3;
-                                    Py[((w + 1) * 3 + c) * kThreads + tid] = q;
+                                    float q = pv[c][0].x * wy0;
+                                    q = q + pv[c][1].x * wy1;
+                                    q = q + pv[c][2].x * wy2;
+                                    q = q + pv[c][3].x * wy3;
+                                    Py[(w * 3 + c) * kThreads + tid] = q;
+                                }
+                                if (w + 1 < nw) {
+#pragma unroll
+                                    for (int c = 0; c < 3; c++) {
+                                        float q = pv[c][0].y * wy0;
+                                        q = q + pv[c][1].y * wy1;
+                                        q = q + pv[c][2].y * wy2;
+                                        q = q + pv[c][3].y * wy3;
+                                        Py[((w + 1) * 3 + c) * kThreads + tid] = q;
+                                    }
+                                }
+                            }
+                        } else {
+                            for (int w = 0; w < nw; w++) {
+#pragma unroll
+                                for (int c = 0; c < 3; c++) {
+                                    const float *pp = Px + ((c * NV + vrel) * NW + w) * TXP + xs;
+                                    const int vs = NW * TXP;
+                                    float q = pp[0] * wy0;
+                                    q = q + pp[vs] * wy1;
+                                    q = q + pp[2 * vs] * wy2;
+                                    q = q + pp[3 * vs] * wy3;
+                                    Py[(w * 3 + c) * kThreads + tid] = q;
                                 }
                             }
                         }
-                    } else {
-                        for (int w = 0; w < nw; w++) {
+                        const int s0 = st_run[6 * r], ns = st_run[6 * r + 1];
+                        const float rys = __int_as_float(st_run[6 * r + 4]);
+                        // Voxel pairs (Z, Z+1): two independent chains per sample.
+                        for (int Z = 0; Z < TZ; Z += 2) {
+                            if (__all(colbad || ((zmask >> Z) & 3u) == 3u)) continue;
+                            float dd0 = dsum[Z * kThreads + tid], dd1 = dsum[(Z + 1) * kThreads + tid];
+                            TermIn a0, a1;
+                            a0.rx = a1.rx = rxs; a0.ry = a1.ry = rys;
+                            a0.dx = a1.dx = dxs; a0.dy = a1.dy = dys;
+                            a0.x0 = a1.x0 = x0; a0.y0 = a1.y0 = y0;
+                            a0.Xc = a1.Xc = Xc; a0.Yc = a1.Yc = Yc;
+                            a0.Zc = zcen[Z];
+                            a1.Zc = zcen[Z + 1];
+                            const float *zrow0 = ztab + Z * noz * 8, *zrow1 = zrow0 + noz * 8;
+                            const float *Pyt = Py + tid;
+                            for (int s = s0; s < s0 + ns; s++) {
+                                const int izb = st_smp[2 * s];
+                                a0.rz = a1.rz = __int_as_float(st_smp[2 * s + 1]);
+                                const float *zt0 = (const float *)((const char *)zrow0 + izb);
+                                const float *zt1 = (const float *)((const char *)zrow1 + izb);
+                                const int wr0 = __float_as_int(zt0[0]), wr1 = __float_as_int(zt1[0]);
+                                a0.z0 = zt0[6]; a0.dz = zt0[7];
+                                a1.z0 = zt1[6]; a1.dz = zt1[7];
+                                // ---- z pass (interpolateG :154-167, innermost sum last)
+                                float G0[3], G1[3];
 #pragma unroll
-                            for (int c = 0; c < 3; c++) {
-                                const float *pp = Px + ((c * NV + vrel) * NW + w) * TXP + xs;
-                                const int vs = NW * TXP;
-                                float q = pp[0] * wy0;
-                                q = q + pp[vs] * wy1;
-                                q = q + pp[2 * vs] * wy2;
-                                q = q + pp[3 * vs] * wy3;
-                                Py[(w * 3 + c) * kThreads + tid] = q;
+                                for (int c = 0; c < 3; c++) {
+                                    const float *q0 = Pyt + wr0 + c * kThreads;
+                                    const float *q1 = Pyt + wr1 + c * kThreads;
+                                    float s0v = q0[0] * zt0[1];
+                                    float s1v = q1[0] * zt1[1];
+                                    s0v = s0v + q0[3 * kThreads] * zt0[2];
+                                    s1v = s1v + q1[3 * kThreads] * zt1[2];
+                                    s0v = s0v + q0[6 * kThreads] * zt0[3];
+                                    s1v = s1v + q1[6 * kThreads] * zt1[3];
+                                    s0v = s0v + q0[9 * kThreads] * zt0[4];
+                                    s1v = s1v + q1[9 * kThreads] * zt1[4];
+                                    G0[c] = s0v;
+                                    G1[c] = s1v;
+                                }
+                                bool b0, b1;
+                                float t0 = term_fast<RP2, kIntInput<Tin>>(G0[0], G0[1], G0[2], a0, sR, sInvR, b0);
+                                float t1 = term_fast<RP2, kIntInput<Tin>>(G1[0], G1[1], G1[2], a1, sR, sInvR, b1);
+                                b0 = b0 || !fast_ok;
+                                b1 = b1 || !fast_ok;
+                                if (b0 || b1) {
+                                    if (b0) t0 = term_general<RP2>(G0[0], G0[1], G0[2], a0, sR, sInvR);
+                                    if (b1) t1 = term_general<RP2>(G1[0], G1[1], G1[2], a1, sR, sInvR);
+                                }
+                                dd0 = dd0 + t0;
+                                dd1 = dd1 + t1;
                             }
+                            dsum[Z * kThreads + tid] = dd0;
+                            dsum[(Z + 1) * kThreads + tid] = dd1;
                         }
                     }
-                    const int s0 = st_run[6 * r], ns = st_run[6 * r + 1];
-                    const float rys = __int_as_float(st_run[6 * r + 4]);
-                    // Voxel pairs (Z, Z+1): two independent chains per sample.
-                    for (int Z = 0; Z < TZ; Z += 2) {
-                        if (__all(colbad || ((zmask >> Z) & 3u) == 3u)) continue;
-                        float dd0 = dsum[Z * kThreads + tid], dd1 = dsum[(Z + 1) * kThreads + tid];
-                        TermIn a0, a1;
-                        a0.rx = a1.rx = rxs; a0.ry = a1.ry = rys;
-                        a0.dx = a1.dx = dxs; a0.dy = a1.dy = dys;
-                        a0.x0 = a1.x0 = x0; a0.y0 = a1.y0 = y0;
-                        a0.Xc = a1.Xc = Xc; a0.Yc = a1.Yc = Yc;
-                        a0.Zc = zcen[Z];
-                        a1.Zc = zcen[Z + 1];
-                        const float *zrow0 = ztab + Z * noz * 8, *zrow1 = zrow0 + noz * 8;
-                        const float *Pyt = Py + tid;
-                        for (int s = s0; s < s0 + ns; s++) {
-                            const int izb = st_smp[2 * s];
-                            a0.rz = a1.rz = __int_as_float(st_smp[2 * s + 1]);
-                            const float *zt0 = (const float *)((const char *)zrow0 + izb);
-                            const float *zt1 = (const float *)((const char *)zrow1 + izb);
-                            const int wr0 = __float_as_int(zt0[0]), wr1 = __float_as_int(zt1[0]);
-                            a0.z0 = zt0[6]; a0.dz = zt0[7];
-                            a1.z0 = zt1[6]; a1.dz = zt1[7];
-                            // ---- z pass (interpolateG :154-167, innermost sum last)
-                            float G0[3], G1[3];
-#pragma unroll
-                            for (int c = 0; c < 3; c++) {
-                                const float *q0 = Pyt + wr0 + c * kThreads;
-                                const float *q1 = Pyt + wr1 + c * kThreads;
-                                float s0v = q0[0] * zt0[1];
-                                float s1v = q1[0] * zt1[1];
-                                s0v = s0v + q0[3 * kThreads] * zt0[2];
-                                s1v = s1v + q1[3 * kThreads] * zt1[2];
-                                s0v = s0v + q0[6 * kThreads] * zt0[3];
-                                s1v = s1v + q1[6 * kThreads] * zt1[3];
-                                s0v = s0v + q0[9 * kThreads] * zt0[4];
-                                s1v = s1v + q1[9 * kThreads] * zt1[4];
-                                G0[c] = s0v;
-                                G1[c] = s1v;
-                            }
-                            bool b0, b1;
-                            float t0 = term_fast<RP2, kIntInput<Tin>>(G0[0], G0[1], G0[2], a0, sR, sInvR, b0);
-                            float t1 = term_fast<RP2, kIntInput<Tin>>(G1[0], G1[1], G1[2], a1, sR, sInvR, b1);
-                            b0 = b0 || !fast_ok;
-                            b1 = b1 || !fast_ok;
-                            if (b0 || b1) {
-                                if (b0) t0 = term_general<RP2>(G0[0], G0[1], G0[2], a0, sR, sInvR);
-                                if (b1) t1 = term_general<RP2>(G1[0], G1[1], G1[2], a1, sR, sInvR);
-                            }
-                            dd0 = dd0 + t0;
-                            dd1 = dd1 + t1;
-                        }
-                        dsum[Z * kThreads + tid] = dd0;
-                        dsum[(Z + 1) * kThreads + tid] = dd1;
-                    }
+                    __syncthreads();
                 }
-                __syncthreads();
-            }
 
-            // ---- per-frame radiality value (kernel.cu:244-251)
-            float rv[TZ];
+
+                // ---- per-frame radiality value (kernel.cu:244-251)
 #pragma unroll
-            for (int Z = 0; Z < TZ; Z++) {
-                float dd = (colbad || ((zmask >> Z) & 1u)) ? __builtin_nanf("") : dsum[Z * kThreads + tid] / sDiv;
-                if (P.renorm == 1) dd = 0.5f + (dd / 2.0f);
-                rv[Z] = (P.remove_con == 1) ? fmaxf(dd, 0.0f) : dd;
+                for (int Z = 0; Z < TZ; Z++) {
+                    float dd = (colbad || ((zmask >> Z) & 1u)) ? __builtin_nanf("") : dsum[Z * kThreads + tid] / sDiv;
+                    if (P.renorm == 1) dd = 0.5f + (dd / 2.0f);
+                    rv[Z] = (P.remove_con == 1) ? fmaxf(dd, 0.0f) : dd;
+                }
             }
 
             // ---- temporal accumulation (same order as the reference)
