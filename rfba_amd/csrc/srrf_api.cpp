@@ -439,8 +439,8 @@ void axis_valid(int b0, int b1, const float *off, int noff, int M, int dim, int 
 // passes (0.20) and the sample passes (0.62), which a wave (an 8 x 8 quadrant
 // of the tile's columns) skips per pair of z slices when none of its voxels
 // has samples in range.  A tile without one (idle) pays idle_cost: 0.18, or
-// almost nothing where the kernel skips its halo and intensities (integer
-// input or no intensity weighting).  The costs are separable per axis, so
+// almost nothing where the kernel skips its frame body (integer input or no
+// intensity weighting).  The costs are separable per axis, so
 // each axis contributes its tile count n, its tiles with a valid voxel e and
 // its active fraction a (quadrant halves for x and y, slice pairs for z).
 // Only phases whose tiles span no more low-res taps than phase 0's are
@@ -569,7 +569,7 @@ int plan_geometry(const std::vector<DevItem> &items, const DevSamples &s, const 
     pl.phase.resize(items.size());
     for (size_t i = 0; i < items.size(); i++) {
         const auto &it = items[i];
-        const auto ph = tile_phases(it, s, TZ, (dtype == RFBA_U16 || c.IntensityW != 1) ? 0.02 : 0.18);
+        const auto ph = tile_phases(it, s, TZ, (dtype == RFBA_U16 || c.IntensityW != 1) ? 0.005 : 0.18);
         pl.phase[i] = ph;
         axis_extent(it.bx0, it.bx1, ph[0], kTX, s.oxmin, s.oxmax, it.M, NU);
         axis_extent(it.by0, it.by1, ph[1], kTY, s.oymin, s.oymax, it.M, NV);
