@@ -396,3 +396,42 @@ def test_c3_shaped_batch_vs_oracle(gpu, method):
     b.close()
     d_src.close()
     assert nbox == 3 * len(blocks)
+
+
+@pytest.mark.parametrize("method,ndirs", [(1, 6), (3, 26), (0, 26)])
+def test_batch_unusual_geometries(gpu, method, ndirs):
+    # Magnifications 1, 5 and 6 (tile phases at every residue of 16), single
+    # frames, and blocks only 3 voxels wide or deep (every voxel NaN by
+    # definition, every tile idle) in one batch: every SR voxel is written,
+    # the batched and the per-block form agree bit for bit, and both equal
+    # the oracle evaluated on each block as its own stack.
+    src = series((7, 14, 15, 16), 6, 41)
+    specs = [((0, 0, 0), (9, 10, 11), 7, 1), ((3, 2, 1), (7, 8, 9), 1, 5), ((5, 5, 5), (6, 5, 4), 2, 6),
+             ((1, 1, 1), (3, 12, 10), 7, 3), ((2, 3, 4), (11, 9, 3), 5, 2), ((0, 4, 0), (16, 4, 14), 3, 4)]
+    c = conf(Method=method, IntensityW=1, ITC=0, TRACOrder=2)
+    smp = srrf.directions(ndirs, 0.5)
+    d_src, blocks = _c3_blocks(src, specs)
+    b = srrf.Batch(blocks, srrf.RFBA_U16, c, smp)
+    outs = {}
+    for run in ("per_block", "batched"):
+        for bl in blocks:
+            bl.out.fill_byte(0x7F)
+        if run == "per_block":
+            b.run_per_block()
+        else:
+            b.run(use_graph=True)
+        outs[run] = []
+        for bl in blocks:
+            (dx, dy, dz), M = bl.dims_xyz, bl.mag
+            got = bl.out.to_array((dz * M, dy * M, dx * M), np.float32)
+            assert not np.any(got.view(np.uint32) == 0x7F7F7F7F), (run, bl.origin_xyz, bl.dims_xyz, M)
+            outs[run].append(got)
+    for i, bl in enumerate(blocks):
+        (ox, oy, oz), (dx, dy, dz) = bl.origin_xyz, bl.dims_xyz
+        lab = "block %s+%s T=%d M=%d" % (bl.origin_xyz, bl.dims_xyz, bl.frames, bl.mag)
+        assert np.array_equal(outs["batched"][i], outs["per_block"][i], equal_nan=True), lab
+        sub = np.ascontiguousarray(src[:bl.frames, oz:oz + dz, oy:oy + dy, ox:ox + dx]).astype(np.float32)
+        o = oracle.srrf_stack(sub, c.replace(Magnification=bl.mag), smp.xyz, smp.divisor)
+        assert np.array_equal(outs["batched"][i], o, equal_nan=True), lab
+    b.close()
+    d_src.close()
